@@ -1,0 +1,331 @@
+"""MI355X-native stabilized-FEM formation and memLS solve for CRIMSON Flowsolver.
+
+The product is the C-ABI library libcrimson_amd.so (HIP kernels for gfx950,
+declared in include/crimson_amd.h).  This package is its host-side mirror:
+ctypes bindings whose names and argument meanings follow the reference's
+interfaces for this path:
+
+* ``ElmGMRInterior`` -- the interior part of ``ElmGMR`` (elmgmr.f90:1-255):
+  flux reconstruction, e3 formation, residual scatter and fillsparseI.
+* ``memLS_LS_CREATE`` / ``memLS_LHS_CREATE`` / ``memLS_BC_CREATE`` /
+  ``memLS_SOLVE`` -- memLS.f90:1379,1080,272,1947 with the same argument
+  order, optional arguments and in/out semantics (``Ri`` is overwritten by the
+  solution).
+
+Errors raise ``CrimsonError`` carrying the library's status code, the
+counterpart of the reference's PRINT + STOP.  There is no CPU fallback: if the
+library is missing or no GPU is visible, the calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcrimson_amd.so")
+
+LS_TYPE_CG, LS_TYPE_GMRES, LS_TYPE_NS = 1, 2, 3
+BC_TYPE_Dir, BC_TYPE_Neu = 0, 1
+
+ERRORS = {1: "EARG", 2: "EUNSUPPORTED", 3: "EGRAPH", 4: "ENODIAG", 5: "EHIP", 6: "ENOMEM",
+          7: "ESINGULAR", 8: "ESTATE"}
+
+EXPORTS = (
+    "crimson_last_error", "crimson_device_count", "crimson_form_setup", "crimson_form_interior",
+    "crimson_form_interior_dev", "crimson_form_interior_to_lhs", "crimson_form_bc3res_dev",
+    "crimson_form_timing", "crimson_form_free", "crimson_genadj", "crimson_ls_create", "crimson_ls_free",
+    "crimson_lhs_create", "crimson_lhs_free", "crimson_bc_create", "crimson_ls_solve",
+    "crimson_ls_solve_dev", "crimson_lhs_set_val", "crimson_lhs_get_val", "crimson_ls_stats",
+    "crimson_lhs_spmv_dev",
+)
+
+
+class CrimsonError(RuntimeError):
+    def __init__(self, code, where, text):
+        super().__init__(f"{where}: {ERRORS.get(code, code)}: {text}")
+        self.code = code
+
+
+class FormParams(C.Structure):
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("Delt", C.c_double),
+                ("alfi", C.c_double), ("almi", C.c_double), ("gami", C.c_double),
+                ("dtsfct", C.c_double), ("taucfct", C.c_double), ("flmpl", C.c_double),
+                ("flmpr", C.c_double), ("itau", C.c_int), ("iconvflow", C.c_int),
+                ("idiff", C.c_int), ("ipord", C.c_int), ("precision", C.c_int),
+                ("aspect_fp64", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it was not built.
+
+    PyTorch ships its own HIP runtime with the same SONAME as ROCm's
+    (libamdhip64.so.7).  Importing torch first makes this library bind to that
+    already-loaded runtime, so a process that also allocates through torch
+    holds a single HIP runtime instead of two that contend for the device."""
+    global _lib
+    if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                              "(make -C crimsonflowsolver_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.crimson_last_error.argtypes = [C.c_char_p, C.c_int]
+        L.crimson_device_count.argtypes = []
+        L.crimson_form_setup.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp,
+                                         C.POINTER(FormParams), C.POINTER(C.c_void_p)]
+        L.crimson_form_interior.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]
+        L.crimson_form_interior_dev.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]
+        L.crimson_form_interior_to_lhs.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.crimson_form_bc3res_dev.argtypes = [vp, vp, C.c_int]
+        L.crimson_form_timing.argtypes = [vp, dp]
+        L.crimson_form_free.argtypes = [vp]
+        L.crimson_genadj.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, ip]
+        L.crimson_ls_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                        vp, vp, vp, C.POINTER(C.c_void_p)]
+        L.crimson_ls_free.argtypes = [vp]
+        L.crimson_lhs_create.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int,
+                                         C.POINTER(C.c_void_p)]
+        L.crimson_lhs_free.argtypes = [vp]
+        L.crimson_bc_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
+        L.crimson_ls_solve.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp]
+        L.crimson_ls_solve_dev.argtypes = [vp, vp, C.c_int, vp, vp, vp]
+        L.crimson_lhs_set_val.argtypes = [vp, C.c_int, vp]
+        L.crimson_lhs_get_val.argtypes = [vp, C.c_int, vp]
+        L.crimson_ls_stats.argtypes = [vp, C.c_int, dp]
+        L.crimson_lhs_spmv_dev.argtypes = [vp, C.c_int, vp, vp, C.c_int, dp]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().crimson_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc, where):
+    if rc != 0:
+        raise CrimsonError(rc, where, last_error())
+
+
+def device_count() -> int:
+    return lib().crimson_device_count()
+
+
+def _f(a, dt=np.float64):
+    return np.asfortranarray(a, dtype=dt)
+
+
+def _p(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):  # torch tensor on the device
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def form_params(rho=1.06, mu=0.04, delt=1e-3, rhoinf=0.5, precision=64, aspect_fp64=20.0):
+    """Material and generalized-alpha parameters (itrPC.f90:16-37)."""
+    if rhoinf < 0 or rhoinf > 1:
+        almi = alfi = gami = 1.0
+    else:
+        almi = (3.0 - rhoinf) / (1.0 + rhoinf) / 2.0
+        alfi = 1.0 / (1.0 + rhoinf)
+        gami = 0.5 + almi - alfi
+    return FormParams(rho, mu, delt, alfi, almi, gami, 1.0, 1.0, 0.0, 0.0, 0, 2, 1, 1,
+                      precision, aspect_fp64)
+
+
+def genadj(ien, nshg):
+    """Native genadj: (colm, rowp) 1-based."""
+    ien = _f(ien, np.int32)
+    nnz = C.c_int()
+    colm = np.zeros(nshg + 1, np.int32)
+    lib().crimson_genadj(nshg, ien.shape[0], _p(ien), _p(colm), None, 0, C.byref(nnz))
+    rowp = np.zeros(nnz.value, np.int32)
+    check(lib().crimson_genadj(nshg, ien.shape[0], _p(ien), _p(colm), _p(rowp), nnz.value,
+                               C.byref(nnz)), "genadj")
+    return colm, rowp
+
+
+class ElmGMRInterior:
+    """Interior ElmGMR on the GPU (elmgmr.f90:82-255).
+
+    ``ien`` (nel,4) 1-based, ``colm`` (nshg+1) and ``rowp`` (nnz) 1-based as
+    genadj builds them, ``iBC`` (nshg) optional."""
+
+    def __init__(self, ien, colm, rowp, nshg, numnp=None, iBC=None, params=None):
+        self.ien = _f(ien, np.int32)
+        self.colm = _f(colm, np.int32)
+        self.rowp = _f(rowp, np.int32)
+        self.nshg = nshg
+        self.numnp = nshg if numnp is None else numnp
+        self.nnz = self.rowp.shape[0]
+        self.params = params if params is not None else form_params()
+        ibc = None if iBC is None else _f(iBC, np.int32)
+        h = C.c_void_p()
+        check(lib().crimson_form_setup(nshg, self.numnp, self.ien.shape[0], _p(self.ien), _p(self.colm),
+                                       _p(self.rowp), self.nnz, _p(ibc), C.byref(self.params), C.byref(h)),
+              "crimson_form_setup")
+        self.h = h
+
+    def __call__(self, y, ac, x, lhs=1, want_q=False):
+        """Returns res(nshg,4), lhsK(9,nnz), lhsP(4,nnz)[, qres(nshg,9)]."""
+        y, ac, x = _f(y), _f(ac), _f(x)
+        res = np.zeros((self.nshg, 4), order="F")
+        K = np.zeros((9, self.nnz), order="F") if lhs else None
+        P = np.zeros((4, self.nnz), order="F") if lhs else None
+        q = np.zeros((self.nshg, 9), order="F") if want_q else None
+        check(lib().crimson_form_interior(self.h, _p(y), _p(ac), _p(x), _p(res), _p(K), _p(P), _p(q),
+                                          1 if lhs else 0), "crimson_form_interior")
+        return (res, K, P, q) if want_q else (res, K, P)
+
+    def dev(self, y, ac, x, res, lhsK=None, lhsP=None, qres=None, lhs=1):
+        """Device-pointer form on torch tensors laid out as the host arrays."""
+        check(lib().crimson_form_interior_dev(self.h, _p(y), _p(ac), _p(x), _p(res), _p(lhsK), _p(lhsP),
+                                              _p(qres), 1 if lhs else 0), "crimson_form_interior_dev")
+
+    def to_lhs(self, y, ac, x, res4, lhs_handle):
+        """Formation into the memLS matrix of ``lhs_handle``; ``res4`` receives
+        Res4(4,nshg) (a (nshg,4) C-ordered torch tensor)."""
+        check(lib().crimson_form_interior_to_lhs(self.h, _p(y), _p(ac), _p(x), _p(res4), lhs_handle.h),
+              "crimson_form_interior_to_lhs")
+
+    def bc3res(self, res, res4_layout=False):
+        check(lib().crimson_form_bc3res_dev(self.h, _p(res), 1 if res4_layout else 0),
+              "crimson_form_bc3res_dev")
+
+    def timing(self):
+        out = (C.c_double * 4)()
+        check(lib().crimson_form_timing(self.h, out), "crimson_form_timing")
+        return dict(flux_ms=out[0], e3_ms=out[1], assembly_ms=out[2], total_ms=out[3])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().crimson_form_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class memLS_lsType:
+    """Opaque memLS_lsType (memLS.f90:190-202) with the public stats fields."""
+
+    def __init__(self, h, LS_type):
+        self.h = h
+        self.LS_type = LS_type
+
+    def stats(self, sub="RI"):
+        out = (C.c_double * 8)()
+        check(lib().crimson_ls_stats(self.h, {"RI": 0, "GM": 1, "CG": 2}[sub], out), "crimson_ls_stats")
+        keys = ("iNorm", "fNorm", "itr", "suc", "dB", "callD", "Resm", "Resc")
+        return dict(zip(keys, list(out)))
+
+    @property
+    def RI(self):
+        return self.stats("RI")
+
+    @property
+    def GM(self):
+        return self.stats("GM")
+
+    @property
+    def CG(self):
+        return self.stats("CG")
+
+    def __del__(self):
+        try:
+            lib().crimson_ls_free(self.h)
+        except Exception:
+            pass
+
+
+class memLS_lhsType:
+    """Opaque memLS_lhsType (memLS.f90:145-160)."""
+
+    def __init__(self, h, nNo, nnz, nFaces):
+        self.h = h
+        self.nNo, self.nnz, self.nFaces = nNo, nnz, nFaces
+        self._keep = []
+
+    def spmv_dev(self, dof, x, y, reps=1):
+        ms = C.c_double()
+        check(lib().crimson_lhs_spmv_dev(self.h, dof, _p(x), _p(y), reps, C.byref(ms)), "crimson_lhs_spmv_dev")
+        return ms.value
+
+    def set_val(self, dof, Val):
+        Val = _f(Val)
+        check(lib().crimson_lhs_set_val(self.h, dof, _p(Val)), "crimson_lhs_set_val")
+
+    def get_val(self, dof=4):
+        out = np.zeros((dof * dof, self.nnz), order="F")
+        check(lib().crimson_lhs_get_val(self.h, dof, _p(out)), "crimson_lhs_get_val")
+        return out
+
+    def __del__(self):
+        try:
+            lib().crimson_lhs_free(self.h)
+        except Exception:
+            pass
+
+
+def memLS_LS_CREATE(LS_type, relTol=None, absTol=None, maxItr=None, dimKry=None, relTolIn=None,
+                    absTolIn=None, maxItrIn=None):
+    present = 0
+    for bit, v in ((1, relTol), (2, absTol), (4, maxItr), (8, dimKry), (16, relTolIn), (32, absTolIn),
+                   (64, maxItrIn)):
+        if v is not None:
+            present |= bit
+    rti = None if relTolIn is None else np.asarray(relTolIn, np.float64)
+    ati = None if absTolIn is None else np.asarray(absTolIn, np.float64)
+    mti = None if maxItrIn is None else np.asarray(maxItrIn, np.int32)
+    h = C.c_void_p()
+    check(lib().crimson_ls_create(LS_type, present, relTol or 0.0, absTol or 0.0, maxItr or 0, dimKry or 0,
+                                  _p(rti), _p(ati), _p(mti), C.byref(h)), "memLS_LS_CREATE")
+    return memLS_lsType(h, LS_type)
+
+
+def memLS_LHS_CREATE(gnNo, nNo, nnz, gNodes, rowPtr, colPtr, nFaces):
+    rowPtr, colPtr = _f(rowPtr, np.int32), _f(colPtr, np.int32)
+    g = None if gNodes is None else _f(gNodes, np.int32)
+    h = C.c_void_p()
+    check(lib().crimson_lhs_create(gnNo, nNo, nnz, _p(g), _p(rowPtr), _p(colPtr), nFaces, C.byref(h)),
+          "memLS_LHS_CREATE")
+    return memLS_lhsType(h, nNo, nnz, nFaces)
+
+
+def memLS_BC_CREATE(lhs, faIn, nNo, dof, BC_type, gNodes, Val=None):
+    g = _f(gNodes, np.int32)
+    v = None if Val is None else _f(Val)
+    check(lib().crimson_bc_create(lhs.h, faIn, nNo, dof, BC_type, _p(g), _p(v)), "memLS_BC_CREATE")
+
+
+def memLS_SOLVE(lhs, ls, dof, Ri, Val, incL=None, res=None):
+    """Ri (dof,nNo) Fortran-ordered float64 is overwritten with the solution."""
+    if not (isinstance(Ri, np.ndarray) and Ri.dtype == np.float64 and Ri.flags.f_contiguous):
+        raise CrimsonError(1, "memLS_SOLVE", "Ri must be a Fortran-ordered float64 array (in/out)")
+    Val = _f(Val)
+    il = None if incL is None else _f(incL, np.int32)
+    rs = None if res is None else _f(res)
+    check(lib().crimson_ls_solve(lhs.h, ls.h, dof, _p(Ri), _p(Val), _p(il), _p(rs)), "memLS_SOLVE")
+
+
+def memLS_SOLVE_dev(lhs, ls, dof, Ri_dev, incL=None, res=None):
+    """Device form: the matrix is the one held by ``lhs``; Ri_dev is a torch
+    tensor (nNo, dof) C-ordered = (dof, nNo) Fortran-ordered, in/out."""
+    il = None if incL is None else _f(incL, np.int32)
+    rs = None if res is None else _f(res)
+    check(lib().crimson_ls_solve_dev(lhs.h, ls.h, dof, _p(Ri_dev), _p(il), _p(rs)), "memLS_SOLVE_dev")

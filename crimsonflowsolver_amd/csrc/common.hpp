@@ -1,0 +1,117 @@
+// Shared internals of libcrimson_amd: error reporting, device buffers and the
+// memLS matrix handle that both the formation and the solver touch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/crimson_amd.h"
+
+namespace crimson {
+
+void set_error(const char *fmt, ...);
+const std::string &last_error();
+
+#define CR_HIP(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      crimson::set_error("%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                               \
+      return CRIMSON_EHIP;                                                        \
+    }                                                                             \
+  } while (0)
+
+#define CR_TRY(expr)            \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != CRIMSON_OK) return rc_; \
+  } while (0)
+
+// Owning device allocation.
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) {
+    if (count == n && p) return CRIMSON_OK;
+    release();
+    if (count == 0) return CRIMSON_OK;
+    hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+      return CRIMSON_ENOMEM;
+    }
+    n = count;
+    return CRIMSON_OK;
+  }
+  int upload(const T *h, size_t count, hipStream_t s) {
+    CR_TRY(alloc(count));
+    if (count) CR_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return CRIMSON_OK;
+  }
+};
+
+// Number of workgroups of `threads` covering n items.
+inline unsigned grid_for(size_t n, unsigned threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+}  // namespace crimson
+
+// memLS face (memLS_faceType, memLS.f90:83-100), single task.
+struct CrFace {
+  int nNo = 0, dof = 0, bGrp = CRIMSON_BC_DIR;
+  bool created = false, coupled = false, inc = true;
+  double res = 0.0, nS = 0.0;
+  std::vector<int> glob;      // 0-based node
+  std::vector<double> val;    // (dof, nNo)
+  crimson::DBuf<int> d_glob;
+  crimson::DBuf<double> d_val;   // (dof, nNo)
+  crimson::DBuf<double> d_valM;  // (dof, nNo) = val * W
+};
+
+// memLS matrix handle.  The matrix of block size vdof (1..4) is held as
+// vdof*vdof planes of nnz doubles, plane m = row-major entry (m / vdof,
+// m % vdof) of every block, i.e. Val(m+1, :) of the reference layout.
+struct crimson_lhs {
+  int nNo = 0, nnz = 0, nFaces = 0;
+  hipStream_t stream = nullptr;
+  std::vector<int> h_rowptr, h_col, h_diag, h_tr;  // 0-based
+  crimson::DBuf<int> d_rowptr, d_col, d_diag, d_tr;
+  int vdof = 0;                   // block size of the stored matrix
+  crimson::DBuf<double> d_V;      // planes [vdof*vdof][nnz]
+  bool gt_is_d = false;           // Gt == mD planes (formation-produced)
+  crimson::DBuf<double> d_Gt;     // [3][nnz] when !gt_is_d
+  std::vector<CrFace> face;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};

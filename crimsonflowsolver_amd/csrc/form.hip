@@ -1,0 +1,984 @@
+// Stabilized-FEM interior formation for P1 tets on gfx950 (the interior part
+// of the reference's ElmGMR, elmgmr.f90:82-255).
+//
+// Pipeline, all deterministic (no atomics; every sum runs in element order):
+//   flux_elem   element-parallel: metric + velocity gradients -> per-element
+//               flux record (lumped-mass weights and the viscous flux q_dI of
+//               e3q.f90:100-145)
+//   flux_node   node-parallel gather over the node's elements in element
+//               order: qres, rmass, q = qres * (1/rmass)  (asiq.f90 + qpbc.f90)
+//   e3_elem     element-parallel e3 (e3.f90 .. e3lhs.f90): all four
+//               quadrature points in registers; writes a 48-double element
+//               record: shape gradients, the scalar part S_ab of the 3x3
+//               velocity blocks, the element residual rl, and four scalars
+//               from which every tangent entry is rebuilt
+//   asm_node    node-parallel: residual rows (res += rl, asigmr.f90:116) and
+//               the diagonal block of every row
+//   asm_edge    edge-parallel: both off-diagonal blocks (i,j) and (j,i) of an
+//               edge from the same element contributions (fillsparseI,
+//               fillsparse.f90:29-63, and the -G^T placement of
+//               solfar.f90:225-235 without its transpose search)
+//
+// The element record factorisation is exact algebra on the reference's
+// lazy tangent (e3lhs.f90:85-402) for P1 elements, whose metric and shape
+// gradients do not vary over the element:
+//   K^{ij}_ab = delta_ij S_ab + Tc g_a,i g_b,j + Mu g_a,j g_b,i
+//   S_ab = tsFct sum_q N_a N_b + rho tlW sum_q N_a (u_q.g_b)
+//          + Mu (g_a.g_b) + rho tlW sum_q tauM_q (u_q.g_a)(u_q.g_b)
+//   G_ab,i = tlW g_b,i sum_q N_a ,   C_ab = Cm (g_a.g_b)
+// with Tc = tlW sum_q tauC_q, Mu = 4 tlW mu, Cm = tlW sum_q tauM_q / rho,
+// tlW = alpha_f gamma Delt WdetJ, tsFct = alpha_m WdetJ rho.
+// K^{ij} is stored at lhsK(i + 3(j-1)) exactly as e3lhs.f90 places it.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace {
+
+// 4-point rule (symtet.c:22-23): qp q has barycentric (r,s,t) and
+// N_4 = 1 - r - s - t evaluated left to right (uniformP.c:22-25).
+constexpr double kA4 = 0.5854101966249685;
+constexpr double kB4 = 0.1381966011250150;
+constexpr double kQwt = (4.0 / 3.0) * 0.25;  // genint.f90:85
+__host__ __device__ constexpr double nq(int q, int a) {
+  return a == 3 ? (q == 0 ? 1.0 - kA4 - kB4 - kB4
+                          : q == 1 ? 1.0 - kB4 - kA4 - kB4
+                          : q == 2 ? 1.0 - kB4 - kB4 - kA4
+                                   : 1.0 - kB4 - kB4 - kB4)
+                : (a == q ? kA4 : kB4);
+}
+__host__ __device__ constexpr double sum_n(int a) { return nq(0, a) + nq(1, a) + nq(2, a) + nq(3, a); }
+__host__ __device__ constexpr double mm(int a, int b) {
+  return nq(0, a) * nq(0, b) + nq(1, a) * nq(1, b) + nq(2, a) * nq(2, b) + nq(3, a) * nq(3, b);
+}
+
+constexpr int kRec = 48;   // doubles per element record
+constexpr int kFRec = 10;  // doubles per flux record
+// record layout
+constexpr int R_G = 0;     // g[a][j]   a*3+j            (12)
+constexpr int R_S = 12;    // S[a][b]   12 + a*4+b       (16)
+constexpr int R_RL = 28;   // rl[a][k]  28 + a*4+k, k: mom1..3, cont (16)
+constexpr int R_TLW = 44, R_TC = 45, R_MU = 46, R_CM = 47;
+
+struct FormConst {
+  double rho, mu, Delt, lhsFct, lhmFct, Dtgl, dtsfct, taucfct;
+};
+
+// Workgroup index remapped so that each XCD (blocks are dealt round-robin
+// over the 8 XCDs) walks a contiguous range of elements and keeps the nodes
+// it gathers in its own L2.
+__device__ inline unsigned xcd_block() {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned per = nb / 8u;
+  if (per == 0 || b >= per * 8u) return b;
+  return (b % 8u) * per + b / 8u;
+}
+
+// e3metric (e3metric.f90:28-81) specialised to P1: dxdxi_ij =
+// (x_j,i - x_4,i)/2 since shgl = dN/dxi / 2 (genshp.f90:35-36).
+template <class T>
+__device__ inline void p1_metric(const T (&xl)[4][3], T (&dx)[3][3], T &WdetJ, T (&g)[4][3]) {
+  T d[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d[i][j] = T(0.5) * xl[j][i] - T(0.5) * xl[3][i];
+  dx[0][0] = d[1][1] * d[2][2] - d[2][1] * d[1][2];
+  dx[0][1] = d[2][1] * d[0][2] - d[0][1] * d[2][2];
+  dx[0][2] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+  const T tmp = T(1) / (dx[0][0] * d[0][0] + dx[0][1] * d[1][0] + dx[0][2] * d[2][0]);
+  dx[0][0] *= tmp;
+  dx[0][1] *= tmp;
+  dx[0][2] *= tmp;
+  dx[1][0] = (d[1][2] * d[2][0] - d[1][0] * d[2][2]) * tmp;
+  dx[1][1] = (d[0][0] * d[2][2] - d[2][0] * d[0][2]) * tmp;
+  dx[1][2] = (d[1][0] * d[0][2] - d[0][0] * d[1][2]) * tmp;
+  dx[2][0] = (d[1][0] * d[2][1] - d[1][1] * d[2][0]) * tmp;
+  dx[2][1] = (d[2][0] * d[0][1] - d[0][0] * d[2][1]) * tmp;
+  dx[2][2] = (d[0][0] * d[1][1] - d[0][1] * d[1][0]) * tmp;
+  WdetJ = T(kQwt) / tmp;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    g[0][j] = T(0.5) * dx[0][j];
+    g[1][j] = T(0.5) * dx[1][j];
+    g[2][j] = T(0.5) * dx[2][j];
+    g[3][j] = T(-0.5) * dx[0][j] + T(-0.5) * dx[1][j] + T(-0.5) * dx[2][j];
+  }
+}
+
+// e3gijd, tet branch (e3stab.f90:493-527)
+template <class T>
+__device__ inline void gijd_tet(const T (&x)[3][3], T (&gg)[6]) {
+  const T c1 = T(1.259921049894873e+00), c2 = T(6.299605249474365e-01);
+  T t1, t2, t3;
+  t1 = c1 * x[0][0] + c2 * (x[1][0] + x[2][0]);
+  t2 = c1 * x[1][0] + c2 * (x[0][0] + x[2][0]);
+  t3 = c1 * x[2][0] + c2 * (x[0][0] + x[1][0]);
+  gg[0] = x[0][0] * t1 + x[1][0] * t2 + x[2][0] * t3;
+  t1 = c1 * x[0][1] + c2 * (x[1][1] + x[2][1]);
+  t2 = c1 * x[1][1] + c2 * (x[0][1] + x[2][1]);
+  t3 = c1 * x[2][1] + c2 * (x[0][1] + x[1][1]);
+  gg[1] = x[0][1] * t1 + x[1][1] * t2 + x[2][1] * t3;
+  gg[3] = x[0][0] * t1 + x[1][0] * t2 + x[2][0] * t3;
+  t1 = c1 * x[0][2] + c2 * (x[1][2] + x[2][2]);
+  t2 = c1 * x[1][2] + c2 * (x[0][2] + x[2][2]);
+  t3 = c1 * x[2][2] + c2 * (x[0][2] + x[1][2]);
+  gg[2] = x[0][2] * t1 + x[1][2] * t2 + x[2][2] * t3;
+  gg[4] = x[0][1] * t1 + x[1][1] * t2 + x[2][1] * t3;
+  gg[5] = x[0][0] * t1 + x[1][0] * t2 + x[2][0] * t3;
+}
+
+// ---------------------------------------------------------------- flux pass
+// e3q per element (e3q.f90:66-145): qdi = mu (grad u + grad u^T) is the same
+// at every quadrature point of a P1 element; the lumped-mass weights are
+// m_a = sum_q N_a(q) WdetJ.  Record: m[4], qdi in slots (11,12,13,22,23,33).
+__global__ void __launch_bounds__(256) flux_elem(int nel, const int *__restrict__ ien,
+                                                 const double *__restrict__ x, int numnp,
+                                                 const double *__restrict__ y, int nshg,
+                                                 double mu, double *__restrict__ frec) {
+  const int e = xcd_block() * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  double xl[4][3], ul[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int n = ien[a * (size_t)nel + e];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      xl[a][i] = x[n + (size_t)numnp * i];
+      ul[a][i] = y[n + (size_t)nshg * i];
+    }
+  }
+  double dx[3][3], W, g[4][3];
+  p1_metric(xl, dx, W, g);
+  double gu[3][3];  // gu[j][i] = d u_i / d x_j
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      gu[j][i] = g[0][j] * ul[0][i] + g[1][j] * ul[1][i] + g[2][j] * ul[2][i] + g[3][j] * ul[3][i];
+  double *r = frec + (size_t)e * kFRec;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    r[a] = nq(0, a) * W + nq(1, a) * W + nq(2, a) * W + nq(3, a) * W;
+  r[4] = 2.0 * mu * gu[0][0];
+  r[5] = mu * (gu[0][1] + gu[1][0]);
+  r[6] = mu * (gu[0][2] + gu[2][0]);
+  r[7] = 2.0 * mu * gu[1][1];
+  r[8] = mu * (gu[1][2] + gu[2][1]);
+  r[9] = 2.0 * mu * gu[2][2];
+}
+
+// qres/rmass gather and qpbc inversion.  q(nshg,9) column-major in the
+// reference's component order: (1,4,7) = (q11,q12,q13), ... so component
+// k = i + 3 (j-1) holds mu(du_i/dx_j + du_j/dx_i).
+__global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict__ inc_ptr,
+                                                 const int *__restrict__ inc,
+                                                 const double *__restrict__ frec,
+                                                 double *__restrict__ q) {
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= nshg) return;
+  double s[6] = {0, 0, 0, 0, 0, 0}, rm = 0.0;
+  for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) {
+    const int ea = inc[p];
+    const double *r = frec + (size_t)(ea >> 2) * kFRec;
+    const double m = r[ea & 3];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] += m * r[4 + k];
+    rm += m;
+  }
+  const double inv = 1.0 / rm;
+  const size_t N = nshg;
+  q[i + 0 * N] = inv * s[0];
+  q[i + 1 * N] = inv * s[1];
+  q[i + 2 * N] = inv * s[2];
+  q[i + 3 * N] = inv * s[1];
+  q[i + 4 * N] = inv * s[3];
+  q[i + 5 * N] = inv * s[4];
+  q[i + 6 * N] = inv * s[2];
+  q[i + 7 * N] = inv * s[4];
+  q[i + 8 * N] = inv * s[5];
+}
+
+// ---------------------------------------------------------------- e3
+template <bool LHS>
+__global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ ien,
+                                               const double *__restrict__ x, int numnp,
+                                               const double *__restrict__ y,
+                                               const double *__restrict__ ac,
+                                               const double *__restrict__ q, int nshg,
+                                               FormConst c, double *__restrict__ rec) {
+  const int e = xcd_block() * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  const size_t N = nshg;
+  int nd[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) nd[a] = ien[a * (size_t)nel + e];
+
+  double g[4][3], W, gg[6];
+  {
+    double xl[4][3], dx[3][3];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) xl[a][i] = x[nd[a] + (size_t)numnp * i];
+    p1_metric(xl, dx, W, g);
+    gijd_tet(dx, gg);
+  }
+  // nodal velocity and pressure; gradients (e3ivar.f90:180-198)
+  double ul[4][3], pl[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ul[a][i] = y[nd[a] + N * i];
+    pl[a] = y[nd[a] + N * 3];
+  }
+  double gu[3][3], gp[3];  // gu[j][i] = du_i/dx_j
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      gu[j][i] = g[0][j] * ul[0][i] + g[1][j] * ul[1][i] + g[2][j] * ul[2][i] + g[3][j] * ul[3][i];
+    gp[j] = g[0][j] * pl[0] + g[1][j] * pl[1] + g[2][j] * pl[2] + g[3][j] * pl[3];
+  }
+  // divergence of the reconstructed flux (e3ivar.f90:355-367)
+  double divq[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double qa[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) qa[k] = q[nd[a] + N * k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) divq[i] += g[a][0] * qa[i] + g[a][1] * qa[i + 3] + g[a][2] * qa[i + 6];
+  }
+  double al[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) al[a][i] = ac[nd[a] + N * i];
+
+  const double rho = c.rho, mu = c.mu, rhoinv = 1.0 / rho;
+  const double rnu = mu * rhoinv;
+  const double dts = c.Dtgl * c.dtsfct;
+  const double GG = gg[0] * gg[0] + gg[1] * gg[1] + gg[2] * gg[2] +
+                    2.0 * (gg[3] * gg[3] + gg[4] * gg[4] + gg[5] * gg[5]);
+  const double tau0 = (2.0 * dts) * (2.0 * dts);
+  const double visc = 36.0 * (rnu * rnu) * GG;  // fff = 36 for ipord = 1
+  const double trg = gg[0] + gg[1] + gg[2];
+  const double divu = gu[0][0] + gu[1][1] + gu[2][2];
+  const double ff = c.taucfct / c.dtsfct;
+
+  double RNa[4][3], RGN[3][3], SU[3] = {0, 0, 0}, TC = 0.0, TM = 0.0, SUPG[10];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) RNa[a][i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) RGN[i][j] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) SUPG[k] = 0.0;
+
+#pragma unroll
+  for (int qp = 0; qp < 4; ++qp) {
+    const double N0 = nq(qp, 0), N1 = nq(qp, 1), N2 = nq(qp, 2), N3 = nq(qp, 3);
+    double u[3], aq[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      u[i] = N0 * ul[0][i] + N1 * ul[1][i] + N2 * ul[2][i] + N3 * ul[3][i];
+      aq[i] = N0 * al[0][i] + N1 * al[1][i] + N2 * al[2][i] + N3 * al[3][i];
+    }
+    const double pq = N0 * pl[0] + N1 * pl[1] + N2 * pl[2] + N3 * pl[3];
+    // strong residual (e3res.f90:775-789)
+    double rL[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      rL[i] = (aq[i] + u[0] * gu[0][i] + u[1] * gu[1][i] + u[2] * gu[2][i]) * rho + gp[i] - divq[i];
+    // tau, itau = 0 (e3stab.f90:108-133)
+    const double uGu = u[0] * (gg[0] * u[0] + gg[3] * u[1] + gg[5] * u[2]) +
+                       u[1] * (gg[3] * u[0] + gg[1] * u[1] + gg[4] * u[2]) +
+                       u[2] * (gg[5] * u[0] + gg[4] * u[1] + gg[2] * u[2]);
+    const double fact = sqrt((tau0 + uGu) + visc);
+    const double tauC = rho * 0.125 * fact / trg * ff;
+    const double tauM = 1.0 / fact;
+    double tauBar = rL[0] * (gg[0] * rL[0] + gg[3] * rL[1] + gg[5] * rL[2]) +
+                    rL[1] * (gg[3] * rL[0] + gg[1] * rL[1] + gg[4] * rL[2]) +
+                    rL[2] * (gg[5] * rL[0] + gg[4] * rL[1] + gg[2] * rL[2]);
+    tauBar = (tauBar != 0.0) ? tauM / sqrt(tauBar) : 0.0;
+    double ub[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ub[i] = u[i] - tauM * rL[i] * rhoinv;
+    // e3Res, iconvflow = 2 (e3res.f90:99-286)
+    const double tmp = -pq + tauC * divu;
+    double lg[3];  // L . grad u_i
+#pragma unroll
+    for (int i = 0; i < 3; ++i) lg[i] = tauBar * (rL[0] * gu[0][i] + rL[1] * gu[1][i] + rL[2] * gu[2][i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double rna = (aq[i] + ub[0] * gu[0][i] + ub[1] * gu[1][i] + ub[2] * gu[2][i]) * rho;
+      RNa[0][i] += N0 * rna;
+      RNa[1][i] += N1 * rna;
+      RNa[2][i] += N2 * rna;
+      RNa[3][i] += N3 * rna;
+      const double tl = tauM * rL[i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double v = mu * (gu[i][j] + gu[j][i]) + tl * u[j] + lg[i] * rL[j];
+        if (i == j) v += tmp;
+        RGN[i][j] += v;
+      }
+      SU[i] += ub[i];
+    }
+    if (LHS) {
+      TC += tauC;
+      TM += tauM;
+      double z[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) z[a] = u[0] * g[a][0] + u[1] * g[a][1] + u[2] * g[a][2];
+      int k = 0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = a; b < 4; ++b) SUPG[k++] += tauM * z[a] * z[b];
+    }
+  }
+
+  double *r = rec + (size_t)e * kRec;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    r[R_RL + a * 4 + 3] = W * (g[a][0] * SU[0] + g[a][1] * SU[1] + g[a][2] * SU[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      r[R_RL + a * 4 + i] = -W * (RNa[a][i] + g[a][0] * RGN[i][0] + g[a][1] * RGN[i][1] + g[a][2] * RGN[i][2]);
+  }
+  if (!LHS) return;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r[R_G + a * 3 + j] = g[a][j];
+  const double tlW = c.lhsFct * W;
+  const double tsFct = c.lhmFct * W * rho;
+  const double Mu = 4.0 * (tlW * mu);
+  const double rt = rho * tlW;
+  // w[c][b] = u_c . g_b : convection sum_q N_a (u_q.g_b) = sum_c mm(a,c) w[c][b]
+  double w[4][4];
+#pragma unroll
+  for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) w[cc][b] = ul[cc][0] * g[b][0] + ul[cc][1] * g[b][1] + ul[cc][2] * g[b][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const double conv = mm(a, 0) * w[0][b] + mm(a, 1) * w[1][b] + mm(a, 2) * w[2][b] + mm(a, 3) * w[3][b];
+      const int lo = a < b ? a : b, hi = a < b ? b : a;
+      const int k = lo * 4 - (lo * (lo - 1)) / 2 + (hi - lo);  // index of (lo,hi) in SUPG
+      const double gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
+      r[R_S + a * 4 + b] = tsFct * mm(a, b) + rt * conv + Mu * gab + rt * SUPG[k];
+    }
+  r[R_TLW] = tlW;
+  r[R_TC] = tlW * TC;
+  r[R_MU] = Mu;
+  r[R_CM] = tlW * TM / rho;
+}
+
+// ---------------------------------------------------------------- assembly
+// Output modes: 0 = reference layout lhsK(9,nnz)/lhsP(4,nnz);
+// 1 = memLS planes V[16][nnz] (row-major 4x4 blocks of solfar's Val4).
+struct AsmOut {
+  double *lhsK, *lhsP;  // mode 0
+  double *V;            // mode 1
+  size_t nnz;
+};
+
+// Write one block given K (PHASTA column-major order m = i + 3j), the
+// pressure-gradient column G(k) = lhsP(1:3) of this block, C = lhsP(4), and
+// the transposed block's lhsP(1:3) (for Val4's -G^T column).
+template <int MODE>
+__device__ inline void put_block(const AsmOut &o, int k, const double (&K)[9], const double (&Gk)[3],
+                                 double C, const double (&Gtr)[3]) {
+  if (MODE == 0) {
+    double *pk = o.lhsK + (size_t)k * 9;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) pk[m] = K[m];
+    double *pp = o.lhsP + (size_t)k * 4;
+    pp[0] = Gk[0];
+    pp[1] = Gk[1];
+    pp[2] = Gk[2];
+    pp[3] = C;
+  } else {
+    // solfar.f90:216-221: Val4(1:3)=lhsK(1:3), Val4(5:7)=lhsK(4:6),
+    // Val4(9:11)=lhsK(7:9), Val4(13:15)=lhsP(1:3), Val4(16)=lhsP(4);
+    // Val4(4:12:4) = -lhsP(1:3) of the transposed block (:225-235).
+    const size_t S = o.nnz;
+    double *V = o.V + k;
+    V[0 * S] = K[0];
+    V[1 * S] = K[1];
+    V[2 * S] = K[2];
+    V[3 * S] = -Gtr[0];
+    V[4 * S] = K[3];
+    V[5 * S] = K[4];
+    V[6 * S] = K[5];
+    V[7 * S] = -Gtr[1];
+    V[8 * S] = K[6];
+    V[9 * S] = K[7];
+    V[10 * S] = K[8];
+    V[11 * S] = -Gtr[2];
+    V[12 * S] = Gk[0];
+    V[13 * S] = Gk[1];
+    V[14 * S] = Gk[2];
+    V[15 * S] = C;
+  }
+}
+
+// MODE 0 writes res(nshg,4) like ElmGMR; MODE 1 writes Res4(4,nshg), the
+// transposed copy SolFlow hands to memLS (solfar.f90:206-210).
+template <int MODE, bool LHS>
+__global__ void __launch_bounds__(256) asm_node(int nshg, const int *__restrict__ inc_ptr,
+                                                const int *__restrict__ inc,
+                                                const int *__restrict__ diagk,
+                                                const double *__restrict__ rec,
+                                                double *__restrict__ res, AsmOut o) {
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= nshg) return;
+  double rs[4] = {0, 0, 0, 0};
+  double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[3] = {0, 0, 0}, C = 0.0;
+  for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) {
+    const int ea = inc[p];
+    const int a = ea & 3;
+    const double *r = rec + (size_t)(ea >> 2) * kRec;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rs[k] += r[R_RL + a * 4 + k];
+    if (LHS) {
+      const double ga[3] = {r[R_G + a * 3 + 0], r[R_G + a * 3 + 1], r[R_G + a * 3 + 2]};
+      const double S = r[R_S + a * 5];
+      const double TM = r[R_TC] + r[R_MU];
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) K[ii + 3 * jj] += (ii == jj ? S : 0.0) + TM * ga[ii] * ga[jj];
+      const double tn = r[R_TLW] * sum_n(a);
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) G[ii] += tn * ga[ii];
+      C += r[R_CM] * (ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2]);
+    }
+  }
+  const size_t N = nshg;
+  if (MODE == 0) {
+    res[i + 0 * N] = rs[0];
+    res[i + 1 * N] = rs[1];
+    res[i + 2 * N] = rs[2];
+    res[i + 3 * N] = rs[3];
+  } else {
+    res[4 * (size_t)i + 0] = rs[0];
+    res[4 * (size_t)i + 1] = rs[1];
+    res[4 * (size_t)i + 2] = rs[2];
+    res[4 * (size_t)i + 3] = rs[3];
+  }
+  if (LHS) put_block<MODE>(o, diagk[i], K, G, C, G);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) asm_edge(int nedge, const int *__restrict__ ek,
+                                                const int *__restrict__ ec_ptr,
+                                                const int *__restrict__ ec,
+                                                const double *__restrict__ rec, AsmOut o) {
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= nedge) return;
+  double Kij[9], Kji[9], Gij[3] = {0, 0, 0}, Gji[3] = {0, 0, 0}, C = 0.0;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    Kij[m] = 0.0;
+    Kji[m] = 0.0;
+  }
+  for (int p = ec_ptr[t]; p < ec_ptr[t + 1]; ++p) {
+    const int eab = ec[p];
+    const int a = (eab >> 2) & 3, b = eab & 3;
+    const double *r = rec + (size_t)(eab >> 4) * kRec;
+    const double ga[3] = {r[R_G + a * 3 + 0], r[R_G + a * 3 + 1], r[R_G + a * 3 + 2]};
+    const double gb[3] = {r[R_G + b * 3 + 0], r[R_G + b * 3 + 1], r[R_G + b * 3 + 2]};
+    const double Sab = r[R_S + a * 4 + b], Sba = r[R_S + b * 4 + a];
+    const double Tc = r[R_TC], Mu = r[R_MU];
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+        // K^{ii,jj}_ab = d S_ab + Tc g_a,ii g_b,jj + Mu g_a,jj g_b,ii
+        const double x1 = Tc * ga[ii] * gb[jj] + Mu * ga[jj] * gb[ii];
+        const double x2 = Tc * gb[ii] * ga[jj] + Mu * gb[jj] * ga[ii];
+        Kij[ii + 3 * jj] += (ii == jj ? Sab : 0.0) + x1;
+        Kji[ii + 3 * jj] += (ii == jj ? Sba : 0.0) + x2;
+      }
+    const double tl = r[R_TLW];
+    const double ta = tl * sum_n(a), tb = tl * sum_n(b);
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+      Gij[ii] += ta * gb[ii];
+      Gji[ii] += tb * ga[ii];
+    }
+    C += r[R_CM] * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+  }
+  put_block<MODE>(o, ek[2 * t], Kij, Gij, C, Gji);
+  put_block<MODE>(o, ek[2 * t + 1], Kji, Gji, C, Gij);
+}
+
+template <int MODE>
+__global__ void zero_blocks(int n, const int *__restrict__ ks, AsmOut o) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[3] = {0, 0, 0};
+  put_block<MODE>(o, ks[t], K, G, 0.0, G);
+}
+
+// bc3Res for codes 0/7 on the velocity bits, bit 2 (pressure) and bit 11
+// (bc3res.f90:34-80).
+__global__ void bc3res_kernel(int nshg, const int *__restrict__ ibc, double *__restrict__ res, int res4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshg) return;
+  const int c = ibc[i];
+  const size_t N = nshg;
+  // component k of node i: res(i,k) or Res4(k,i)
+  const size_t s = res4 ? 1 : N, b = res4 ? 4 * (size_t)i : (size_t)i;
+  if ((c >> 2) & 1) res[b + 3 * s] = 0.0;
+  if (((c >> 3) & 7) == 7 || ((c >> 11) & 1)) {
+    res[b] = 0.0;
+    res[b + s] = 0.0;
+    res[b + 2 * s] = 0.0;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- handle
+struct crimson_form {
+  int nshg = 0, numnp = 0, nel = 0, nnz = 0, nedge = 0, norphan = 0;
+  crimson_form_params prm{};
+  hipStream_t stream = nullptr;
+  crimson::DBuf<int> d_ien, d_inc_ptr, d_inc, d_diagk, d_ek, d_ec_ptr, d_ec, d_orphan, d_ibc;
+  crimson::DBuf<double> d_frec, d_q, d_rec;
+  crimson::DBuf<double> s_y, s_ac, s_x, s_res, s_K, s_P, s_q;  // staging of the host API
+  bool has_ibc = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float ms[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int validate_params(const crimson_form_params *p) {
+  if (!p) {
+    crimson::set_error("null parameters");
+    return CRIMSON_EARG;
+  }
+  if (p->itau != 0 || p->iconvflow != 2 || p->idiff != 1 || p->ipord != 1 || p->flmpl != 0.0 ||
+      p->flmpr != 0.0) {
+    crimson::set_error(
+        "unsupported flags (itau=%d iconvflow=%d idiff=%d ipord=%d flmpl=%g flmpr=%g); supported: "
+        "itau=0 iconvflow=2 idiff=1 ipord=1 flmpl=flmpr=0",
+        p->itau, p->iconvflow, p->idiff, p->ipord, p->flmpl, p->flmpr);
+    return CRIMSON_EUNSUPPORTED;
+  }
+  if (p->precision != 64) {
+    crimson::set_error("formation precision %d not available in this build (64 only)", p->precision);
+    return CRIMSON_EUNSUPPORTED;
+  }
+  if (!(p->rho > 0) || !(p->mu >= 0) || !(p->Delt > 0) || !(p->dtsfct > 0)) {
+    crimson::set_error("non-physical parameters rho=%g mu=%g Delt=%g dtsfct=%g", p->rho, p->mu, p->Delt,
+                       p->dtsfct);
+    return CRIMSON_EARG;
+  }
+  return CRIMSON_OK;
+}
+
+FormConst make_const(const crimson_form_params &p) {
+  FormConst c;
+  c.rho = p.rho;
+  c.mu = p.mu;
+  c.Delt = p.Delt;
+  c.lhsFct = p.alfi * p.gami * p.Delt;   // e3lhs.f90:85
+  c.lhmFct = p.almi * (1.0 - p.flmpl);   // e3lhs.f90:86
+  c.Dtgl = 1.0 / p.Delt;                 // itrPC.f90:71
+  c.dtsfct = p.dtsfct;
+  c.taucfct = p.taucfct;
+  return c;
+}
+
+// position of column `col` in 0-based CSR row `row`, or -1
+inline int find_col(const std::vector<int> &rp, const std::vector<int> &cols, int row, int col) {
+  auto b = cols.begin() + rp[row], e = cols.begin() + rp[row + 1];
+  auto it = std::lower_bound(b, e, col);
+  if (it == e || *it != col) return -1;
+  return (int)(it - cols.begin());
+}
+
+int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
+                  double *d_res, int lhs, int mode, const AsmOut &o, double *d_qout) {
+  const unsigned T = 256;
+  hipStream_t s = f->stream;
+  const FormConst c = make_const(f->prm);
+  CR_HIP(hipEventRecord(f->ev[0], s));
+  if (f->nel > 0) {
+    flux_elem<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg,
+                                                         f->prm.mu, f->d_frec.p);
+  }
+  flux_node<<<crimson::grid_for(f->nshg, T), T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_frec.p,
+                                                        d_qout ? d_qout : f->d_q.p);
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipEventRecord(f->ev[1], s));
+  const double *dq = d_qout ? d_qout : f->d_q.p;
+  if (f->nel > 0) {
+    if (lhs)
+      e3_elem<true><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, d_ac,
+                                                                dq, f->nshg, c, f->d_rec.p);
+    else
+      e3_elem<false><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, d_ac,
+                                                                 dq, f->nshg, c, f->d_rec.p);
+  }
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipEventRecord(f->ev[2], s));
+  const unsigned gn = crimson::grid_for(f->nshg, T), ge = crimson::grid_for(f->nedge, T);
+  if (!lhs) {
+    asm_node<0, false><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_diagk.p, f->d_rec.p, d_res, o);
+  } else if (mode == 0) {
+    asm_node<0, true><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_diagk.p, f->d_rec.p, d_res, o);
+    if (f->nedge) asm_edge<0><<<ge, T, 0, s>>>(f->nedge, f->d_ek.p, f->d_ec_ptr.p, f->d_ec.p, f->d_rec.p, o);
+    if (f->norphan) zero_blocks<0><<<crimson::grid_for(f->norphan, T), T, 0, s>>>(f->norphan, f->d_orphan.p, o);
+  } else {
+    asm_node<1, true><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_diagk.p, f->d_rec.p, d_res, o);
+    if (f->nedge) asm_edge<1><<<ge, T, 0, s>>>(f->nedge, f->d_ek.p, f->d_ec_ptr.p, f->d_ec.p, f->d_rec.p, o);
+    if (f->norphan) zero_blocks<1><<<crimson::grid_for(f->norphan, T), T, 0, s>>>(f->norphan, f->d_orphan.p, o);
+  }
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipEventRecord(f->ev[3], s));
+  CR_HIP(hipStreamSynchronize(s));
+  CR_HIP(hipEventElapsedTime(&f->ms[0], f->ev[0], f->ev[1]));
+  CR_HIP(hipEventElapsedTime(&f->ms[1], f->ev[1], f->ev[2]));
+  CR_HIP(hipEventElapsedTime(&f->ms[2], f->ev[2], f->ev[3]));
+  CR_HIP(hipEventElapsedTime(&f->ms[3], f->ev[0], f->ev[3]));
+  return CRIMSON_OK;
+}
+
+}  // namespace
+
+extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, const int *colm,
+                                  const int *rowp, int nnz, const int *iBC, const crimson_form_params *prm,
+                                  crimson_form **out) {
+  if (!out || !ien || !colm || !rowp || nshg <= 0 || numnp < nshg || nel < 0 || nnz <= 0) {
+    crimson::set_error("crimson_form_setup: invalid sizes or null pointers (nshg=%d numnp=%d nel=%d nnz=%d)",
+                       nshg, numnp, nel, nnz);
+    return CRIMSON_EARG;
+  }
+  if ((long long)nel * 16 >= (1LL << 31)) {
+    crimson::set_error("crimson_form_setup: nel=%d exceeds the 2^27 element limit of the packed maps", nel);
+    return CRIMSON_EARG;
+  }
+  CR_TRY(validate_params(prm));
+  // CSR (0-based) and checks
+  std::vector<int> rp(nshg + 1), cols(nnz);
+  for (int i = 0; i <= nshg; ++i) rp[i] = colm[i] - 1;
+  if (rp[0] != 0 || rp[nshg] != nnz) {
+    crimson::set_error("colm must start at 1 and end at nnz+1 (got %d .. %d, nnz=%d)", colm[0], colm[nshg], nnz);
+    return CRIMSON_EARG;
+  }
+  for (int i = 0; i < nshg; ++i) {
+    if (rp[i + 1] < rp[i]) {
+      crimson::set_error("colm not monotone at row %d", i + 1);
+      return CRIMSON_EARG;
+    }
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      cols[k] = rowp[k] - 1;
+      if (cols[k] < 0 || cols[k] >= nshg || (k > rp[i] && cols[k] <= cols[k - 1])) {
+        crimson::set_error("rowp row %d not strictly ascending within [1,nshg]", i + 1);
+        return CRIMSON_EARG;
+      }
+    }
+  }
+  std::vector<int> diagk(nshg);
+  for (int i = 0; i < nshg; ++i) {
+    diagk[i] = find_col(rp, cols, i, i);
+    if (diagk[i] < 0) {
+      crimson::set_error("row %d has no diagonal block", i + 1);
+      return CRIMSON_ENODIAG;
+    }
+  }
+  if (iBC) {
+    for (int i = 0; i < nshg; ++i) {
+      const int vb = (iBC[i] >> 3) & 7;
+      if (vb != 0 && vb != 7) {
+        crimson::set_error("node %d: partial velocity BC code %d (bc3lhs projection) is not supported", i + 1, vb);
+        return CRIMSON_EUNSUPPORTED;
+      }
+      if ((iBC[i] >> 10) & 1) {
+        crimson::set_error("node %d: periodic BC (iBC bit 10) is not supported", i + 1);
+        return CRIMSON_EUNSUPPORTED;
+      }
+    }
+  }
+  // node incidences in element order
+  std::vector<int> inc_ptr(nshg + 1, 0), inc((size_t)nel * 4);
+  for (int a = 0; a < 4; ++a)
+    for (int e = 0; e < nel; ++e) {
+      const int n = ien[(size_t)a * nel + e] - 1;
+      if (n < 0 || n >= nshg) {
+        crimson::set_error("ien(%d,%d)=%d out of range [1,%d]", e + 1, a + 1, n + 1, nshg);
+        return CRIMSON_EARG;
+      }
+      inc_ptr[n + 1]++;
+    }
+  for (int i = 0; i < nshg; ++i) inc_ptr[i + 1] += inc_ptr[i];
+  {
+    std::vector<int> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+    for (int e = 0; e < nel; ++e)
+      for (int a = 0; a < 4; ++a) {
+        const int n = ien[(size_t)a * nel + e] - 1;
+        inc[fill[n]++] = e * 4 + a;
+      }
+  }
+  // edges (i<j) with both block positions; contributions in element order
+  std::vector<int> edge_of_k(nnz, -1), ek;
+  int nedge = 0;
+  for (int i = 0; i < nshg; ++i)
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = cols[k];
+      if (j <= i) continue;
+      const int kt = find_col(rp, cols, j, i);
+      if (kt < 0) {
+        crimson::set_error("graph not symmetric: block (%d,%d) present, (%d,%d) missing", i + 1, j + 1, j + 1, i + 1);
+        return CRIMSON_EGRAPH;
+      }
+      edge_of_k[k] = nedge++;
+      ek.push_back(k);
+      ek.push_back(kt);
+    }
+  std::vector<int> ec_ptr(nedge + 1, 0), ec((size_t)nel * 6);
+  std::vector<char> touched(nnz, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<int> fill;
+    if (pass == 1) fill.assign(ec_ptr.begin(), ec_ptr.end() - 1);
+    for (int e = 0; e < nel; ++e) {
+      int nd[4];
+      for (int a = 0; a < 4; ++a) nd[a] = ien[(size_t)a * nel + e] - 1;
+      for (int p = 0; p < 4; ++p)
+        for (int qq = p + 1; qq < 4; ++qq) {
+          int a = p, b = qq;
+          if (nd[a] == nd[b]) {
+            crimson::set_error("element %d repeats node %d", e + 1, nd[a] + 1);
+            return CRIMSON_EARG;
+          }
+          if (nd[a] > nd[b]) std::swap(a, b);
+          const int k = find_col(rp, cols, nd[a], nd[b]);
+          if (k < 0) {
+            crimson::set_error("element %d: pair (%d,%d) missing from the CSR graph", e + 1, nd[a] + 1, nd[b] + 1);
+            return CRIMSON_EGRAPH;
+          }
+          const int t = edge_of_k[k];
+          if (pass == 0) {
+            ec_ptr[t + 1]++;
+            touched[k] = 1;
+            touched[ek[2 * t + 1]] = 1;
+          } else {
+            ec[fill[t]++] = e * 16 + a * 4 + b;
+          }
+        }
+    }
+    if (pass == 0)
+      for (int t = 0; t < nedge; ++t) ec_ptr[t + 1] += ec_ptr[t];
+  }
+  for (int i = 0; i < nshg; ++i) touched[diagk[i]] = 1;
+  std::vector<int> orphan;
+  for (int k = 0; k < nnz; ++k)
+    if (!touched[k]) orphan.push_back(k);
+
+  crimson_form *f = new crimson_form();
+  f->nshg = nshg;
+  f->numnp = numnp;
+  f->nel = nel;
+  f->nnz = nnz;
+  f->nedge = nedge;
+  f->norphan = (int)orphan.size();
+  f->prm = *prm;
+  int rc = CRIMSON_OK;
+  auto fail = [&](int code) {
+    crimson_form_free(f);
+    return code;
+  };
+  if (hipStreamCreate(&f->stream) != hipSuccess) {
+    crimson::set_error("hipStreamCreate failed (is a GPU visible?)");
+    f->stream = nullptr;
+    return fail(CRIMSON_EHIP);
+  }
+  for (auto &ev : f->ev)
+    if (hipEventCreate(&ev) != hipSuccess) {
+      crimson::set_error("hipEventCreate failed");
+      return fail(CRIMSON_EHIP);
+    }
+  std::vector<int> ien0((size_t)nel * 4);
+  for (size_t k = 0; k < ien0.size(); ++k) ien0[k] = ien[k] - 1;
+  hipStream_t s = f->stream;
+  if ((rc = f->d_ien.upload(ien0.data(), ien0.size(), s)) ||
+      (rc = f->d_inc_ptr.upload(inc_ptr.data(), inc_ptr.size(), s)) ||
+      (rc = f->d_inc.upload(inc.data(), inc.size(), s)) ||
+      (rc = f->d_diagk.upload(diagk.data(), diagk.size(), s)) ||
+      (rc = f->d_ek.upload(ek.data(), ek.size(), s)) ||
+      (rc = f->d_ec_ptr.upload(ec_ptr.data(), ec_ptr.size(), s)) ||
+      (rc = f->d_ec.upload(ec.data(), ec.size(), s)) ||
+      (rc = f->d_orphan.upload(orphan.data(), orphan.size(), s)) ||
+      (rc = f->d_frec.alloc((size_t)std::max(nel, 1) * kFRec)) ||
+      (rc = f->d_q.alloc((size_t)nshg * 9)) || (rc = f->d_rec.alloc((size_t)std::max(nel, 1) * kRec)))
+    return fail(rc);
+  if (iBC) {
+    f->has_ibc = true;
+    if ((rc = f->d_ibc.upload(iBC, nshg, s))) return fail(rc);
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    crimson::set_error("upload of the formation maps failed");
+    return fail(CRIMSON_EHIP);
+  }
+  *out = f;
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_form_free(crimson_form *f) {
+  if (!f) return CRIMSON_OK;
+  if (f->stream) (void)hipStreamSynchronize(f->stream);
+  for (auto &ev : f->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (f->stream) (void)hipStreamDestroy(f->stream);
+  delete f;
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_form_interior_dev(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
+                                         double *d_res, double *d_lhsK, double *d_lhsP, double *d_qres, int lhs) {
+  if (!f || !d_y || !d_ac || !d_x || !d_res || (lhs && (!d_lhsK || !d_lhsP))) {
+    crimson::set_error("crimson_form_interior_dev: null argument");
+    return CRIMSON_EARG;
+  }
+  AsmOut o{d_lhsK, d_lhsP, nullptr, (size_t)f->nnz};
+  return run_formation(f, d_y, d_ac, d_x, d_res, lhs ? 1 : 0, 0, o, d_qres);
+}
+
+extern "C" int crimson_form_interior(crimson_form *f, const double *y, const double *ac, const double *x,
+                                     double *res, double *lhsK, double *lhsP, double *qres, int lhs) {
+  if (!f || !y || !ac || !x || !res || (lhs && (!lhsK || !lhsP))) {
+    crimson::set_error("crimson_form_interior: null argument");
+    return CRIMSON_EARG;
+  }
+  hipStream_t s = f->stream;
+  const size_t N = f->nshg;
+  CR_TRY(f->s_y.upload(y, N * 4, s));
+  CR_TRY(f->s_ac.upload(ac, N * 4, s));
+  CR_TRY(f->s_x.upload(x, (size_t)f->numnp * 3, s));
+  CR_TRY(f->s_res.alloc(N * 4));
+  if (lhs) {
+    CR_TRY(f->s_K.alloc((size_t)f->nnz * 9));
+    CR_TRY(f->s_P.alloc((size_t)f->nnz * 4));
+  }
+  AsmOut o{f->s_K.p, f->s_P.p, nullptr, (size_t)f->nnz};
+  CR_TRY(run_formation(f, f->s_y.p, f->s_ac.p, f->s_x.p, f->s_res.p, lhs ? 1 : 0, 0, o, nullptr));
+  CR_HIP(hipMemcpyAsync(res, f->s_res.p, N * 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (lhs) {
+    CR_HIP(hipMemcpyAsync(lhsK, f->s_K.p, (size_t)f->nnz * 9 * sizeof(double), hipMemcpyDeviceToHost, s));
+    CR_HIP(hipMemcpyAsync(lhsP, f->s_P.p, (size_t)f->nnz * 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  if (qres) CR_HIP(hipMemcpyAsync(qres, f->d_q.p, N * 9 * sizeof(double), hipMemcpyDeviceToHost, s));
+  CR_HIP(hipStreamSynchronize(s));
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_form_interior_to_lhs(crimson_form *f, const double *d_y, const double *d_ac,
+                                            const double *d_x, double *d_res, crimson_lhs *lhs) {
+  if (!f || !lhs || !d_y || !d_ac || !d_x || !d_res) {
+    crimson::set_error("crimson_form_interior_to_lhs: null argument");
+    return CRIMSON_EARG;
+  }
+  if (lhs->nNo != f->nshg || lhs->nnz != f->nnz) {
+    crimson::set_error("lhs (nNo=%d nnz=%d) does not match the formation graph (nshg=%d nnz=%d)", lhs->nNo,
+                       lhs->nnz, f->nshg, f->nnz);
+    return CRIMSON_EARG;
+  }
+  CR_TRY(lhs->d_V.alloc((size_t)16 * f->nnz));
+  lhs->vdof = 4;
+  lhs->gt_is_d = true;  // Gt = -mG^T = mD for the solfar packing
+  AsmOut o{nullptr, nullptr, lhs->d_V.p, (size_t)f->nnz};
+  return run_formation(f, d_y, d_ac, d_x, d_res, 1, 1, o, nullptr);
+}
+
+extern "C" int crimson_form_bc3res_dev(crimson_form *f, double *d_res, int res4_layout) {
+  if (!f || !d_res) {
+    crimson::set_error("crimson_form_bc3res_dev: null argument");
+    return CRIMSON_EARG;
+  }
+  if (!f->has_ibc) return CRIMSON_OK;
+  bc3res_kernel<<<crimson::grid_for(f->nshg, 256), 256, 0, f->stream>>>(f->nshg, f->d_ibc.p, d_res,
+                                                                         res4_layout ? 1 : 0);
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipStreamSynchronize(f->stream));
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_form_timing(const crimson_form *f, double *out4) {
+  if (!f || !out4) {
+    crimson::set_error("crimson_form_timing: null argument");
+    return CRIMSON_EARG;
+  }
+  for (int k = 0; k < 4; ++k) out4[k] = f->ms[k];
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_genadj(int nshg, int nel, const int *ien, int *colm, int *rowp, int maxnnz, int *nnz_out) {
+  if (nshg <= 0 || nel < 0 || !ien || !colm || !nnz_out) {
+    crimson::set_error("crimson_genadj: invalid arguments");
+    return CRIMSON_EARG;
+  }
+  // bucket (i, j) pairs per row, then sort and deduplicate each row
+  std::vector<int> cnt(nshg + 1, 0);
+  for (int a = 0; a < 4; ++a)
+    for (int e = 0; e < nel; ++e) {
+      const int n = ien[(size_t)a * nel + e];
+      if (n < 1 || n > nshg) {
+        crimson::set_error("ien(%d,%d)=%d out of range", e + 1, a + 1, n);
+        return CRIMSON_EARG;
+      }
+      cnt[n] += 4;
+    }
+  for (int i = 0; i < nshg; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int> buf(cnt[nshg]);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int e = 0; e < nel; ++e)
+    for (int a = 0; a < 4; ++a) {
+      const int i = ien[(size_t)a * nel + e] - 1;
+      for (int b = 0; b < 4; ++b) buf[fill[i]++] = ien[(size_t)b * nel + e];
+    }
+  long long nnz = 0;
+  colm[0] = 1;
+  std::vector<int> rowlen(nshg);
+  for (int i = 0; i < nshg; ++i) {
+    auto b = buf.begin() + cnt[i], e = buf.begin() + cnt[i + 1];
+    std::sort(b, e);
+    auto u = std::unique(b, e);
+    rowlen[i] = (int)(u - b);
+    if (rowlen[i] == 0) rowlen[i] = 1;  // isolated node: keep its diagonal
+    nnz += rowlen[i];
+  }
+  *nnz_out = (int)nnz;
+  if (nnz > maxnnz || !rowp) {
+    crimson::set_error("crimson_genadj: need %lld column entries, capacity %d", nnz, maxnnz);
+    return CRIMSON_EARG;
+  }
+  int pos = 0;
+  for (int i = 0; i < nshg; ++i) {
+    if (cnt[i + 1] == cnt[i]) {
+      rowp[pos++] = i + 1;
+    } else {
+      for (int k = 0; k < rowlen[i]; ++k) rowp[pos++] = buf[cnt[i] + k];
+    }
+    colm[i + 1] = pos + 1;
+  }
+  return CRIMSON_OK;
+}

@@ -1,0 +1,1291 @@
+// memLS on gfx950: PRECOND, NSSOLVER (GMRES + CG on the Schur complement +
+// least-squares combination), GMRESV and CGRADV/CGRADS (memLS.f90).
+//
+// Design:
+//  * The matrix lives as vdof*vdof planes of nnz doubles (plane m = entry
+//    (m / vdof, m % vdof) of every row-major block).  Each plane is read with
+//    16 lanes per CSR row, so every load instruction covers 128 contiguous
+//    bytes.
+//  * PRECOND's symmetric diagonal scaling is not written back into Val: every
+//    SpMV applies y_r(i) = W_r(i) sum_j A_rc(i,j) W_c(j) x_c(j) + fix_r(i) x_r(i)
+//    where fix marks the diagonal entries memLS forces to 1 (zero diagonal,
+//    Dirichlet-masked DOF; memLS.f90:1815-1820,1923-1929).  This saves the two
+//    read+write sweeps over Val per solve.
+//  * Dots and norms are two-stage deterministic reductions (per-workgroup
+//    partials in a fixed order, then one workgroup per result).
+//  * GMRES orthogonalises with classical Gram-Schmidt and a DGKS second pass
+//    (taken when ||w'|| < ||w||/sqrt 2), computing all i projections of a
+//    step in one sweep.  memLS uses modified Gram-Schmidt (memLS.f90:743-746);
+//    both produce the same Krylov space, and solutions agree to the solver
+//    tolerance.  Givens rotations, the convergence test and the triangular
+//    solve run on the host in FP64 exactly as memLS.f90:750-775.
+//  * CG iterations run device-resident with a device-side convergence flag;
+//    the host enqueues them in batches and reads the flag between batches.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace {
+
+constexpr int kRowLanes = 16;  // lanes per CSR row in the SpMV kernels
+constexpr int kRedThreads = 256;
+constexpr int kChunk = 1024;   // vector entries per workgroup in the dot sweeps
+
+struct SubLs {
+  double absTol = 0, relTol = 0, iNorm = 0, fNorm = 0, dB = 0, callD = 0;
+  int suc = 0, mItr = 0, sD = 0, itr = 0;
+};
+
+// A scaled view of a sub-block (rows r0.., cols c0..) of the stored matrix.
+struct MatView {
+  const double *base;  // plane 0
+  size_t nnz;
+  int plane[16];       // plane index of (r, c), r < NR, c < NC: plane[r * 4 + c]
+  const double *W;     // W(wdof, nNo)
+  const double *fix;   // fix(wdof, nNo) or null
+  int wdof, r0, c0;
+};
+
+template <int NR, int NC>
+__global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restrict__ rowptr,
+                                                   const int *__restrict__ col, MatView A,
+                                                   const double *__restrict__ x, double *__restrict__ y,
+                                                   double ya, const double *__restrict__ z, double za) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = gid / kRowLanes;
+  const int lane = threadIdx.x % kRowLanes;
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  if (row < nNo) {
+    const int ke = rowptr[row + 1];
+    for (int k = rowptr[row] + lane; k < ke; k += kRowLanes) {
+      const int j = col[k];
+      double xv[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) xv[c] = A.W[A.c0 + c + (size_t)A.wdof * j] * x[c + (size_t)NC * j];
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[r] += A.base[(size_t)A.plane[r * 4 + c] * A.nnz + k] * xv[c];
+    }
+  }
+#pragma unroll
+  for (int off = kRowLanes / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] += __shfl_xor(acc[r], off, kRowLanes);
+  if (row < nNo && lane == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      double v = A.W[A.r0 + r + (size_t)A.wdof * row] * acc[r];
+      if (A.fix && NR == NC && A.r0 == A.c0) v += A.fix[A.r0 + r + (size_t)A.wdof * row] * x[r + (size_t)NC * row];
+      v *= ya;
+      if (z) v += za * z[r + (size_t)NR * row];
+      y[r + (size_t)NR * row] = v;
+    }
+  }
+}
+
+// Wave-level sum (64 lanes) then across the 4 waves of a 256-thread group.
+__device__ inline double block_sum(double v, double *sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return t;
+}
+
+// partial[blk][j] = sum over this block's chunk of U_j . v for j < m, and
+// partial[blk][m] = v . v when self != 0.  U_j = U0 + j * ustride.
+__global__ void __launch_bounds__(256) mdot_kernel(size_t L, const double *__restrict__ v,
+                                                   const double *__restrict__ U0, size_t ustride, int m,
+                                                   int self, double *__restrict__ partial, int pstride,
+                                                   const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double sh[4];
+  const size_t base = (size_t)blockIdx.x * kChunk;
+  double vv[kChunk / 256];
+#pragma unroll
+  for (int t = 0; t < kChunk / 256; ++t) {
+    const size_t e = base + threadIdx.x + 256 * t;
+    vv[t] = e < L ? v[e] : 0.0;
+  }
+  double *out = partial + (size_t)blockIdx.x * pstride;
+  for (int j = 0; j < m; ++j) {
+    const double *u = U0 + ustride * j;
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < kChunk / 256; ++t) {
+      const size_t e = base + threadIdx.x + 256 * t;
+      if (e < L) s += u[e] * vv[t];
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[j] = s;
+  }
+  if (self) {
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < kChunk / 256; ++t) s += vv[t] * vv[t];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[m] = s;
+  }
+}
+
+// out[j] = sum_blk partial[blk][j], j < m, one workgroup per j, fixed order.
+__global__ void __launch_bounds__(256) reduce_kernel(int nblk, const double *__restrict__ partial, int pstride,
+                                                     double *__restrict__ out, const int *__restrict__ gate) {
+  __shared__ double sh[4];
+  const int j = blockIdx.x;
+  if (gate && *gate == 0) {
+    if (threadIdx.x == 0) out[j] = 0.0;
+    return;
+  }
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256) s += partial[(size_t)b * pstride + j];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[j] = s;
+}
+
+// y -= sum_j coef[j] U_j (coef on the device), then the block's partial of
+// y . y into partial[blk * pstride].
+__global__ void __launch_bounds__(256) msub_norm_kernel(size_t L, double *__restrict__ y,
+                                                        const double *__restrict__ U0, size_t ustride, int m,
+                                                        const double *__restrict__ coef,
+                                                        double *__restrict__ partial, int pstride,
+                                                        const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double sh[4];
+  __shared__ double cs[256];
+  for (int j = threadIdx.x; j < m && j < 256; j += 256) cs[j] = coef[j];
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * kChunk;
+  double s = 0.0;
+#pragma unroll
+  for (int t = 0; t < kChunk / 256; ++t) {
+    const size_t e = base + threadIdx.x + 256 * t;
+    if (e < L) {
+      double w = y[e];
+      for (int j = 0; j < m; ++j) w -= cs[j] * U0[ustride * j + e];
+      y[e] = w;
+      s += w * w;
+    }
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.x * pstride] = s;
+}
+
+// y += sum_j coef[j] U_j, coef on the device.
+__global__ void maxpy_kernel(size_t L, double *__restrict__ y, const double *__restrict__ U0, size_t ustride,
+                             int m, const double *__restrict__ coef, double sign) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L) return;
+  double w = y[e];
+  for (int j = 0; j < m; ++j) w += sign * coef[j] * U0[ustride * j + e];
+  y[e] = w;
+}
+
+// y = a x + b y
+__global__ void axpby_kernel(size_t L, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L) return;
+  y[e] = a * x[e] + (b == 0.0 ? 0.0 : b * y[e]);
+}
+
+// y = x * W (or x / ... ) elementwise
+__global__ void mul_kernel(size_t L, const double *__restrict__ x, const double *__restrict__ w,
+                           double *__restrict__ y) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L) return;
+  y[e] = x[e] * w[e];
+}
+
+// Strided component extraction/insert between (dof,N) and (n,N) vectors.
+__global__ void extract_kernel(int N, int dof, int c0, int nc, const double *__restrict__ src,
+                               double *__restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int c = 0; c < nc; ++c) dst[c + (size_t)nc * i] = src[c0 + c + (size_t)dof * i];
+}
+__global__ void insert_kernel(int N, int dof, int c0, int nc, const double *__restrict__ src,
+                              double *__restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int c = 0; c < nc; ++c) dst[c0 + c + (size_t)dof * i] = src[c + (size_t)nc * i];
+}
+
+// PRECOND part 1: W from the diagonal, fix for zero diagonals.
+__global__ void precond_diag_kernel(int N, int dof, int vdof, const double *__restrict__ V, size_t nnz,
+                                    const int *__restrict__ diag, double *__restrict__ W, double *__restrict__ fix) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int d = diag[i];
+  for (int r = 0; r < dof; ++r) {
+    const double a = V[(size_t)(r * vdof + r) * nnz + d];
+    W[r + (size_t)dof * i] = (a == 0.0) ? 1.0 : 1.0 / sqrt(fabs(a));
+    fix[r + (size_t)dof * i] = (a == 0.0) ? 1.0 : 0.0;
+  }
+}
+// Dirichlet masking of W by a face's val (memLS.f90:1831-1841)
+__global__ void precond_face_kernel(int n, int dof, int fdof, const int *__restrict__ glob,
+                                    const double *__restrict__ val, double *__restrict__ W) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int nc = fdof < dof ? fdof : dof;
+  const int Ac = glob[a];
+  for (int i = 0; i < nc; ++i) W[i + (size_t)dof * Ac] *= val[i + (size_t)fdof * a];
+}
+// After masking: diagonal entries that scale to zero are forced to one.
+__global__ void precond_fix_kernel(int N, int dof, int vdof, const double *__restrict__ V, size_t nnz,
+                                   const int *__restrict__ diag, const double *__restrict__ W,
+                                   double *__restrict__ fix) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int d = diag[i];
+  for (int r = 0; r < dof; ++r) {
+    const double a = V[(size_t)(r * vdof + r) * nnz + d];
+    const double w = W[r + (size_t)dof * i];
+    if (a == 0.0 || (a * w) * w == 0.0) fix[r + (size_t)dof * i] = 1.0;
+  }
+}
+// valM = val * W for a coupled face
+__global__ void face_valm_kernel(int n, int dof, int fdof, const int *__restrict__ glob,
+                                 const double *__restrict__ val, const double *__restrict__ W,
+                                 double *__restrict__ valM) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int nc = fdof < dof ? fdof : dof;
+  for (int i = 0; i < nc; ++i) valM[i + (size_t)fdof * a] = val[i + (size_t)fdof * a] * W[i + (size_t)dof * glob[a]];
+}
+// S = sum_a sum_i valM(i,a) X(i, glob(a)) in one workgroup (faces are small)
+__global__ void __launch_bounds__(256) face_dot_kernel(int n, int xdof, int fdof, const int *__restrict__ glob,
+                                                       const double *__restrict__ valM,
+                                                       const double *__restrict__ X, double *__restrict__ out) {
+  __shared__ double sh[4];
+  const int nc = fdof < xdof ? fdof : xdof;
+  double s = 0.0;
+  for (int a = threadIdx.x; a < n; a += 256)
+    for (int i = 0; i < nc; ++i) s += valM[i + (size_t)fdof * a] * X[i + (size_t)xdof * glob[a]];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) *out = s;
+}
+// Y(i, glob(a)) += valM(i,a) * coef * S
+__global__ void face_axpy_kernel(int n, int xdof, int fdof, const int *__restrict__ glob,
+                                 const double *__restrict__ valM, const double *__restrict__ S, double coef,
+                                 double *__restrict__ Y) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int nc = fdof < xdof ? fdof : xdof;
+  const double s = coef * *S;
+  for (int i = 0; i < nc; ++i) Y[i + (size_t)xdof * glob[a]] += valM[i + (size_t)fdof * a] * s;
+}
+// DGKS decision: gate = ||w'||^2 < 0.5 ||w||^2
+__global__ void dgks_kernel(const double *__restrict__ n0, const double *__restrict__ n1, int *__restrict__ gate) {
+  *gate = (*n1 < 0.5 * *n0) ? 1 : 0;
+}
+// AoS (vdof*vdof, nnz) -> planes, and Gt = -mG^T via the transpose map
+__global__ void aos_to_planes_kernel(size_t nnz, int m, const double *__restrict__ aos, double *__restrict__ V) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  for (int p = 0; p < m; ++p) V[(size_t)p * nnz + k] = aos[(size_t)m * k + p];
+}
+__global__ void gt_kernel(size_t nnz, const double *__restrict__ V, const int *__restrict__ tr, double *__restrict__ Gt) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const int l = tr[k];
+  Gt[0 * nnz + l] = -V[3 * nnz + k];
+  Gt[1 * nnz + l] = -V[7 * nnz + k];
+  Gt[2 * nnz + l] = -V[11 * nnz + k];
+}
+
+// ----- CG (device-resident iterations with a convergence flag) -----
+// scal layout: 0 err, 1 errO, 2 eps2, 3 den, 4 alpha, 5 done, 6 itr, 7 mItr, 8 suc
+__global__ void cg_check_kernel(double *__restrict__ scal) {
+  if (scal[5] != 0.0) return;
+  if (scal[6] >= scal[7]) {
+    scal[5] = 1.0;
+    return;
+  }
+  if (scal[0] < scal[2]) {
+    scal[8] = 1.0;
+    scal[5] = 1.0;
+    return;
+  }
+  scal[1] = scal[0];  // errO = err
+  scal[6] += 1.0;
+}
+__global__ void cg_alpha_kernel(double *__restrict__ scal) {
+  if (scal[5] != 0.0) return;
+  scal[4] = scal[1] / scal[3];
+}
+// X += alpha P ; R -= alpha SP ; partial of R.R
+__global__ void __launch_bounds__(256) cg_update_kernel(size_t L, const double *__restrict__ scal,
+                                                        double *__restrict__ X, double *__restrict__ R,
+                                                        const double *__restrict__ P, const double *__restrict__ SP,
+                                                        double *__restrict__ partial) {
+  if (scal[5] != 0.0) return;
+  __shared__ double sh[4];
+  const double alpha = scal[4];
+  const size_t base = (size_t)blockIdx.x * kChunk;
+  double s = 0.0;
+#pragma unroll
+  for (int t = 0; t < kChunk / 256; ++t) {
+    const size_t e = base + threadIdx.x + 256 * t;
+    if (e < L) {
+      X[e] = X[e] + alpha * P[e];
+      const double r = R[e] - alpha * SP[e];
+      R[e] = r;
+      s += r * r;
+    }
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+__global__ void cg_err_kernel(double *__restrict__ scal, const double *__restrict__ rr) {
+  if (scal[5] != 0.0) return;
+  const double n = sqrt(*rr);
+  scal[0] = n * n;  // err = NORMS(R)**2
+}
+// P = R + err/errO P
+__global__ void cg_p_kernel(size_t L, const double *__restrict__ scal, const double *__restrict__ R,
+                            double *__restrict__ P) {
+  if (scal[5] != 0.0) return;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L) return;
+  P[e] = R[e] + scal[0] / scal[1] * P[e];
+}
+}  // namespace
+
+struct crimson_ls {
+  SubLs GM, CG, RI;
+  int LS_type = 0, Resm = 0, Resc = 0;
+};
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Per-solve context: device workspaces sized for this lhs and dof.
+struct Ctx {
+  crimson_lhs *lhs;
+  hipStream_t s;
+  int N, dof;
+  crimson::DBuf<double> W, fix, partial, dsc, hcol;
+  crimson::DBuf<int> gate;
+  std::vector<double> hbuf;
+  bool coupled = false;
+  int nblk_max = 0;
+
+  int init(crimson_lhs *l, int d) {
+    lhs = l;
+    s = l->stream;
+    N = l->nNo;
+    dof = d;
+    const size_t L = (size_t)dof * N;
+    nblk_max = (int)((L + kChunk - 1) / kChunk);
+    CR_TRY(W.alloc(L));
+    CR_TRY(fix.alloc(L));
+    CR_TRY(partial.alloc((size_t)nblk_max * 256));
+    CR_TRY(dsc.alloc(64));
+    CR_TRY(hcol.alloc(4 * 256));
+    CR_TRY(gate.alloc(4));
+    return CRIMSON_OK;
+  }
+};
+
+MatView view(const Ctx &c, const double *base, int vdof, int r0, int c0, int nr, int nc, const double *W,
+             const double *fix, int wdof) {
+  MatView A;
+  A.base = base;
+  A.nnz = (size_t)c.lhs->nnz;
+  for (int k = 0; k < 16; ++k) A.plane[k] = 0;
+  for (int r = 0; r < nr; ++r)
+    for (int cc = 0; cc < nc; ++cc) A.plane[r * 4 + cc] = (r0 + r) * vdof + (c0 + cc);
+  A.W = W;
+  A.fix = fix;
+  A.wdof = wdof;
+  A.r0 = r0;
+  A.c0 = c0;
+  return A;
+}
+
+// y = ya * (A x) + za * z
+int spmv(Ctx &c, const MatView &A, int nr, int nc, const double *x, double *y, double ya = 1.0,
+         const double *z = nullptr, double za = 0.0) {
+  const unsigned T = 256;
+  const unsigned G = crimson::grid_for((size_t)c.N * kRowLanes, T);
+  const int *rp = c.lhs->d_rowptr.p, *cl = c.lhs->d_col.p;
+#define CR_SPMV(R_, C_) spmv_kernel<R_, C_><<<G, T, 0, c.s>>>(c.N, rp, cl, A, x, y, ya, z, za)
+  if (nr == 3 && nc == 3) CR_SPMV(3, 3);
+  else if (nr == 4 && nc == 4) CR_SPMV(4, 4);
+  else if (nr == 3 && nc == 1) CR_SPMV(3, 1);
+  else if (nr == 1 && nc == 3) CR_SPMV(1, 3);
+  else if (nr == 1 && nc == 1) CR_SPMV(1, 1);
+  else if (nr == 2 && nc == 2) CR_SPMV(2, 2);
+  else if (nr == 2 && nc == 1) CR_SPMV(2, 1);
+  else if (nr == 1 && nc == 2) CR_SPMV(1, 2);
+  else {
+    crimson::set_error("spmv block %dx%d not instantiated", nr, nc);
+    return CRIMSON_EUNSUPPORTED;
+  }
+#undef CR_SPMV
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+
+// dots of v with U_j (j < m) [+ v.v] -> dst on device (m [+1] values)
+int mdot(Ctx &c, size_t L, const double *v, const double *U0, size_t ustride, int m, bool self, double *dst,
+         const int *gate = nullptr) {
+  const int nblk = (int)((L + kChunk - 1) / kChunk);
+  const int pst = m + (self ? 1 : 0);
+  if (pst > 256) {
+    crimson::set_error("more than 255 simultaneous projections");
+    return CRIMSON_EUNSUPPORTED;
+  }
+  mdot_kernel<<<nblk, 256, 0, c.s>>>(L, v, U0, ustride, m, self ? 1 : 0, c.partial.p, pst, gate);
+  reduce_kernel<<<pst, 256, 0, c.s>>>(nblk, c.partial.p, pst, dst, gate);
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+
+int axpby(Ctx &c, size_t L, double a, const double *x, double b, double *y) {
+  axpby_kernel<<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, a, x, b, y);
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+
+int sync_read(Ctx &c, const double *d, double *h, size_t n) {
+  CR_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
+  CR_HIP(hipStreamSynchronize(c.s));
+  return CRIMSON_OK;
+}
+
+double host_norm(Ctx &c, size_t L, const double *v, int *rc) {
+  *rc = mdot(c, L, v, nullptr, 0, 0, true, c.dsc.p + 40);
+  if (*rc) return 0.0;
+  double h = 0;
+  *rc = sync_read(c, c.dsc.p + 40, &h, 1);
+  return std::sqrt(h);
+}
+
+// ADDBCMUL (memLS.f90:206-268) over all coupled faces, X (xdof, N) -> Y.
+int addbcmul(Ctx &c, bool pre, int xdof, const double *X, double *Y) {
+  if (!c.coupled) return CRIMSON_OK;
+  auto &faces = c.lhs->face;
+  // all face dots first (PRE reads the unmodified vector, as unCondU does)
+  for (size_t f = 0; f < faces.size(); ++f) {
+    const CrFace &fa = faces[f];
+    if (!fa.coupled) continue;
+    face_dot_kernel<<<1, 256, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p, fa.d_valM.p, X, c.dsc.p + 48 + f % 16);
+  }
+  for (size_t f = 0; f < faces.size(); ++f) {
+    const CrFace &fa = faces[f];
+    if (!fa.coupled) continue;
+    const double coef = pre ? -fa.res / (1.0 + fa.res * fa.nS) : fa.res;
+    face_axpy_kernel<<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p,
+                                                                        fa.d_valM.p, c.dsc.p + 48 + f % 16, coef, Y);
+  }
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+
+// GMRES (memLS.f90:681-788) for the operator x -> A x + ADD(x), left
+// preconditioned by PRE when faces are coupled; R, X are (nd, N) device.
+struct Workspace {
+  crimson::DBuf<double> U;   // (sD+1) vectors
+  crimson::DBuf<double> tmp;
+};
+
+int gmres_dev(Ctx &c, const MatView &A, int nd, SubLs &ls, const double *R, double *X, Workspace &ws, bool pre_ok,
+              bool count_first = false) {
+  const size_t L = (size_t)nd * c.N;
+  const int sD = ls.sD;
+  CR_TRY(ws.U.alloc(L * (sD + 1)));
+  double *U = ws.U.p;
+  std::vector<double> h((size_t)(sD + 1) * sD, 0.0), y(sD), cs(sD), sn(sD), err(sD + 1);
+  auto H = [&](int i, int j) -> double & { return h[(size_t)i + (size_t)(sD + 1) * j]; };
+  const double t0 = now_s();
+  ls.suc = 0;
+  CR_HIP(hipMemsetAsync(X, 0, L * sizeof(double), c.s));
+  double eps = 0.0;
+  int rc = CRIMSON_OK;
+  std::vector<double> hc(2 * (sD + 2) + 4);
+  for (int l = 1; l <= ls.mItr; ++l) {
+    if (l == 1) {
+      // X = 0 here, so R - A X is R (GMRESV still counts the product)
+      if (count_first) ls.itr = ls.itr + 1;
+      CR_HIP(hipMemcpyAsync(U, R, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+    } else {
+      ls.itr = ls.itr + 1;
+      CR_TRY(spmv(c, A, nd, nd, X, U));
+      CR_TRY(addbcmul(c, false, nd, X, U));
+      CR_TRY(axpby(c, L, 1.0, R, -1.0, U));
+    }
+    if (c.coupled && pre_ok) {
+      CR_HIP(hipMemcpyAsync(ws.tmp.p, U, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+      CR_TRY(addbcmul(c, true, nd, ws.tmp.p, U));
+    }
+    err[0] = host_norm(c, L, U, &rc);
+    if (rc) return rc;
+    if (l == 1) {
+      eps = err[0];
+      if (eps <= ls.absTol) {
+        ls.callD = 0.0;
+        ls.dB = 0.0;
+        return CRIMSON_OK;
+      }
+      ls.iNorm = eps;
+      ls.fNorm = eps;
+      eps = std::max(ls.absTol, ls.relTol * eps);
+    }
+    ls.dB = ls.fNorm;
+    CR_TRY(axpby(c, L, 1.0 / err[0], U, 0.0, U));
+    int i;
+    for (i = 1; i <= sD; ++i) {
+      ls.itr = ls.itr + 1;
+      double *ui = U + L * (size_t)i;
+      const double *uprev = U + L * (size_t)(i - 1);
+      CR_TRY(spmv(c, A, nd, nd, uprev, ui));
+      CR_TRY(addbcmul(c, false, nd, uprev, ui));
+      if (c.coupled && pre_ok) {
+        CR_HIP(hipMemcpyAsync(ws.tmp.p, ui, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+        CR_TRY(addbcmul(c, true, nd, ws.tmp.p, ui));
+      }
+      // CGS pass 1: h1 = U^T w, n0 = w.w ; w -= U h1 ; n1 = w.w
+      double *h1 = c.hcol.p, *h2 = c.hcol.p + 256, *nrm = c.hcol.p + 512;
+      const int nblk = (int)((L + kChunk - 1) / kChunk);
+      CR_TRY(mdot(c, L, ui, U, L, i, true, h1));  // h1[i] = n0
+      msub_norm_kernel<<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h1, c.partial.p, 1, nullptr);
+      reduce_kernel<<<1, 256, 0, c.s>>>(nblk, c.partial.p, 1, nrm + 1, nullptr);
+      dgks_kernel<<<1, 1, 0, c.s>>>(h1 + i, nrm + 1, c.gate.p);
+      // DGKS pass 2 (gated)
+      CR_TRY(mdot(c, L, ui, U, L, i, false, h2, c.gate.p));
+      msub_norm_kernel<<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h2, c.partial.p, 1, c.gate.p);
+      reduce_kernel<<<1, 256, 0, c.s>>>(nblk, c.partial.p, 1, nrm + 2, c.gate.p);
+      CR_HIP(hipGetLastError());
+      // read h1[0..i], h2[0..i-1], n1, n2, gate
+      CR_HIP(hipMemcpyAsync(hc.data(), h1, (i + 1) * sizeof(double), hipMemcpyDeviceToHost, c.s));
+      CR_HIP(hipMemcpyAsync(hc.data() + (sD + 2), h2, i * sizeof(double), hipMemcpyDeviceToHost, c.s));
+      double nn[3];
+      CR_HIP(hipMemcpyAsync(nn, nrm, 3 * sizeof(double), hipMemcpyDeviceToHost, c.s));
+      int gate = 0;
+      CR_HIP(hipMemcpyAsync(&gate, c.gate.p, sizeof(int), hipMemcpyDeviceToHost, c.s));
+      CR_HIP(hipStreamSynchronize(c.s));
+      for (int j = 1; j <= i; ++j) H(j - 1, i - 1) = hc[j - 1] + (gate ? hc[(sD + 2) + j - 1] : 0.0);
+      H(i, i - 1) = std::sqrt(gate ? nn[2] : nn[1]);
+      CR_TRY(axpby(c, L, 1.0 / H(i, i - 1), ui, 0.0, ui));
+      for (int j = 1; j <= i - 1; ++j) {
+        const double tmp = cs[j - 1] * H(j - 1, i - 1) + sn[j - 1] * H(j, i - 1);
+        H(j, i - 1) = -sn[j - 1] * H(j - 1, i - 1) + cs[j - 1] * H(j, i - 1);
+        H(j - 1, i - 1) = tmp;
+      }
+      const double tmp = std::sqrt(H(i - 1, i - 1) * H(i - 1, i - 1) + H(i, i - 1) * H(i, i - 1));
+      cs[i - 1] = H(i - 1, i - 1) / tmp;
+      sn[i - 1] = H(i, i - 1) / tmp;
+      H(i - 1, i - 1) = tmp;
+      H(i, i - 1) = 0.0;
+      err[i] = -sn[i - 1] * err[i - 1];
+      err[i - 1] = cs[i - 1] * err[i - 1];
+      if (std::fabs(err[i]) < eps) {
+        ls.suc = 1;
+        break;
+      }
+    }
+    if (i > sD) i = sD;
+    for (int j = 0; j < i; ++j) y[j] = err[j];
+    for (int j = i; j >= 1; --j) {
+      for (int k = j + 1; k <= i; ++k) y[j - 1] = y[j - 1] - H(j - 1, k - 1) * y[k - 1];
+      y[j - 1] = y[j - 1] / H(j - 1, j - 1);
+    }
+    CR_HIP(hipMemcpyAsync(c.hcol.p + 768, y.data(), i * sizeof(double), hipMemcpyHostToDevice, c.s));
+    maxpy_kernel<<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, X, U, L, i, c.hcol.p + 768, 1.0);
+    CR_HIP(hipGetLastError());
+    CR_HIP(hipStreamSynchronize(c.s));  // y is a host buffer read by the copy above
+    ls.fNorm = std::fabs(err[i]);
+    if (ls.suc) break;
+  }
+  ls.callD = now_s() - t0 + ls.callD;
+  ls.dB = 10.0 * std::log(ls.fNorm / ls.dB);
+  return CRIMSON_OK;
+}
+
+// Generic CG driver on device vectors (memLS CGRAD / CGRADV / CGRADS).
+// op(P, SP) computes SP for the iteration; R is overwritten with X.
+template <class Op>
+int cg_dev(Ctx &c, size_t L, SubLs &ls, double *R, Op op, double *X, double *P, double *SP, bool schur) {
+  const double t0 = now_s();
+  double *scal = c.dsc.p;  // 0 err 1 errO 2 eps2 3 den 4 alpha 5 done 6 itr 7 mItr 8 suc
+  ls.suc = 0;
+  int rc = CRIMSON_OK;
+  const double errO0 = host_norm(c, L, R, &rc);
+  if (rc) return rc;
+  ls.iNorm = errO0;
+  double eps = std::max(ls.absTol, ls.relTol * errO0);
+  eps = eps * eps;
+  double hs[9] = {errO0 * errO0, errO0 * errO0, eps, 0, 0, 0, 0, (double)ls.mItr, 0};
+  CR_HIP(hipMemcpyAsync(scal, hs, sizeof(hs), hipMemcpyHostToDevice, c.s));
+  CR_HIP(hipMemsetAsync(X, 0, L * sizeof(double), c.s));
+  CR_HIP(hipMemcpyAsync(P, R, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+  const int nblk = (int)((L + kChunk - 1) / kChunk);
+  const int batch = 8;
+  double back[9];
+  for (;;) {
+    for (int b = 0; b < batch; ++b) {
+      cg_check_kernel<<<1, 1, 0, c.s>>>(scal);
+      CR_TRY(op(P, SP, scal));
+      // den = P . SP
+      mdot_kernel<<<nblk, 256, 0, c.s>>>(L, SP, P, 0, 1, 0, c.partial.p, 1, nullptr);
+      reduce_kernel<<<1, 256, 0, c.s>>>(nblk, c.partial.p, 1, scal + 3, nullptr);
+      cg_alpha_kernel<<<1, 1, 0, c.s>>>(scal);
+      cg_update_kernel<<<nblk, 256, 0, c.s>>>(L, scal, X, R, P, SP, c.partial.p);
+      reduce_kernel<<<1, 256, 0, c.s>>>(nblk, c.partial.p, 1, scal + 9, nullptr);
+      cg_err_kernel<<<1, 1, 0, c.s>>>(scal, scal + 9);
+      cg_p_kernel<<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, scal, R, P);
+    }
+    CR_HIP(hipGetLastError());
+    CR_HIP(hipMemcpyAsync(back, scal, sizeof(back), hipMemcpyDeviceToHost, c.s));
+    CR_HIP(hipStreamSynchronize(c.s));
+    if (back[5] != 0.0) break;
+  }
+  // the flag is raised at the top of the iteration after the last one
+  CR_HIP(hipMemcpyAsync(R, X, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+  const double err = back[0], errO = back[1];
+  ls.suc = back[8] != 0.0;
+  ls.fNorm = std::sqrt(err);
+  ls.dB = 5.0 * std::log(err / errO);
+  const int done_itr = (int)back[6];
+  if (schur)
+    ls.itr = ls.itr + done_itr;  // CGRAD accumulates (memLS.f90:428)
+  else
+    ls.itr = done_itr;
+  ls.callD = now_s() - t0 + (schur ? ls.callD : 0.0);
+  return CRIMSON_OK;
+}
+
+// NSSOLVER (memLS.f90:1460-1759) on device; Ri (4,N) device in/out
+// (preconditioned RHS in, preconditioned solution out).
+int nssolver_dev(Ctx &c, crimson_ls *ls, double *Ri) {
+  crimson_lhs *l = c.lhs;
+  const int N = c.N, nsd = 3;
+  const size_t mn = (size_t)nsd * N;
+  int iB = ls->RI.mItr;
+  const int nB = 2 * iB;
+  crimson::DBuf<double> Rm, Rc, Rmi, Rci, U, P, MU, MP, t1, t2, t3, t4, dxB;
+  CR_TRY(Rm.alloc(mn));
+  CR_TRY(Rc.alloc(N));
+  CR_TRY(Rmi.alloc(mn));
+  CR_TRY(Rci.alloc(N));
+  CR_TRY(U.alloc(mn * iB));
+  CR_TRY(P.alloc((size_t)N * iB));
+  CR_TRY(MU.alloc(mn * nB));
+  CR_TRY(MP.alloc((size_t)N * nB));
+  CR_TRY(t1.alloc(mn));
+  CR_TRY(t2.alloc(N));
+  CR_TRY(t3.alloc(N));
+  CR_TRY(t4.alloc(N));
+  CR_TRY(dxB.alloc(nB));
+  Workspace ws;
+  CR_TRY(ws.tmp.alloc(mn));
+  const unsigned gN = crimson::grid_for(N, 256);
+  extract_kernel<<<gN, 256, 0, c.s>>>(N, 4, 0, 3, Ri, Rmi.p);
+  extract_kernel<<<gN, 256, 0, c.s>>>(N, 4, 3, 1, Ri, Rci.p);
+  CR_HIP(hipMemcpyAsync(Rm.p, Rmi.p, mn * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+  CR_HIP(hipMemcpyAsync(Rc.p, Rci.p, N * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+  int rc = CRIMSON_OK;
+  const double a1 = host_norm(c, mn, Rm.p, &rc);
+  if (rc) return rc;
+  const double a2 = host_norm(c, N, Rc.p, &rc);
+  if (rc) return rc;
+  double eps = std::sqrt(a1 * a1 + a2 * a2);
+  ls->RI.iNorm = eps;
+  ls->RI.fNorm = eps;
+  ls->CG.callD = 0.0;
+  ls->GM.callD = 0.0;
+  ls->CG.itr = 0;
+  ls->GM.itr = 0;
+  ls->RI.suc = 0;
+  const double t0 = now_s();
+  eps = std::max(ls->RI.absTol, ls->RI.relTol * eps);
+  // DEPART: views on the 4x4 planes (memLS.f90:1612-1679); Gt = -mG^T
+  const double *V = l->d_V.p;
+  const MatView mK = view(c, V, 4, 0, 0, 3, 3, c.W.p, c.fix.p, 4);
+  const MatView mG = view(c, V, 4, 0, 3, 3, 1, c.W.p, nullptr, 4);
+  const MatView mD = view(c, V, 4, 3, 0, 1, 3, c.W.p, nullptr, 4);
+  const MatView mL = view(c, V, 4, 3, 3, 1, 1, c.W.p, c.fix.p, 4);
+  MatView Gt = mD;
+  if (!l->gt_is_d) {
+    CR_TRY(l->d_Gt.alloc((size_t)3 * l->nnz));
+    gt_kernel<<<crimson::grid_for(l->nnz, 256), 256, 0, c.s>>>(l->nnz, V, l->d_tr.p, l->d_Gt.p);
+    CR_HIP(hipGetLastError());
+    Gt = view(c, l->d_Gt.p, 1, 0, 0, 1, 3, c.W.p, nullptr, 4);
+    Gt.plane[0] = 0;
+    Gt.plane[1] = 1;
+    Gt.plane[2] = 2;
+    Gt.r0 = 3;
+    Gt.c0 = 0;
+  }
+  // BCPRE (memLS.f90:1683-1716): nS = ||valM||^2 over nsd components
+  for (auto &fa : l->face) {
+    if (!fa.coupled) continue;
+    std::vector<double> vm((size_t)fa.dof * fa.nNo);
+    CR_HIP(hipMemcpyAsync(vm.data(), fa.d_valM.p, vm.size() * sizeof(double), hipMemcpyDeviceToHost, c.s));
+    CR_HIP(hipStreamSynchronize(c.s));
+    fa.nS = 0.0;
+    const int nc = std::min(fa.dof, nsd);
+    for (int a = 0; a < fa.nNo; ++a)
+      for (int i = 0; i < nc; ++i) fa.nS = fa.nS + vm[i + (size_t)fa.dof * a] * vm[i + (size_t)fa.dof * a];
+  }
+  // Schur operator of CGRAD: SP = L P - Gt (G P + PRE)
+  auto schur_op = [&](const double *Pv, double *SPv, const double *scal) -> int {
+    (void)scal;
+    CR_TRY(spmv(c, mG, 3, 1, Pv, t1.p));
+    if (c.coupled) {
+      CR_HIP(hipMemcpyAsync(ws.tmp.p, t1.p, mn * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+      CR_TRY(addbcmul(c, true, 3, ws.tmp.p, t1.p));
+    }
+    CR_TRY(spmv(c, Gt, 1, 3, t1.p, t2.p));
+    CR_TRY(spmv(c, mL, 1, 1, Pv, SPv, 1.0, t2.p, -1.0));
+    return CRIMSON_OK;
+  };
+  std::vector<double> A((size_t)nB * nB, 0.0), Bv(nB, 0.0), xB(nB, 0.0);
+  std::vector<double> dots(2 * nB + 4);
+  int i, iBB = 0;
+  for (i = 1; i <= ls->RI.mItr; ++i) {
+    iB = 2 * i - 1;
+    iBB = 2 * i;
+    ls->RI.dB = ls->RI.fNorm;
+    double *Ui = U.p + mn * (i - 1), *Pi = P.p + (size_t)N * (i - 1);
+    double *MUb = MU.p + mn * (iB - 1), *MUbb = MU.p + mn * (iBB - 1);
+    double *MPb = MP.p + (size_t)N * (iB - 1), *MPbb = MP.p + (size_t)N * (iBB - 1);
+    CR_TRY(gmres_dev(c, mK, 3, ls->GM, Rm.p, Ui, ws, true));
+    CR_TRY(spmv(c, mD, 1, 3, Ui, Pi, -1.0, Rc.p, 1.0));  // P = Rc - D U
+    CR_TRY(cg_dev(c, (size_t)N, ls->CG, Pi, schur_op, t3.p, t4.p, MPb, true));
+    CR_TRY(spmv(c, mG, 3, 1, Pi, MUb));
+    CR_TRY(axpby(c, mn, 1.0, Rm.p, 0.0, MUbb));
+    CR_TRY(axpby(c, mn, -1.0, MUb, 1.0, MUbb));  // MU(iBB) = Rm - MU(iB)
+    CR_TRY(gmres_dev(c, mK, 3, ls->GM, MUbb, Ui, ws, true));
+    CR_TRY(spmv(c, mK, 3, 3, Ui, MUbb));
+    CR_TRY(addbcmul(c, false, 3, Ui, MUbb));
+    CR_TRY(spmv(c, mL, 1, 1, Pi, MPb));
+    CR_TRY(spmv(c, mD, 1, 3, Ui, MPbb));
+    // Gram matrix rows iB, iBB: dots with all previous columns and the RHS
+    for (int k = iB; k <= iBB; ++k) {
+      const double *MUk = MU.p + mn * (k - 1), *MPk = MP.p + (size_t)N * (k - 1);
+      // MU_j . MU_k for j < k, then MU_k.MU_k ; plus MU_k . Rmi
+      CR_TRY(mdot(c, mn, MUk, MU.p, mn, k - 1, true, c.hcol.p));
+      CR_TRY(mdot(c, mn, MUk, Rmi.p, 0, 1, false, c.hcol.p + 256));
+      CR_TRY(mdot(c, N, MPk, MP.p, N, k - 1, true, c.hcol.p + 512));
+      CR_TRY(mdot(c, N, MPk, Rci.p, 0, 1, false, c.hcol.p + 768));
+      std::vector<double> h1(k + 1), h2(k + 1), r1(1), r2(1);
+      CR_HIP(hipMemcpyAsync(h1.data(), c.hcol.p, (k) * sizeof(double), hipMemcpyDeviceToHost, c.s));
+      CR_HIP(hipMemcpyAsync(r1.data(), c.hcol.p + 256, sizeof(double), hipMemcpyDeviceToHost, c.s));
+      CR_HIP(hipMemcpyAsync(h2.data(), c.hcol.p + 512, (k) * sizeof(double), hipMemcpyDeviceToHost, c.s));
+      CR_HIP(hipMemcpyAsync(r2.data(), c.hcol.p + 768, sizeof(double), hipMemcpyDeviceToHost, c.s));
+      CR_HIP(hipStreamSynchronize(c.s));
+      for (int j = 1; j <= k - 1; ++j) {
+        A[(j - 1) + (size_t)nB * (k - 1)] = h1[j - 1] + h2[j - 1];
+        A[(k - 1) + (size_t)nB * (j - 1)] = A[(j - 1) + (size_t)nB * (k - 1)];
+      }
+      const double n1 = std::sqrt(h1[k - 1]), n2 = std::sqrt(h2[k - 1]);
+      A[(k - 1) + (size_t)nB * (k - 1)] = n1 * n1 + n2 * n2;
+      Bv[k - 1] = r1[0] + r2[0];
+    }
+    for (int k = 0; k < iBB; ++k) xB[k] = Bv[k];
+    // GE (memLS.f90:607-677)
+    {
+      const int Nn = iBB;
+      if (Nn == 2) {
+        const double a11 = A[0], a21 = A[1], a12 = A[(size_t)nB], a22 = A[(size_t)nB + 1];
+        const double pivot = a11 * a22 - a21 * a12;
+        const double saveEl = (xB[0] * a22 - xB[1] * a12) / pivot;
+        xB[1] = (xB[1] * a11 - xB[0] * a21) / pivot;
+        xB[0] = saveEl;
+      } else {
+        std::vector<double> Cm((size_t)Nn * (Nn + 1));
+        auto CC = [&](int r, int cc) -> double & { return Cm[r + (size_t)Nn * cc]; };
+        for (int cc = 0; cc < Nn; ++cc)
+          for (int r = 0; r < Nn; ++r) CC(r, cc) = A[r + (size_t)nB * cc];
+        for (int r = 0; r < Nn; ++r) CC(r, Nn) = xB[r];
+        for (int m = 0; m < Nn - 1; ++m) {
+          int ipv = m;
+          double pivot = std::fabs(CC(m, m));
+          for (int r = m + 1; r < Nn; ++r)
+            if (std::fabs(CC(r, m)) > pivot) {
+              ipv = r;
+              pivot = std::fabs(CC(r, m));
+            }
+          if (pivot < 2.0 * 2.220446049250313e-16) {
+            crimson::set_error("memLS GE: singular matrix");
+            return CRIMSON_ESINGULAR;
+          }
+          if (ipv != m)
+            for (int cc = m; cc <= Nn; ++cc) std::swap(CC(m, cc), CC(ipv, cc));
+          for (int r = m + 1; r < Nn; ++r) {
+            const double sv = CC(r, m) / CC(m, m);
+            CC(r, m) = 0.0;
+            for (int cc = m + 1; cc <= Nn; ++cc) CC(r, cc) = CC(r, cc) - sv * CC(m, cc);
+          }
+        }
+        for (int j = Nn - 1; j >= 0; --j) {
+          for (int r = j + 1; r < Nn; ++r) CC(j, Nn) = CC(j, Nn) - CC(j, r) * CC(r, Nn);
+          CC(j, Nn) = CC(j, Nn) / CC(j, j);
+        }
+        for (int r = 0; r < Nn; ++r) xB[r] = CC(r, Nn);
+      }
+    }
+    double sxb = 0.0;
+    for (int k = 0; k < iBB; ++k) sxb = sxb + xB[k] * Bv[k];
+    ls->RI.fNorm = std::sqrt(ls->RI.iNorm * ls->RI.iNorm - sxb);
+    if (ls->RI.fNorm < eps) {
+      ls->RI.suc = 1;
+      break;
+    }
+    CR_HIP(hipMemcpyAsync(dxB.p, xB.data(), iBB * sizeof(double), hipMemcpyHostToDevice, c.s));
+    CR_TRY(axpby(c, mn, 1.0, Rmi.p, 0.0, Rm.p));
+    CR_TRY(axpby(c, N, 1.0, Rci.p, 0.0, Rc.p));
+    maxpy_kernel<<<crimson::grid_for(mn, 256), 256, 0, c.s>>>(mn, Rm.p, MU.p, mn, iBB, dxB.p, -1.0);
+    maxpy_kernel<<<crimson::grid_for(N, 256), 256, 0, c.s>>>(N, Rc.p, MP.p, N, iBB, dxB.p, -1.0);
+    CR_HIP(hipGetLastError());
+    CR_HIP(hipStreamSynchronize(c.s));
+  }
+  if (i > ls->RI.mItr) {
+    ls->RI.itr = ls->RI.mItr;
+  } else {
+    ls->RI.itr = i;
+    CR_HIP(hipMemcpyAsync(dxB.p, xB.data(), iBB * sizeof(double), hipMemcpyHostToDevice, c.s));
+    CR_TRY(axpby(c, N, 1.0, Rci.p, 0.0, Rc.p));
+    maxpy_kernel<<<crimson::grid_for(N, 256), 256, 0, c.s>>>(N, Rc.p, MP.p, N, iBB, dxB.p, -1.0);
+    CR_HIP(hipGetLastError());
+  }
+  {
+    const double rcn = host_norm(c, N, Rc.p, &rc) / ls->RI.fNorm;
+    if (rc) return rc;
+    ls->Resc = (int)std::lround(100.0 * rcn * rcn);
+    ls->Resm = 100 - ls->Resc;
+  }
+  // solution: Rmi = sum xB(2i) U_i, Rci = sum xB(2i-1) P_i
+  {
+    const int nit = ls->RI.itr;
+    std::vector<double> cu(nit), cp(nit);
+    for (int k = 1; k <= nit; ++k) {
+      cu[k - 1] = xB[2 * k - 1];
+      cp[k - 1] = xB[2 * k - 2];
+    }
+    CR_HIP(hipMemcpyAsync(c.hcol.p, cu.data(), nit * sizeof(double), hipMemcpyHostToDevice, c.s));
+    CR_HIP(hipMemcpyAsync(c.hcol.p + 256, cp.data(), nit * sizeof(double), hipMemcpyHostToDevice, c.s));
+    CR_HIP(hipMemsetAsync(Rmi.p, 0, mn * sizeof(double), c.s));
+    CR_HIP(hipMemsetAsync(Rci.p, 0, N * sizeof(double), c.s));
+    maxpy_kernel<<<crimson::grid_for(mn, 256), 256, 0, c.s>>>(mn, Rmi.p, U.p, mn, nit, c.hcol.p, 1.0);
+    maxpy_kernel<<<crimson::grid_for(N, 256), 256, 0, c.s>>>(N, Rci.p, P.p, N, nit, c.hcol.p + 256, 1.0);
+    insert_kernel<<<gN, 256, 0, c.s>>>(N, 4, 0, 3, Rmi.p, Ri);
+    insert_kernel<<<gN, 256, 0, c.s>>>(N, 4, 3, 1, Rci.p, Ri);
+    CR_HIP(hipGetLastError());
+    CR_HIP(hipStreamSynchronize(c.s));
+  }
+  ls->RI.callD = now_s() - t0;
+  ls->RI.dB = 10.0 * std::log(ls->RI.fNorm / ls->RI.dB);
+  return CRIMSON_OK;
+}
+
+int solve_dev(crimson_lhs *l, crimson_ls *ls, int dof, double *d_Ri, const int *incL, const double *res) {
+  if (l->vdof != dof || !l->d_V.p) {
+    crimson::set_error("lhs holds no matrix of block size %d (stored: %d)", dof, l->vdof);
+    return CRIMSON_ESTATE;
+  }
+  if (ls->LS_type == CRIMSON_LS_NS && dof != 4) {
+    crimson::set_error("LS_TYPE_NS needs dof = 4 (got %d)", dof);
+    return CRIMSON_EUNSUPPORTED;
+  }
+  Ctx c;
+  CR_TRY(c.init(l, dof));
+  // face flags (memLS.f90:1965-1986)
+  for (int f = 0; f < l->nFaces; ++f) {
+    CrFace &fa = l->face[f];
+    fa.inc = true;
+    if (incL && incL[f] == 0) fa.inc = false;
+    fa.coupled = false;
+    if (!fa.inc || !fa.created) continue;
+    if (fa.bGrp == CRIMSON_BC_NEU && res && res[f] != 0.0) {
+      fa.res = res[f];
+      fa.coupled = true;
+      c.coupled = true;
+    }
+  }
+  const int N = l->nNo;
+  const size_t L = (size_t)dof * N;
+  const unsigned gN = crimson::grid_for(N, 256);
+  // PRECOND (memLS.f90:1763-1943) as the W/fix vectors
+  precond_diag_kernel<<<gN, 256, 0, c.s>>>(N, dof, l->vdof, l->d_V.p, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
+  for (auto &fa : l->face)
+    if (fa.created && fa.inc && fa.bGrp == CRIMSON_BC_DIR && fa.nNo > 0)
+      precond_face_kernel<<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, dof, fa.dof, fa.d_glob.p,
+                                                                           fa.d_val.p, c.W.p);
+  precond_fix_kernel<<<gN, 256, 0, c.s>>>(N, dof, l->vdof, l->d_V.p, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
+  for (auto &fa : l->face)
+    if (fa.coupled) {
+      CR_TRY(fa.d_valM.alloc((size_t)fa.dof * fa.nNo));
+      face_valm_kernel<<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, dof, fa.dof, fa.d_glob.p,
+                                                                         fa.d_val.p, c.W.p, fa.d_valM.p);
+    }
+  mul_kernel<<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, d_Ri, c.W.p, d_Ri);  // R = W R
+  CR_HIP(hipGetLastError());
+  int rc = CRIMSON_OK;
+  if (ls->LS_type == CRIMSON_LS_NS) {
+    rc = nssolver_dev(c, ls, d_Ri);
+  } else if (ls->LS_type == CRIMSON_LS_GMRES) {
+    // GMRESV (memLS.f90:792-883): no PRE application, restart from X
+    Workspace ws;
+    CR_TRY(ws.tmp.alloc(L));
+    crimson::DBuf<double> X;
+    CR_TRY(X.alloc(L));
+    const MatView A = view(c, l->d_V.p, dof, 0, 0, dof, dof, c.W.p, c.fix.p, dof);
+    ls->RI.itr = 0;
+    const double nr = host_norm(c, L, d_Ri, &rc);
+    if (rc) return rc;
+    if (nr <= ls->RI.absTol) {
+      ls->RI.iNorm = nr;
+      ls->RI.fNorm = nr;
+      ls->RI.suc = 0;
+      ls->RI.callD = 0.0;
+      ls->RI.dB = 0.0;
+      CR_HIP(hipMemsetAsync(d_Ri, 0, L * sizeof(double), c.s));
+    } else {
+      // GMRESV counts the residual product of every cycle (memLS.f90:827-829)
+      rc = gmres_dev(c, A, dof, ls->RI, d_Ri, X.p, ws, false, true);
+      if (rc) return rc;
+      CR_HIP(hipMemcpyAsync(d_Ri, X.p, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+    }
+  } else if (ls->LS_type == CRIMSON_LS_CG) {
+    const MatView A = view(c, l->d_V.p, dof, 0, 0, dof, dof, c.W.p, c.fix.p, dof);
+    crimson::DBuf<double> X, P, SP;
+    CR_TRY(X.alloc(L));
+    CR_TRY(P.alloc(L));
+    CR_TRY(SP.alloc(L));
+    auto op = [&](const double *Pv, double *SPv, const double *) -> int { return spmv(c, A, dof, dof, Pv, SPv); };
+    rc = cg_dev(c, L, ls->RI, d_Ri, op, X.p, P.p, SP.p, false);
+  } else {
+    crimson::set_error("LS_type %d not defined", ls->LS_type);
+    return CRIMSON_EARG;
+  }
+  if (rc) return rc;
+  mul_kernel<<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, d_Ri, c.W.p, d_Ri);  // R = R W
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipStreamSynchronize(c.s));
+  return CRIMSON_OK;
+}
+
+}  // namespace
+
+extern "C" int crimson_ls_create(int lsType, int present, double relTol, double absTol, int maxItr, int dimKry,
+                                 const double *relTolIn, const double *absTolIn, const int *maxItrIn,
+                                 crimson_ls **out) {
+  if (!out) {
+    crimson::set_error("crimson_ls_create: null output");
+    return CRIMSON_EARG;
+  }
+  crimson_ls *ls = new crimson_ls();
+  ls->LS_type = lsType;
+  switch (lsType) {  // memLS.f90:1394-1416
+    case CRIMSON_LS_NS:
+      ls->RI.relTol = 4e-1;
+      ls->GM.relTol = 1e-2;
+      ls->CG.relTol = 1e-1;
+      ls->RI.mItr = 10;
+      ls->GM.mItr = 3;
+      ls->CG.mItr = 500;
+      ls->GM.sD = 50;
+      break;
+    case CRIMSON_LS_GMRES:
+      ls->RI.relTol = 1e-2;
+      ls->RI.mItr = 2;
+      ls->RI.sD = 150;
+      break;
+    case CRIMSON_LS_CG:
+      ls->RI.relTol = 1e-4;
+      ls->RI.mItr = 1000;
+      break;
+    default:
+      delete ls;
+      crimson::set_error("Solver type LS_TYPE %d is not defined", lsType);
+      return CRIMSON_EARG;
+  }
+  ls->RI.absTol = 1e-10;
+  ls->GM.absTol = 1e-10;
+  ls->CG.absTol = 1e-10;
+  if (present & 1) ls->RI.relTol = relTol;
+  if (present & 2) ls->RI.absTol = absTol;
+  if (present & 4) ls->RI.mItr = maxItr;
+  if (present & 8) {
+    ls->RI.sD = dimKry;
+    ls->GM.sD = dimKry;
+  }
+  if ((present & 16) && relTolIn) {
+    ls->GM.relTol = relTolIn[0];
+    ls->CG.relTol = relTolIn[1];
+  }
+  if ((present & 32) && absTolIn) {
+    ls->GM.absTol = absTolIn[0];
+    ls->CG.absTol = absTolIn[1];
+  }
+  if ((present & 64) && maxItrIn) {
+    ls->GM.mItr = maxItrIn[0];
+    ls->CG.mItr = maxItrIn[1];
+  }
+  // NS keeps 2*maxItr least-squares columns, each a row of one projection
+  // sweep (at most 255 per sweep); GMRES keeps dimKry+1 basis vectors.
+  const bool bad_ns = lsType == CRIMSON_LS_NS && (ls->RI.mItr > 127 || ls->GM.sD < 1 || ls->GM.sD > 250);
+  const bool bad_gm = lsType == CRIMSON_LS_GMRES && (ls->RI.sD < 1 || ls->RI.sD > 250);
+  if (ls->RI.mItr < 1 || bad_ns || bad_gm) {
+    crimson::set_error("solver limits out of range (maxItr %d, dimKry %d/%d; supported: maxItr >= 1, "
+                       "NS maxItr <= 127, dimKry 1..250)",
+                       ls->RI.mItr, ls->RI.sD, ls->GM.sD);
+    delete ls;
+    return CRIMSON_EUNSUPPORTED;
+  }
+  *out = ls;
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_ls_free(crimson_ls *ls) {
+  delete ls;
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_lhs_create(int gnNo, int nNo, int nnz, const int *gNodes, const int *rowPtr,
+                                  const int *colPtr, int nFaces, crimson_lhs **out) {
+  if (!out || !rowPtr || !colPtr || nNo <= 0 || nnz <= 0 || nFaces < 0 || gnNo != nNo) {
+    crimson::set_error("crimson_lhs_create: invalid arguments (single task: gnNo must equal nNo)");
+    return CRIMSON_EARG;
+  }
+  if (gNodes)
+    for (int a = 0; a < nNo; ++a)
+      if (gNodes[a] != a + 1) {
+        crimson::set_error("crimson_lhs_create: gNodes must be the identity on one task");
+        return CRIMSON_EUNSUPPORTED;
+      }
+  crimson_lhs *l = new crimson_lhs();
+  l->nNo = nNo;
+  l->nnz = nnz;
+  l->nFaces = nFaces;
+  l->face.resize(nFaces);
+  l->h_rowptr.resize(nNo + 1);
+  l->h_col.resize(nnz);
+  l->h_diag.assign(nNo, -1);
+  l->h_tr.assign(nnz, -1);
+  auto fail = [&](int code) {
+    crimson_lhs_free(l);
+    return code;
+  };
+  for (int i = 0; i <= nNo; ++i) l->h_rowptr[i] = rowPtr[i] - 1;
+  if (l->h_rowptr[0] != 0 || l->h_rowptr[nNo] != nnz) {
+    crimson::set_error("rowPtr must run from 1 to nnz+1");
+    return fail(CRIMSON_EARG);
+  }
+  for (int k = 0; k < nnz; ++k) {
+    l->h_col[k] = colPtr[k] - 1;
+    if (l->h_col[k] < 0 || l->h_col[k] >= nNo) {
+      crimson::set_error("colPtr(%d)=%d out of range", k + 1, colPtr[k]);
+      return fail(CRIMSON_EARG);
+    }
+  }
+  for (int i = 0; i < nNo; ++i) {
+    if (l->h_rowptr[i + 1] < l->h_rowptr[i]) {
+      crimson::set_error("rowPtr not monotone at %d", i + 1);
+      return fail(CRIMSON_EARG);
+    }
+    for (int k = l->h_rowptr[i]; k < l->h_rowptr[i + 1]; ++k) {
+      if (l->h_col[k] == i && l->h_diag[i] < 0) l->h_diag[i] = k;
+      // transpose position (the DEPART search, memLS.f90:1668-1679, done once)
+      const int j = l->h_col[k];
+      for (int m = l->h_rowptr[j]; m < l->h_rowptr[j + 1]; ++m)
+        if (l->h_col[m] == i) {
+          l->h_tr[k] = m;
+          break;
+        }
+    }
+    if (l->h_diag[i] < 0) {
+      crimson::set_error("row %d has no diagonal block", i + 1);
+      return fail(CRIMSON_ENODIAG);
+    }
+  }
+  for (int k = 0; k < nnz; ++k)
+    if (l->h_tr[k] < 0) {
+      crimson::set_error("CSR graph not symmetric at entry %d", k + 1);
+      return fail(CRIMSON_EGRAPH);
+    }
+  if (hipStreamCreate(&l->stream) != hipSuccess) {
+    l->stream = nullptr;
+    crimson::set_error("hipStreamCreate failed (is a GPU visible?)");
+    return fail(CRIMSON_EHIP);
+  }
+  if (hipEventCreate(&l->ev0) != hipSuccess || hipEventCreate(&l->ev1) != hipSuccess) {
+    crimson::set_error("hipEventCreate failed");
+    return fail(CRIMSON_EHIP);
+  }
+  int rc;
+  if ((rc = l->d_rowptr.upload(l->h_rowptr.data(), nNo + 1, l->stream)) ||
+      (rc = l->d_col.upload(l->h_col.data(), nnz, l->stream)) ||
+      (rc = l->d_diag.upload(l->h_diag.data(), nNo, l->stream)) ||
+      (rc = l->d_tr.upload(l->h_tr.data(), nnz, l->stream)))
+    return fail(rc);
+  if (hipStreamSynchronize(l->stream) != hipSuccess) {
+    crimson::set_error("upload of the lhs structure failed");
+    return fail(CRIMSON_EHIP);
+  }
+  *out = l;
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_lhs_free(crimson_lhs *l) {
+  if (!l) return CRIMSON_OK;
+  if (l->stream) (void)hipStreamSynchronize(l->stream);
+  if (l->ev0) (void)hipEventDestroy(l->ev0);
+  if (l->ev1) (void)hipEventDestroy(l->ev1);
+  hipStream_t s = l->stream;
+  delete l;
+  if (s) (void)hipStreamDestroy(s);
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_bc_create(crimson_lhs *l, int faIn, int nNo, int dof, int bcType, const int *gNodes,
+                                 const double *val) {
+  if (!l || faIn < 1 || faIn > l->nFaces || nNo < 0 || dof < 1 || (nNo > 0 && !gNodes) ||
+      (bcType != CRIMSON_BC_DIR && bcType != CRIMSON_BC_NEU)) {
+    crimson::set_error("crimson_bc_create: invalid arguments (faIn=%d of %d)", faIn, l ? l->nFaces : 0);
+    return CRIMSON_EARG;
+  }
+  CrFace &fa = l->face[faIn - 1];
+  fa.nNo = nNo;
+  fa.dof = dof;
+  fa.bGrp = bcType;
+  fa.res = 0.0;
+  fa.created = true;
+  fa.glob.resize(nNo);
+  for (int a = 0; a < nNo; ++a) {
+    fa.glob[a] = gNodes[a] - 1;
+    if (fa.glob[a] < 0 || fa.glob[a] >= l->nNo) {
+      crimson::set_error("gNodes(%d)=%d out of range", a + 1, gNodes[a]);
+      return CRIMSON_EARG;
+    }
+  }
+  fa.val.assign((size_t)dof * nNo, 0.0);
+  if (val) std::memcpy(fa.val.data(), val, sizeof(double) * dof * nNo);
+  CR_TRY(fa.d_glob.upload(fa.glob.data(), nNo, l->stream));
+  CR_TRY(fa.d_val.upload(fa.val.data(), fa.val.size(), l->stream));
+  CR_TRY(fa.d_valM.alloc(fa.val.size()));
+  CR_HIP(hipStreamSynchronize(l->stream));
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_lhs_set_val(crimson_lhs *l, int dof, const double *Val) {
+  if (!l || !Val || dof < 1 || dof > 4) {
+    crimson::set_error("crimson_lhs_set_val: invalid arguments");
+    return CRIMSON_EARG;
+  }
+  const size_t m = (size_t)dof * dof, nnz = l->nnz;
+  crimson::DBuf<double> aos;
+  CR_TRY(aos.upload(Val, m * nnz, l->stream));
+  CR_TRY(l->d_V.alloc(m * nnz));
+  aos_to_planes_kernel<<<crimson::grid_for(nnz, 256), 256, 0, l->stream>>>(nnz, (int)m, aos.p, l->d_V.p);
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipStreamSynchronize(l->stream));
+  l->vdof = dof;
+  l->gt_is_d = false;
+  return CRIMSON_OK;
+}
+
+__global__ void planes_to_aos_kernel(size_t nnz, int m, const double *__restrict__ V, double *__restrict__ aos) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  for (int p = 0; p < m; ++p) aos[(size_t)m * k + p] = V[(size_t)p * nnz + k];
+}
+
+extern "C" int crimson_lhs_get_val(crimson_lhs *l, int dof, double *Val) {
+  if (!l || !Val || dof != l->vdof || !l->d_V.p) {
+    crimson::set_error("crimson_lhs_get_val: no matrix of block size %d held", dof);
+    return CRIMSON_EARG;
+  }
+  const size_t m = (size_t)dof * dof, nnz = l->nnz;
+  crimson::DBuf<double> aos;
+  CR_TRY(aos.alloc(m * nnz));
+  planes_to_aos_kernel<<<crimson::grid_for(nnz, 256), 256, 0, l->stream>>>(nnz, (int)m, l->d_V.p, aos.p);
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipMemcpyAsync(Val, aos.p, m * nnz * sizeof(double), hipMemcpyDeviceToHost, l->stream));
+  CR_HIP(hipStreamSynchronize(l->stream));
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_ls_solve_dev(crimson_lhs *l, crimson_ls *ls, int dof, double *d_Ri, const int *incL,
+                                    const double *res) {
+  if (!l || !ls || !d_Ri) {
+    crimson::set_error("crimson_ls_solve_dev: null argument");
+    return CRIMSON_EARG;
+  }
+  return solve_dev(l, ls, dof, d_Ri, incL, res);
+}
+
+extern "C" int crimson_ls_solve(crimson_lhs *l, crimson_ls *ls, int dof, double *Ri, const double *Val,
+                                const int *incL, const double *res) {
+  if (!l || !ls || !Ri || !Val) {
+    crimson::set_error("crimson_ls_solve: null argument");
+    return CRIMSON_EARG;
+  }
+  CR_TRY(crimson_lhs_set_val(l, dof, Val));
+  crimson::DBuf<double> dR;
+  const size_t L = (size_t)dof * l->nNo;
+  CR_TRY(dR.upload(Ri, L, l->stream));
+  CR_TRY(solve_dev(l, ls, dof, dR.p, incL, res));
+  CR_HIP(hipMemcpyAsync(Ri, dR.p, L * sizeof(double), hipMemcpyDeviceToHost, l->stream));
+  CR_HIP(hipStreamSynchronize(l->stream));
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_ls_stats(const crimson_ls *ls, int sub, double *o) {
+  if (!ls || !o || sub < 0 || sub > 2) {
+    crimson::set_error("crimson_ls_stats: invalid arguments");
+    return CRIMSON_EARG;
+  }
+  const SubLs &s = sub == 0 ? ls->RI : (sub == 1 ? ls->GM : ls->CG);
+  o[0] = s.iNorm;
+  o[1] = s.fNorm;
+  o[2] = s.itr;
+  o[3] = s.suc;
+  o[4] = s.dB;
+  o[5] = s.callD;
+  o[6] = ls->Resm;
+  o[7] = ls->Resc;
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_lhs_spmv_dev(crimson_lhs *l, int dof, const double *d_x, double *d_y, int reps, double *ms) {
+  if (!l || !d_x || !d_y || reps < 1 || !ms || !l->d_V.p || (dof != 3 && dof != 4) || l->vdof != 4) {
+    crimson::set_error("crimson_lhs_spmv_dev: invalid arguments (needs a 4x4 matrix, dof 3 or 4)");
+    return CRIMSON_EARG;
+  }
+  Ctx c;
+  CR_TRY(c.init(l, 4));
+  const unsigned gN = crimson::grid_for(l->nNo, 256);
+  precond_diag_kernel<<<gN, 256, 0, c.s>>>(l->nNo, 4, 4, l->d_V.p, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
+  const MatView A = view(c, l->d_V.p, 4, 0, 0, dof, dof, c.W.p, c.fix.p, 4);
+  CR_TRY(spmv(c, A, dof, dof, d_x, d_y));  // warm-up
+  CR_HIP(hipEventRecord(l->ev0, c.s));
+  for (int r = 0; r < reps; ++r) CR_TRY(spmv(c, A, dof, dof, d_x, d_y));
+  CR_HIP(hipEventRecord(l->ev1, c.s));
+  CR_HIP(hipEventSynchronize(l->ev1));
+  float t = 0.f;
+  CR_HIP(hipEventElapsedTime(&t, l->ev0, l->ev1));
+  *ms = t / reps;
+  return CRIMSON_OK;
+}

@@ -1,0 +1,185 @@
+/*
+ * crimson_amd.h -- C ABI of the MI355X-native formation and memLS solve path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success or
+ * one of the CRIMSON_E* codes; crimson_last_error() returns a text for the
+ * last failure on the calling thread.  Host-pointer entry points copy their
+ * inputs to the GPU and their outputs back, and are synchronous at return.
+ * *_dev entry points take device pointers and return once their work is
+ * complete on the handle's stream.
+ *
+ * Index base: like the reference, ien/colm/rowp/gNodes are 1-based.
+ * Layouts: Fortran column-major, y(nshg,4) columns (u1,u2,u3,p);
+ * lhsK(9,nnz) and lhsP(4,nnz) as fillsparseI writes them; Val(dof*dof,nnz)
+ * row-major blocks as memLS reads them; Ri(dof,nNo).
+ */
+#ifndef CRIMSON_AMD_H
+#define CRIMSON_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CRIMSON_OK = 0,
+  CRIMSON_EARG = 1,         /* invalid argument or shape */
+  CRIMSON_EUNSUPPORTED = 2, /* flag or BC combination outside the supported set */
+  CRIMSON_EGRAPH = 3,       /* element pair missing from the CSR graph, or graph not symmetric */
+  CRIMSON_ENODIAG = 4,      /* a CSR row without its diagonal block */
+  CRIMSON_EHIP = 5,         /* HIP runtime error (no device, launch failure) */
+  CRIMSON_ENOMEM = 6,       /* device allocation failed */
+  CRIMSON_ESINGULAR = 7,    /* memLS GE pivot below 2*eps ('Singular matrix') */
+  CRIMSON_ESTATE = 8        /* handle used out of order */
+};
+
+/* Copies the text of the calling thread's last error into buf (len bytes). */
+int crimson_last_error(char *buf, int len);
+/* Number of HIP devices visible (0 without a GPU); never fails. */
+int crimson_device_count(void);
+
+/* ------------------------------------------------------------------ */
+/* Element formation: interior part of ElmGMR.                         */
+
+typedef struct {
+  double rho;     /* datmat(1,1,1) */
+  double mu;      /* datmat(1,2,1), Newtonian */
+  double Delt;    /* time step */
+  double alfi;    /* alpha_f */
+  double almi;    /* alpha_m */
+  double gami;    /* gamma */
+  double dtsfct;  /* Time Constant for tau (1) */
+  double taucfct; /* C factor for tauC (1) */
+  double flmpl;   /* LHS lumped-mass fraction: only 0 supported */
+  double flmpr;   /* RHS lumped-mass fraction: only 0 supported */
+  int itau;       /* only 0 */
+  int iconvflow;  /* only 2 (advective form) */
+  int idiff;      /* only 1 (global flux reconstruction, row-sum lumping) */
+  int ipord;      /* only 1 (P1 tets, 4-point rule) */
+  int precision;  /* 64: FP64 formation; 32: FP32 formation with FP32
+                     accumulation (FP64 kept for tau on elements whose
+                     aspect ratio exceeds aspect_fp64) */
+  double aspect_fp64; /* aspect-ratio threshold of the FP64 tau island */
+} crimson_form_params;
+
+typedef struct crimson_form crimson_form;
+
+/* Replaces the setup ElmGMR relies on: the element blocks mien(iblk)%p
+ * concatenated (elmgmr.f90:134-148), the CSR graph from genadj
+ * (genadj.f90:1-84) and the BC codes bc3lhs reads (bc3lhs.f90:44-49).
+ * ien(nel,4) 1-based column-major; colm(nshg+1)/rowp(nnz) 1-based;
+ * iBC(nshg) may be NULL.  Builds the element->nonzero maps once. */
+int crimson_form_setup(int nshg, int numnp, int nel, const int *ien,
+                       const int *colm, const int *rowp, int nnz,
+                       const int *iBC, const crimson_form_params *prm,
+                       crimson_form **out);
+
+/* ElmGMR interior (elmgmr.f90:82-255): AsIq + qpbc, AsIGMR/e3 with the
+ * residual scatter, bc3lhs, fillsparseI.  Host arrays.  res(nshg,4) is
+ * overwritten; lhsK(9,nnz) and lhsP(4,nnz) are overwritten when lhs == 1
+ * (the reference zeroes them first, elmgmr.f90:127-130). qres(nshg,9) is an
+ * optional output of the reconstructed diffusive flux (may be NULL). */
+int crimson_form_interior(crimson_form *f, const double *y, const double *ac,
+                          const double *x, double *res, double *lhsK,
+                          double *lhsP, double *qres, int lhs);
+
+/* Same on device pointers; inputs and outputs are FP64 device arrays in the
+ * layouts above whatever the handle's formation precision. */
+int crimson_form_interior_dev(crimson_form *f, const double *d_y,
+                              const double *d_ac, const double *d_x,
+                              double *d_res, double *d_lhsK, double *d_lhsP,
+                              double *d_qres, int lhs);
+
+/* Device-resident fused form: formation writes the memLS matrix of `lhs`
+ * directly (the Val4 packing of solfar.f90:212-235, with the transposed
+ * -G blocks taken from the same element contributions instead of the
+ * transpose search) and the residual as d_Res4 (4,nshg), the transposed
+ * copy SolFlow passes to memLS_SOLVE (solfar.f90:206-210). */
+struct crimson_lhs;
+int crimson_form_interior_to_lhs(crimson_form *f, const double *d_y,
+                                 const double *d_ac, const double *d_x,
+                                 double *d_Res4, struct crimson_lhs *lhs);
+
+/* bc3Res for 0- and 3-component velocity codes plus the pressure bit
+ * (bc3res.f90:34-80) on a device residual: res(nshg,4) when res4_layout is
+ * 0, Res4(4,nshg) when it is 1.  A no-op for a handle set up without iBC. */
+int crimson_form_bc3res_dev(crimson_form *f, double *d_res, int res4_layout);
+
+/* Wall time (ms) of the last formation's kernels, measured with HIP events
+ * on the handle's stream: out[0] flux pass, out[1] element kernel,
+ * out[2] assembly, out[3] total. */
+int crimson_form_timing(const crimson_form *f, double *out4);
+int crimson_form_free(crimson_form *f);
+
+/* Native genadj (genadj.f90:1-84): node-adjacency CSR including self, rows
+ * ascending.  colm must hold nshg+1 ints; rowp must hold maxnnz ints.
+ * Returns CRIMSON_EARG if maxnnz is too small (nnz_out holds the need). */
+int crimson_genadj(int nshg, int nel, const int *ien, int *colm, int *rowp,
+                   int maxnnz, int *nnz_out);
+
+/* ------------------------------------------------------------------ */
+/* memLS (memLS.f90).                                                   */
+
+enum { CRIMSON_LS_CG = 1, CRIMSON_LS_GMRES = 2, CRIMSON_LS_NS = 3 };
+enum { CRIMSON_BC_DIR = 0, CRIMSON_BC_NEU = 1 };
+
+typedef struct crimson_lhs crimson_lhs;
+typedef struct crimson_ls crimson_ls;
+
+/* memLS_LS_CREATE (memLS.f90:1379-1442).  `present` marks the optional
+ * arguments given: 1 relTol, 2 absTol, 4 maxItr, 8 dimKry, 16 relTolIn,
+ * 32 absTolIn, 64 maxItrIn. */
+int crimson_ls_create(int lsType, int present, double relTol, double absTol,
+                      int maxItr, int dimKry, const double *relTolIn,
+                      const double *absTolIn, const int *maxItrIn,
+                      crimson_ls **out);
+int crimson_ls_free(crimson_ls *ls);
+
+/* memLS_LHS_CREATE (memLS.f90:1080-1136), single task: gnNo == nNo,
+ * gNodes is the identity (may be NULL), rowPtr(nNo+1) and colPtr(nnz)
+ * 1-based (itrdrv.f90:934 passes rowPtr=colm, colPtr=rowp). */
+int crimson_lhs_create(int gnNo, int nNo, int nnz, const int *gNodes,
+                       const int *rowPtr, const int *colPtr, int nFaces,
+                       crimson_lhs **out);
+int crimson_lhs_free(crimson_lhs *lhs);
+
+/* memLS_BC_CREATE (memLS.f90:272-316); val(dof,nNo) may be NULL (zeros). */
+int crimson_bc_create(crimson_lhs *lhs, int faIn, int nNo, int dof,
+                      int bcType, const int *gNodes, const double *val);
+
+/* memLS_SOLVE (memLS.f90:1947-2020) on host arrays: Ri(dof,nNo) in: the
+ * right-hand side, out: the solution; Val(dof*dof,nnz) is read (its
+ * contents on return are those passed in: the scaling is applied on the
+ * GPU copy).  incL(nFaces) and res(nFaces) may be NULL. dof must be 4 for
+ * LS_TYPE_NS; 1..4 for GMRES and CG. */
+int crimson_ls_solve(crimson_lhs *lhs, crimson_ls *ls, int dof, double *Ri,
+                     const double *Val, const int *incL, const double *res);
+
+/* memLS_SOLVE on device data: the matrix is the one already held by `lhs`
+ * (from crimson_form_interior_to_lhs or crimson_lhs_set_val), d_Ri(dof,nNo)
+ * device in/out. */
+int crimson_ls_solve_dev(crimson_lhs *lhs, crimson_ls *ls, int dof,
+                         double *d_Ri, const int *incL, const double *res);
+
+/* Uploads Val(dof*dof,nnz) (host, row-major blocks) into the lhs. */
+int crimson_lhs_set_val(crimson_lhs *lhs, int dof, const double *Val);
+
+/* Downloads the held matrix as Val(dof*dof,nnz) row-major blocks (the
+ * layout solfar.f90:212-235 packs), e.g. after crimson_form_interior_to_lhs. */
+int crimson_lhs_get_val(crimson_lhs *lhs, int dof, double *Val);
+
+/* Stats of the last solve: sub 0 = RI, 1 = GM, 2 = CG; out[0] iNorm,
+ * out[1] fNorm, out[2] itr, out[3] suc, out[4] dB, out[5] callD (s),
+ * out[6] Resm, out[7] Resc. */
+int crimson_ls_stats(const crimson_ls *ls, int sub, double *out8);
+
+/* Measurement hooks for the Krylov kernels on the lhs's stream: one SpMV of
+ * the scaled 3x3 momentum block (dof 3) or the full 4x4 matrix (dof 4) on
+ * device vectors d_x -> d_y (dof,nNo), repeated `reps` times; returns the
+ * average kernel time in ms in *ms (HIP events around the launches). */
+int crimson_lhs_spmv_dev(crimson_lhs *lhs, int dof, const double *d_x,
+                         double *d_y, int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

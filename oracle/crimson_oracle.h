@@ -1,0 +1,136 @@
+/*
+ * crimson_oracle.h -- CPU restatement of CRIMSON Flowsolver's stabilized-FEM
+ * formation and memLS linear-solve path (FP64), used ONLY as test
+ * infrastructure: the checker in tests/, smoke() and the CPU-baseline leg of
+ * bench.py.  Nothing in the product library links or calls this code.
+ *
+ * Every function cites the reference file:line it restates.  Array layouts
+ * follow the reference exactly: Fortran column-major, 1-based node numbers in
+ * ien/colm/rowp/gNodes, node-major fields y(nshg,ndof).
+ *
+ * Supported flag set (rejected otherwise by the callers of this oracle):
+ *   P1 tets (nshl=4), 4-point rule, itau=0, iconvflow=2, idiff=1, ipord=1,
+ *   Newtonian (nnwType=0), no ALE (uMesh=0), flmpl=flmpr=0, no body force,
+ *   no periodic nodes (iBC bit 10), velocity BCs with 0 or 3 components.
+ */
+#ifndef CRIMSON_ORACLE_H
+#define CRIMSON_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  double rho;     /* datmat(1,1,1) */
+  double mu;      /* datmat(1,2,1) */
+  double Delt;    /* Delt(itseq) */
+  double alfi;    /* alpha_f */
+  double almi;    /* alpha_m */
+  double gami;    /* gamma */
+  double dtsfct;  /* Time Constant (1) */
+  double taucfct; /* C factor (1) */
+  double flmpl;   /* LHS lumped mass fraction (0) */
+  double flmpr;   /* RHS lumped mass fraction (0) */
+  int ibksiz;     /* element block size (255) */
+  int lhs;        /* 1: form the tangent */
+} cfo_params;
+
+/* error codes (0 = ok) */
+enum {
+  CFO_OK = 0,
+  CFO_EARG = 1,        /* bad argument */
+  CFO_EUNSUPPORTED = 2,/* flag/BC combination outside the supported set */
+  CFO_ESEARCH = 3,     /* sparseloc target missing from the row */
+  CFO_ENNZ = 4,        /* nnz capacity exceeded */
+  CFO_ESINGULAR = 5,   /* GE pivot below 2 eps (memLS 'Singular matrix') */
+  CFO_ENODIAG = 6      /* a CSR row without its diagonal block */
+};
+
+/* generalized-alpha constants from rho_inf, itrPC.f90:16-37 */
+void cfo_time_params(double rhoinf, double *almi, double *alfi, double *gami);
+
+/* e3 over one element block (e3.f90:1-176 with getshp, getdiff, e3ivar,
+ * e3metric, e3stab/e3gijd, e3Res, e3resStrongPDE_ALE, e3LHS).
+ * yl(npro,4,4) in (p,u1,u2,u3) order, acl(npro,4,4), xl(npro,4,3),
+ * ql(npro,4,9).  rl(npro,4,4), xKebe(npro,9,4,4), xGoC(npro,4,4,4) are
+ * ACCUMULATED into (the caller zeroes them, asigmr.f90:87-96). */
+void cfo_e3_block(int npro, const double *yl, const double *acl,
+                  const double *xl, const double *ql, double *rl,
+                  double *xKebe, double *xGoC, const cfo_params *p);
+
+/* e3q over one block (e3q.f90:1-211, e3qvar.f90:1-148): ql(npro,4,9) and
+ * rmassl(npro,4) accumulated. */
+void cfo_e3q_block(int npro, const double *yl, const double *xl, double *ql,
+                   double *rmassl, const cfo_params *p);
+
+/* Interior part of ElmGMR (elmgmr.f90:82-255): AsIq block loop, qpbc,
+ * zeroing, AsIGMR/e3 block loop with residual scatter, bc3lhs (no-op for the
+ * supported BC codes) and fillsparseI.  ien(nel,4) 1-based; y,ac (nshg,4);
+ * x (numnp,3); colm (nshg+1), rowp (nnz) 1-based.  Outputs res (nshg,4),
+ * lhsK (9,nnz), lhsP (4,nnz) (lhs==1 only), qres (nshg,9) optional. */
+int cfo_elmgmr_interior(int nshg, int numnp, int nel, const int *ien,
+                        const double *y, const double *ac, const double *x,
+                        const int *iBC, const int *colm, const int *rowp,
+                        int nnz, double *res, double *lhsK, double *lhsP,
+                        double *qres, const cfo_params *p);
+
+/* Same with the diffusive flux q supplied (skips AsIq/qpbc); used to
+ * differentiate the element residual with q frozen. */
+int cfo_elmgmr_interior_q(int nshg, int numnp, int nel, const int *ien,
+                          const double *y, const double *ac, const double *x,
+                          const double *q, const int *iBC, const int *colm,
+                          const int *rowp, int nnz, double *res, double *lhsK,
+                          double *lhsP, const cfo_params *p);
+
+/* bc3Res without periodicity (bc3res.f90:1-80). BC(nshg,ndofBC). */
+int cfo_bc3res(int nshg, const int *iBC, const double *BC, int ndofBC,
+               double *res);
+
+/* genadj + Asadj (genadj.f90:1-84, asadj.f90:1-60): colm(nshg+1), rowp. */
+int cfo_genadj(int nshg, int nel, const int *ien, int *colm, int *rowp,
+               int maxnnz, int *nnz_out);
+
+/* Val4/Res4 packing of SolFlow (solfar.f90:206-235). */
+void cfo_pack_val4(int nshg, const int *colm, const int *rowp,
+                   const double *lhsK, const double *lhsP, double *Val4);
+
+/* ---------------- memLS (memLS.f90) ---------------- */
+enum { CFO_LS_CG = 1, CFO_LS_GMRES = 2, CFO_LS_NS = 3 };
+enum { CFO_BC_DIR = 0, CFO_BC_NEU = 1 };
+
+typedef struct cfo_lhs cfo_lhs;
+typedef struct cfo_ls cfo_ls;
+
+/* memLS_LHS_CREATE, single task (memLS.f90:1080-1136). rowPtr(nNo+1) and
+ * colPtr(nnz) 1-based, as passed from itrdrv.f90:934 (rowPtr=colm,
+ * colPtr=rowp). */
+int cfo_lhs_create(int nNo, int nnz, const int *rowPtr, const int *colPtr,
+                   int nFaces, cfo_lhs **out);
+void cfo_lhs_free(cfo_lhs *lhs);
+/* memLS_BC_CREATE (memLS.f90:272-342); val may be NULL (=> zeros). */
+int cfo_bc_create(cfo_lhs *lhs, int faIn, int nNo, int dof, int bcType,
+                  const int *gNodes, const double *val);
+/* memLS_LS_CREATE (memLS.f90:1379-1442).  'present' bit mask:
+ * 1 relTol, 2 absTol, 4 maxItr, 8 dimKry, 16 relTolIn, 32 absTolIn,
+ * 64 maxItrIn. */
+int cfo_ls_create(int lsType, int present, double relTol, double absTol,
+                  int maxItr, int dimKry, const double *relTolIn,
+                  const double *absTolIn, const int *maxItrIn, cfo_ls **out);
+void cfo_ls_free(cfo_ls *ls);
+/* memLS_SOLVE (memLS.f90:1947-2020). Ri(dof,nNo) in/out, Val(dof*dof,nnz)
+ * in/out (rescaled in place as the reference does).  incL/res may be NULL. */
+int cfo_ls_solve(cfo_lhs *lhs, cfo_ls *ls, int dof, double *Ri, double *Val,
+                 const int *incL, const double *res);
+/* stats: for sub = 0 (RI), 1 (GM), 2 (CG): out[0]=iNorm out[1]=fNorm
+ * out[2]=itr out[3]=suc out[4]=dB out[5]=callD; plus Resm/Resc in out[6..7] */
+void cfo_ls_stats(const cfo_ls *ls, int sub, double *out);
+
+/* Flop count of one element formation in the reference formulation
+ * (flux pass + e3 with lhs=1 + scatter adds), counted stage by stage from
+ * the restatement above (+,-,*,/,sqrt each 1). */
+double cfo_flops_per_element(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

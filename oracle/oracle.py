@@ -1,0 +1,239 @@
+"""ctypes bindings of the CPU restatement (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, always as the checker or the timed CPU
+baseline, never as part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+ORACLE_HEADER_NOTE = "CPU restatement of the reference path; test infrastructure only"
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB
+
+
+class Params(C.Structure):
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("Delt", C.c_double),
+                ("alfi", C.c_double), ("almi", C.c_double), ("gami", C.c_double),
+                ("dtsfct", C.c_double), ("taucfct", C.c_double),
+                ("flmpl", C.c_double), ("flmpr", C.c_double),
+                ("ibksiz", C.c_int), ("lhs", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB)
+        dp = np.ctypeslib.ndpointer(np.float64, flags="F_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="F_CONTIGUOUS")
+        L.cfo_elmgmr_interior.argtypes = [C.c_int, C.c_int, C.c_int, ip, dp, dp, dp,
+                                          C.c_void_p, ip, ip, C.c_int, dp, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.POINTER(Params)]
+        L.cfo_elmgmr_interior_q.argtypes = [C.c_int, C.c_int, C.c_int, ip, dp, dp, dp, dp,
+                                            C.c_void_p, ip, ip, C.c_int, dp, C.c_void_p,
+                                            C.c_void_p, C.POINTER(Params)]
+        L.cfo_e3_block.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp, C.POINTER(Params)]
+        L.cfo_e3_block.restype = None
+        L.cfo_e3q_block.argtypes = [C.c_int, dp, dp, dp, dp, C.POINTER(Params)]
+        L.cfo_e3q_block.restype = None
+        L.cfo_bc3res.argtypes = [C.c_int, ip, C.c_void_p, C.c_int, dp]
+        L.cfo_genadj.argtypes = [C.c_int, C.c_int, ip, ip, ip, C.c_int, C.POINTER(C.c_int)]
+        L.cfo_pack_val4.argtypes = [C.c_int, ip, ip, dp, dp, dp]
+        L.cfo_pack_val4.restype = None
+        L.cfo_lhs_create.argtypes = [C.c_int, C.c_int, ip, ip, C.c_int, C.POINTER(C.c_void_p)]
+        L.cfo_lhs_free.argtypes = [C.c_void_p]
+        L.cfo_lhs_free.restype = None
+        L.cfo_bc_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_void_p]
+        L.cfo_ls_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
+        L.cfo_ls_free.argtypes = [C.c_void_p]
+        L.cfo_ls_free.restype = None
+        L.cfo_ls_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p]
+        L.cfo_ls_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        L.cfo_ls_stats.restype = None
+        L.cfo_time_params.argtypes = [C.c_double] + [C.POINTER(C.c_double)] * 3
+        L.cfo_time_params.restype = None
+        L.cfo_flops_per_element.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def time_params(rhoinf: float):
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    lib().cfo_time_params(rhoinf, C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value  # almi, alfi, gami
+
+
+def make_params(rho=1.06, mu=0.04, delt=1e-3, rhoinf=0.5, lhs=1, ibksiz=255):
+    almi, alfi, gami = time_params(rhoinf)
+    return Params(rho, mu, delt, alfi, almi, gami, 1.0, 1.0, 0.0, 0.0, ibksiz, lhs)
+
+
+def _f(a, dt=np.float64):
+    return np.asfortranarray(a, dtype=dt)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with code {rc}")
+
+
+def genadj(ien, nshg):
+    """colm(nshg+1), rowp(nnz), 1-based (genadj.f90)."""
+    ien = _f(ien, np.int32)
+    nel = ien.shape[0]
+    cap = 32 * nshg
+    colm = np.zeros(nshg + 1, np.int32)
+    rowp = np.zeros(cap, np.int32)
+    nnz = C.c_int()
+    _check(lib().cfo_genadj(nshg, nel, ien, colm, rowp, cap, C.byref(nnz)), "genadj")
+    return colm, rowp[: nnz.value].copy()
+
+
+def elmgmr_interior(ien, y, ac, x, colm, rowp, params, ibc=None, want_q=False, q=None):
+    """Interior ElmGMR: returns res(nshg,4), lhsK(9,nnz), lhsP(4,nnz)[, qres]."""
+    ien = _f(ien, np.int32)
+    y, ac, x = _f(y), _f(ac), _f(x)
+    nshg = y.shape[0]
+    nnz = rowp.shape[0]
+    res = np.zeros((nshg, 4), order="F")
+    lhsK = np.zeros((9, nnz), order="F") if params.lhs == 1 else None
+    lhsP = np.zeros((4, nnz), order="F") if params.lhs == 1 else None
+    ibc_a = None if ibc is None else _f(ibc, np.int32)
+    if q is not None:
+        rc = lib().cfo_elmgmr_interior_q(nshg, x.shape[0], ien.shape[0], ien, y, ac, x, _f(q),
+                                         _ptr(ibc_a), _f(colm, np.int32), _f(rowp, np.int32), nnz,
+                                         res, _ptr(lhsK), _ptr(lhsP), C.byref(params))
+        _check(rc, "elmgmr_interior_q")
+        return res, lhsK, lhsP
+    qres = np.zeros((nshg, 9), order="F") if want_q else None
+    rc = lib().cfo_elmgmr_interior(nshg, x.shape[0], ien.shape[0], ien, y, ac, x, _ptr(ibc_a),
+                                   _f(colm, np.int32), _f(rowp, np.int32), nnz, res,
+                                   _ptr(lhsK), _ptr(lhsP), _ptr(qres), C.byref(params))
+    _check(rc, "elmgmr_interior")
+    return (res, lhsK, lhsP, qres) if want_q else (res, lhsK, lhsP)
+
+
+def e3_block(yl, acl, xl, ql, params):
+    """One e3 call on SoA block arrays (npro,4,4)... returns rl, xKebe, xGoC."""
+    n = yl.shape[0]
+    rl = np.zeros((n, 4, 4), order="F")
+    K = np.zeros((n, 9, 4, 4), order="F")
+    G = np.zeros((n, 4, 4, 4), order="F")
+    lib().cfo_e3_block(n, _f(yl), _f(acl), _f(xl), _f(ql), rl, K, G, C.byref(params))
+    return rl, K, G
+
+
+def e3q_block(yl, xl, params):
+    n = yl.shape[0]
+    ql = np.zeros((n, 4, 9), order="F")
+    rm = np.zeros((n, 4), order="F")
+    lib().cfo_e3q_block(n, _f(yl), _f(xl), ql, rm, C.byref(params))
+    return ql, rm
+
+
+def bc3res(ibc, res, bc=None):
+    res = _f(res).copy(order="F")
+    bca = None if bc is None else _f(bc)
+    _check(lib().cfo_bc3res(res.shape[0], _f(ibc, np.int32), _ptr(bca),
+                            0 if bc is None else bc.shape[1], res), "bc3res")
+    return res
+
+
+def pack_val4(colm, rowp, lhsK, lhsP):
+    nshg = colm.shape[0] - 1
+    val = np.zeros((16, rowp.shape[0]), order="F")
+    lib().cfo_pack_val4(nshg, _f(colm, np.int32), _f(rowp, np.int32), _f(lhsK), _f(lhsP), val)
+    return val
+
+
+LS_TYPE_CG, LS_TYPE_GMRES, LS_TYPE_NS = 1, 2, 3
+BC_TYPE_Dir, BC_TYPE_Neu = 0, 1
+
+
+class MemLS:
+    """memLS_LHS_CREATE / BC_CREATE / LS_CREATE / SOLVE on the CPU."""
+
+    def __init__(self, colm, rowp, nfaces, ls_type=LS_TYPE_NS, relTol=None, absTol=None,
+                 maxItr=None, dimKry=None, relTolIn=None, absTolIn=None, maxItrIn=None):
+        L = lib()
+        self.nNo = colm.shape[0] - 1
+        self.nnz = rowp.shape[0]
+        self._keep = []
+        h = C.c_void_p()
+        _check(L.cfo_lhs_create(self.nNo, self.nnz, _f(colm, np.int32), _f(rowp, np.int32), nfaces,
+                                C.byref(h)), "lhs_create")
+        self.lhs = h
+        present = 0
+        vals = dict(relTol=0.0, absTol=0.0, maxItr=0, dimKry=0)
+        for bit, name, v in ((1, "relTol", relTol), (2, "absTol", absTol), (4, "maxItr", maxItr),
+                             (8, "dimKry", dimKry)):
+            if v is not None:
+                present |= bit
+                vals[name] = v
+        arrs = []
+        for bit, v, dt in ((16, relTolIn, np.float64), (32, absTolIn, np.float64), (64, maxItrIn, np.int32)):
+            if v is not None:
+                present |= bit
+                a = np.asarray(v, dtype=dt)
+                arrs.append(a)
+            else:
+                arrs.append(None)
+        self._keep.extend(arrs)
+        ls = C.c_void_p()
+        _check(L.cfo_ls_create(ls_type, present, vals["relTol"], vals["absTol"], vals["maxItr"],
+                               vals["dimKry"], _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]),
+                               C.byref(ls)), "ls_create")
+        self.ls = ls
+
+    def bc_create(self, faIn, gNodes, dof, bc_type, val=None):
+        g = _f(gNodes, np.int32)
+        v = None if val is None else _f(val)
+        self._keep.append(v)
+        _check(lib().cfo_bc_create(self.lhs, faIn, g.shape[0], dof, bc_type, g, _ptr(v)), "bc_create")
+
+    def solve(self, Ri, Val, incL=None, res=None):
+        """Returns the solution (dof,nNo); Ri and Val are copied first."""
+        Ri = _f(Ri).copy(order="F")
+        Val = _f(Val).copy(order="F")
+        il = None if incL is None else _f(incL, np.int32)
+        rs = None if res is None else _f(res)
+        _check(lib().cfo_ls_solve(self.lhs, self.ls, Ri.shape[0], Ri, Val, _ptr(il), _ptr(rs)), "ls_solve")
+        return Ri
+
+    def stats(self, sub=0):
+        out = (C.c_double * 8)()
+        lib().cfo_ls_stats(self.ls, sub, out)
+        keys = ("iNorm", "fNorm", "itr", "suc", "dB", "callD", "Resm", "Resc")
+        return dict(zip(keys, list(out)))
+
+    def __del__(self):
+        try:
+            lib().cfo_lhs_free(self.lhs)
+            lib().cfo_ls_free(self.ls)
+        except Exception:
+            pass
+
+
+def flops_per_element():
+    return lib().cfo_flops_per_element()
