@@ -1,0 +1,242 @@
+"""Benchmark of the stabilized-FEM formation + memLS path on one MI355X.
+
+One step = one FP64 interior formation (flux reconstruction, e3 over all
+elements, assembly straight into the memLS matrix and Res4) on the cylinder
+built from a 64x64x320 hex grid clipped to r <= 1 cm and split into Kuhn tets,
+with inputs resident in HBM.  value = elements formed per second (Melem/s).
+
+Also reported on the same mesh: the Krylov SpMV bandwidth of the scaled 3x3
+momentum block and of the full 4x4 matrix, and the wall time of FP64
+timesteps (3 nonlinear iterations of formation + NS solve + update each,
+Kspace 100, memLS NS tolerance 1e-8, resistance outlet 1333 dyn s/cm^5).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Melem/s formed + Krylov SpMV GB/s; mixed vs FP64 GPU timestep time at 1e-8"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_problem(grid, cfa, M):
+    import numpy as np
+    t0 = time.time()
+    m = M.cylinder_mesh(*grid)
+    colm, rowp = cfa.genadj(m.ien, m.numnp)
+    y, ac = M.poiseuille_state(m)
+    ibc, bc = M.dirichlet_codes(m)
+    log(f"mesh {grid}: nel={m.nel} nshg={m.numnp} nnz={rowp.size} ({time.time() - t0:.1f}s)")
+    return m, colm, rowp, y, ac, ibc, bc
+
+
+def inflow_mean(t, seed=7, period=1.0):
+    """Mean inflow velocity: 10 cm/s plus 3 seeded harmonics (+/- 30 %)."""
+    from crimsonflowsolver_amd.mesh import uniform_pm1
+    r = uniform_pm1(seed, 6)
+    u = 10.0
+    for k in range(3):
+        u += 3.0 * 0.5 * (1 + r[k]) / (k + 1) * math.sin(2 * math.pi * (k + 1) * t / period + math.pi * r[3 + k])
+    return u
+
+
+def cpu_baseline(sample_grid, target_s=15.0):
+    """Oracle (CPU restatement, one core) formation on a bounded slab of the
+    same cross-section; returns Melem/s."""
+    from crimsonflowsolver_amd import mesh as M
+    from oracle import oracle as O
+    m = M.cylinder_mesh(*sample_grid)
+    colm, rowp = O.genadj(m.ien, m.numnp)
+    y, ac = M.poiseuille_state(m)
+    p = O.make_params()
+    O.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, p)  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        O.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, p)
+        n += 1
+        if time.perf_counter() - t0 >= target_s:
+            break
+    dt = time.perf_counter() - t0
+    return m.nel * n / dt / 1e6, m.nel, n, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", default="64,64,320")
+    ap.add_argument("--timesteps", type=int, default=1, help="FP64 timesteps to time (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0: skip)")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the formation/solve path here has no multi-GPU form "
+              "(one GPU holds the whole mesh; see DESIGN.md for how it would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import crimsonflowsolver_amd as cfa
+    from crimsonflowsolver_amd import mesh as M
+
+    if cfa.device_count() < 1 or not torch.cuda.is_available():
+        raise SystemExit("bench.py: no GPU visible")
+    dev = torch.device("cuda:0")
+    grid = tuple(int(v) for v in args.grid.split(","))
+    m, colm, rowp, y, ac, ibc, bc = build_problem(grid, cfa, M)
+    N, nel, nnz = m.numnp, m.nel, rowp.size
+    prm = cfa.form_params()
+    t0 = time.time()
+    form = cfa.ElmGMRInterior(m.ien, colm, rowp, N, iBC=ibc, params=prm)
+    log(f"formation setup {time.time() - t0:.1f}s")
+
+    # faces: 1 = Dirichlet (inlet + wall, all velocity components),
+    # 2 = resistance outlet (Neumann, val = NABI)
+    fixed = np.nonzero(ibc)[0]
+    out = np.nonzero(m.outlet & (ibc == 0))[0]
+    nabi = m.nabi()
+    lhs = cfa.memLS_LHS_CREATE(N, N, nnz, None, colm, rowp, 2)
+    cfa.memLS_BC_CREATE(lhs, 1, fixed.size, 3, cfa.BC_TYPE_Dir, fixed + 1, np.zeros((3, fixed.size), order="F"))
+    cfa.memLS_BC_CREATE(lhs, 2, out.size, 3, cfa.BC_TYPE_Neu, out + 1, np.asfortranarray(nabi[out].T))
+
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)  # (n,k) Fortran -> (k,n) C
+    dy, dac, dx = T(y), T(ac), T(m.x)
+    dres4 = torch.zeros((N, 4), dtype=torch.float64, device=dev)       # Res4(4,N) Fortran
+
+    # ---- formation throughput
+    for _ in range(args.warmup):
+        form.to_lhs(dy, dac, dx, dres4, lhs)
+    torch.cuda.synchronize()
+    tm = np.zeros(4)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        form.to_lhs(dy, dac, dx, dres4, lhs)
+        tt = form.timing()
+        tm += [tt["flux_ms"], tt["e3_ms"], tt["assembly_ms"], tt["total_ms"]]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    tm /= args.steps
+    ms_step = elapsed / args.steps * 1e3
+    melems = nel / (ms_step * 1e-3) / 1e6
+    log(f"formation: {ms_step:.3f} ms/step wall, events flux {tm[0]:.3f} e3 {tm[1]:.3f} asm {tm[2]:.3f} "
+        f"total {tm[3]:.3f} ms -> {melems:.1f} Melem/s")
+
+    # ---- Krylov SpMV (scaled, as inside the solver)
+    x3 = torch.rand((N, 3), dtype=torch.float64, device=dev)
+    y3 = torch.empty_like(x3)
+    x4 = torch.rand((N, 4), dtype=torch.float64, device=dev)
+    y4 = torch.empty_like(x4)
+    ms3 = lhs.spmv_dev(3, x3, y3, args.spmv_reps)
+    ms4 = lhs.spmv_dev(4, x4, y4, args.spmv_reps)
+    b3 = nnz * (72 + 4) + N * (8 + 24 + 24)
+    b4 = nnz * (128 + 4) + N * (8 + 32 + 32)
+    gbs3, gbs4 = b3 / (ms3 * 1e-3) / 1e9, b4 / (ms4 * 1e-3) / 1e9
+    log(f"spmv 3x3: {ms3:.4f} ms {gbs3:.0f} GB/s ; 4x4: {ms4:.4f} ms {gbs4:.0f} GB/s")
+
+    # ---- FP64 timesteps (10 x 3 nonlinear iterations in the workload; timed here: args.timesteps)
+    ts_ms = None
+    ts_info = {}
+    if args.timesteps > 0:
+        ls = cfa.memLS_LS_CREATE(cfa.LS_TYPE_NS, relTol=1e-8, dimKry=100, maxItr=10,
+                                 relTolIn=[0.01, 0.2], maxItrIn=[1, 200])
+        R = 1333.0
+        faceres = np.array([0.0, R * prm.alfi * prm.gami * prm.Delt])
+        dibc = torch.from_numpy(ibc.astype(np.int32)).to(dev)
+        BCh = np.zeros((N, 7), order="F")
+        rr = (m.x[:, 0] ** 2 + m.x[:, 1] ** 2)
+        inl = m.inlet & ~m.wall
+        dBC = T(BCh)  # (7,N) C == BC(N,7) Fortran
+        dglob = torch.from_numpy(out.astype(np.int32)).to(dev)
+        dnabi = torch.from_numpy(np.ascontiguousarray(nabi[out])).to(dev)   # (nf,3) C == (3,nf) F
+        dq = torch.zeros(1, dtype=torch.float64, device=dev)
+        yold, acold = dy.clone(), dac.clone()
+        ya, aca = torch.empty_like(dy), torch.empty_like(dac)
+        step_times, its = [], []
+        for n in range(args.timesteps):
+            t = (n + 1) * prm.Delt
+            prof = np.zeros(N)
+            prof[inl] = 2.0 * inflow_mean(t) * np.clip(1.0 - rr[inl], 0.0, None)
+            dBC[4] = torch.from_numpy(prof).to(dev)   # BC(:,5) = u_z
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            form.predict(yold, acold, dy, dac)
+            form.itr_bc(dibc, dBC, dy)
+            for it in range(3):
+                form.yalpha(yold, acold, dy, dac, ya, aca)
+                form.sync()
+                form.to_lhs(ya, aca, dx, dres4, lhs)
+                form.bc3res(dres4, res4_layout=True)
+                form.elmpvsq(dglob, dnabi, R, ya, dres4, dq)
+                form.sync()
+                cfa.memLS_SOLVE_dev(lhs, ls, 4, dres4, res=faceres)
+                its.append((ls.RI["itr"], ls.GM["itr"], ls.CG["itr"], ls.RI["fNorm"] / ls.RI["iNorm"]))
+                form.correct(dy, dac, dres4)
+                form.itr_bc(dibc, dBC, dy)
+                form.sync()
+            yold.copy_(dy)
+            acold.copy_(dac)
+            torch.cuda.synchronize()
+            step_times.append(time.perf_counter() - t0)
+            log(f"timestep {n + 1}: {step_times[-1] * 1e3:.1f} ms, solver (RI,GM,CG itr, rel) {its[-3:]}")
+        ts_ms = float(np.mean(step_times) * 1e3)
+        ts_info = {"timesteps_timed": args.timesteps, "nonlinear_iterations": 3,
+                   "solver_iterations": [list(map(float, v)) for v in its]}
+
+    # ---- roofline of the dominant kernel (from the live HIP-event timings)
+    from oracle import oracle as O  # the oracle's hand count of the reference formulation
+    fpe = O.flops_per_element()
+    phases = {"flux": tm[0], "e3_elem": tm[1], "assembly": tm[2]}
+    dominant = max(phases, key=phases.get)
+    if dominant == "e3_elem":
+        ach = fpe * nel / (tm[1] * 1e-3) / 1e12
+        roof = {"kernel": "e3_elem", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+    else:
+        # assembly: record reads (48 doubles/element) + block writes (16 doubles/nnz) + Res4
+        byt = nel * 48 * 8 + nnz * 16 * 8 + N * 4 * 8 + nel * 6 * 4 + N * 4 * 4
+        ach = byt / (tm[2] * 1e-3) / 1e9
+        roof = {"kernel": "assembly", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        sample = (grid[0], grid[1], max(4, grid[2] // 16))
+        v, sn, nrep, dt = cpu_baseline(sample, args.cpu_seconds)
+        cpu = {"value": v, "unit": "Melem/s", "cores": 1, "kind": "port",
+               "sample": f"oracle interior formation (flux+e3+fillsparseI) on the {sample[0]}x{sample[1]}x{sample[2]} "
+                         f"slab ({sn} tets), {nrep} calls in {dt:.1f}s, one core"}
+        log(f"cpu baseline: {v:.3f} Melem/s")
+
+    line = {
+        "metric": METRIC, "value": melems, "unit": "Melem/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"FP64 interior formation (flux pass + e3 + assembly into the memLS matrix) on the "
+                               f"cylinder r=1 L=10 from a {grid[0]}x{grid[1]}x{grid[2]} hex grid clipped by hex-centroid "
+                               f"radius, 6 Kuhn tets per hex",
+                   "nel": nel, "nshg": N, "nnzb": nnz, "rho": 1.06, "mu": 0.04, "dt": 1e-3, "rho_inf": 0.5},
+        "phase_ms": {"flux": tm[0], "e3_elem": tm[1], "assembly": tm[2], "events_total": tm[3]},
+        "krylov_spmv_gbs": {"bsr3x3": gbs3, "bsr4x4": gbs4, "ms3": ms3, "ms4": ms4},
+        "timestep_ms_fp64": ts_ms, **ts_info,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,7 +38,8 @@ EXPORTS = (
     "crimson_form_timing", "crimson_form_free", "crimson_genadj", "crimson_ls_create", "crimson_ls_free",
     "crimson_lhs_create", "crimson_lhs_free", "crimson_bc_create", "crimson_ls_solve",
     "crimson_ls_solve_dev", "crimson_lhs_set_val", "crimson_lhs_get_val", "crimson_ls_stats",
-    "crimson_lhs_spmv_dev",
+    "crimson_lhs_spmv_dev", "crimson_itr_predict_dev", "crimson_itr_yalpha_dev", "crimson_itr_correct_dev",
+    "crimson_itr_bc_dev", "crimson_elmpvsq_dev", "crimson_form_sync",
 )
 
 
@@ -102,6 +103,12 @@ def lib():
         L.crimson_lhs_get_val.argtypes = [vp, C.c_int, vp]
         L.crimson_ls_stats.argtypes = [vp, C.c_int, dp]
         L.crimson_lhs_spmv_dev.argtypes = [vp, C.c_int, vp, vp, C.c_int, dp]
+        L.crimson_itr_predict_dev.argtypes = [vp, vp, vp, vp, vp, C.c_double]
+        L.crimson_itr_yalpha_dev.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double]
+        L.crimson_itr_correct_dev.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_double]
+        L.crimson_itr_bc_dev.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.crimson_elmpvsq_dev.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp, vp, vp]
+        L.crimson_form_sync.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -203,6 +210,30 @@ class ElmGMRInterior:
     def bc3res(self, res, res4_layout=False):
         check(lib().crimson_form_bc3res_dev(self.h, _p(res), 1 if res4_layout else 0),
               "crimson_form_bc3res_dev")
+
+    # steps either side of formation and solve (itrPC.f90, itrbc.f90, ElmpvsQ)
+    def predict(self, yold, acold, y, ac):
+        check(lib().crimson_itr_predict_dev(self.h, _p(yold), _p(acold), _p(y), _p(ac), self.params.gami),
+              "itrPredict")
+
+    def yalpha(self, yold, acold, y, ac, ya, aca):
+        check(lib().crimson_itr_yalpha_dev(self.h, _p(yold), _p(acold), _p(y), _p(ac), _p(ya), _p(aca),
+                                           self.params.alfi, self.params.almi), "itrYAlpha")
+
+    def correct(self, y, ac, res4):
+        p = self.params
+        check(lib().crimson_itr_correct_dev(self.h, _p(y), _p(ac), _p(res4), p.gami, p.alfi, p.Delt),
+              "itrCorrect")
+
+    def itr_bc(self, ibc_dev, bc_dev, y):
+        check(lib().crimson_itr_bc_dev(self.h, _p(ibc_dev), _p(bc_dev), int(bc_dev.shape[0]), _p(y)), "itrBC")
+
+    def elmpvsq(self, glob_dev, nabi_dev, R, y, res4, q_dev):
+        check(lib().crimson_elmpvsq_dev(self.h, int(glob_dev.numel()), _p(glob_dev), _p(nabi_dev), R, _p(y),
+                                        _p(res4), _p(q_dev)), "ElmpvsQ")
+
+    def sync(self):
+        check(lib().crimson_form_sync(self.h), "crimson_form_sync")
 
     def timing(self):
         out = (C.c_double * 4)()

@@ -88,6 +88,12 @@ inline unsigned grid_for(size_t n, unsigned threads) {
 
 }  // namespace crimson
 
+struct crimson_form;
+namespace crimson {
+hipStream_t form_stream(const crimson_form *f);
+int form_nshg(const crimson_form *f);
+}  // namespace crimson
+
 // memLS face (memLS_faceType, memLS.f90:83-100), single task.
 struct CrFace {
   int nNo = 0, dof = 0, bGrp = CRIMSON_BC_DIR;

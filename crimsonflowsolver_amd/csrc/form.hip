@@ -665,6 +665,11 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
 
 }  // namespace
 
+namespace crimson {
+hipStream_t form_stream(const crimson_form *f) { return f->stream; }
+int form_nshg(const crimson_form *f) { return f->nshg; }
+}  // namespace crimson
+
 extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, const int *colm,
                                   const int *rowp, int nnz, const int *iBC, const crimson_form_params *prm,
                                   crimson_form **out) {
@@ -936,8 +941,9 @@ extern "C" int crimson_genadj(int nshg, int nel, const int *ien, int *colm, int 
     crimson::set_error("crimson_genadj: invalid arguments");
     return CRIMSON_EARG;
   }
-  // bucket (i, j) pairs per row, then sort and deduplicate each row
-  std::vector<int> cnt(nshg + 1, 0);
+  // node -> element incidences, then per row the sorted unique union of the
+  // incident elements' nodes (what Asadj + genadj's sort produce)
+  std::vector<int> ptr(nshg + 1, 0);
   for (int a = 0; a < 4; ++a)
     for (int e = 0; e < nel; ++e) {
       const int n = ien[(size_t)a * nel + e];
@@ -945,40 +951,34 @@ extern "C" int crimson_genadj(int nshg, int nel, const int *ien, int *colm, int 
         crimson::set_error("ien(%d,%d)=%d out of range", e + 1, a + 1, n);
         return CRIMSON_EARG;
       }
-      cnt[n] += 4;
+      ptr[n]++;
     }
-  for (int i = 0; i < nshg; ++i) cnt[i + 1] += cnt[i];
-  std::vector<int> buf(cnt[nshg]);
-  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-  for (int e = 0; e < nel; ++e)
-    for (int a = 0; a < 4; ++a) {
-      const int i = ien[(size_t)a * nel + e] - 1;
-      for (int b = 0; b < 4; ++b) buf[fill[i]++] = ien[(size_t)b * nel + e];
-    }
+  for (int i = 0; i < nshg; ++i) ptr[i + 1] += ptr[i];
+  std::vector<int> inc((size_t)ptr[nshg]);
+  {
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < nel; ++e)
+      for (int a = 0; a < 4; ++a) inc[fill[ien[(size_t)a * nel + e] - 1]++] = e;
+  }
+  std::vector<int> buf;
   long long nnz = 0;
   colm[0] = 1;
-  std::vector<int> rowlen(nshg);
   for (int i = 0; i < nshg; ++i) {
-    auto b = buf.begin() + cnt[i], e = buf.begin() + cnt[i + 1];
-    std::sort(b, e);
-    auto u = std::unique(b, e);
-    rowlen[i] = (int)(u - b);
-    if (rowlen[i] == 0) rowlen[i] = 1;  // isolated node: keep its diagonal
-    nnz += rowlen[i];
+    buf.clear();
+    for (int p = ptr[i]; p < ptr[i + 1]; ++p)
+      for (int b = 0; b < 4; ++b) buf.push_back(ien[(size_t)b * nel + inc[p]]);
+    if (buf.empty()) buf.push_back(i + 1);  // isolated node keeps its diagonal
+    std::sort(buf.begin(), buf.end());
+    const size_t len = std::unique(buf.begin(), buf.end()) - buf.begin();
+    if (rowp && nnz + (long long)len <= maxnnz)
+      std::copy(buf.begin(), buf.begin() + len, rowp + nnz);
+    nnz += len;
+    colm[i + 1] = (int)(nnz + 1);
   }
   *nnz_out = (int)nnz;
   if (nnz > maxnnz || !rowp) {
     crimson::set_error("crimson_genadj: need %lld column entries, capacity %d", nnz, maxnnz);
     return CRIMSON_EARG;
-  }
-  int pos = 0;
-  for (int i = 0; i < nshg; ++i) {
-    if (cnt[i + 1] == cnt[i]) {
-      rowp[pos++] = i + 1;
-    } else {
-      for (int k = 0; k < rowlen[i]; ++k) rowp[pos++] = buf[cnt[i] + k];
-    }
-    colm[i + 1] = pos + 1;
   }
   return CRIMSON_OK;
 }

@@ -117,6 +117,37 @@ int crimson_genadj(int nshg, int nel, const int *ien, int *colm, int *rowp,
                    int maxnnz, int *nnz_out);
 
 /* ------------------------------------------------------------------ */
+/* The steps either side of formation and solve in one nonlinear
+ * iteration (itrdrv.f90:1284-1528), on device arrays y(nshg,4),
+ * ac(nshg,4) and the memLS increment Res4(4,nshg).  Asynchronous on the
+ * formation handle's stream; crimson_form_sync waits for them. */
+
+/* itrPredict, ipred = 1 (itrPC.f90:104-108). */
+int crimson_itr_predict_dev(crimson_form *f, const double *d_yold,
+                            const double *d_acold, double *d_y, double *d_ac,
+                            double gami);
+/* itrYAlpha (itrPC.f90:295-365), non-ALE, ndof = 4. */
+int crimson_itr_yalpha_dev(crimson_form *f, const double *d_yold,
+                           const double *d_acold, const double *d_y,
+                           const double *d_ac, double *d_yAlpha,
+                           double *d_acAlpha, double alfi, double almi);
+/* itrCorrect (itrPC.f90:164-192) with solinc = Res4^T. */
+int crimson_itr_correct_dev(crimson_form *f, double *d_y, double *d_ac,
+                            const double *d_Res4, double gami, double alfi,
+                            double Delt);
+/* itrBC (itrbc.f90:1-100) for velocity codes 0/7 and the pressure bit;
+ * d_iBC(nshg), d_BC(nshg,ndofBC) column-major, ndofBC >= 5. */
+int crimson_itr_bc_dev(crimson_form *f, const int *d_iBC, const double *d_BC,
+                       int ndofBC, double *d_y);
+/* Resistance-outlet term of ElmpvsQ (elmgmr.f90:1152-1176): Q = sum NABI.u
+ * over the face nodes d_glob (0-based), Res4(1:3,i) += -Q R NABI(i,1:3);
+ * d_nabi(3,nface); Q is left in *d_q. */
+int crimson_elmpvsq_dev(crimson_form *f, int nface, const int *d_glob,
+                        const double *d_nabi, double R, const double *d_y,
+                        double *d_Res4, double *d_q);
+int crimson_form_sync(crimson_form *f);
+
+/* ------------------------------------------------------------------ */
 /* memLS (memLS.f90).                                                   */
 
 enum { CRIMSON_LS_CG = 1, CRIMSON_LS_GMRES = 2, CRIMSON_LS_NS = 3 };
