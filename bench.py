@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--timesteps", type=int, default=1, help="FP64 timesteps to time (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0: skip)")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run one plain device copy of this many MB first (calibrates PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the formation/solve path here has no multi-GPU form "
@@ -118,6 +120,15 @@ def main():
     dy, dac, dx = T(y), T(ac), T(m.x)
     dres4 = torch.zeros((N, 4), dtype=torch.float64, device=dev)       # Res4(4,N) Fortran
 
+    if args.copy_mb > 0:
+        n = args.copy_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
+        del src, dst
+
     # ---- formation throughput
     for _ in range(args.warmup):
         form.to_lhs(dy, dac, dx, dres4, lhs)
@@ -141,8 +152,8 @@ def main():
     y3 = torch.empty_like(x3)
     x4 = torch.rand((N, 4), dtype=torch.float64, device=dev)
     y4 = torch.empty_like(x4)
-    ms3 = lhs.spmv_dev(3, x3, y3, args.spmv_reps)
-    ms4 = lhs.spmv_dev(4, x4, y4, args.spmv_reps)
+    ms3 = lhs.spmv_dev(3, x3, y3, max(1, args.spmv_reps))
+    ms4 = lhs.spmv_dev(4, x4, y4, max(1, args.spmv_reps))
     b3 = nnz * (72 + 4) + N * (8 + 24 + 24)
     b4 = nnz * (128 + 4) + N * (8 + 32 + 32)
     gbs3, gbs4 = b3 / (ms3 * 1e-3) / 1e9, b4 / (ms4 * 1e-3) / 1e9

@@ -12,12 +12,12 @@
 //               record: shape gradients, the scalar part S_ab of the 3x3
 //               velocity blocks, the element residual rl, and four scalars
 //               from which every tangent entry is rebuilt
-//   asm_node    node-parallel: residual rows (res += rl, asigmr.f90:116) and
-//               the diagonal block of every row
-//   asm_edge    edge-parallel: both off-diagonal blocks (i,j) and (j,i) of an
-//               edge from the same element contributions (fillsparseI,
-//               fillsparse.f90:29-63, and the -G^T placement of
-//               solfar.f90:225-235 without its transpose search)
+//   asm_row     row-parallel, 16 lanes per CSR row, one lane per block: the
+//               blocks of the row from the row's element incidences in element
+//               order (fillsparseI, fillsparse.f90:29-63), the -G^T column of
+//               Val4 from the same contributions (solfar.f90:225-235 without
+//               its transpose search) and the residual row (asigmr.f90:116)
+//   res_node    the residual rows alone when no tangent is formed
 //
 // The element record factorisation is exact algebra on the reference's
 // lazy tangent (e3lhs.f90:85-402) for P1 elements, whose metric and shape
@@ -58,13 +58,16 @@ __host__ __device__ constexpr double mm(int a, int b) {
   return nq(0, a) * nq(0, b) + nq(1, a) * nq(1, b) + nq(2, a) * nq(2, b) + nq(3, a) * nq(3, b);
 }
 
-constexpr int kRec = 48;   // doubles per element record
-constexpr int kFRec = 10;  // doubles per flux record
-// record layout
-constexpr int R_G = 0;     // g[a][j]   a*3+j            (12)
-constexpr int R_S = 12;    // S[a][b]   12 + a*4+b       (16)
-constexpr int R_RL = 28;   // rl[a][k]  28 + a*4+k, k: mom1..3, cont (16)
-constexpr int R_TLW = 44, R_TC = 45, R_MU = 46, R_CM = 47;
+// Element record: 48 doubles = three 128-byte lines.  Line 0 holds what
+// every tangent entry needs (shape gradients and the four element scalars);
+// node a's S row and residual sit together in half a line at R_A(a).
+constexpr int kRec = 48;
+constexpr int R_G = 0;                                // g[a][j] at a*3+j (12)
+constexpr int R_TLW = 12, R_TC = 13, R_MU = 14, R_CM = 15;
+__host__ __device__ constexpr int R_A(int a) { return 16 + 8 * a; }  // S[a][0..3], rl[a][0..3]
+constexpr int kFRec = 16;  // flux record: one 128-byte line (m[4], qdi[6], pad)
+constexpr int kRowLanes = 16;   // lanes per CSR row in the assembly
+constexpr int kNodeLanes = 4;   // lanes per node in the flux/residual gathers
 
 struct FormConst {
   double rho, mu, Delt, lhsFct, lhmFct, Dtgl, dtsfct, taucfct;
@@ -181,17 +184,27 @@ __global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict
                                                  const int *__restrict__ inc,
                                                  const double *__restrict__ frec,
                                                  double *__restrict__ q) {
-  const int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= nshg) return;
+  const int gid = xcd_block() * blockDim.x + threadIdx.x;
+  const int i = gid / kNodeLanes, lane = gid % kNodeLanes;
   double s[6] = {0, 0, 0, 0, 0, 0}, rm = 0.0;
-  for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) {
-    const int ea = inc[p];
-    const double *r = frec + (size_t)(ea >> 2) * kFRec;
-    const double m = r[ea & 3];
+  if (i < nshg) {
+    for (int p = inc_ptr[i] + lane; p < inc_ptr[i + 1]; p += kNodeLanes) {
+      const int ea = inc[p];
+      const double *r = frec + (size_t)(ea >> 2) * kFRec;
+      const double m = r[ea & 3];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) s[k] += m * r[4 + k];
-    rm += m;
+      for (int k = 0; k < 6; ++k) s[k] += m * r[4 + k];
+      rm += m;
+    }
   }
+  // fixed-order reduction over the node's lanes: (0+1)+(2+3)
+#pragma unroll
+  for (int off = 1; off < kNodeLanes; off <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] += __shfl_xor(s[k], off, kNodeLanes);
+    rm += __shfl_xor(rm, off, kNodeLanes);
+  }
+  if (i >= nshg || lane != 0) return;
   const double inv = 1.0 / rm;
   const size_t N = nshg;
   q[i + 0 * N] = inv * s[0];
@@ -352,10 +365,10 @@ __global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ 
   double *r = rec + (size_t)e * kRec;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    r[R_RL + a * 4 + 3] = W * (g[a][0] * SU[0] + g[a][1] * SU[1] + g[a][2] * SU[2]);
+    r[R_A(a) + 4 + 3] = W * (g[a][0] * SU[0] + g[a][1] * SU[1] + g[a][2] * SU[2]);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-      r[R_RL + a * 4 + i] = -W * (RNa[a][i] + g[a][0] * RGN[i][0] + g[a][1] * RGN[i][1] + g[a][2] * RGN[i][2]);
+      r[R_A(a) + 4 + i] = -W * (RNa[a][i] + g[a][0] * RGN[i][0] + g[a][1] * RGN[i][1] + g[a][2] * RGN[i][2]);
   }
   if (!LHS) return;
 #pragma unroll
@@ -380,7 +393,7 @@ __global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ 
       const int lo = a < b ? a : b, hi = a < b ? b : a;
       const int k = lo * 4 - (lo * (lo - 1)) / 2 + (hi - lo);  // index of (lo,hi) in SUPG
       const double gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
-      r[R_S + a * 4 + b] = tsFct * mm(a, b) + rt * conv + Mu * gab + rt * SUPG[k];
+      r[R_A(a) + b] = tsFct * mm(a, b) + rt * conv + Mu * gab + rt * SUPG[k];
     }
   r[R_TLW] = tlW;
   r[R_TC] = tlW * TC;
@@ -437,40 +450,14 @@ __device__ inline void put_block(const AsmOut &o, int k, const double (&K)[9], c
   }
 }
 
-// MODE 0 writes res(nshg,4) like ElmGMR; MODE 1 writes Res4(4,nshg), the
-// transposed copy SolFlow hands to memLS (solfar.f90:206-210).
-template <int MODE, bool LHS>
-__global__ void __launch_bounds__(256) asm_node(int nshg, const int *__restrict__ inc_ptr,
-                                                const int *__restrict__ inc,
-                                                const int *__restrict__ diagk,
-                                                const double *__restrict__ rec,
-                                                double *__restrict__ res, AsmOut o) {
-  const int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= nshg) return;
-  double rs[4] = {0, 0, 0, 0};
-  double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[3] = {0, 0, 0}, C = 0.0;
-  for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) {
-    const int ea = inc[p];
-    const int a = ea & 3;
-    const double *r = rec + (size_t)(ea >> 2) * kRec;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) rs[k] += r[R_RL + a * 4 + k];
-    if (LHS) {
-      const double ga[3] = {r[R_G + a * 3 + 0], r[R_G + a * 3 + 1], r[R_G + a * 3 + 2]};
-      const double S = r[R_S + a * 5];
-      const double TM = r[R_TC] + r[R_MU];
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-        for (int ii = 0; ii < 3; ++ii) K[ii + 3 * jj] += (ii == jj ? S : 0.0) + TM * ga[ii] * ga[jj];
-      const double tn = r[R_TLW] * sum_n(a);
-#pragma unroll
-      for (int ii = 0; ii < 3; ++ii) G[ii] += tn * ga[ii];
-      C += r[R_CM] * (ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2]);
-    }
-  }
-  const size_t N = nshg;
+// Residual rows only (lhs = 0): res_i = sum of rl_a over the node's
+// incidences (the scatter of asigmr.f90:116).  MODE 0 writes res(nshg,4) like
+// ElmGMR; MODE 1 writes Res4(4,nshg), the transposed copy SolFlow hands to
+// memLS (solfar.f90:206-210).
+template <int MODE>
+__device__ inline void put_res(double *res, int i, int nshg, const double (&rs)[4]) {
   if (MODE == 0) {
+    const size_t N = nshg;
     res[i + 0 * N] = rs[0];
     res[i + 1 * N] = rs[1];
     res[i + 2 * N] = rs[2];
@@ -481,59 +468,106 @@ __global__ void __launch_bounds__(256) asm_node(int nshg, const int *__restrict_
     res[4 * (size_t)i + 2] = rs[2];
     res[4 * (size_t)i + 3] = rs[3];
   }
-  if (LHS) put_block<MODE>(o, diagk[i], K, G, C, G);
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) asm_edge(int nedge, const int *__restrict__ ek,
-                                                const int *__restrict__ ec_ptr,
-                                                const int *__restrict__ ec,
-                                                const double *__restrict__ rec, AsmOut o) {
-  const int t = xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= nedge) return;
-  double Kij[9], Kji[9], Gij[3] = {0, 0, 0}, Gji[3] = {0, 0, 0}, C = 0.0;
+__global__ void __launch_bounds__(256) res_node(int nshg, const int *__restrict__ inc_ptr,
+                                                const int *__restrict__ inc, const double *__restrict__ rec,
+                                                double *__restrict__ res) {
+  const int gid = xcd_block() * blockDim.x + threadIdx.x;
+  const int i = gid / kNodeLanes, lane = gid % kNodeLanes;
+  double rs[4] = {0, 0, 0, 0};
+  if (i < nshg)
+    for (int p = inc_ptr[i] + lane; p < inc_ptr[i + 1]; p += kNodeLanes) {
+      const int ea = inc[p];
+      const double *r = rec + (size_t)(ea >> 2) * kRec + R_A(ea & 3) + 4;
 #pragma unroll
-  for (int m = 0; m < 9; ++m) {
-    Kij[m] = 0.0;
-    Kji[m] = 0.0;
-  }
-  for (int p = ec_ptr[t]; p < ec_ptr[t + 1]; ++p) {
-    const int eab = ec[p];
-    const int a = (eab >> 2) & 3, b = eab & 3;
-    const double *r = rec + (size_t)(eab >> 4) * kRec;
-    const double ga[3] = {r[R_G + a * 3 + 0], r[R_G + a * 3 + 1], r[R_G + a * 3 + 2]};
-    const double gb[3] = {r[R_G + b * 3 + 0], r[R_G + b * 3 + 1], r[R_G + b * 3 + 2]};
-    const double Sab = r[R_S + a * 4 + b], Sba = r[R_S + b * 4 + a];
-    const double Tc = r[R_TC], Mu = r[R_MU];
-#pragma unroll
-    for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-      for (int ii = 0; ii < 3; ++ii) {
-        // K^{ii,jj}_ab = d S_ab + Tc g_a,ii g_b,jj + Mu g_a,jj g_b,ii
-        const double x1 = Tc * ga[ii] * gb[jj] + Mu * ga[jj] * gb[ii];
-        const double x2 = Tc * gb[ii] * ga[jj] + Mu * gb[jj] * ga[ii];
-        Kij[ii + 3 * jj] += (ii == jj ? Sab : 0.0) + x1;
-        Kji[ii + 3 * jj] += (ii == jj ? Sba : 0.0) + x2;
-      }
-    const double tl = r[R_TLW];
-    const double ta = tl * sum_n(a), tb = tl * sum_n(b);
-#pragma unroll
-    for (int ii = 0; ii < 3; ++ii) {
-      Gij[ii] += ta * gb[ii];
-      Gji[ii] += tb * ga[ii];
+      for (int k = 0; k < 4; ++k) rs[k] += r[k];
     }
-    C += r[R_CM] * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
-  }
-  put_block<MODE>(o, ek[2 * t], Kij, Gij, C, Gji);
-  put_block<MODE>(o, ek[2 * t + 1], Kji, Gji, C, Gij);
+#pragma unroll
+  for (int off = 1; off < kNodeLanes; off <<= 1)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rs[k] += __shfl_xor(rs[k], off, kNodeLanes);
+  if (i < nshg && lane == 0) put_res<MODE>(res, i, nshg, rs);
 }
 
+// Row-parallel assembly: kRowLanes lanes per CSR row, lane s owns the row's
+// block s (and s + 16, ... for long rows).  All lanes of a row walk the row's
+// element incidences in element order (the accumulation order of
+// fillsparseI), so the record lines of an incidence are fetched once for the
+// row; slot[p] holds, byte b, the position in row i of node ien(e,b).  The
+// lane owning the diagonal block also sums the residual row.  Every block of
+// the row is written, so blocks no element touches come out zero (the
+// reference zeroes lhsK/lhsP first, elmgmr.f90:127-130).
 template <int MODE>
-__global__ void zero_blocks(int n, const int *__restrict__ ks, AsmOut o) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[3] = {0, 0, 0};
-  put_block<MODE>(o, ks[t], K, G, 0.0, G);
+__global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__ rowptr,
+                                               const int *__restrict__ diagk, const int *__restrict__ inc_ptr,
+                                               const int *__restrict__ inc, const unsigned *__restrict__ slot,
+                                               const double *__restrict__ rec, double *__restrict__ res,
+                                               AsmOut o) {
+  const int gid = xcd_block() * blockDim.x + threadIdx.x;
+  const int row = gid / kRowLanes, lane = gid % kRowLanes;
+  if (row >= nshg) return;
+  const int k0 = rowptr[row], nslot = rowptr[row + 1] - k0, sdiag = diagk[row] - k0;
+  const int p0 = inc_ptr[row], p1 = inc_ptr[row + 1];
+  for (int s0 = 0; s0 < nslot; s0 += kRowLanes) {
+    const int s = s0 + lane;
+    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Gij[3] = {0, 0, 0}, Gji[3] = {0, 0, 0}, C = 0.0;
+    double rs[4] = {0, 0, 0, 0};
+    for (int c0 = p0; c0 < p1; c0 += kRowLanes) {
+      // the row's next 16 incidences, one per lane, then broadcast in order
+      const int pc = c0 + lane;
+      const int my_ea = pc < p1 ? inc[pc] : 0;
+      const unsigned my_sm = pc < p1 ? slot[pc] : 0xffffffffu;
+      const int nt = min(kRowLanes, p1 - c0);
+      for (int t = 0; t < nt; ++t) {
+        const int ea = __shfl(my_ea, t, kRowLanes);
+        const unsigned sm = (unsigned)__shfl((int)my_sm, t, kRowLanes);
+        const int a = ea & 3;
+        // line 0 of the record, same address for all lanes of the row
+        const double *r = rec + (size_t)(ea >> 2) * kRec;
+        double gl[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) gl[k] = r[R_G + k];
+        const double tl = r[R_TLW], Tc = r[R_TC], Mu = r[R_MU], Cm = r[R_CM];
+        int b = -1;
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb)
+          if ((int)((sm >> (8 * bb)) & 255u) == s) b = bb;
+        if (b < 0) continue;
+        double ga[3], gb[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          ga[j] = a == 0 ? gl[j] : a == 1 ? gl[3 + j] : a == 2 ? gl[6 + j] : gl[9 + j];
+          gb[j] = b == 0 ? gl[j] : b == 1 ? gl[3 + j] : b == 2 ? gl[6 + j] : gl[9 + j];
+        }
+        const double Sab = r[R_A(a) + b];
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+          for (int ii = 0; ii < 3; ++ii)
+            // K^{ii,jj}_ab = delta S_ab + Tc g_a,ii g_b,jj + Mu g_a,jj g_b,ii
+            K[ii + 3 * jj] += (ii == jj ? Sab : 0.0) + (Tc * ga[ii] * gb[jj] + Mu * ga[jj] * gb[ii]);
+        const double sa = a == 0 ? sum_n(0) : a == 1 ? sum_n(1) : a == 2 ? sum_n(2) : sum_n(3);
+        const double sb = b == 0 ? sum_n(0) : b == 1 ? sum_n(1) : b == 2 ? sum_n(2) : sum_n(3);
+        const double ta = tl * sa, tb = tl * sb;
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          Gij[ii] += ta * gb[ii];
+          Gji[ii] += tb * ga[ii];
+        }
+        C += Cm * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+        if (b == a) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) rs[k] += r[R_A(a) + 4 + k];
+        }
+      }
+    }
+    if (s < nslot) {
+      put_block<MODE>(o, k0 + s, K, Gij, C, Gji);
+      if (s == sdiag) put_res<MODE>(res, row, nshg, rs);
+    }
+  }
 }
 
 // bc3Res for codes 0/7 on the velocity bits, bit 2 (pressure) and bit 11
@@ -557,10 +591,11 @@ __global__ void bc3res_kernel(int nshg, const int *__restrict__ ibc, double *__r
 
 // ---------------------------------------------------------------- handle
 struct crimson_form {
-  int nshg = 0, numnp = 0, nel = 0, nnz = 0, nedge = 0, norphan = 0;
+  int nshg = 0, numnp = 0, nel = 0, nnz = 0;
   crimson_form_params prm{};
   hipStream_t stream = nullptr;
-  crimson::DBuf<int> d_ien, d_inc_ptr, d_inc, d_diagk, d_ek, d_ec_ptr, d_ec, d_orphan, d_ibc;
+  crimson::DBuf<int> d_ien, d_inc_ptr, d_inc, d_diagk, d_rowptr, d_ibc;
+  crimson::DBuf<unsigned> d_slot;
   crimson::DBuf<double> d_frec, d_q, d_rec;
   crimson::DBuf<double> s_y, s_ac, s_x, s_res, s_K, s_P, s_q;  // staging of the host API
   bool has_ibc = false;
@@ -626,7 +661,8 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
     flux_elem<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg,
                                                          f->prm.mu, f->d_frec.p);
   }
-  flux_node<<<crimson::grid_for(f->nshg, T), T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_frec.p,
+  flux_node<<<crimson::grid_for((size_t)f->nshg * kNodeLanes, T), T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p,
+                                                                            f->d_frec.p,
                                                         d_qout ? d_qout : f->d_q.p);
   CR_HIP(hipGetLastError());
   CR_HIP(hipEventRecord(f->ev[1], s));
@@ -641,17 +677,19 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
   }
   CR_HIP(hipGetLastError());
   CR_HIP(hipEventRecord(f->ev[2], s));
-  const unsigned gn = crimson::grid_for(f->nshg, T), ge = crimson::grid_for(f->nedge, T);
+  const unsigned gn = crimson::grid_for((size_t)f->nshg * kNodeLanes, T);
+  const unsigned gr = crimson::grid_for((size_t)f->nshg * kRowLanes, T);
   if (!lhs) {
-    asm_node<0, false><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_diagk.p, f->d_rec.p, d_res, o);
+    if (mode == 0)
+      res_node<0><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
+    else
+      res_node<1><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
   } else if (mode == 0) {
-    asm_node<0, true><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_diagk.p, f->d_rec.p, d_res, o);
-    if (f->nedge) asm_edge<0><<<ge, T, 0, s>>>(f->nedge, f->d_ek.p, f->d_ec_ptr.p, f->d_ec.p, f->d_rec.p, o);
-    if (f->norphan) zero_blocks<0><<<crimson::grid_for(f->norphan, T), T, 0, s>>>(f->norphan, f->d_orphan.p, o);
+    asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p, f->d_slot.p,
+                                f->d_rec.p, d_res, o);
   } else {
-    asm_node<1, true><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_diagk.p, f->d_rec.p, d_res, o);
-    if (f->nedge) asm_edge<1><<<ge, T, 0, s>>>(f->nedge, f->d_ek.p, f->d_ec_ptr.p, f->d_ec.p, f->d_rec.p, o);
-    if (f->norphan) zero_blocks<1><<<crimson::grid_for(f->norphan, T), T, 0, s>>>(f->norphan, f->d_orphan.p, o);
+    asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p, f->d_slot.p,
+                                f->d_rec.p, d_res, o);
   }
   CR_HIP(hipGetLastError());
   CR_HIP(hipEventRecord(f->ev[3], s));
@@ -744,68 +782,42 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
         inc[fill[n]++] = e * 4 + a;
       }
   }
-  // edges (i<j) with both block positions; contributions in element order
-  std::vector<int> edge_of_k(nnz, -1), ek;
-  int nedge = 0;
-  for (int i = 0; i < nshg; ++i)
-    for (int k = rp[i]; k < rp[i + 1]; ++k) {
-      const int j = cols[k];
-      if (j <= i) continue;
-      const int kt = find_col(rp, cols, j, i);
-      if (kt < 0) {
-        crimson::set_error("graph not symmetric: block (%d,%d) present, (%d,%d) missing", i + 1, j + 1, j + 1, i + 1);
-        return CRIMSON_EGRAPH;
-      }
-      edge_of_k[k] = nedge++;
-      ek.push_back(k);
-      ek.push_back(kt);
+  // slot map: for incidence p = (e,a) of row i, byte b = position of
+  // ien(e,b) in row i (the sparseloc search of fillsparse.f90:41, done once)
+  std::vector<unsigned> slot(inc.size());
+  for (int i = 0; i < nshg; ++i) {
+    if (rp[i + 1] - rp[i] > 255) {
+      crimson::set_error("row %d has %d blocks; at most 255 supported", i + 1, rp[i + 1] - rp[i]);
+      return CRIMSON_EUNSUPPORTED;
     }
-  std::vector<int> ec_ptr(nedge + 1, 0), ec((size_t)nel * 6);
-  std::vector<char> touched(nnz, 0);
-  for (int pass = 0; pass < 2; ++pass) {
-    std::vector<int> fill;
-    if (pass == 1) fill.assign(ec_ptr.begin(), ec_ptr.end() - 1);
-    for (int e = 0; e < nel; ++e) {
-      int nd[4];
-      for (int a = 0; a < 4; ++a) nd[a] = ien[(size_t)a * nel + e] - 1;
-      for (int p = 0; p < 4; ++p)
-        for (int qq = p + 1; qq < 4; ++qq) {
-          int a = p, b = qq;
-          if (nd[a] == nd[b]) {
-            crimson::set_error("element %d repeats node %d", e + 1, nd[a] + 1);
-            return CRIMSON_EARG;
-          }
-          if (nd[a] > nd[b]) std::swap(a, b);
-          const int k = find_col(rp, cols, nd[a], nd[b]);
-          if (k < 0) {
-            crimson::set_error("element %d: pair (%d,%d) missing from the CSR graph", e + 1, nd[a] + 1, nd[b] + 1);
-            return CRIMSON_EGRAPH;
-          }
-          const int t = edge_of_k[k];
-          if (pass == 0) {
-            ec_ptr[t + 1]++;
-            touched[k] = 1;
-            touched[ek[2 * t + 1]] = 1;
-          } else {
-            ec[fill[t]++] = e * 16 + a * 4 + b;
-          }
+    for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) {
+      const int e = inc[p] >> 2;
+      unsigned sm = 0;
+      for (int b = 0; b < 4; ++b) {
+        const int j = ien[(size_t)b * nel + e] - 1;
+        const int k = find_col(rp, cols, i, j);
+        if (k < 0) {
+          crimson::set_error("element %d: pair (%d,%d) missing from the CSR graph", e + 1, i + 1, j + 1);
+          return CRIMSON_EGRAPH;
         }
+        sm |= (unsigned)(k - rp[i]) << (8 * b);
+      }
+      slot[p] = sm;
     }
-    if (pass == 0)
-      for (int t = 0; t < nedge; ++t) ec_ptr[t + 1] += ec_ptr[t];
   }
-  for (int i = 0; i < nshg; ++i) touched[diagk[i]] = 1;
-  std::vector<int> orphan;
-  for (int k = 0; k < nnz; ++k)
-    if (!touched[k]) orphan.push_back(k);
+  for (int e = 0; e < nel; ++e)
+    for (int a = 0; a < 4; ++a)
+      for (int b = a + 1; b < 4; ++b)
+        if (ien[(size_t)a * nel + e] == ien[(size_t)b * nel + e]) {
+          crimson::set_error("element %d repeats node %d", e + 1, ien[(size_t)a * nel + e]);
+          return CRIMSON_EARG;
+        }
 
   crimson_form *f = new crimson_form();
   f->nshg = nshg;
   f->numnp = numnp;
   f->nel = nel;
   f->nnz = nnz;
-  f->nedge = nedge;
-  f->norphan = (int)orphan.size();
   f->prm = *prm;
   int rc = CRIMSON_OK;
   auto fail = [&](int code) {
@@ -829,10 +841,8 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
       (rc = f->d_inc_ptr.upload(inc_ptr.data(), inc_ptr.size(), s)) ||
       (rc = f->d_inc.upload(inc.data(), inc.size(), s)) ||
       (rc = f->d_diagk.upload(diagk.data(), diagk.size(), s)) ||
-      (rc = f->d_ek.upload(ek.data(), ek.size(), s)) ||
-      (rc = f->d_ec_ptr.upload(ec_ptr.data(), ec_ptr.size(), s)) ||
-      (rc = f->d_ec.upload(ec.data(), ec.size(), s)) ||
-      (rc = f->d_orphan.upload(orphan.data(), orphan.size(), s)) ||
+      (rc = f->d_rowptr.upload(rp.data(), rp.size(), s)) ||
+      (rc = f->d_slot.upload(slot.data(), slot.size(), s)) ||
       (rc = f->d_frec.alloc((size_t)std::max(nel, 1) * kFRec)) ||
       (rc = f->d_q.alloc((size_t)nshg * 9)) || (rc = f->d_rec.alloc((size_t)std::max(nel, 1) * kRec)))
     return fail(rc);

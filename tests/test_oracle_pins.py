@@ -1,0 +1,482 @@
+"""Pins of the oracle (the CPU restatement of e3*.f90, e3q.f90, qpbc.f90,
+fillsparse.f90, genadj.f90, solfar.f90 and memLS.f90) against what the
+mathematics fixes: closed forms, invariances, finite differences of the
+residual, brute-force assembly and direct solves.  CPU only.
+
+The reference's own tests hold no kernel-level golden vectors (end-to-end
+gtests only) and its Fortran cannot be compiled in this image, so these are
+the checks that pin each oracle function; each is chosen so that a plausible
+mistake (a dropped term, a wrong sign or index, a transposed operand) fails.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+RHO, MU, DT = 1.06, 0.04, 1e-3
+
+
+def tet_block(x4, y=None, ac=None, q=None):
+    """Reference block arrays for one element, in e3's argument order:
+    yl(1,4,4) in (p,u1,u2,u3) order, acl(1,4,4), xl(1,4,3), ql(1,4,9)."""
+    xl = np.asfortranarray(np.asarray(x4, float).reshape(1, 4, 3))
+    yl = np.zeros((1, 4, 4), order="F") if y is None else np.asfortranarray(y.reshape(1, 4, 4))
+    acl = np.zeros((1, 4, 4), order="F") if ac is None else np.asfortranarray(ac.reshape(1, 4, 4))
+    ql = np.zeros((1, 4, 9), order="F") if q is None else np.asfortranarray(q.reshape(1, 4, 9))
+    return yl, acl, xl, ql
+
+
+def p1_grads(x4):
+    """Exact P1 gradients and volume of a tet with nodes x4[0..3]."""
+    x4 = np.asarray(x4, float)
+    J = np.stack([x4[0] - x4[3], x4[1] - x4[3], x4[2] - x4[3]], axis=1)  # dx/d(r,s,t)
+    Ji = np.linalg.inv(J)
+    g = np.zeros((4, 3))
+    g[:3] = Ji
+    g[3] = -Ji.sum(axis=0)
+    return g, np.linalg.det(J) / 6.0
+
+
+def random_tet(rng, stretch=1.0):
+    while True:
+        x = rng.normal(size=(4, 3)) * np.array([1.0, 1.0, stretch])
+        g, V = p1_grads(x)
+        if V > 0.05:
+            return x
+
+
+@pytest.fixture(scope="module")
+def O(oracle_lib):
+    return oracle_lib
+
+
+def lhs_fct(O, p=None):
+    p = p or O.make_params()
+    return p.alfi * p.gami * p.Delt, p
+
+
+def test_time_params_closed_form(O):
+    almi, alfi, gami = O.time_params(0.5)
+    assert (almi, alfi, gami) == pytest.approx((5 / 6, 2 / 3, 2 / 3), abs=1e-15)
+    assert O.time_params(-1.0) == (1.0, 1.0, 1.0)  # backward Euler branch
+    almi, alfi, gami = O.time_params(0.0)
+    assert (almi, alfi, gami) == pytest.approx((1.5, 1.0, 1.0), abs=1e-15)
+
+
+def test_zero_state_mass_grad_and_C(O):
+    """u=p=a=q=0: rl = 0; G_ab,i = lhsFct V/4 g_b,i; the mass part of the
+    diagonal of K is the P1 consistent mass alpha_m rho V (1+d_ab)/20."""
+    rng = np.random.default_rng(1)
+    x = random_tet(rng)
+    g, V = p1_grads(x)
+    lf, p = lhs_fct(O)
+    rl, K, G = O.e3_block(*tet_block(x), p)
+    assert np.abs(rl).max() == 0.0
+    for a in range(4):
+        for b in range(4):
+            np.testing.assert_allclose(G[0, :3, a, b], lf * V / 4 * g[b], rtol=1e-12, atol=1e-15)
+    # mass: K^{11} - K^{22} removes the isotropic parts; use the u-free
+    # diagonal: K^{ii} = m_ab + lf V [mu (g_a.g_b + g_ai g_bi) + tauC g_ai g_bi]
+    # so K^{11} + K^{22} + K^{33} = 3 m_ab + lf V [(3mu + mu + tauC) g_a.g_b]
+    tr = K[0, 0] + K[0, 4] + K[0, 8]
+    m = p.almi * RHO * V * (1 + np.eye(4)) / 20.0
+    gg = g @ g.T
+    # the remaining isotropic factor is common to all (a,b): solve for it
+    coef = (tr - 3 * m) / (lf * V * gg)
+    assert np.ptp(coef) < 1e-9 * np.abs(coef).max()
+    assert coef[0, 0] > 4 * MU  # 4 mu + tauC with tauC > 0
+
+
+def test_tau_on_unit_right_tet(O):
+    """Hand-evaluated tau at u = 0 on the unit right tet: dxi/dx = 2 I (the
+    bi-unit mapping of shgl = dN/dxi / 2), G = 4 [[c1,c2,c2],[c2,c1,c2],
+    [c2,c2,c1]] with c1 = 2^(1/3), c2 = c1/2, so G:G = 72 2^(2/3) and
+    tr G = 12 2^(1/3); tauM = [(2/dt)^2 + 36 nu^2 G:G]^(-1/2),
+    tauC = rho/8 sqrt(...)/trG."""
+    x = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 0]], float)
+    g, V = p1_grads(x)
+    lf, p = lhs_fct(O)
+    rl, K, G = O.e3_block(*tet_block(x), p)
+    nu = MU / RHO
+    s = (2 / DT) ** 2 + 36 * nu ** 2 * 72 * 2 ** (2 / 3)
+    tauM, tauC = s ** -0.5, RHO / 8 * np.sqrt(s) / (12 * 2 ** (1 / 3))
+    # C_ab = lf V tauM / rho g_a.g_b (PSPG Laplacian)
+    np.testing.assert_allclose(G[0, 3], lf * V * tauM / RHO * (g @ g.T), rtol=1e-12)
+    # grad-div: K^{12}_ab - K^{21}_ab = lf V (tauC - mu) (g_a,2 g_b,1 - g_a,1 g_b,2)
+    d = K[0, 1] - K[0, 3]
+    want = lf * V * (tauC - MU) * (np.outer(g[:, 1], g[:, 0]) - np.outer(g[:, 0], g[:, 1]))
+    np.testing.assert_allclose(d, want, rtol=1e-10, atol=1e-14 * np.abs(want).max())
+
+
+def test_constant_state_patch(O):
+    """Constant u and p, a = q = 0: L = 0, ubar = u, tauBar = 0, so
+    rl_a,i = V p g_a,i and rl_a,4 = V u.g_a exactly; each sums to 0."""
+    rng = np.random.default_rng(2)
+    x = random_tet(rng)
+    g, V = p1_grads(x)
+    u, pr = np.array([3.0, -1.5, 7.0]), 123.0
+    y = np.zeros((4, 4))
+    y[:, 0] = pr
+    y[:, 1:] = u
+    rl, K, G = O.e3_block(*tet_block(x, y=y), O.make_params())
+    np.testing.assert_allclose(rl[0, :, :3], V * pr * g, rtol=1e-12, atol=1e-12 * V * pr)
+    np.testing.assert_allclose(rl[0, :, 3], V * g @ u, rtol=1e-12, atol=1e-12 * V * np.abs(u).max())
+    assert np.abs(rl[0].sum(axis=0)).max() < 1e-11 * V * pr
+
+
+def _perm_case(rng):
+    x = random_tet(rng)
+    y = rng.normal(size=(4, 4)) * np.array([100, 10, 10, 10])
+    ac = rng.normal(size=(4, 4)) * 50
+    q = rng.normal(size=(4, 9))
+    return x, y, ac, q
+
+
+def test_node_permutation_invariance(O):
+    """The c1/c2 correction of e3gijd (e3stab.f90:493-527) makes tau
+    independent of the element's node order: permuting nodes permutes rl
+    rows and K/G blocks.  An uncorrected metric fails this by ~1e-1."""
+    rng = np.random.default_rng(3)
+    x, y, ac, q = _perm_case(rng)
+    p = O.make_params()
+    rl0, K0, G0 = O.e3_block(*tet_block(x, y, ac, q), p)
+    scale = (np.abs(rl0).max(), np.abs(K0).max(), np.abs(G0).max())
+    for perm in itertools.permutations(range(4)):
+        P = list(perm)
+        # an odd permutation reverses the orientation: WdetJ changes sign and
+        # every integral with it, while tau (metric only) is unchanged
+        sgn = np.linalg.det(np.eye(4)[P])
+        rl, K, G = O.e3_block(*tet_block(x[P], y[P], ac[P], q[P]), p)
+        assert np.abs(rl[0] - sgn * rl0[0][P]).max() <= 1e-12 * scale[0]
+        assert np.abs(K[0] - sgn * K0[0][:, P][:, :, P]).max() <= 1e-12 * scale[1]
+        assert np.abs(G[0] - sgn * G0[0][:, P][:, :, P]).max() <= 1e-12 * scale[2]
+
+
+def test_rotation_equivariance(O):
+    """Rotating coordinates, velocities, accelerations and the flux tensor
+    rotates the momentum residual and every 3x3 block (K -> Q K Q^T)."""
+    rng = np.random.default_rng(4)
+    x, y, ac, q = _perm_case(rng)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    p = O.make_params()
+    rl0, K0, G0 = O.e3_block(*tet_block(x, y, ac, q), p)
+    y2, ac2 = y.copy(), ac.copy()
+    y2[:, 1:] = y[:, 1:] @ Q.T
+    ac2[:, 1:] = ac[:, 1:] @ Q.T
+    # q(:, i + 3(j-1)) = tau_ij  ->  Q tau Q^T
+    tau = q.reshape(4, 3, 3).transpose(0, 2, 1)  # tau[a][i][j]
+    tau2 = np.einsum("ik,akl,jl->aij", Q, tau, Q)
+    q2 = tau2.transpose(0, 2, 1).reshape(4, 9)
+    rl, K, G = O.e3_block(*tet_block(x @ Q.T, y2, ac2, q2), p)
+    np.testing.assert_allclose(rl[0, :, :3], rl0[0, :, :3] @ Q.T, atol=1e-11 * np.abs(rl0).max())
+    np.testing.assert_allclose(rl[0, :, 3], rl0[0, :, 3], atol=1e-11 * np.abs(rl0).max())
+    Kb0 = K0[0].reshape(3, 3, 4, 4, order="F")  # [i][j] = K(i + 3j)
+    Kb = K[0].reshape(3, 3, 4, 4, order="F")
+    np.testing.assert_allclose(Kb, np.einsum("ik,klab,jl->ijab", Q, Kb0, Q), atol=1e-11 * np.abs(K0).max())
+
+
+def _fd_setup(O, state):
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(4, 4, 5)
+    colm, rowp = O.genadj(m.ien, m.numnp)
+    rng = np.random.default_rng(5)
+    n = m.numnp
+    y = np.zeros((n, 4), order="F")
+    ac = np.zeros((n, 4), order="F")
+    if state == "zero":
+        pass
+    elif state == "constu":
+        y[:, :3] = [4.0, -2.0, 9.0]
+        y[:, 3] = 50.0
+    else:
+        y[:] = rng.normal(size=(n, 4)) * [5, 5, 20, 100]
+        ac[:, :3] = rng.normal(size=(n, 3)) * 30
+    q = np.asfortranarray(rng.normal(size=(n, 9)) * (0 if state != "generic" else 0.3))
+    return m, colm, rowp, y, ac, q
+
+
+def _dense_blocks(colm, rowp, lhsK, lhsP, n):
+    K = np.zeros((n, 3, n, 3))
+    Gp = np.zeros((n, 3, n))   # momentum row, pressure column: -G^T
+    D = np.zeros((n, n, 3))    # continuity row, velocity column: lhsP(1:3)
+    C = np.zeros((n, n))
+    for i in range(n):
+        for k in range(colm[i] - 1, colm[i + 1] - 1):
+            j = rowp[k] - 1
+            K[i, :, j, :] = lhsK[:, k].reshape(3, 3, order="F")  # [eq i][var j] = lhsK(i + 3j)
+            D[i, j, :] = lhsP[:3, k]
+            C[i, j] = lhsP[3, k]
+            Gp[j, :, i] = -lhsP[:3, k]
+    return K, Gp, D, C
+
+
+def _fd(fun, base, idx, h):
+    bp, bm = base.copy(order="F"), base.copy(order="F")
+    bp[idx] += h
+    bm[idx] -= h
+    return (fun(bp) - fun(bm)) / (2 * h)
+
+
+def _p1_mass(m):
+    """Assembled P1 consistent mass, V (1 + d_ab) / 20 per element (numpy)."""
+    n = m.numnp
+    M = np.zeros((n, n))
+    x = m.x[m.ien - 1]
+    V = np.einsum("ei,ei->e", x[:, 0] - x[:, 3], np.cross(x[:, 1] - x[:, 3], x[:, 2] - x[:, 3])) / 6.0
+    loc = (1 + np.eye(4)) / 20.0
+    for e in range(m.nel):
+        idx = m.ien[e] - 1
+        M[np.ix_(idx, idx)] += V[e] * loc
+    return M
+
+
+@pytest.mark.parametrize("state", ["zero", "constu"])
+def test_tangent_momentum_block_is_exact_jacobian(O, state):
+    """At u = 0 or at constant u, p with a = 0 (so L = 0), the lazy tangent's
+    momentum-velocity block is exact: K = alpha_m rho M - d(res)/d(y_alpha)
+    * alpha_f gamma Delt (mass, convection, viscous, grad-div and SUPG terms),
+    with K^{ij} at lhsK(i + 3(j-1)) for equation i, variable j.  Central
+    differences of the assembled residual with q frozen pin index placement,
+    signs and factors; the mass is the closed-form P1 mass.  (Perturbing the
+    acceleration as well would add the SUPG acceleration term tauM rho N_b
+    (u.grad N_a), which the lazy tangent omits; at u = 0 it vanishes and the
+    zero-state case also checks d(res)/d(ac) = -mass.)"""
+    m, colm, rowp, y, ac, q = _fd_setup(O, state)
+    p = O.make_params()
+    n = m.numnp
+    res0, lhsK, lhsP = O.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, p, q=q)
+    K, Gp, D, C = _dense_blocks(colm, rowp, lhsK, lhsP, n)
+    Mm = p.almi * RHO * _p1_mass(m)
+    lf = p.alfi * p.gami * p.Delt
+    rng = np.random.default_rng(6)
+    for node in rng.choice(n, 4, replace=False):
+        for c in range(3):
+            def R(dv, which):
+                yy = y.copy(order="F")
+                aa = ac.copy(order="F")
+                if which == "y":
+                    yy[node, c] += lf * dv
+                else:
+                    aa[node, c] += p.almi * dv
+                return O.elmgmr_interior(m.ien, yy, aa, m.x, colm, rowp, O.make_params(lhs=0), q=q)[0]
+            h = 1e-3
+            Jy = -(R(h, "y") - R(-h, "y")) / (2 * h)
+            want = K[:, :, node, c].copy()
+            want[:, c] -= Mm[:, node]
+            err = np.abs(Jy[:, :3] - want).max()
+            assert err <= 1e-7 * np.abs(want).max(), (node, c, err)
+            if state == "zero":
+                Ja = -(R(h, "a") - R(-h, "a")) / (2 * h)
+                mass = np.zeros((n, 3))
+                mass[:, c] = Mm[:, node]
+                assert np.abs(Ja[:, :3] - mass).max() <= 1e-7 * np.abs(Mm).max()
+
+
+def test_tangent_pressure_and_continuity_columns(O):
+    """d(res_mom)/d(p) = -(G^T) (Val4's column 4, solfar.f90:225-235) at u = 0,
+    and d(res_cont)/d(p) = -C (the PSPG block) at a generic state."""
+    for state in ("zero", "generic"):
+        m, colm, rowp, y, ac, q = _fd_setup(O, state)
+        p = O.make_params()
+        n = m.numnp
+        _, lhsK, lhsP = O.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, p, q=q)
+        K, Gp, D, C = _dense_blocks(colm, rowp, lhsK, lhsP, n)
+        lf = p.alfi * p.gami * p.Delt
+        for node in (0, n // 2, n - 1):
+            def R(dv):
+                yy = y.copy(order="F")
+                yy[node, 3] += lf * dv   # pressure: y_alpha(4) = y(4), Delta p scaled by gami Delt alfi
+                return O.elmgmr_interior(m.ien, yy, ac, m.x, colm, rowp, O.make_params(lhs=0), q=q)[0]
+            h = 1.0
+            J = -(R(h) - R(-h)) / (2 * h)
+            np.testing.assert_allclose(J[:, 3], C[:, node], atol=1e-8 * np.abs(C).max())
+            if state == "zero":
+                np.testing.assert_allclose(J[:, :3], Gp[:, :, node], atol=1e-8 * np.abs(Gp).max())
+
+
+def test_flux_reconstruction_exact_for_linear_velocity(O):
+    """qpbc's lumped L2 projection of mu (grad u + grad u^T) reproduces a
+    constant tensor exactly at every node for globally linear u."""
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(5, 5, 6)
+    colm, rowp = O.genadj(m.ien, m.numnp)
+    A = np.array([[0.3, -1.2, 2.0], [0.7, 0.1, -0.4], [1.5, 0.9, -0.4]])  # du_i/dx_j
+    y = np.zeros((m.numnp, 4), order="F")
+    y[:, :3] = m.x @ A.T + 1.0
+    _, _, _, q = O.elmgmr_interior(m.ien, y, np.zeros_like(y), m.x, colm, rowp, O.make_params(lhs=0), want_q=True)
+    tau = MU * (A + A.T)
+    want = tau.T.reshape(-1, order="F")  # q(:, i + 3(j-1)) = tau_ij
+    np.testing.assert_allclose(q, np.tile(want, (m.numnp, 1)), atol=1e-12)
+
+
+def test_patch_test_interior_nodes(O):
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(5, 5, 6)
+    colm, rowp = O.genadj(m.ien, m.numnp)
+    y = np.zeros((m.numnp, 4), order="F")
+    y[:, :3] = [1.0, 2.0, -3.0]
+    y[:, 3] = 11.0
+    res, _, _ = O.elmgmr_interior(m.ien, y, np.zeros_like(y), m.x, colm, rowp, O.make_params())
+    interior = ~(m.inlet | m.outlet | m.wall)
+    assert interior.sum() > 0
+    assert np.abs(res[interior]).max() < 1e-12
+
+
+def test_assembly_brute_force(O):
+    """fillsparseI and the residual scatter equal a dense sum of the per-
+    element e3 outputs; genadj equals the set of element node pairs."""
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(4, 4, 4)
+    n = m.numnp
+    colm, rowp = O.genadj(m.ien, n)
+    rp, cc = M.csr_graph(m.ien - 1, n)
+    assert np.array_equal(colm - 1, rp) and np.array_equal(rowp - 1, cc)
+    rng = np.random.default_rng(7)
+    y = np.asfortranarray(rng.normal(size=(n, 4)) * [3, 3, 10, 50])
+    ac = np.asfortranarray(rng.normal(size=(n, 4)) * 20)
+    p = O.make_params()
+    res, lhsK, lhsP, q = O.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, p, want_q=True)
+    ien0 = m.ien - 1
+    yl = np.asfortranarray(np.stack([y[ien0][:, :, 3], y[ien0][:, :, 0], y[ien0][:, :, 1], y[ien0][:, :, 2]], axis=2))
+    acl = np.asfortranarray(np.stack([ac[ien0][:, :, 3], ac[ien0][:, :, 0], ac[ien0][:, :, 1], ac[ien0][:, :, 2]], axis=2))
+    xl = np.asfortranarray(m.x[ien0])
+    ql = np.asfortranarray(q[ien0])
+    rl, K, G = O.e3_block(yl, acl, xl, ql, p)
+    dres = np.zeros((n, 4))
+    Kd = np.zeros((9, n, n))
+    Pd = np.zeros((4, n, n))
+    for e in range(m.nel):
+        for a in range(4):
+            dres[ien0[e, a]] += rl[e, a]
+            for b in range(4):
+                Kd[:, ien0[e, a], ien0[e, b]] += K[e, :, a, b]
+                Pd[:, ien0[e, a], ien0[e, b]] += G[e, :, a, b]
+    np.testing.assert_allclose(res, dres, atol=1e-12 * np.abs(dres).max())
+    for i in range(n):
+        for k in range(colm[i] - 1, colm[i + 1] - 1):
+            j = rowp[k] - 1
+            np.testing.assert_allclose(lhsK[:, k], Kd[:, i, j], atol=1e-12 * np.abs(Kd).max())
+            np.testing.assert_allclose(lhsP[:, k], Pd[:, i, j], atol=1e-12 * np.abs(Pd).max())
+    assert np.count_nonzero(np.abs(Kd).sum(axis=0)) <= rowp.size
+
+
+def test_val4_packing_dense(O):
+    """solfar's Val4: rows 1-3/cols 1-3 = lhsK in stored order, (r,4) =
+    -lhsP(r) of the transposed block, row 4 = lhsP."""
+    rng = np.random.default_rng(8)
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(3, 3, 3)
+    colm, rowp = O.genadj(m.ien, m.numnp)
+    nnz = rowp.size
+    K = np.asfortranarray(rng.normal(size=(9, nnz)))
+    P = np.asfortranarray(rng.normal(size=(4, nnz)))
+    V = O.pack_val4(colm, rowp, K, P)
+    pos = {}
+    for i in range(m.numnp):
+        for k in range(colm[i] - 1, colm[i + 1] - 1):
+            pos[(i, rowp[k] - 1)] = k
+    for (i, j), k in pos.items():
+        blk = V[:, k].reshape(4, 4)
+        assert np.array_equal(blk[:3, :3].ravel(), K[:, k])
+        assert np.array_equal(blk[3], P[:, k])
+        assert np.array_equal(blk[:3, 3], -P[:3, pos[(j, i)]])
+
+
+def test_bc3res_codes(O):
+    n = 6
+    rng = np.random.default_rng(9)
+    res = np.asfortranarray(rng.normal(size=(n, 4)))
+    ibc = np.array([0, 56, 4, 60, 1 << 11, 8 | 16], np.int32)  # none, uvw, p, uvw+p, bit 11, partial(3)
+    bc = np.asfortranarray(rng.normal(size=(n, 7)))
+    out = O.bc3res(ibc, res, bc)
+    want = res.copy()
+    want[1, :3] = 0
+    want[2, 3] = 0
+    want[3, :] = 0
+    want[4, :3] = 0
+    want[5, 2] = res[5, 2] - bc[5, 3] * res[5, 0] - bc[5, 5] * res[5, 1]
+    want[5, :2] = 0
+    np.testing.assert_allclose(out, want, atol=1e-15)
+
+
+def _spd_system(n, rng):
+    A = rng.normal(size=(n, n))
+    return A @ A.T + n * np.eye(n)
+
+
+def test_memls_gmres_cg_match_direct_solve(O):
+    """GMRESV (LS_TYPE_GMRES) and CGRADV (LS_TYPE_CG) on a small block system
+    built from the formation, against numpy's direct solve of the same
+    diagonally scaled system (PRECOND, memLS.f90:1763-1943)."""
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(3, 3, 4)
+    n = m.numnp
+    colm, rowp = O.genadj(m.ien, n)
+    y, ac = M.poiseuille_state(m)
+    res, K, P = O.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, O.make_params())
+    val = O.pack_val4(colm, rowp, K, P)
+    A = np.zeros((4 * n, 4 * n))
+    for i in range(n):
+        for k in range(colm[i] - 1, colm[i + 1] - 1):
+            j = rowp[k] - 1
+            A[4 * i:4 * i + 4, 4 * j:4 * j + 4] = val[:, k].reshape(4, 4)
+    W = 1 / np.sqrt(np.abs(np.diag(A)))
+    R = np.asfortranarray(res.T)
+    xd = (W * np.linalg.solve(W[:, None] * A * W[None, :], W * R.T.ravel())).reshape(n, 4).T
+    ls = O.MemLS(colm, rowp, 0, O.LS_TYPE_GMRES, relTol=1e-12, dimKry=4 * n, maxItr=5)
+    x = ls.solve(R, val)
+    assert ls.stats(0)["suc"] == 1
+    np.testing.assert_allclose(x, xd, atol=1e-8 * np.abs(xd).max())
+    # CG on an SPD scalar system with the same sparsity
+    rng = np.random.default_rng(10)
+    S = np.zeros((n, n))
+    for i in range(n):
+        for k in range(colm[i] - 1, colm[i + 1] - 1):
+            S[i, rowp[k] - 1] = -rng.uniform(0.1, 1.0)
+    S = (S + S.T) / 2
+    np.fill_diagonal(S, np.abs(S).sum(axis=1) + 1.0)
+    vals = np.asfortranarray([[S[i, rowp[k] - 1] for i in range(n) for k in range(colm[i] - 1, colm[i + 1] - 1)]])
+    b = np.asfortranarray(rng.normal(size=(1, n)))
+    cg = O.MemLS(colm, rowp, 0, O.LS_TYPE_CG, relTol=1e-13, maxItr=500)
+    x = cg.solve(b, vals)
+    assert cg.stats(0)["suc"] == 1
+    np.testing.assert_allclose(x[0], np.linalg.solve(S, b[0]), rtol=1e-9, atol=1e-12)
+
+
+def test_memls_ns_solver_and_dirichlet_masking(O):
+    """NSSOLVER with a Dirichlet face: constrained DOFs return exactly zero,
+    the rest match the direct solution of the masked scaled system."""
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(4, 4, 6)
+    n = m.numnp
+    colm, rowp = O.genadj(m.ien, n)
+    y, ac = M.poiseuille_state(m)
+    res, K, P = O.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, O.make_params())
+    ibc, bc = M.dirichlet_codes(m)
+    res = O.bc3res(ibc, res, bc)
+    val = O.pack_val4(colm, rowp, K, P)
+    fixed = np.nonzero(ibc)[0]
+    # inner tolerances of 1e-6 drive iNorm^2 - xB.B below zero in rounding and
+    # fNorm to NaN (memLS.f90:1545 takes the square root unguarded)
+    ls = O.MemLS(colm, rowp, 1, O.LS_TYPE_NS, relTol=1e-6, dimKry=60, maxItr=10, relTolIn=[1e-3, 1e-3],
+                 maxItrIn=[10, 500])
+    ls.bc_create(1, fixed + 1, 3, O.BC_TYPE_Dir, np.zeros((3, fixed.size), order="F"))
+    x = ls.solve(np.asfortranarray(res.T), val)
+    assert ls.stats(0)["suc"] == 1
+    assert np.all(x[:3, fixed] == 0.0)
+    A = np.zeros((4 * n, 4 * n))
+    for i in range(n):
+        for k in range(colm[i] - 1, colm[i + 1] - 1):
+            A[4 * i:4 * i + 4, 4 * (rowp[k] - 1):4 * (rowp[k] - 1) + 4] = val[:, k].reshape(4, 4)
+    W = 1 / np.sqrt(np.abs(np.diag(A)))
+    for c in range(3):
+        W[4 * fixed + c] = 0
+    As = W[:, None] * A * W[None, :]
+    z = np.diag(As) == 0
+    As[z, z] = 1
+    xd = (W * np.linalg.solve(As, W * res.ravel())).reshape(n, 4).T
+    np.testing.assert_allclose(x, xd, atol=1e-5 * np.abs(xd).max())
+
+
+def test_flop_count_constant(O):
+    assert 6000 < O.flops_per_element() < 10000
