@@ -499,74 +499,106 @@ __global__ void __launch_bounds__(256) res_node(int nshg, const int *__restrict_
 // lane owning the diagonal block also sums the residual row.  Every block of
 // the row is written, so blocks no element touches come out zero (the
 // reference zeroes lhsK/lhsP first, elmgmr.f90:127-130).
+// Contribution of element incidence (record r, local nodes a, b) to the
+// block (ien(e,a), ien(e,b)): K^{ii,jj} += delta S_ab + Tc g_a,ii g_b,jj +
+// Mu g_a,jj g_b,ii, lhsP(1:3) += tlW sum_q N_a g_b, the transposed block's
+// lhsP(1:3) += tlW sum_q N_b g_a, C += Cm g_a.g_b (e3lhs.f90:171-402).
+struct BlockAcc {
+  double K[9], Gij[3], Gji[3], C;
+};
+__device__ inline void block_zero(BlockAcc &A) {
+#pragma unroll
+  for (int m = 0; m < 9; ++m) A.K[m] = 0.0;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) A.Gij[m] = A.Gji[m] = 0.0;
+  A.C = 0.0;
+}
+__device__ inline double sum_n_rt(int a) {
+  return a == 0 ? sum_n(0) : a == 1 ? sum_n(1) : a == 2 ? sum_n(2) : sum_n(3);
+}
+__device__ inline void block_add(BlockAcc &A, const double *__restrict__ r, int a, int b) {
+  const double ga[3] = {r[R_G + a * 3 + 0], r[R_G + a * 3 + 1], r[R_G + a * 3 + 2]};
+  const double gb[3] = {r[R_G + b * 3 + 0], r[R_G + b * 3 + 1], r[R_G + b * 3 + 2]};
+  const double Sab = r[R_A(a) + b], Tc = r[R_TC], Mu = r[R_MU], tl = r[R_TLW], Cm = r[R_CM];
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii)
+      A.K[ii + 3 * jj] += (ii == jj ? Sab : 0.0) + (Tc * ga[ii] * gb[jj] + Mu * ga[jj] * gb[ii]);
+  const double ta = tl * sum_n_rt(a), tb = tl * sum_n_rt(b);
+#pragma unroll
+  for (int ii = 0; ii < 3; ++ii) {
+    A.Gij[ii] += ta * gb[ii];
+    A.Gji[ii] += tb * ga[ii];
+  }
+  A.C += Cm * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+}
+
+// Row-parallel assembly, kRowLanes lanes per CSR row.
+// Phase A: lane s owns off-diagonal block s of the row and walks that block's
+// own contribution list (lst: (local incidence index << 2 | b), in element
+// order, built once from the sparseloc positions); this is fillsparseI's
+// accumulation (fillsparse.f90:29-63) for that block.  Phase B: the diagonal
+// block (every incidence, b = a) and the residual row (asigmr.f90:116) are
+// split over the row's lanes and combined by a fixed shuffle tree.  Every
+// block of the row is written, so blocks no element touches come out zero
+// (the reference zeroes lhsK/lhsP first, elmgmr.f90:127-130).
 template <int MODE>
 __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__ rowptr,
                                                const int *__restrict__ diagk, const int *__restrict__ inc_ptr,
-                                               const int *__restrict__ inc, const unsigned *__restrict__ slot,
+                                               const int *__restrict__ inc, const int *__restrict__ lst_ptr,
+                                               const unsigned short *__restrict__ lst,
                                                const double *__restrict__ rec, double *__restrict__ res,
                                                AsmOut o) {
   const int gid = xcd_block() * blockDim.x + threadIdx.x;
   const int row = gid / kRowLanes, lane = gid % kRowLanes;
-  if (row >= nshg) return;
-  const int k0 = rowptr[row], nslot = rowptr[row + 1] - k0, sdiag = diagk[row] - k0;
-  const int p0 = inc_ptr[row], p1 = inc_ptr[row + 1];
-  for (int s0 = 0; s0 < nslot; s0 += kRowLanes) {
-    const int s = s0 + lane;
-    double K[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Gij[3] = {0, 0, 0}, Gji[3] = {0, 0, 0}, C = 0.0;
-    double rs[4] = {0, 0, 0, 0};
-    for (int c0 = p0; c0 < p1; c0 += kRowLanes) {
-      // the row's next 16 incidences, one per lane, then broadcast in order
-      const int pc = c0 + lane;
-      const int my_ea = pc < p1 ? inc[pc] : 0;
-      const unsigned my_sm = pc < p1 ? slot[pc] : 0xffffffffu;
-      const int nt = min(kRowLanes, p1 - c0);
-      for (int t = 0; t < nt; ++t) {
-        const int ea = __shfl(my_ea, t, kRowLanes);
-        const unsigned sm = (unsigned)__shfl((int)my_sm, t, kRowLanes);
-        const int a = ea & 3;
-        // line 0 of the record, same address for all lanes of the row
-        const double *r = rec + (size_t)(ea >> 2) * kRec;
-        double gl[12];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) gl[k] = r[R_G + k];
-        const double tl = r[R_TLW], Tc = r[R_TC], Mu = r[R_MU], Cm = r[R_CM];
-        int b = -1;
-#pragma unroll
-        for (int bb = 0; bb < 4; ++bb)
-          if ((int)((sm >> (8 * bb)) & 255u) == s) b = bb;
-        if (b < 0) continue;
-        double ga[3], gb[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          ga[j] = a == 0 ? gl[j] : a == 1 ? gl[3 + j] : a == 2 ? gl[6 + j] : gl[9 + j];
-          gb[j] = b == 0 ? gl[j] : b == 1 ? gl[3 + j] : b == 2 ? gl[6 + j] : gl[9 + j];
-        }
-        const double Sab = r[R_A(a) + b];
-#pragma unroll
-        for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-          for (int ii = 0; ii < 3; ++ii)
-            // K^{ii,jj}_ab = delta S_ab + Tc g_a,ii g_b,jj + Mu g_a,jj g_b,ii
-            K[ii + 3 * jj] += (ii == jj ? Sab : 0.0) + (Tc * ga[ii] * gb[jj] + Mu * ga[jj] * gb[ii]);
-        const double sa = a == 0 ? sum_n(0) : a == 1 ? sum_n(1) : a == 2 ? sum_n(2) : sum_n(3);
-        const double sb = b == 0 ? sum_n(0) : b == 1 ? sum_n(1) : b == 2 ? sum_n(2) : sum_n(3);
-        const double ta = tl * sa, tb = tl * sb;
-#pragma unroll
-        for (int ii = 0; ii < 3; ++ii) {
-          Gij[ii] += ta * gb[ii];
-          Gji[ii] += tb * ga[ii];
-        }
-        C += Cm * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
-        if (b == a) {
-#pragma unroll
-          for (int k = 0; k < 4; ++k) rs[k] += r[R_A(a) + 4 + k];
-        }
-      }
+  const bool live = row < nshg;
+  int k0 = 0, nslot = 0, sdiag = -1, p0 = 0, p1 = 0;
+  if (live) {
+    k0 = rowptr[row];
+    nslot = rowptr[row + 1] - k0;
+    sdiag = diagk[row] - k0;
+    p0 = inc_ptr[row];
+    p1 = inc_ptr[row + 1];
+  }
+  // phase A: off-diagonal blocks
+  for (int s = lane; s < nslot; s += kRowLanes) {
+    if (s == sdiag) continue;
+    BlockAcc A;
+    block_zero(A);
+    const int q1 = lst_ptr[k0 + s + 1];
+    for (int q = lst_ptr[k0 + s]; q < q1; ++q) {
+      const unsigned v = lst[q];
+      const int ea = inc[p0 + (int)(v >> 2)];
+      block_add(A, rec + (size_t)(ea >> 2) * kRec, ea & 3, (int)(v & 3u));
     }
-    if (s < nslot) {
-      put_block<MODE>(o, k0 + s, K, Gij, C, Gji);
-      if (s == sdiag) put_res<MODE>(res, row, nshg, rs);
-    }
+    put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
+  }
+  // phase B: diagonal block and residual row, incidences strided over lanes
+  BlockAcc D;
+  block_zero(D);
+  double rs[4] = {0, 0, 0, 0};
+  for (int p = p0 + lane; p < p1; p += kRowLanes) {
+    const int ea = inc[p];
+    const int a = ea & 3;
+    const double *r = rec + (size_t)(ea >> 2) * kRec;
+    block_add(D, r, a, a);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rs[k] += r[R_A(a) + 4 + k];
+  }
+#pragma unroll
+  for (int off = kRowLanes / 2; off > 0; off >>= 1) {
+#pragma unroll
+    for (int m = 0; m < 9; ++m) D.K[m] += __shfl_xor(D.K[m], off, kRowLanes);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) D.Gij[m] += __shfl_xor(D.Gij[m], off, kRowLanes);
+    D.C += __shfl_xor(D.C, off, kRowLanes);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rs[k] += __shfl_xor(rs[k], off, kRowLanes);
+  }
+  if (live && lane == 0) {
+    put_block<MODE>(o, k0 + sdiag, D.K, D.Gij, D.C, D.Gij);
+    put_res<MODE>(res, row, nshg, rs);
   }
 }
 
@@ -595,7 +627,8 @@ struct crimson_form {
   crimson_form_params prm{};
   hipStream_t stream = nullptr;
   crimson::DBuf<int> d_ien, d_inc_ptr, d_inc, d_diagk, d_rowptr, d_ibc;
-  crimson::DBuf<unsigned> d_slot;
+  crimson::DBuf<int> d_lst_ptr;
+  crimson::DBuf<unsigned short> d_lst;
   crimson::DBuf<double> d_frec, d_q, d_rec;
   crimson::DBuf<double> s_y, s_ac, s_x, s_res, s_K, s_P, s_q;  // staging of the host API
   bool has_ibc = false;
@@ -685,11 +718,11 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
     else
       res_node<1><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
   } else if (mode == 0) {
-    asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p, f->d_slot.p,
-                                f->d_rec.p, d_res, o);
+    asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
+                                f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p, d_res, o);
   } else {
-    asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p, f->d_slot.p,
-                                f->d_rec.p, d_res, o);
+    asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
+                                f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p, d_res, o);
   }
   CR_HIP(hipGetLastError());
   CR_HIP(hipEventRecord(f->ev[3], s));
@@ -782,28 +815,43 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
         inc[fill[n]++] = e * 4 + a;
       }
   }
-  // slot map: for incidence p = (e,a) of row i, byte b = position of
-  // ien(e,b) in row i (the sparseloc search of fillsparse.f90:41, done once)
-  std::vector<unsigned> slot(inc.size());
+  // Contribution lists of the off-diagonal blocks: for block k of row i, the
+  // entries (t << 2 | b) of the row's incidences t = (e,a) whose node
+  // ien(e,b) is the block's column, in element order (sparseloc,
+  // fillsparse.f90:41, resolved once).
+  std::vector<int> lst_ptr(nnz + 1, 0);
   for (int i = 0; i < nshg; ++i) {
-    if (rp[i + 1] - rp[i] > 255) {
-      crimson::set_error("row %d has %d blocks; at most 255 supported", i + 1, rp[i + 1] - rp[i]);
+    if (inc_ptr[i + 1] - inc_ptr[i] >= (1 << 14)) {
+      crimson::set_error("node %d belongs to %d elements; at most 16383 supported", i + 1, inc_ptr[i + 1] - inc_ptr[i]);
       return CRIMSON_EUNSUPPORTED;
     }
     for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) {
-      const int e = inc[p] >> 2;
-      unsigned sm = 0;
+      const int e = inc[p] >> 2, a = inc[p] & 3;
       for (int b = 0; b < 4; ++b) {
+        if (b == a) continue;
         const int j = ien[(size_t)b * nel + e] - 1;
         const int k = find_col(rp, cols, i, j);
         if (k < 0) {
           crimson::set_error("element %d: pair (%d,%d) missing from the CSR graph", e + 1, i + 1, j + 1);
           return CRIMSON_EGRAPH;
         }
-        sm |= (unsigned)(k - rp[i]) << (8 * b);
+        lst_ptr[k + 1]++;
       }
-      slot[p] = sm;
     }
+  }
+  for (int k = 0; k < nnz; ++k) lst_ptr[k + 1] += lst_ptr[k];
+  std::vector<unsigned short> lst((size_t)lst_ptr[nnz]);
+  {
+    std::vector<int> fill(lst_ptr.begin(), lst_ptr.end() - 1);
+    for (int i = 0; i < nshg; ++i)
+      for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) {
+        const int e = inc[p] >> 2, a = inc[p] & 3;
+        for (int b = 0; b < 4; ++b) {
+          if (b == a) continue;
+          const int k = find_col(rp, cols, i, ien[(size_t)b * nel + e] - 1);
+          lst[fill[k]++] = (unsigned short)(((p - inc_ptr[i]) << 2) | b);
+        }
+      }
   }
   for (int e = 0; e < nel; ++e)
     for (int a = 0; a < 4; ++a)
@@ -842,7 +890,8 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
       (rc = f->d_inc.upload(inc.data(), inc.size(), s)) ||
       (rc = f->d_diagk.upload(diagk.data(), diagk.size(), s)) ||
       (rc = f->d_rowptr.upload(rp.data(), rp.size(), s)) ||
-      (rc = f->d_slot.upload(slot.data(), slot.size(), s)) ||
+      (rc = f->d_lst_ptr.upload(lst_ptr.data(), lst_ptr.size(), s)) ||
+      (rc = f->d_lst.upload(lst.data(), lst.size(), s)) ||
       (rc = f->d_frec.alloc((size_t)std::max(nel, 1) * kFRec)) ||
       (rc = f->d_q.alloc((size_t)nshg * 9)) || (rc = f->d_rec.alloc((size_t)std::max(nel, 1) * kRec)))
     return fail(rc);
