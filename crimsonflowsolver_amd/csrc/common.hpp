@@ -115,7 +115,9 @@ struct crimson_lhs {
   std::vector<int> h_rowptr, h_col, h_diag, h_tr;  // 0-based
   crimson::DBuf<int> d_rowptr, d_col, d_diag, d_tr;
   int vdof = 0;                   // block size of the stored matrix
+  int prec = 64;                  // 64: d_V holds the matrix; 32: d_V32 does
   crimson::DBuf<double> d_V;      // planes [vdof*vdof][nnz]
+  crimson::DBuf<float> d_V32;     // FP32 planes [vdof*vdof][nnz]
   bool gt_is_d = false;           // Gt == mD planes (formation-produced)
   crimson::DBuf<double> d_Gt;     // [3][nnz] when !gt_is_d
   std::vector<CrFace> face;

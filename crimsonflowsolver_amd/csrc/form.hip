@@ -401,21 +401,177 @@ __global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ 
   r[R_CM] = tlW * TM / rho;
 }
 
+// ---------------------------------------------------------------- FP32 tangent
+// The mixed-precision formation keeps the residual in FP64 (flux pass, e3
+// residual, res_node) and forms the lazy tangent in FP32 with FP32
+// accumulation.  The tangent needs no strong residual (tauBar does not enter
+// e3LHS and the tangent's tau depends on u and the metric only), so this
+// kernel evaluates the metric, tau at the 4 quadrature points and the element
+// record S/g/scalars alone.  Elements whose aspect ratio exceeds a threshold
+// are evaluated in FP64 and rounded to FP32 at the end (metric and tau on
+// skewed elements are where FP32 loses most).
+// FP32 record: 32 floats = one 128-byte line: g[a][j] at a*3+j, tlW, Tc, Mu,
+// Cm at 12..15, S[a][0..3] at 16 + 4a.
+constexpr int kRec32 = 32;
+__host__ __device__ constexpr int R32_S(int a) { return 16 + 4 * a; }
+
+// Longest edge over shortest altitude, normalised to 1 for the regular tet.
+__device__ inline float tet_aspect(const double (&xd)[4][3]) {
+  float x[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) x[a][i] = (float)xd[a][i];
+  auto sub = [&](int a, int b, float (&d)[3]) {
+    d[0] = x[a][0] - x[b][0];
+    d[1] = x[a][1] - x[b][1];
+    d[2] = x[a][2] - x[b][2];
+  };
+  auto cross2 = [](const float (&u)[3], const float (&v)[3]) {
+    const float c0 = u[1] * v[2] - u[2] * v[1], c1 = u[2] * v[0] - u[0] * v[2], c2 = u[0] * v[1] - u[1] * v[0];
+    return c0 * c0 + c1 * c1 + c2 * c2;
+  };
+  float e01[3], e02[3], e03[3], e12[3], e13[3];
+  sub(1, 0, e01);
+  sub(2, 0, e02);
+  sub(3, 0, e03);
+  sub(2, 1, e12);
+  sub(3, 1, e13);
+  float e23[3];
+  sub(3, 2, e23);
+  auto l2 = [](const float (&d)[3]) { return d[0] * d[0] + d[1] * d[1] + d[2] * d[2]; };
+  const float L2 = fmaxf(fmaxf(fmaxf(l2(e01), l2(e02)), fmaxf(l2(e03), l2(e12))), fmaxf(l2(e13), l2(e23)));
+  // squared doubled areas of the 4 faces
+  const float A2 = fmaxf(fmaxf(cross2(e01, e02), cross2(e01, e03)), fmaxf(cross2(e02, e03), cross2(e12, e13)));
+  float c[3];
+  c[0] = e02[1] * e03[2] - e02[2] * e03[1];
+  c[1] = e02[2] * e03[0] - e02[0] * e03[2];
+  c[2] = e02[0] * e03[1] - e02[1] * e03[0];
+  const float six_v = fabsf(e01[0] * c[0] + e01[1] * c[1] + e01[2] * c[2]);
+  // h_min = 3V / A_max = six_v / sqrt(A2);  aspect = L_max / h_min / sqrt(3/2)
+  if (!(six_v > 0.0f)) return 3.0e38f;
+  return sqrtf(L2) * sqrtf(A2) / six_v * 0.8164965809277260f;
+}
+
+template <class T>
+__device__ inline void tangent_body(const double (&xd)[4][3], const double (&ud)[4][3], const FormConst &c,
+                                    float *__restrict__ r) {
+  T xl[4][3], ul[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      xl[a][i] = (T)xd[a][i];
+      ul[a][i] = (T)ud[a][i];
+    }
+  T dx[3][3], W, g[4][3], gg[6];
+  p1_metric(xl, dx, W, g);
+  gijd_tet(dx, gg);
+  const T rho = (T)c.rho, mu = (T)c.mu, rhoinv = T(1) / rho;
+  const T rnu = mu * rhoinv;
+  const T dts = (T)(c.Dtgl * c.dtsfct);
+  const T GG = gg[0] * gg[0] + gg[1] * gg[1] + gg[2] * gg[2] + T(2) * (gg[3] * gg[3] + gg[4] * gg[4] + gg[5] * gg[5]);
+  const T tau0 = (T(2) * dts) * (T(2) * dts);
+  const T visc = T(36) * (rnu * rnu) * GG;
+  const T trg = gg[0] + gg[1] + gg[2];
+  const T ff = (T)(c.taucfct / c.dtsfct);
+  T TC = T(0), TM = T(0), SUPG[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) SUPG[k] = T(0);
+#pragma unroll
+  for (int qp = 0; qp < 4; ++qp) {
+    const T N0 = (T)nq(qp, 0), N1 = (T)nq(qp, 1), N2 = (T)nq(qp, 2), N3 = (T)nq(qp, 3);
+    T u[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) u[i] = N0 * ul[0][i] + N1 * ul[1][i] + N2 * ul[2][i] + N3 * ul[3][i];
+    const T uGu = u[0] * (gg[0] * u[0] + gg[3] * u[1] + gg[5] * u[2]) +
+                  u[1] * (gg[3] * u[0] + gg[1] * u[1] + gg[4] * u[2]) +
+                  u[2] * (gg[5] * u[0] + gg[4] * u[1] + gg[2] * u[2]);
+    const T fact = sqrt((tau0 + uGu) + visc);
+    const T tauC = rho * T(0.125) * fact / trg * ff;
+    const T tauM = T(1) / fact;
+    TC += tauC;
+    TM += tauM;
+    T z[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) z[a] = u[0] * g[a][0] + u[1] * g[a][1] + u[2] * g[a][2];
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = a; b < 4; ++b) SUPG[k++] += tauM * z[a] * z[b];
+  }
+  const T tlW = (T)c.lhsFct * W;
+  const T tsFct = (T)c.lhmFct * W * rho;
+  const T Mu = T(4) * (tlW * mu);
+  const T rt = rho * tlW;
+  T w[4][4];
+#pragma unroll
+  for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) w[cc][b] = ul[cc][0] * g[b][0] + ul[cc][1] * g[b][1] + ul[cc][2] * g[b][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const T conv = (T)mm(a, 0) * w[0][b] + (T)mm(a, 1) * w[1][b] + (T)mm(a, 2) * w[2][b] + (T)mm(a, 3) * w[3][b];
+      const int lo = a < b ? a : b, hi = a < b ? b : a;
+      const int k = lo * 4 - (lo * (lo - 1)) / 2 + (hi - lo);
+      const T gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
+      r[R32_S(a) + b] = (float)(tsFct * (T)mm(a, b) + rt * conv + Mu * gab + rt * SUPG[k]);
+    }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r[R_G + a * 3 + j] = (float)g[a][j];
+  r[R_TLW] = (float)tlW;
+  r[R_TC] = (float)(tlW * TC);
+  r[R_MU] = (float)Mu;
+  r[R_CM] = (float)(tlW * TM / rho);
+}
+
+__global__ void __launch_bounds__(256) e3_tangent32(int nel, const int *__restrict__ ien,
+                                                    const double *__restrict__ x, int numnp,
+                                                    const double *__restrict__ y, int nshg, FormConst c,
+                                                    float aspect_fp64, float *__restrict__ rec) {
+  const int e = xcd_block() * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  double xl[4][3], ul[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int n = ien[a * (size_t)nel + e];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      xl[a][i] = x[n + (size_t)numnp * i];
+      ul[a][i] = y[n + (size_t)nshg * i];
+    }
+  }
+  float r[kRec32];
+  if (tet_aspect(xl) > aspect_fp64)
+    tangent_body<double>(xl, ul, c, r);
+  else
+    tangent_body<float>(xl, ul, c, r);
+  float4 *o = reinterpret_cast<float4 *>(rec + (size_t)e * kRec32);
+#pragma unroll
+  for (int k = 0; k < kRec32 / 4; ++k) o[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+}
+
 // ---------------------------------------------------------------- assembly
 // Output modes: 0 = reference layout lhsK(9,nnz)/lhsP(4,nnz);
 // 1 = memLS planes V[16][nnz] (row-major 4x4 blocks of solfar's Val4).
 struct AsmOut {
   double *lhsK, *lhsP;  // mode 0
-  double *V;            // mode 1
+  double *V;            // mode 1, FP64 matrix
+  float *V32;           // mode 1, FP32 matrix
   size_t nnz;
 };
 
 // Write one block given K (PHASTA column-major order m = i + 3j), the
 // pressure-gradient column G(k) = lhsP(1:3) of this block, C = lhsP(4), and
 // the transposed block's lhsP(1:3) (for Val4's -G^T column).
-template <int MODE>
-__device__ inline void put_block(const AsmOut &o, int k, const double (&K)[9], const double (&Gk)[3],
-                                 double C, const double (&Gtr)[3]) {
+template <int MODE, class T>
+__device__ inline void put_block(const AsmOut &o, int k, const T (&K)[9], const T (&Gk)[3], T C,
+                                 const T (&Gtr)[3]) {
   if (MODE == 0) {
     double *pk = o.lhsK + (size_t)k * 9;
 #pragma unroll
@@ -430,7 +586,7 @@ __device__ inline void put_block(const AsmOut &o, int k, const double (&K)[9], c
     // Val4(9:11)=lhsK(7:9), Val4(13:15)=lhsP(1:3), Val4(16)=lhsP(4);
     // Val4(4:12:4) = -lhsP(1:3) of the transposed block (:225-235).
     const size_t S = o.nnz;
-    double *V = o.V + k;
+    T *V = (sizeof(T) == 4 ? (T *)o.V32 : (T *)o.V) + k;
     V[0 * S] = K[0];
     V[1 * S] = K[1];
     V[2 * S] = K[2];
@@ -491,41 +647,41 @@ __global__ void __launch_bounds__(256) res_node(int nshg, const int *__restrict_
   if (i < nshg && lane == 0) put_res<MODE>(res, i, nshg, rs);
 }
 
-// Row-parallel assembly: kRowLanes lanes per CSR row, lane s owns the row's
-// block s (and s + 16, ... for long rows).  All lanes of a row walk the row's
-// element incidences in element order (the accumulation order of
-// fillsparseI), so the record lines of an incidence are fetched once for the
-// row; slot[p] holds, byte b, the position in row i of node ien(e,b).  The
-// lane owning the diagonal block also sums the residual row.  Every block of
-// the row is written, so blocks no element touches come out zero (the
-// reference zeroes lhsK/lhsP first, elmgmr.f90:127-130).
-// Contribution of element incidence (record r, local nodes a, b) to the
-// block (ien(e,a), ien(e,b)): K^{ii,jj} += delta S_ab + Tc g_a,ii g_b,jj +
-// Mu g_a,jj g_b,ii, lhsP(1:3) += tlW sum_q N_a g_b, the transposed block's
-// lhsP(1:3) += tlW sum_q N_b g_a, C += Cm g_a.g_b (e3lhs.f90:171-402).
+// Row-parallel assembly.  Contribution of element incidence (staged record r,
+// local nodes a, b) to the block (ien(e,a), ien(e,b)): K^{ii,jj} += delta S_ab
+// + Tc g_a,ii g_b,jj + Mu g_a,jj g_b,ii, lhsP(1:3) += tlW sum_q N_a g_b, the
+// transposed block's lhsP(1:3) += tlW sum_q N_b g_a, C += Cm g_a.g_b
+// (e3lhs.f90:171-402).
+template <class T>
 struct BlockAcc {
-  double K[9], Gij[3], Gji[3], C;
+  T K[9], Gij[3], Gji[3], C;
 };
-__device__ inline void block_zero(BlockAcc &A) {
+template <class T>
+__device__ inline void block_zero(BlockAcc<T> &A) {
 #pragma unroll
-  for (int m = 0; m < 9; ++m) A.K[m] = 0.0;
+  for (int m = 0; m < 9; ++m) A.K[m] = T(0);
 #pragma unroll
-  for (int m = 0; m < 3; ++m) A.Gij[m] = A.Gji[m] = 0.0;
-  A.C = 0.0;
+  for (int m = 0; m < 3; ++m) A.Gij[m] = A.Gji[m] = T(0);
+  A.C = T(0);
 }
 __device__ inline double sum_n_rt(int a) {
   return a == 0 ? sum_n(0) : a == 1 ? sum_n(1) : a == 2 ? sum_n(2) : sum_n(3);
 }
-__device__ inline void block_add(BlockAcc &A, const double *__restrict__ r, int a, int b) {
-  const double ga[3] = {r[R_G + a * 3 + 0], r[R_G + a * 3 + 1], r[R_G + a * 3 + 2]};
-  const double gb[3] = {r[R_G + b * 3 + 0], r[R_G + b * 3 + 1], r[R_G + b * 3 + 2]};
-  const double Sab = r[R_A(a) + b], Tc = r[R_TC], Mu = r[R_MU], tl = r[R_TLW], Cm = r[R_CM];
+// Staged incidence: record line 0 (g, tlW, Tc, Mu, Cm) at 0..15 and node a's
+// S row at 16..19 (the residual row goes straight to the lane that sums it).
+constexpr int kStage = 20;
+constexpr int ST_S = 16;
+template <class T>
+__device__ inline void block_add(BlockAcc<T> &A, const T *r, int a, int b) {
+  const T ga[3] = {r[R_G + a * 3 + 0], r[R_G + a * 3 + 1], r[R_G + a * 3 + 2]};
+  const T gb[3] = {r[R_G + b * 3 + 0], r[R_G + b * 3 + 1], r[R_G + b * 3 + 2]};
+  const T Sab = r[ST_S + b], Tc = r[R_TC], Mu = r[R_MU], tl = r[R_TLW], Cm = r[R_CM];
 #pragma unroll
   for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
     for (int ii = 0; ii < 3; ++ii)
-      A.K[ii + 3 * jj] += (ii == jj ? Sab : 0.0) + (Tc * ga[ii] * gb[jj] + Mu * ga[jj] * gb[ii]);
-  const double ta = tl * sum_n_rt(a), tb = tl * sum_n_rt(b);
+      A.K[ii + 3 * jj] += (ii == jj ? Sab : T(0)) + (Tc * ga[ii] * gb[jj] + Mu * ga[jj] * gb[ii]);
+  const T ta = tl * T(sum_n_rt(a)), tb = tl * T(sum_n_rt(b));
 #pragma unroll
   for (int ii = 0; ii < 3; ++ii) {
     A.Gij[ii] += ta * gb[ii];
@@ -534,15 +690,56 @@ __device__ inline void block_add(BlockAcc &A, const double *__restrict__ r, int 
   A.C += Cm * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
 }
 
-// Row-parallel assembly, kRowLanes lanes per CSR row.
-// Phase A: lane s owns off-diagonal block s of the row and walks that block's
-// own contribution list (lst: (local incidence index << 2 | b), in element
-// order, built once from the sparseloc positions); this is fillsparseI's
-// accumulation (fillsparse.f90:29-63) for that block.  Phase B: the diagonal
-// block (every incidence, b = a) and the residual row (asigmr.f90:116) are
-// split over the row's lanes and combined by a fixed shuffle tree.  Every
-// block of the row is written, so blocks no element touches come out zero
-// (the reference zeroes lhsK/lhsP first, elmgmr.f90:127-130).
+// Diagonal contribution (b = a): the transposed block is the block itself,
+// so only K, lhsP(1:3) and C are accumulated.
+template <class T>
+struct DiagAcc {
+  T K[9], G[3], C;
+};
+template <class T>
+__device__ inline void diag_zero(DiagAcc<T> &A) {
+#pragma unroll
+  for (int m = 0; m < 9; ++m) A.K[m] = T(0);
+  A.G[0] = A.G[1] = A.G[2] = A.C = T(0);
+}
+template <class T>
+__device__ inline void diag_add(DiagAcc<T> &A, const T *r, int a) {
+  const T ga[3] = {r[R_G + a * 3 + 0], r[R_G + a * 3 + 1], r[R_G + a * 3 + 2]};
+  const T Saa = r[ST_S + a], Tc = r[R_TC], Mu = r[R_MU], tl = r[R_TLW], Cm = r[R_CM];
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii)
+      A.K[ii + 3 * jj] += (ii == jj ? Saa : T(0)) + (Tc * ga[ii] * ga[jj] + Mu * ga[jj] * ga[ii]);
+  const T ta = tl * T(sum_n_rt(a));
+#pragma unroll
+  for (int ii = 0; ii < 3; ++ii) A.G[ii] += ta * ga[ii];
+  A.C += Cm * (ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2]);
+}
+
+// Orders LDS accesses of the lanes of one wavefront (the lanes of a CSR row
+// always share a wavefront).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Row-parallel assembly, kRowLanes lanes per CSR row.  The row's element
+// incidences are taken in chunks of kRowLanes, in element order; each chunk's
+// record data (line 0 plus the row node's S and residual half line) is staged
+// in LDS with whole-line loads, 8 lanes per record line, so every record line
+// is read once per row.  Then:
+//  - lane s accumulates off-diagonal block s (and s + 16, ... for long rows)
+//    from its contribution list (lst: local incidence << 4 | a << 2 | b, in
+//    element order, built once from the sparseloc positions): fillsparseI's
+//    accumulation (fillsparse.f90:29-63) for that block;
+//  - lane t adds incidence t's diagonal contribution (b = a) and residual row
+//    (asigmr.f90:116); the lanes' partial sums are combined by a fixed
+//    shuffle tree.
+// Every block of the row is written, so blocks no element touches come out
+// zero (the reference zeroes lhsK/lhsP first, elmgmr.f90:127-130).
+constexpr int kRowsPerBlock = 256 / kRowLanes;
 template <int MODE>
 __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__ rowptr,
                                                const int *__restrict__ diagk, const int *__restrict__ inc_ptr,
@@ -550,56 +747,175 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
                                                const unsigned short *__restrict__ lst,
                                                const double *__restrict__ rec, double *__restrict__ res,
                                                AsmOut o) {
+  __shared__ double st_all[kRowsPerBlock][kRowLanes][kStage];
   const int gid = xcd_block() * blockDim.x + threadIdx.x;
   const int row = gid / kRowLanes, lane = gid % kRowLanes;
+  double(*st)[kStage] = st_all[threadIdx.x / kRowLanes];
   const bool live = row < nshg;
-  int k0 = 0, nslot = 0, sdiag = -1, p0 = 0, p1 = 0;
+  int k0 = 0, nslot = 0, sdiag = -1, p0 = 0, ninc = 0;
   if (live) {
     k0 = rowptr[row];
     nslot = rowptr[row + 1] - k0;
     sdiag = diagk[row] - k0;
     p0 = inc_ptr[row];
-    p1 = inc_ptr[row + 1];
+    ninc = inc_ptr[row + 1] - p0;
   }
-  // phase A: off-diagonal blocks
-  for (int s = lane; s < nslot; s += kRowLanes) {
-    if (s == sdiag) continue;
-    BlockAcc A;
-    block_zero(A);
-    const int q1 = lst_ptr[k0 + s + 1];
-    for (int q = lst_ptr[k0 + s]; q < q1; ++q) {
-      const unsigned v = lst[q];
-      const int ea = inc[p0 + (int)(v >> 2)];
-      block_add(A, rec + (size_t)(ea >> 2) * kRec, ea & 3, (int)(v & 3u));
-    }
-    put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
-  }
-  // phase B: diagonal block and residual row, incidences strided over lanes
-  BlockAcc D;
-  block_zero(D);
+  const int nchunk = (ninc + kRowLanes - 1) / kRowLanes;
+  const int ngroup = (nslot + kRowLanes - 1) / kRowLanes;
+  DiagAcc<double> D;
+  diag_zero(D);
   double rs[4] = {0, 0, 0, 0};
-  for (int p = p0 + lane; p < p1; p += kRowLanes) {
-    const int ea = inc[p];
-    const int a = ea & 3;
-    const double *r = rec + (size_t)(ea >> 2) * kRec;
-    block_add(D, r, a, a);
+  for (int g = 0; g < ngroup; ++g) {
+    const int s = lane + g * kRowLanes;
+    const bool own = s < nslot && s != sdiag;
+    BlockAcc<double> A;
+    block_zero(A);
+    int q = own ? lst_ptr[k0 + s] : 0;
+    const int q1 = own ? lst_ptr[k0 + s + 1] : 0;
+    for (int c = 0; c < nchunk; ++c) {
+      const int tb = c * kRowLanes;
+      const int nt = min(kRowLanes, ninc - tb);
+      const int ea = lane < nt ? inc[p0 + tb + lane] : 0;
+      wave_lds_sync();  // the previous chunk's readers are done
 #pragma unroll
-    for (int k = 0; k < 4; ++k) rs[k] += r[R_A(a) + 4 + k];
+      for (int j = 0; j < 8; ++j) {  // line 0: 8 pieces of 16 bytes per record
+        const int piece = j * kRowLanes + lane;
+        const int t = piece >> 3, part = piece & 7;
+        const int et = __shfl(ea, t, kRowLanes);
+        if (t < nt) {
+          const double2 v = *reinterpret_cast<const double2 *>(rec + (size_t)(et >> 2) * kRec + 2 * part);
+          st[t][2 * part] = v.x;
+          st[t][2 * part + 1] = v.y;
+        }
+      }
+      // node a's S row (lanes 0..7 of each pair of incidences) and, for the
+      // incidence's own lane, its residual row
+      double2 rl01 = make_double2(0.0, 0.0), rl23 = rl01;
+      if (lane < nt) {
+        const double *ra = rec + (size_t)(ea >> 2) * kRec + R_A(ea & 3);
+        const double2 s01 = *reinterpret_cast<const double2 *>(ra);
+        const double2 s23 = *reinterpret_cast<const double2 *>(ra + 2);
+        if (g == 0) {
+          rl01 = *reinterpret_cast<const double2 *>(ra + 4);
+          rl23 = *reinterpret_cast<const double2 *>(ra + 6);
+        }
+        st[lane][ST_S + 0] = s01.x;
+        st[lane][ST_S + 1] = s01.y;
+        st[lane][ST_S + 2] = s23.x;
+        st[lane][ST_S + 3] = s23.y;
+      }
+      wave_lds_sync();
+      if (g == 0 && lane < nt) {
+        const int a = ea & 3;
+        diag_add(D, st[lane], a);
+        rs[0] += rl01.x;
+        rs[1] += rl01.y;
+        rs[2] += rl23.x;
+        rs[3] += rl23.y;
+      }
+      for (; q < q1; ++q) {
+        const unsigned v = lst[q];
+        const int t = (int)(v >> 4) - tb;
+        if (t >= kRowLanes) break;
+        block_add(A, st[t], (int)((v >> 2) & 3u), (int)(v & 3u));
+      }
+    }
+    if (own) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
   }
 #pragma unroll
   for (int off = kRowLanes / 2; off > 0; off >>= 1) {
 #pragma unroll
     for (int m = 0; m < 9; ++m) D.K[m] += __shfl_xor(D.K[m], off, kRowLanes);
 #pragma unroll
-    for (int m = 0; m < 3; ++m) D.Gij[m] += __shfl_xor(D.Gij[m], off, kRowLanes);
+    for (int m = 0; m < 3; ++m) D.G[m] += __shfl_xor(D.G[m], off, kRowLanes);
     D.C += __shfl_xor(D.C, off, kRowLanes);
 #pragma unroll
     for (int k = 0; k < 4; ++k) rs[k] += __shfl_xor(rs[k], off, kRowLanes);
   }
   if (live && lane == 0) {
-    put_block<MODE>(o, k0 + sdiag, D.K, D.Gij, D.C, D.Gij);
+    put_block<MODE>(o, k0 + sdiag, D.K, D.G, D.C, D.G);
     put_res<MODE>(res, row, nshg, rs);
   }
+}
+
+// FP32 tangent assembly (no residual rows): the structure of asm_row with
+// FP32 records (one line per element: g, scalars, S), FP32 accumulation and
+// FP32 planes (MODE 1) or FP64 lhsK/lhsP (MODE 0).
+template <int MODE>
+__global__ void __launch_bounds__(256) asm_row32(int nshg, const int *__restrict__ rowptr,
+                                                 const int *__restrict__ diagk, const int *__restrict__ inc_ptr,
+                                                 const int *__restrict__ inc, const int *__restrict__ lst_ptr,
+                                                 const unsigned short *__restrict__ lst,
+                                                 const float *__restrict__ rec, AsmOut o) {
+  __shared__ float st_all[kRowsPerBlock][kRowLanes][kStage];
+  const int gid = xcd_block() * blockDim.x + threadIdx.x;
+  const int row = gid / kRowLanes, lane = gid % kRowLanes;
+  float(*st)[kStage] = st_all[threadIdx.x / kRowLanes];
+  const bool live = row < nshg;
+  int k0 = 0, nslot = 0, sdiag = -1, p0 = 0, ninc = 0;
+  if (live) {
+    k0 = rowptr[row];
+    nslot = rowptr[row + 1] - k0;
+    sdiag = diagk[row] - k0;
+    p0 = inc_ptr[row];
+    ninc = inc_ptr[row + 1] - p0;
+  }
+  const int nchunk = (ninc + kRowLanes - 1) / kRowLanes;
+  const int ngroup = (nslot + kRowLanes - 1) / kRowLanes;
+  DiagAcc<float> D;
+  diag_zero(D);
+  for (int g = 0; g < ngroup; ++g) {
+    const int s = lane + g * kRowLanes;
+    const bool own = s < nslot && s != sdiag;
+    BlockAcc<float> A;
+    block_zero(A);
+    int q = own ? lst_ptr[k0 + s] : 0;
+    const int q1 = own ? lst_ptr[k0 + s + 1] : 0;
+    for (int c = 0; c < nchunk; ++c) {
+      const int tb = c * kRowLanes;
+      const int nt = min(kRowLanes, ninc - tb);
+      const int ea = lane < nt ? inc[p0 + tb + lane] : 0;
+      wave_lds_sync();
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {  // line 0 of the record: 4 pieces of 16 bytes
+        const int piece = j * kRowLanes + lane;
+        const int t = piece >> 2, part = piece & 3;
+        const int et = __shfl(ea, t, kRowLanes);
+        if (t < nt) {
+          const float4 v = *reinterpret_cast<const float4 *>(rec + (size_t)(et >> 2) * kRec32 + 4 * part);
+          st[t][4 * part + 0] = v.x;
+          st[t][4 * part + 1] = v.y;
+          st[t][4 * part + 2] = v.z;
+          st[t][4 * part + 3] = v.w;
+        }
+      }
+      if (lane < nt) {
+        const float4 v = *reinterpret_cast<const float4 *>(rec + (size_t)(ea >> 2) * kRec32 + R32_S(ea & 3));
+        st[lane][ST_S + 0] = v.x;
+        st[lane][ST_S + 1] = v.y;
+        st[lane][ST_S + 2] = v.z;
+        st[lane][ST_S + 3] = v.w;
+      }
+      wave_lds_sync();
+      if (g == 0 && lane < nt) diag_add(D, st[lane], ea & 3);
+      for (; q < q1; ++q) {
+        const unsigned v = lst[q];
+        const int t = (int)(v >> 4) - tb;
+        if (t >= kRowLanes) break;
+        block_add(A, st[t], (int)((v >> 2) & 3u), (int)(v & 3u));
+      }
+    }
+    if (own) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
+  }
+#pragma unroll
+  for (int off = kRowLanes / 2; off > 0; off >>= 1) {
+#pragma unroll
+    for (int m = 0; m < 9; ++m) D.K[m] += __shfl_xor(D.K[m], off, kRowLanes);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) D.G[m] += __shfl_xor(D.G[m], off, kRowLanes);
+    D.C += __shfl_xor(D.C, off, kRowLanes);
+  }
+  if (live && lane == 0) put_block<MODE>(o, k0 + sdiag, D.K, D.G, D.C, D.G);
 }
 
 // bc3Res for codes 0/7 on the velocity bits, bit 2 (pressure) and bit 11
@@ -630,6 +946,7 @@ struct crimson_form {
   crimson::DBuf<int> d_lst_ptr;
   crimson::DBuf<unsigned short> d_lst;
   crimson::DBuf<double> d_frec, d_q, d_rec;
+  crimson::DBuf<float> d_rec32;  // FP32 tangent records (precision 32)
   crimson::DBuf<double> s_y, s_ac, s_x, s_res, s_K, s_P, s_q;  // staging of the host API
   bool has_ibc = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -651,9 +968,14 @@ int validate_params(const crimson_form_params *p) {
         p->itau, p->iconvflow, p->idiff, p->ipord, p->flmpl, p->flmpr);
     return CRIMSON_EUNSUPPORTED;
   }
-  if (p->precision != 64) {
-    crimson::set_error("formation precision %d not available in this build (64 only)", p->precision);
+  if (p->precision != 64 && p->precision != 32) {
+    crimson::set_error("formation precision %d not available (64: FP64; 32: FP64 residual, FP32 tangent)",
+                       p->precision);
     return CRIMSON_EUNSUPPORTED;
+  }
+  if (p->precision == 32 && !(p->aspect_fp64 >= 1.0)) {
+    crimson::set_error("aspect_fp64 = %g: the FP64 tau threshold must be >= 1", p->aspect_fp64);
+    return CRIMSON_EARG;
   }
   if (!(p->rho > 0) || !(p->mu >= 0) || !(p->Delt > 0) || !(p->dtsfct > 0)) {
     crimson::set_error("non-physical parameters rho=%g mu=%g Delt=%g dtsfct=%g", p->rho, p->mu, p->Delt,
@@ -700,19 +1022,34 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
   CR_HIP(hipGetLastError());
   CR_HIP(hipEventRecord(f->ev[1], s));
   const double *dq = d_qout ? d_qout : f->d_q.p;
+  // mixed precision: FP64 residual (e3 without tangent, res_node), FP32 tangent
+  const bool t32 = lhs && f->prm.precision == 32;
   if (f->nel > 0) {
-    if (lhs)
+    if (lhs && !t32)
       e3_elem<true><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, d_ac,
                                                                 dq, f->nshg, c, f->d_rec.p);
     else
       e3_elem<false><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, d_ac,
                                                                  dq, f->nshg, c, f->d_rec.p);
+    if (t32)
+      e3_tangent32<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg, c,
+                                                              (float)f->prm.aspect_fp64, f->d_rec32.p);
   }
   CR_HIP(hipGetLastError());
   CR_HIP(hipEventRecord(f->ev[2], s));
   const unsigned gn = crimson::grid_for((size_t)f->nshg * kNodeLanes, T);
   const unsigned gr = crimson::grid_for((size_t)f->nshg * kRowLanes, T);
-  if (!lhs) {
+  if (t32) {
+    if (mode == 0) {
+      res_node<0><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
+      asm_row32<0><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
+                                    f->d_lst_ptr.p, f->d_lst.p, f->d_rec32.p, o);
+    } else {
+      res_node<1><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
+      asm_row32<1><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
+                                    f->d_lst_ptr.p, f->d_lst.p, f->d_rec32.p, o);
+    }
+  } else if (!lhs) {
     if (mode == 0)
       res_node<0><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
     else
@@ -816,13 +1153,13 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
       }
   }
   // Contribution lists of the off-diagonal blocks: for block k of row i, the
-  // entries (t << 2 | b) of the row's incidences t = (e,a) whose node
+  // entries (t << 4 | a << 2 | b) of the row's incidences t = (e,a) whose node
   // ien(e,b) is the block's column, in element order (sparseloc,
   // fillsparse.f90:41, resolved once).
   std::vector<int> lst_ptr(nnz + 1, 0);
   for (int i = 0; i < nshg; ++i) {
-    if (inc_ptr[i + 1] - inc_ptr[i] >= (1 << 14)) {
-      crimson::set_error("node %d belongs to %d elements; at most 16383 supported", i + 1, inc_ptr[i + 1] - inc_ptr[i]);
+    if (inc_ptr[i + 1] - inc_ptr[i] >= (1 << 12)) {
+      crimson::set_error("node %d belongs to %d elements; at most 4095 supported", i + 1, inc_ptr[i + 1] - inc_ptr[i]);
       return CRIMSON_EUNSUPPORTED;
     }
     for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) {
@@ -849,7 +1186,7 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
         for (int b = 0; b < 4; ++b) {
           if (b == a) continue;
           const int k = find_col(rp, cols, i, ien[(size_t)b * nel + e] - 1);
-          lst[fill[k]++] = (unsigned short)(((p - inc_ptr[i]) << 2) | b);
+          lst[fill[k]++] = (unsigned short)(((p - inc_ptr[i]) << 4) | (a << 2) | b);
         }
       }
   }
@@ -895,6 +1232,7 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
       (rc = f->d_frec.alloc((size_t)std::max(nel, 1) * kFRec)) ||
       (rc = f->d_q.alloc((size_t)nshg * 9)) || (rc = f->d_rec.alloc((size_t)std::max(nel, 1) * kRec)))
     return fail(rc);
+  if (prm->precision == 32 && (rc = f->d_rec32.alloc((size_t)std::max(nel, 1) * kRec32))) return fail(rc);
   if (iBC) {
     f->has_ibc = true;
     if ((rc = f->d_ibc.upload(iBC, nshg, s))) return fail(rc);
@@ -923,7 +1261,7 @@ extern "C" int crimson_form_interior_dev(crimson_form *f, const double *d_y, con
     crimson::set_error("crimson_form_interior_dev: null argument");
     return CRIMSON_EARG;
   }
-  AsmOut o{d_lhsK, d_lhsP, nullptr, (size_t)f->nnz};
+  AsmOut o{d_lhsK, d_lhsP, nullptr, nullptr, (size_t)f->nnz};
   return run_formation(f, d_y, d_ac, d_x, d_res, lhs ? 1 : 0, 0, o, d_qres);
 }
 
@@ -943,7 +1281,7 @@ extern "C" int crimson_form_interior(crimson_form *f, const double *y, const dou
     CR_TRY(f->s_K.alloc((size_t)f->nnz * 9));
     CR_TRY(f->s_P.alloc((size_t)f->nnz * 4));
   }
-  AsmOut o{f->s_K.p, f->s_P.p, nullptr, (size_t)f->nnz};
+  AsmOut o{f->s_K.p, f->s_P.p, nullptr, nullptr, (size_t)f->nnz};
   CR_TRY(run_formation(f, f->s_y.p, f->s_ac.p, f->s_x.p, f->s_res.p, lhs ? 1 : 0, 0, o, nullptr));
   CR_HIP(hipMemcpyAsync(res, f->s_res.p, N * 4 * sizeof(double), hipMemcpyDeviceToHost, s));
   if (lhs) {
@@ -966,10 +1304,17 @@ extern "C" int crimson_form_interior_to_lhs(crimson_form *f, const double *d_y, 
                        lhs->nnz, f->nshg, f->nnz);
     return CRIMSON_EARG;
   }
-  CR_TRY(lhs->d_V.alloc((size_t)16 * f->nnz));
   lhs->vdof = 4;
   lhs->gt_is_d = true;  // Gt = -mG^T = mD for the solfar packing
-  AsmOut o{nullptr, nullptr, lhs->d_V.p, (size_t)f->nnz};
+  if (f->prm.precision == 32) {
+    CR_TRY(lhs->d_V32.alloc((size_t)16 * f->nnz));
+    lhs->prec = 32;
+    AsmOut o{nullptr, nullptr, nullptr, lhs->d_V32.p, (size_t)f->nnz};
+    return run_formation(f, d_y, d_ac, d_x, d_res, 1, 1, o, nullptr);
+  }
+  CR_TRY(lhs->d_V.alloc((size_t)16 * f->nnz));
+  lhs->prec = 64;
+  AsmOut o{nullptr, nullptr, lhs->d_V.p, nullptr, (size_t)f->nnz};
   return run_formation(f, d_y, d_ac, d_x, d_res, 1, 1, o, nullptr);
 }
 
