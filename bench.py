@@ -44,6 +44,26 @@ def build_problem(grid, cfa, M):
     return m, colm, rowp, y, ac, ibc, bc
 
 
+# flops per element of the reference formulation (e3 per quadrature point,
+# the flux pass and the scatter), the oracle's hand count
+# (crimson_oracle.c: cfo_flops_per_element; pinned by the oracle tests)
+FLOPS_PER_ELEMENT = 7038.0
+
+
+def pmc_traffic(grid, kernel):
+    """HBM bytes per dispatch of ``kernel`` from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated against a
+    plain device copy), or None when no summary exists for it."""
+    path = os.path.join(ROOT, "profiles", f"formation_{grid[0]}x{grid[1]}x{grid[2]}_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_dispatch" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def inflow_mean(t, seed=7, period=1.0):
     """Mean inflow velocity: 10 cm/s plus 3 seeded harmonics (+/- 30 %)."""
     from crimsonflowsolver_amd.mesh import uniform_pm1
@@ -83,6 +103,7 @@ def main():
     ap.add_argument("--timesteps", type=int, default=1, help="FP64 timesteps to time (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0: skip)")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision timesteps (0: skip)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MB first (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -116,6 +137,14 @@ def main():
     cfa.memLS_BC_CREATE(lhs, 1, fixed.size, 3, cfa.BC_TYPE_Dir, fixed + 1, np.zeros((3, fixed.size), order="F"))
     cfa.memLS_BC_CREATE(lhs, 2, out.size, 3, cfa.BC_TYPE_Neu, out + 1, np.asfortranarray(nabi[out].T))
 
+    form32 = lhs32 = None
+    if args.mixed:
+        form32 = cfa.ElmGMRInterior(m.ien, colm, rowp, N, iBC=ibc,
+                                    params=cfa.form_params(precision=32, aspect_fp64=20.0))
+        lhs32 = cfa.memLS_LHS_CREATE(N, N, nnz, None, colm, rowp, 2)
+        cfa.memLS_BC_CREATE(lhs32, 1, fixed.size, 3, cfa.BC_TYPE_Dir, fixed + 1, np.zeros((3, fixed.size), order="F"))
+        cfa.memLS_BC_CREATE(lhs32, 2, out.size, 3, cfa.BC_TYPE_Neu, out + 1, np.asfortranarray(nabi[out].T))
+
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)  # (n,k) Fortran -> (k,n) C
     dy, dac, dx = T(y), T(ac), T(m.x)
     dres4 = torch.zeros((N, 4), dtype=torch.float64, device=dev)       # Res4(4,N) Fortran
@@ -147,6 +176,25 @@ def main():
     log(f"formation: {ms_step:.3f} ms/step wall, events flux {tm[0]:.3f} e3 {tm[1]:.3f} asm {tm[2]:.3f} "
         f"total {tm[3]:.3f} ms -> {melems:.1f} Melem/s")
 
+    mixed_form = None
+    if args.mixed:
+        for _ in range(args.warmup):
+            form32.to_lhs(dy, dac, dx, dres4, lhs32)
+        torch.cuda.synchronize()
+        tm32 = np.zeros(4)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            form32.to_lhs(dy, dac, dx, dres4, lhs32)
+            tt = form32.timing()
+            tm32 += [tt["flux_ms"], tt["e3_ms"], tt["assembly_ms"], tt["total_ms"]]
+        torch.cuda.synchronize()
+        ms32 = (time.perf_counter() - t0) / args.steps * 1e3
+        tm32 /= args.steps
+        mixed_form = {"melem_s": nel / (ms32 * 1e-3) / 1e6, "ms_per_step": ms32,
+                      "phase_ms": {"flux": tm32[0], "e3_residual_and_fp32_tangent": tm32[1],
+                                   "res_rows_and_fp32_assembly": tm32[2]}}
+        log(f"mixed formation: {ms32:.3f} ms/step -> {mixed_form['melem_s']:.1f} Melem/s, phases {tm32}")
+
     # ---- Krylov SpMV (scaled, as inside the solver)
     x3 = torch.rand((N, 3), dtype=torch.float64, device=dev)
     y3 = torch.empty_like(x3)
@@ -158,71 +206,99 @@ def main():
     b4 = nnz * (128 + 4) + N * (8 + 32 + 32)
     gbs3, gbs4 = b3 / (ms3 * 1e-3) / 1e9, b4 / (ms4 * 1e-3) / 1e9
     log(f"spmv 3x3: {ms3:.4f} ms {gbs3:.0f} GB/s ; 4x4: {ms4:.4f} ms {gbs4:.0f} GB/s")
+    spmv32 = None
+    if args.mixed:
+        m3 = lhs32.spmv_dev(3, x3, y3, max(1, args.spmv_reps))
+        m4 = lhs32.spmv_dev(4, x4, y4, max(1, args.spmv_reps))
+        c3 = nnz * (36 + 4) + N * (4 + 12 + 12)
+        c4 = nnz * (64 + 4) + N * (4 + 16 + 16)
+        spmv32 = {"bsr3x3": c3 / (m3 * 1e-3) / 1e9, "bsr4x4": c4 / (m4 * 1e-3) / 1e9, "ms3": m3, "ms4": m4}
+        log(f"fp32 spmv 3x3: {m3:.4f} ms {spmv32['bsr3x3']:.0f} GB/s ; 4x4: {m4:.4f} ms {spmv32['bsr4x4']:.0f} GB/s")
 
-    # ---- FP64 timesteps (10 x 3 nonlinear iterations in the workload; timed here: args.timesteps)
+    # ---- timesteps: FP64, then the mixed form from the same initial state
     ts_ms = None
     ts_info = {}
     if args.timesteps > 0:
-        ls = cfa.memLS_LS_CREATE(cfa.LS_TYPE_NS, relTol=1e-8, dimKry=100, maxItr=10,
-                                 relTolIn=[0.01, 0.2], maxItrIn=[1, 200])
         R = 1333.0
         faceres = np.array([0.0, R * prm.alfi * prm.gami * prm.Delt])
         dibc = torch.from_numpy(ibc.astype(np.int32)).to(dev)
-        BCh = np.zeros((N, 7), order="F")
         rr = (m.x[:, 0] ** 2 + m.x[:, 1] ** 2)
         inl = m.inlet & ~m.wall
-        dBC = T(BCh)  # (7,N) C == BC(N,7) Fortran
         dglob = torch.from_numpy(out.astype(np.int32)).to(dev)
         dnabi = torch.from_numpy(np.ascontiguousarray(nabi[out])).to(dev)   # (nf,3) C == (3,nf) F
-        dq = torch.zeros(1, dtype=torch.float64, device=dev)
-        yold, acold = dy.clone(), dac.clone()
-        ya, aca = torch.empty_like(dy), torch.empty_like(dac)
-        step_times, its = [], []
-        for n in range(args.timesteps):
-            t = (n + 1) * prm.Delt
-            prof = np.zeros(N)
-            prof[inl] = 2.0 * inflow_mean(t) * np.clip(1.0 - rr[inl], 0.0, None)
-            dBC[4] = torch.from_numpy(prof).to(dev)   # BC(:,5) = u_z
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            form.predict(yold, acold, dy, dac)
-            form.itr_bc(dibc, dBC, dy)
-            for it in range(3):
-                form.yalpha(yold, acold, dy, dac, ya, aca)
-                form.sync()
-                form.to_lhs(ya, aca, dx, dres4, lhs)
-                form.bc3res(dres4, res4_layout=True)
-                form.elmpvsq(dglob, dnabi, R, ya, dres4, dq)
-                form.sync()
-                cfa.memLS_SOLVE_dev(lhs, ls, 4, dres4, res=faceres)
-                its.append((ls.RI["itr"], ls.GM["itr"], ls.CG["itr"], ls.RI["fNorm"] / ls.RI["iNorm"]))
-                form.correct(dy, dac, dres4)
-                form.itr_bc(dibc, dBC, dy)
-                form.sync()
-            yold.copy_(dy)
-            acold.copy_(dac)
-            torch.cuda.synchronize()
-            step_times.append(time.perf_counter() - t0)
-            log(f"timestep {n + 1}: {step_times[-1] * 1e3:.1f} ms, solver (RI,GM,CG itr, rel) {its[-3:]}")
-        ts_ms = float(np.mean(step_times) * 1e3)
+        y0, ac0 = dy.clone(), dac.clone()
+
+        def run_steps(fm, lh, label):
+            ls = cfa.memLS_LS_CREATE(cfa.LS_TYPE_NS, relTol=1e-8, dimKry=100, maxItr=10,
+                                     relTolIn=[0.01, 0.2], maxItrIn=[1, 200])
+            dBC = T(np.zeros((N, 7), order="F"))  # (7,N) C == BC(N,7) Fortran
+            dq = torch.zeros(1, dtype=torch.float64, device=dev)
+            yc, acc = y0.clone(), ac0.clone()
+            yold, acold = y0.clone(), ac0.clone()
+            ya, aca = torch.empty_like(yc), torch.empty_like(acc)
+            times, its = [], []
+            for n in range(args.timesteps):
+                t = (n + 1) * prm.Delt
+                prof = np.zeros(N)
+                prof[inl] = 2.0 * inflow_mean(t) * np.clip(1.0 - rr[inl], 0.0, None)
+                dBC[4] = torch.from_numpy(prof).to(dev)   # BC(:,5) = u_z
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                fm.predict(yold, acold, yc, acc)
+                fm.itr_bc(dibc, dBC, yc)
+                for it in range(3):
+                    fm.yalpha(yold, acold, yc, acc, ya, aca)
+                    fm.sync()
+                    fm.to_lhs(ya, aca, dx, dres4, lh)
+                    fm.bc3res(dres4, res4_layout=True)
+                    fm.elmpvsq(dglob, dnabi, R, ya, dres4, dq)
+                    fm.sync()
+                    cfa.memLS_SOLVE_dev(lh, ls, 4, dres4, res=faceres)
+                    its.append((ls.RI["itr"], ls.GM["itr"], ls.CG["itr"], ls.RI["fNorm"] / ls.RI["iNorm"]))
+                    fm.correct(yc, acc, dres4)
+                    fm.itr_bc(dibc, dBC, yc)
+                    fm.sync()
+                yold.copy_(yc)
+                acold.copy_(acc)
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t0)
+                log(f"{label} timestep {n + 1}: {times[-1] * 1e3:.1f} ms, solver (RI,GM,CG itr, rel) {its[-3:]}")
+            return times, its, yc
+
+        st64, its64, y64 = run_steps(form, lhs, "fp64")
+        ts_ms = float(np.mean(st64) * 1e3)
         ts_info = {"timesteps_timed": args.timesteps, "nonlinear_iterations": 3,
-                   "solver_iterations": [list(map(float, v)) for v in its]}
+                   "solver_iterations": [list(map(float, v)) for v in its64]}
+        if args.mixed:
+            st32, its32, y32 = run_steps(form32, lhs32, "mixed")
+            d = (y32 - y64).abs()
+            err = {"u": float(d[:, :3].max() / y64[:, :3].abs().max()),
+                   "p": float(d[:, 3].max() / y64[:, 3].abs().max())}
+            ms32 = float(np.mean(st32) * 1e3)
+            ts_info.update({"timestep_ms_mixed": ms32, "mixed_speedup": ts_ms / ms32,
+                            "mixed_rel_err_vs_fp64": err,
+                            "solver_iterations_mixed": [list(map(float, v)) for v in its32]})
+            log(f"mixed: {ms32:.1f} ms/timestep vs fp64 {ts_ms:.1f} -> x{ts_ms / ms32:.2f}; rel err {err}")
 
     # ---- roofline of the dominant kernel (from the live HIP-event timings)
-    from oracle import oracle as O  # the oracle's hand count of the reference formulation
-    fpe = O.flops_per_element()
+    fpe = FLOPS_PER_ELEMENT
     phases = {"flux": tm[0], "e3_elem": tm[1], "assembly": tm[2]}
     dominant = max(phases, key=phases.get)
     if dominant == "e3_elem":
         ach = fpe * nel / (tm[1] * 1e-3) / 1e12
-        roof = {"kernel": "e3_elem", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+        roof = {"kernel": "e3_elem<true>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": pmc_traffic(grid, "e3_elem<true>")}
     else:
-        # assembly: record reads (48 doubles/element) + block writes (16 doubles/nnz) + Res4
-        byt = nel * 48 * 8 + nnz * 16 * 8 + N * 4 * 8 + nel * 6 * 4 + N * 4 * 4
+        # assembly: the element record read once (48 doubles/element), every
+        # block written once (16 doubles/nnz), Res4 (4 doubles/node), and the
+        # incidence (4 B per element-node) and contribution-list (2 B per
+        # off-diagonal contribution, 12 per element) indices
+        byt = nel * 48 * 8 + nnz * 16 * 8 + N * 4 * 8 + nel * 4 * 4 + nel * 12 * 2
         ach = byt / (tm[2] * 1e-3) / 1e9
-        roof = {"kernel": "assembly", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+        roof = {"kernel": "asm_row<1>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": byt,
+                "traffic": pmc_traffic(grid, "asm_row<1>")}
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -243,6 +319,7 @@ def main():
                    "nel": nel, "nshg": N, "nnzb": nnz, "rho": 1.06, "mu": 0.04, "dt": 1e-3, "rho_inf": 0.5},
         "phase_ms": {"flux": tm[0], "e3_elem": tm[1], "assembly": tm[2], "events_total": tm[3]},
         "krylov_spmv_gbs": {"bsr3x3": gbs3, "bsr4x4": gbs4, "ms3": ms3, "ms4": ms4},
+        "krylov_spmv_gbs_fp32": spmv32, "mixed_formation": mixed_form,
         "timestep_ms_fp64": ts_ms, **ts_info,
         "roofline": roof, "cpu_baseline": cpu,
     }

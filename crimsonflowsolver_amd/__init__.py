@@ -39,7 +39,8 @@ EXPORTS = (
     "crimson_lhs_create", "crimson_lhs_free", "crimson_bc_create", "crimson_ls_solve",
     "crimson_ls_solve_dev", "crimson_lhs_set_val", "crimson_lhs_get_val", "crimson_ls_stats",
     "crimson_lhs_spmv_dev", "crimson_itr_predict_dev", "crimson_itr_yalpha_dev", "crimson_itr_correct_dev",
-    "crimson_itr_bc_dev", "crimson_elmpvsq_dev", "crimson_form_sync",
+    "crimson_itr_bc_dev", "crimson_elmpvsq_dev", "crimson_form_sync", "crimson_lhs_set_precision",
+    "crimson_form_vjp_dev",
 )
 
 
@@ -207,6 +208,14 @@ class ElmGMRInterior:
         check(lib().crimson_form_interior_to_lhs(self.h, _p(y), _p(ac), _p(x), _p(res4), lhs_handle.h),
               "crimson_form_interior_to_lhs")
 
+    def vjp(self, y, ac, x, w, ybar):
+        """Adjoint of the interior residual on device tensors laid out like
+        res: ybar <- w^T d res/d y (y = u1,u2,u3,p); returns w^T d res/d mu."""
+        mb = C.c_double()
+        check(lib().crimson_form_vjp_dev(self.h, _p(y), _p(ac), _p(x), _p(w), _p(ybar), C.byref(mb)),
+              "crimson_form_vjp_dev")
+        return mb.value
+
     def bc3res(self, res, res4_layout=False):
         check(lib().crimson_form_bc3res_dev(self.h, _p(res), 1 if res4_layout else 0),
               "crimson_form_bc3res_dev")
@@ -300,6 +309,10 @@ class memLS_lhsType:
     def set_val(self, dof, Val):
         Val = _f(Val)
         check(lib().crimson_lhs_set_val(self.h, dof, _p(Val)), "crimson_lhs_set_val")
+
+    def set_precision(self, bits):
+        """32: hold the matrix in FP32 and solve in FP32 (mixed form); 64: back to FP64."""
+        check(lib().crimson_lhs_set_precision(self.h, int(bits)), "crimson_lhs_set_precision")
 
     def get_val(self, dof=4):
         out = np.zeros((dof * dof, self.nnz), order="F")

@@ -120,6 +120,7 @@ struct crimson_lhs {
   crimson::DBuf<float> d_V32;     // FP32 planes [vdof*vdof][nnz]
   bool gt_is_d = false;           // Gt == mD planes (formation-produced)
   crimson::DBuf<double> d_Gt;     // [3][nnz] when !gt_is_d
+  crimson::DBuf<float> d_Gt32;    // the same for an FP32 matrix
   std::vector<CrFace> face;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

@@ -6,18 +6,20 @@
 //               flux record (lumped-mass weights and the viscous flux q_dI of
 //               e3q.f90:100-145)
 //   flux_node   node-parallel gather over the node's elements in element
-//               order: qres, rmass, q = qres * (1/rmass)  (asiq.f90 + qpbc.f90)
-//   e3_elem     element-parallel e3 (e3.f90 .. e3lhs.f90): all four
-//               quadrature points in registers; writes a 48-double element
-//               record: shape gradients, the scalar part S_ab of the 3x3
-//               velocity blocks, the element residual rl, and four scalars
-//               from which every tangent entry is rebuilt
+//               order: qres, rmass, q = qres * (1/rmass)  (asiq.f90 + qpbc.f90);
+//               writes one 128-byte node record (x, u, p, ac, q)
+//   e3_elem     element-parallel e3 (e3.f90 .. e3lhs.f90) on the four node
+//               records: all four quadrature points in registers; writes a
+//               48-double element record: shape gradients, the scalar part
+//               S_ab of the 3x3 velocity blocks, the element residual rl, and
+//               four scalars from which every tangent entry is rebuilt
 //   asm_row     row-parallel, 16 lanes per CSR row, one lane per block: the
 //               blocks of the row from the row's element incidences in element
 //               order (fillsparseI, fillsparse.f90:29-63), the -G^T column of
 //               Val4 from the same contributions (solfar.f90:225-235 without
 //               its transpose search) and the residual row (asigmr.f90:116)
 //   res_node    the residual rows alone when no tangent is formed
+//   e3_tangent32, asm_row32   the FP32 tangent of the mixed-precision form
 //
 // The element record factorisation is exact algebra on the reference's
 // lazy tangent (e3lhs.f90:85-402) for P1 elements, whose metric and shape
@@ -39,108 +41,15 @@
 
 #include "common.hpp"
 
+#include "form_common.hpp"
+
 namespace {
-
-// 4-point rule (symtet.c:22-23): qp q has barycentric (r,s,t) and
-// N_4 = 1 - r - s - t evaluated left to right (uniformP.c:22-25).
-constexpr double kA4 = 0.5854101966249685;
-constexpr double kB4 = 0.1381966011250150;
-constexpr double kQwt = (4.0 / 3.0) * 0.25;  // genint.f90:85
-__host__ __device__ constexpr double nq(int q, int a) {
-  return a == 3 ? (q == 0 ? 1.0 - kA4 - kB4 - kB4
-                          : q == 1 ? 1.0 - kB4 - kA4 - kB4
-                          : q == 2 ? 1.0 - kB4 - kB4 - kA4
-                                   : 1.0 - kB4 - kB4 - kB4)
-                : (a == q ? kA4 : kB4);
-}
-__host__ __device__ constexpr double sum_n(int a) { return nq(0, a) + nq(1, a) + nq(2, a) + nq(3, a); }
-__host__ __device__ constexpr double mm(int a, int b) {
-  return nq(0, a) * nq(0, b) + nq(1, a) * nq(1, b) + nq(2, a) * nq(2, b) + nq(3, a) * nq(3, b);
-}
-
-// Element record: 48 doubles = three 128-byte lines.  Line 0 holds what
-// every tangent entry needs (shape gradients and the four element scalars);
-// node a's S row and residual sit together in half a line at R_A(a).
-constexpr int kRec = 48;
-constexpr int R_G = 0;                                // g[a][j] at a*3+j (12)
-constexpr int R_TLW = 12, R_TC = 13, R_MU = 14, R_CM = 15;
-__host__ __device__ constexpr int R_A(int a) { return 16 + 8 * a; }  // S[a][0..3], rl[a][0..3]
-constexpr int kFRec = 16;  // flux record: one 128-byte line (m[4], qdi[6], pad)
-constexpr int kRowLanes = 16;   // lanes per CSR row in the assembly
-constexpr int kNodeLanes = 4;   // lanes per node in the flux/residual gathers
-
-struct FormConst {
-  double rho, mu, Delt, lhsFct, lhmFct, Dtgl, dtsfct, taucfct;
-};
-
-// Workgroup index remapped so that each XCD (blocks are dealt round-robin
-// over the 8 XCDs) walks a contiguous range of elements and keeps the nodes
-// it gathers in its own L2.
-__device__ inline unsigned xcd_block() {
-  const unsigned nb = gridDim.x, b = blockIdx.x;
-  const unsigned per = nb / 8u;
-  if (per == 0 || b >= per * 8u) return b;
-  return (b % 8u) * per + b / 8u;
-}
-
-// e3metric (e3metric.f90:28-81) specialised to P1: dxdxi_ij =
-// (x_j,i - x_4,i)/2 since shgl = dN/dxi / 2 (genshp.f90:35-36).
-template <class T>
-__device__ inline void p1_metric(const T (&xl)[4][3], T (&dx)[3][3], T &WdetJ, T (&g)[4][3]) {
-  T d[3][3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) d[i][j] = T(0.5) * xl[j][i] - T(0.5) * xl[3][i];
-  dx[0][0] = d[1][1] * d[2][2] - d[2][1] * d[1][2];
-  dx[0][1] = d[2][1] * d[0][2] - d[0][1] * d[2][2];
-  dx[0][2] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
-  const T tmp = T(1) / (dx[0][0] * d[0][0] + dx[0][1] * d[1][0] + dx[0][2] * d[2][0]);
-  dx[0][0] *= tmp;
-  dx[0][1] *= tmp;
-  dx[0][2] *= tmp;
-  dx[1][0] = (d[1][2] * d[2][0] - d[1][0] * d[2][2]) * tmp;
-  dx[1][1] = (d[0][0] * d[2][2] - d[2][0] * d[0][2]) * tmp;
-  dx[1][2] = (d[1][0] * d[0][2] - d[0][0] * d[1][2]) * tmp;
-  dx[2][0] = (d[1][0] * d[2][1] - d[1][1] * d[2][0]) * tmp;
-  dx[2][1] = (d[2][0] * d[0][1] - d[0][0] * d[2][1]) * tmp;
-  dx[2][2] = (d[0][0] * d[1][1] - d[0][1] * d[1][0]) * tmp;
-  WdetJ = T(kQwt) / tmp;
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    g[0][j] = T(0.5) * dx[0][j];
-    g[1][j] = T(0.5) * dx[1][j];
-    g[2][j] = T(0.5) * dx[2][j];
-    g[3][j] = T(-0.5) * dx[0][j] + T(-0.5) * dx[1][j] + T(-0.5) * dx[2][j];
-  }
-}
-
-// e3gijd, tet branch (e3stab.f90:493-527)
-template <class T>
-__device__ inline void gijd_tet(const T (&x)[3][3], T (&gg)[6]) {
-  const T c1 = T(1.259921049894873e+00), c2 = T(6.299605249474365e-01);
-  T t1, t2, t3;
-  t1 = c1 * x[0][0] + c2 * (x[1][0] + x[2][0]);
-  t2 = c1 * x[1][0] + c2 * (x[0][0] + x[2][0]);
-  t3 = c1 * x[2][0] + c2 * (x[0][0] + x[1][0]);
-  gg[0] = x[0][0] * t1 + x[1][0] * t2 + x[2][0] * t3;
-  t1 = c1 * x[0][1] + c2 * (x[1][1] + x[2][1]);
-  t2 = c1 * x[1][1] + c2 * (x[0][1] + x[2][1]);
-  t3 = c1 * x[2][1] + c2 * (x[0][1] + x[1][1]);
-  gg[1] = x[0][1] * t1 + x[1][1] * t2 + x[2][1] * t3;
-  gg[3] = x[0][0] * t1 + x[1][0] * t2 + x[2][0] * t3;
-  t1 = c1 * x[0][2] + c2 * (x[1][2] + x[2][2]);
-  t2 = c1 * x[1][2] + c2 * (x[0][2] + x[2][2]);
-  t3 = c1 * x[2][2] + c2 * (x[0][2] + x[1][2]);
-  gg[2] = x[0][2] * t1 + x[1][2] * t2 + x[2][2] * t3;
-  gg[4] = x[0][1] * t1 + x[1][1] * t2 + x[2][1] * t3;
-  gg[5] = x[0][0] * t1 + x[1][0] * t2 + x[2][0] * t3;
-}
 
 // ---------------------------------------------------------------- flux pass
 // e3q per element (e3q.f90:66-145): qdi = mu (grad u + grad u^T) is the same
 // at every quadrature point of a P1 element; the lumped-mass weights are
-// m_a = sum_q N_a(q) WdetJ.  Record: m[4], qdi in slots (11,12,13,22,23,33).
+// m_a = sum_q N_a(q) WdetJ.  Record (64 bytes): WdetJ, qdi in slots
+// (11,12,13,22,23,33), pad.
 __global__ void __launch_bounds__(256) flux_elem(int nel, const int *__restrict__ ien,
                                                  const double *__restrict__ x, int numnp,
                                                  const double *__restrict__ y, int nshg,
@@ -165,25 +74,24 @@ __global__ void __launch_bounds__(256) flux_elem(int nel, const int *__restrict_
 #pragma unroll
     for (int i = 0; i < 3; ++i)
       gu[j][i] = g[0][j] * ul[0][i] + g[1][j] * ul[1][i] + g[2][j] * ul[2][i] + g[3][j] * ul[3][i];
-  double *r = frec + (size_t)e * kFRec;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-    r[a] = nq(0, a) * W + nq(1, a) * W + nq(2, a) * W + nq(3, a) * W;
-  r[4] = 2.0 * mu * gu[0][0];
-  r[5] = mu * (gu[0][1] + gu[1][0]);
-  r[6] = mu * (gu[0][2] + gu[2][0]);
-  r[7] = 2.0 * mu * gu[1][1];
-  r[8] = mu * (gu[1][2] + gu[2][1]);
-  r[9] = 2.0 * mu * gu[2][2];
+  double2 *r = reinterpret_cast<double2 *>(frec + (size_t)e * kFRec);
+  r[0] = make_double2(W, 2.0 * mu * gu[0][0]);
+  r[1] = make_double2(mu * (gu[0][1] + gu[1][0]), mu * (gu[0][2] + gu[2][0]));
+  r[2] = make_double2(2.0 * mu * gu[1][1], mu * (gu[1][2] + gu[2][1]));
+  r[3] = make_double2(2.0 * mu * gu[2][2], 0.0);
 }
 
 // qres/rmass gather and qpbc inversion.  q(nshg,9) column-major in the
 // reference's component order: (1,4,7) = (q11,q12,q13), ... so component
-// k = i + 3 (j-1) holds mu(du_i/dx_j + du_j/dx_i).
+// k = i + 3 (j-1) holds mu(du_i/dx_j + du_j/dx_i).  Writes the node record
+// (with the node's x, y, ac) and, when q is not null, q(nshg,9).
 __global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict__ inc_ptr,
                                                  const int *__restrict__ inc,
                                                  const double *__restrict__ frec,
-                                                 double *__restrict__ q) {
+                                                 const double *__restrict__ x, int numnp,
+                                                 const double *__restrict__ y, const double *__restrict__ ac,
+                                                 double *__restrict__ node, double *__restrict__ q,
+                                                 double *__restrict__ invm) {
   const int gid = xcd_block() * blockDim.x + threadIdx.x;
   const int i = gid / kNodeLanes, lane = gid % kNodeLanes;
   double s[6] = {0, 0, 0, 0, 0, 0}, rm = 0.0;
@@ -191,9 +99,9 @@ __global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict
     for (int p = inc_ptr[i] + lane; p < inc_ptr[i + 1]; p += kNodeLanes) {
       const int ea = inc[p];
       const double *r = frec + (size_t)(ea >> 2) * kFRec;
-      const double m = r[ea & 3];
+      const double m = flux_weight(r[0], ea & 3);
 #pragma unroll
-      for (int k = 0; k < 6; ++k) s[k] += m * r[4 + k];
+      for (int k = 0; k < 6; ++k) s[k] += m * r[1 + k];
       rm += m;
     }
   }
@@ -204,9 +112,37 @@ __global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict
     for (int k = 0; k < 6; ++k) s[k] += __shfl_xor(s[k], off, kNodeLanes);
     rm += __shfl_xor(rm, off, kNodeLanes);
   }
-  if (i >= nshg || lane != 0) return;
+  if (i >= nshg) return;
   const double inv = 1.0 / rm;
-  const size_t N = nshg;
+  const size_t N = nshg, NP = numnp;
+  if (invm && lane == 0) invm[i] = inv;
+  // the node's record, four doubles per lane
+  double v[4];
+  if (lane == 0) {
+    v[0] = x[i];
+    v[1] = x[i + NP];
+    v[2] = x[i + 2 * NP];
+    v[3] = y[i];
+  } else if (lane == 1) {
+    v[0] = y[i + N];
+    v[1] = y[i + 2 * N];
+    v[2] = y[i + 3 * N];
+    v[3] = ac[i];
+  } else if (lane == 2) {
+    v[0] = ac[i + N];
+    v[1] = ac[i + 2 * N];
+    v[2] = inv * s[0];
+    v[3] = inv * s[1];
+  } else {
+    v[0] = inv * s[2];
+    v[1] = inv * s[3];
+    v[2] = inv * s[4];
+    v[3] = inv * s[5];
+  }
+  double2 *nr = reinterpret_cast<double2 *>(node + (size_t)i * kNRec + 4 * lane);
+  nr[0] = make_double2(v[0], v[1]);
+  nr[1] = make_double2(v[2], v[3]);
+  if (!q || lane != 0) return;
   q[i + 0 * N] = inv * s[0];
   q[i + 1 * N] = inv * s[1];
   q[i + 2 * N] = inv * s[2];
@@ -221,17 +157,25 @@ __global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict
 // ---------------------------------------------------------------- e3
 template <bool LHS>
 __global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ ien,
-                                               const double *__restrict__ x, int numnp,
-                                               const double *__restrict__ y,
-                                               const double *__restrict__ ac,
-                                               const double *__restrict__ q, int nshg,
-                                               FormConst c, double *__restrict__ rec) {
+                                               const double *__restrict__ node, FormConst c,
+                                               double *__restrict__ rec) {
   const int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= nel) return;
-  const size_t N = nshg;
   int nd[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) nd[a] = ien[a * (size_t)nel + e];
+  // the four node records (localx/localy gathers, localy.f90:54-64)
+  double nl[4][kNRec];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const double2 *r = reinterpret_cast<const double2 *>(node + (size_t)nd[a] * kNRec);
+#pragma unroll
+    for (int k = 0; k < kNRec / 2; ++k) {
+      const double2 v = r[k];
+      nl[a][2 * k] = v.x;
+      nl[a][2 * k + 1] = v.y;
+    }
+  }
 
   double g[4][3], W, gg[6];
   {
@@ -239,7 +183,7 @@ __global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ 
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int i = 0; i < 3; ++i) xl[a][i] = x[nd[a] + (size_t)numnp * i];
+      for (int i = 0; i < 3; ++i) xl[a][i] = nl[a][N_X + i];
     p1_metric(xl, dx, W, g);
     gijd_tet(dx, gg);
   }
@@ -248,8 +192,8 @@ __global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ 
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) ul[a][i] = y[nd[a] + N * i];
-    pl[a] = y[nd[a] + N * 3];
+    for (int i = 0; i < 3; ++i) ul[a][i] = nl[a][N_U + i];
+    pl[a] = nl[a][N_P];
   }
   double gu[3][3], gp[3];  // gu[j][i] = du_i/dx_j
 #pragma unroll
@@ -265,7 +209,7 @@ __global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ 
   for (int a = 0; a < 4; ++a) {
     double qa[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) qa[k] = q[nd[a] + N * k];
+    for (int k = 0; k < 9; ++k) qa[k] = nl[a][N_Q + qsym(k)];
 #pragma unroll
     for (int i = 0; i < 3; ++i) divq[i] += g[a][0] * qa[i] + g[a][1] * qa[i + 3] + g[a][2] * qa[i + 6];
   }
@@ -273,7 +217,7 @@ __global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ 
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) al[a][i] = ac[nd[a] + N * i];
+    for (int i = 0; i < 3; ++i) al[a][i] = nl[a][N_A + i];
 
   const double rho = c.rho, mu = c.mu, rhoinv = 1.0 / rho;
   const double rnu = mu * rhoinv;
@@ -725,7 +669,9 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Row-parallel assembly, kRowLanes lanes per CSR row.  The row's element
+// Row-parallel assembly, kRowLanes lanes per CSR row; rows are taken in the
+// handle's schedule order (neighbouring rows together, so that the element
+// records they share are read from L2).  The row's element
 // incidences are taken in chunks of kRowLanes, in element order; each chunk's
 // record data (line 0 plus the row node's S and residual half line) is staged
 // in LDS with whole-line loads, 8 lanes per record line, so every record line
@@ -746,12 +692,13 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
                                                const int *__restrict__ inc, const int *__restrict__ lst_ptr,
                                                const unsigned short *__restrict__ lst,
                                                const double *__restrict__ rec, double *__restrict__ res,
-                                               AsmOut o) {
+                                               AsmOut o, const int *__restrict__ order) {
   __shared__ double st_all[kRowsPerBlock][kRowLanes][kStage];
   const int gid = xcd_block() * blockDim.x + threadIdx.x;
-  const int row = gid / kRowLanes, lane = gid % kRowLanes;
+  const int slot = gid / kRowLanes, lane = gid % kRowLanes;
+  const bool live = slot < nshg;
+  const int row = live ? order[slot] : slot;
   double(*st)[kStage] = st_all[threadIdx.x / kRowLanes];
-  const bool live = row < nshg;
   int k0 = 0, nslot = 0, sdiag = -1, p0 = 0, ninc = 0;
   if (live) {
     k0 = rowptr[row];
@@ -846,12 +793,14 @@ __global__ void __launch_bounds__(256) asm_row32(int nshg, const int *__restrict
                                                  const int *__restrict__ diagk, const int *__restrict__ inc_ptr,
                                                  const int *__restrict__ inc, const int *__restrict__ lst_ptr,
                                                  const unsigned short *__restrict__ lst,
-                                                 const float *__restrict__ rec, AsmOut o) {
+                                                 const float *__restrict__ rec, AsmOut o,
+                                                 const int *__restrict__ order) {
   __shared__ float st_all[kRowsPerBlock][kRowLanes][kStage];
   const int gid = xcd_block() * blockDim.x + threadIdx.x;
-  const int row = gid / kRowLanes, lane = gid % kRowLanes;
+  const int slot = gid / kRowLanes, lane = gid % kRowLanes;
+  const bool live = slot < nshg;
+  const int row = live ? order[slot] : slot;
   float(*st)[kStage] = st_all[threadIdx.x / kRowLanes];
-  const bool live = row < nshg;
   int k0 = 0, nslot = 0, sdiag = -1, p0 = 0, ninc = 0;
   if (live) {
     k0 = rowptr[row];
@@ -937,21 +886,6 @@ __global__ void bc3res_kernel(int nshg, const int *__restrict__ ibc, double *__r
 
 }  // namespace
 
-// ---------------------------------------------------------------- handle
-struct crimson_form {
-  int nshg = 0, numnp = 0, nel = 0, nnz = 0;
-  crimson_form_params prm{};
-  hipStream_t stream = nullptr;
-  crimson::DBuf<int> d_ien, d_inc_ptr, d_inc, d_diagk, d_rowptr, d_ibc;
-  crimson::DBuf<int> d_lst_ptr;
-  crimson::DBuf<unsigned short> d_lst;
-  crimson::DBuf<double> d_frec, d_q, d_rec;
-  crimson::DBuf<float> d_rec32;  // FP32 tangent records (precision 32)
-  crimson::DBuf<double> s_y, s_ac, s_x, s_res, s_K, s_P, s_q;  // staging of the host API
-  bool has_ibc = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  float ms[4] = {0, 0, 0, 0};
-};
 
 namespace {
 
@@ -985,18 +919,6 @@ int validate_params(const crimson_form_params *p) {
   return CRIMSON_OK;
 }
 
-FormConst make_const(const crimson_form_params &p) {
-  FormConst c;
-  c.rho = p.rho;
-  c.mu = p.mu;
-  c.Delt = p.Delt;
-  c.lhsFct = p.alfi * p.gami * p.Delt;   // e3lhs.f90:85
-  c.lhmFct = p.almi * (1.0 - p.flmpl);   // e3lhs.f90:86
-  c.Dtgl = 1.0 / p.Delt;                 // itrPC.f90:71
-  c.dtsfct = p.dtsfct;
-  c.taucfct = p.taucfct;
-  return c;
-}
 
 // position of column `col` in 0-based CSR row `row`, or -1
 inline int find_col(const std::vector<int> &rp, const std::vector<int> &cols, int row, int col) {
@@ -1006,31 +928,68 @@ inline int find_col(const std::vector<int> &rp, const std::vector<int> &cols, in
   return (int)(it - cols.begin());
 }
 
+// Row schedule of the assembly: the nodes in Morton (Z-curve) order of their
+// coordinates, computed from the first call's x and kept.  Only the order in
+// which rows are processed depends on it, never a result.
+int ensure_order(crimson_form *f, const double *d_x) {
+  if (f->d_order.p) return CRIMSON_OK;
+  const int n = f->nshg;
+  const size_t np = f->numnp;
+  std::vector<double> x(np * 3);
+  CR_HIP(hipMemcpyAsync(x.data(), d_x, x.size() * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+  CR_HIP(hipStreamSynchronize(f->stream));
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = hi[k] = x[k * np];
+    for (int i = 0; i < n; ++i) {
+      lo[k] = std::min(lo[k], x[i + k * np]);
+      hi[k] = std::max(hi[k], x[i + k * np]);
+    }
+  }
+  auto spread = [](unsigned long long v) {  // 21 bits -> every third bit
+    v &= 0x1fffffULL;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+  };
+  std::vector<std::pair<unsigned long long, int>> key(n);
+  for (int i = 0; i < n; ++i) {
+    unsigned long long m = 0;
+    for (int k = 0; k < 3; ++k) {
+      const double span = hi[k] - lo[k];
+      const double t = span > 0 ? (x[i + k * np] - lo[k]) / span : 0.0;
+      const unsigned long long q = (unsigned long long)(t * 2097151.0);
+      m |= spread(q) << k;
+    }
+    key[i] = {m, i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = key[i].second;
+  CR_TRY(f->d_order.upload(order.data(), n, f->stream));
+  CR_HIP(hipStreamSynchronize(f->stream));
+  return CRIMSON_OK;
+}
+
 int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
                   double *d_res, int lhs, int mode, const AsmOut &o, double *d_qout) {
   const unsigned T = 256;
   hipStream_t s = f->stream;
   const FormConst c = make_const(f->prm);
+  if (lhs) CR_TRY(ensure_order(f, d_x));
   CR_HIP(hipEventRecord(f->ev[0], s));
-  if (f->nel > 0) {
-    flux_elem<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg,
-                                                         f->prm.mu, f->d_frec.p);
-  }
-  flux_node<<<crimson::grid_for((size_t)f->nshg * kNodeLanes, T), T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p,
-                                                                            f->d_frec.p,
-                                                        d_qout ? d_qout : f->d_q.p);
-  CR_HIP(hipGetLastError());
+  CR_TRY(crimson::form_flux_pass(f, d_y, d_ac, d_x, d_qout, nullptr));
   CR_HIP(hipEventRecord(f->ev[1], s));
-  const double *dq = d_qout ? d_qout : f->d_q.p;
   // mixed precision: FP64 residual (e3 without tangent, res_node), FP32 tangent
   const bool t32 = lhs && f->prm.precision == 32;
   if (f->nel > 0) {
     if (lhs && !t32)
-      e3_elem<true><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, d_ac,
-                                                                dq, f->nshg, c, f->d_rec.p);
+      e3_elem<true><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, c, f->d_rec.p);
     else
-      e3_elem<false><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, d_ac,
-                                                                 dq, f->nshg, c, f->d_rec.p);
+      e3_elem<false><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, c, f->d_rec.p);
     if (t32)
       e3_tangent32<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg, c,
                                                               (float)f->prm.aspect_fp64, f->d_rec32.p);
@@ -1043,11 +1002,11 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
     if (mode == 0) {
       res_node<0><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
       asm_row32<0><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
-                                    f->d_lst_ptr.p, f->d_lst.p, f->d_rec32.p, o);
+                                    f->d_lst_ptr.p, f->d_lst.p, f->d_rec32.p, o, f->d_order.p);
     } else {
       res_node<1><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
       asm_row32<1><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
-                                    f->d_lst_ptr.p, f->d_lst.p, f->d_rec32.p, o);
+                                    f->d_lst_ptr.p, f->d_lst.p, f->d_rec32.p, o, f->d_order.p);
     }
   } else if (!lhs) {
     if (mode == 0)
@@ -1056,10 +1015,10 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
       res_node<1><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
   } else if (mode == 0) {
     asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
-                                f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p, d_res, o);
+                                f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p, d_res, o, f->d_order.p);
   } else {
     asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
-                                f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p, d_res, o);
+                                f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p, d_res, o, f->d_order.p);
   }
   CR_HIP(hipGetLastError());
   CR_HIP(hipEventRecord(f->ev[3], s));
@@ -1074,6 +1033,20 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
 }  // namespace
 
 namespace crimson {
+// The flux pass (flux_elem + flux_node) on the handle's stream: fills the
+// flux and node records; q(nshg,9) and 1/rmass(nshg) when requested.
+int form_flux_pass(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, double *d_q,
+                   double *d_invm) {
+  const unsigned T = 256;
+  hipStream_t s = f->stream;
+  if (f->nel > 0)
+    flux_elem<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg,
+                                                         f->prm.mu, f->d_frec.p);
+  flux_node<<<crimson::grid_for((size_t)f->nshg * kNodeLanes, T), T, 0, s>>>(
+      f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_frec.p, d_x, f->numnp, d_y, d_ac, f->d_node.p, d_q, d_invm);
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
 hipStream_t form_stream(const crimson_form *f) { return f->stream; }
 int form_nshg(const crimson_form *f) { return f->nshg; }
 }  // namespace crimson
@@ -1230,7 +1203,8 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
       (rc = f->d_lst_ptr.upload(lst_ptr.data(), lst_ptr.size(), s)) ||
       (rc = f->d_lst.upload(lst.data(), lst.size(), s)) ||
       (rc = f->d_frec.alloc((size_t)std::max(nel, 1) * kFRec)) ||
-      (rc = f->d_q.alloc((size_t)nshg * 9)) || (rc = f->d_rec.alloc((size_t)std::max(nel, 1) * kRec)))
+      (rc = f->d_node.alloc((size_t)nshg * kNRec)) || (rc = f->d_q.alloc((size_t)nshg * 9)) ||
+      (rc = f->d_rec.alloc((size_t)std::max(nel, 1) * kRec)))
     return fail(rc);
   if (prm->precision == 32 && (rc = f->d_rec32.alloc((size_t)std::max(nel, 1) * kRec32))) return fail(rc);
   if (iBC) {
@@ -1282,7 +1256,7 @@ extern "C" int crimson_form_interior(crimson_form *f, const double *y, const dou
     CR_TRY(f->s_P.alloc((size_t)f->nnz * 4));
   }
   AsmOut o{f->s_K.p, f->s_P.p, nullptr, nullptr, (size_t)f->nnz};
-  CR_TRY(run_formation(f, f->s_y.p, f->s_ac.p, f->s_x.p, f->s_res.p, lhs ? 1 : 0, 0, o, nullptr));
+  CR_TRY(run_formation(f, f->s_y.p, f->s_ac.p, f->s_x.p, f->s_res.p, lhs ? 1 : 0, 0, o, qres ? f->d_q.p : nullptr));
   CR_HIP(hipMemcpyAsync(res, f->s_res.p, N * 4 * sizeof(double), hipMemcpyDeviceToHost, s));
   if (lhs) {
     CR_HIP(hipMemcpyAsync(lhsK, f->s_K.p, (size_t)f->nnz * 9 * sizeof(double), hipMemcpyDeviceToHost, s));

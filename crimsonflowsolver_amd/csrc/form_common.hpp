@@ -1,0 +1,162 @@
+// Shared definitions of the formation kernels (form.hip) and their adjoint
+// (adjoint.hip): quadrature constants, element/node record layouts, the P1
+// metric and e3gijd, and the formation handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace {
+
+// 4-point rule (symtet.c:22-23): qp q has barycentric (r,s,t) and
+// N_4 = 1 - r - s - t evaluated left to right (uniformP.c:22-25).
+constexpr double kA4 = 0.5854101966249685;
+constexpr double kB4 = 0.1381966011250150;
+constexpr double kQwt = (4.0 / 3.0) * 0.25;  // genint.f90:85
+__host__ __device__ constexpr double nq(int q, int a) {
+  return a == 3 ? (q == 0 ? 1.0 - kA4 - kB4 - kB4
+                          : q == 1 ? 1.0 - kB4 - kA4 - kB4
+                          : q == 2 ? 1.0 - kB4 - kB4 - kA4
+                                   : 1.0 - kB4 - kB4 - kB4)
+                : (a == q ? kA4 : kB4);
+}
+__host__ __device__ constexpr double sum_n(int a) { return nq(0, a) + nq(1, a) + nq(2, a) + nq(3, a); }
+__host__ __device__ constexpr double mm(int a, int b) {
+  return nq(0, a) * nq(0, b) + nq(1, a) * nq(1, b) + nq(2, a) * nq(2, b) + nq(3, a) * nq(3, b);
+}
+
+// Element record: 48 doubles = three 128-byte lines.  Line 0 holds what
+// every tangent entry needs (shape gradients and the four element scalars);
+// node a's S row and residual sit together in half a line at R_A(a).
+constexpr int kRec = 48;
+constexpr int R_G = 0;                                // g[a][j] at a*3+j (12)
+constexpr int R_TLW = 12, R_TC = 13, R_MU = 14, R_CM = 15;
+__host__ __device__ constexpr int R_A(int a) { return 16 + 8 * a; }  // S[a][0..3], rl[a][0..3]
+constexpr int kFRec = 8;  // flux record: half a line (WdetJ, qdi[6], pad)
+// lumped-mass weight of local node a, m_a = sum_q N_a(q) WdetJ (e3q.f90:120-145)
+__host__ __device__ inline double flux_weight(double W, int a) {
+  return a == 0   ? nq(0, 0) * W + nq(1, 0) * W + nq(2, 0) * W + nq(3, 0) * W
+         : a == 1 ? nq(0, 1) * W + nq(1, 1) * W + nq(2, 1) * W + nq(3, 1) * W
+         : a == 2 ? nq(0, 2) * W + nq(1, 2) * W + nq(2, 2) * W + nq(3, 2) * W
+                  : nq(0, 3) * W + nq(1, 3) * W + nq(2, 3) * W + nq(3, 3) * W;
+}
+constexpr int kRowLanes = 16;   // lanes per CSR row in the assembly
+constexpr int kNodeLanes = 4;   // lanes per node in the flux/residual gathers
+
+struct FormConst {
+  double rho, mu, Delt, lhsFct, lhmFct, Dtgl, dtsfct, taucfct;
+};
+
+// Workgroup index remapped so that each XCD (blocks are dealt round-robin
+// over the 8 XCDs) walks a contiguous range of elements and keeps the nodes
+// it gathers in its own L2.
+__device__ inline unsigned xcd_block() {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned per = nb / 8u;
+  if (per == 0 || b >= per * 8u) return b;
+  return (b % 8u) * per + b / 8u;
+}
+
+// e3metric (e3metric.f90:28-81) specialised to P1: dxdxi_ij =
+// (x_j,i - x_4,i)/2 since shgl = dN/dxi / 2 (genshp.f90:35-36).
+template <class T>
+__device__ inline void p1_metric(const T (&xl)[4][3], T (&dx)[3][3], T &WdetJ, T (&g)[4][3]) {
+  T d[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d[i][j] = T(0.5) * xl[j][i] - T(0.5) * xl[3][i];
+  dx[0][0] = d[1][1] * d[2][2] - d[2][1] * d[1][2];
+  dx[0][1] = d[2][1] * d[0][2] - d[0][1] * d[2][2];
+  dx[0][2] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+  const T tmp = T(1) / (dx[0][0] * d[0][0] + dx[0][1] * d[1][0] + dx[0][2] * d[2][0]);
+  dx[0][0] *= tmp;
+  dx[0][1] *= tmp;
+  dx[0][2] *= tmp;
+  dx[1][0] = (d[1][2] * d[2][0] - d[1][0] * d[2][2]) * tmp;
+  dx[1][1] = (d[0][0] * d[2][2] - d[2][0] * d[0][2]) * tmp;
+  dx[1][2] = (d[1][0] * d[0][2] - d[0][0] * d[1][2]) * tmp;
+  dx[2][0] = (d[1][0] * d[2][1] - d[1][1] * d[2][0]) * tmp;
+  dx[2][1] = (d[2][0] * d[0][1] - d[0][0] * d[2][1]) * tmp;
+  dx[2][2] = (d[0][0] * d[1][1] - d[0][1] * d[1][0]) * tmp;
+  WdetJ = T(kQwt) / tmp;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    g[0][j] = T(0.5) * dx[0][j];
+    g[1][j] = T(0.5) * dx[1][j];
+    g[2][j] = T(0.5) * dx[2][j];
+    g[3][j] = T(-0.5) * dx[0][j] + T(-0.5) * dx[1][j] + T(-0.5) * dx[2][j];
+  }
+}
+
+// e3gijd, tet branch (e3stab.f90:493-527)
+template <class T>
+__device__ inline void gijd_tet(const T (&x)[3][3], T (&gg)[6]) {
+  const T c1 = T(1.259921049894873e+00), c2 = T(6.299605249474365e-01);
+  T t1, t2, t3;
+  t1 = c1 * x[0][0] + c2 * (x[1][0] + x[2][0]);
+  t2 = c1 * x[1][0] + c2 * (x[0][0] + x[2][0]);
+  t3 = c1 * x[2][0] + c2 * (x[0][0] + x[1][0]);
+  gg[0] = x[0][0] * t1 + x[1][0] * t2 + x[2][0] * t3;
+  t1 = c1 * x[0][1] + c2 * (x[1][1] + x[2][1]);
+  t2 = c1 * x[1][1] + c2 * (x[0][1] + x[2][1]);
+  t3 = c1 * x[2][1] + c2 * (x[0][1] + x[1][1]);
+  gg[1] = x[0][1] * t1 + x[1][1] * t2 + x[2][1] * t3;
+  gg[3] = x[0][0] * t1 + x[1][0] * t2 + x[2][0] * t3;
+  t1 = c1 * x[0][2] + c2 * (x[1][2] + x[2][2]);
+  t2 = c1 * x[1][2] + c2 * (x[0][2] + x[2][2]);
+  t3 = c1 * x[2][2] + c2 * (x[0][2] + x[1][2]);
+  gg[2] = x[0][2] * t1 + x[1][2] * t2 + x[2][2] * t3;
+  gg[4] = x[0][1] * t1 + x[1][1] * t2 + x[2][1] * t3;
+  gg[5] = x[0][0] * t1 + x[1][0] * t2 + x[2][0] * t3;
+}
+
+// Node record: 16 doubles = one 128-byte line per node, written by flux_node
+// so that the element kernel gathers each node with whole-line loads:
+// x(3), u(3), p, ac(3), and the 6 distinct components of q after qpbc
+// (q11, q12, q13, q22, q23, q33).
+constexpr int kNRec = 16;
+constexpr int N_X = 0, N_U = 3, N_P = 6, N_A = 7, N_Q = 10;
+// q component k = i + 3j (column-major, the reference's order) -> distinct slot
+__host__ __device__ constexpr int qsym(int k) {
+  return k == 0 ? 0 : k == 1 ? 1 : k == 2 ? 2 : k == 3 ? 1 : k == 4 ? 3 : k == 5 ? 4 : k == 6 ? 2 : k == 7 ? 4 : 5;
+}
+
+inline FormConst make_const(const crimson_form_params &p) {
+  FormConst c;
+  c.rho = p.rho;
+  c.mu = p.mu;
+  c.Delt = p.Delt;
+  c.lhsFct = p.alfi * p.gami * p.Delt;   // e3lhs.f90:85
+  c.lhmFct = p.almi * (1.0 - p.flmpl);   // e3lhs.f90:86
+  c.Dtgl = 1.0 / p.Delt;                 // itrPC.f90:71
+  c.dtsfct = p.dtsfct;
+  c.taucfct = p.taucfct;
+  return c;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- handle
+struct crimson_form {
+  int nshg = 0, numnp = 0, nel = 0, nnz = 0;
+  crimson_form_params prm{};
+  hipStream_t stream = nullptr;
+  crimson::DBuf<int> d_ien, d_inc_ptr, d_inc, d_diagk, d_rowptr, d_ibc;
+  crimson::DBuf<int> d_lst_ptr, d_order;  // d_order: assembly row schedule
+  crimson::DBuf<unsigned short> d_lst;
+  crimson::DBuf<double> d_frec, d_node, d_q, d_rec;
+  crimson::DBuf<float> d_rec32;  // FP32 tangent records (precision 32)
+  crimson::DBuf<double> s_y, s_ac, s_x, s_res, s_K, s_P, s_q;  // staging of the host API
+  bool has_ibc = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float ms[4] = {0, 0, 0, 0};
+  // adjoint (VJP) workspaces, allocated on first use
+  crimson::DBuf<double> d_vrec, d_vrec2, d_qbar, d_invm, d_mupart;
+};
+
+namespace crimson {
+int form_flux_pass(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, double *d_q,
+                   double *d_invm);
+}  // namespace crimson

@@ -44,31 +44,34 @@ struct SubLs {
 };
 
 // A scaled view of a sub-block (rows r0.., cols c0..) of the stored matrix.
+// T is the arithmetic type of the solve: the matrix planes, W, the Krylov
+// vectors and every accumulation (FP64, or FP32 in the mixed form).
+template <class T>
 struct MatView {
-  const double *base;  // plane 0
+  const T *base;  // plane 0
   size_t nnz;
-  int plane[16];       // plane index of (r, c), r < NR, c < NC: plane[r * 4 + c]
-  const double *W;     // W(wdof, nNo)
-  const double *fix;   // fix(wdof, nNo) or null
+  int plane[16];  // plane index of (r, c), r < NR, c < NC: plane[r * 4 + c]
+  const T *W;     // W(wdof, nNo)
+  const T *fix;   // fix(wdof, nNo) or null
   int wdof, r0, c0;
 };
 
-template <int NR, int NC>
+template <int NR, int NC, class T>
 __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restrict__ rowptr,
-                                                   const int *__restrict__ col, MatView A,
-                                                   const double *__restrict__ x, double *__restrict__ y,
-                                                   double ya, const double *__restrict__ z, double za) {
+                                                   const int *__restrict__ col, MatView<T> A,
+                                                   const T *__restrict__ x, T *__restrict__ y, T ya,
+                                                   const T *__restrict__ z, T za) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int row = gid / kRowLanes;
   const int lane = threadIdx.x % kRowLanes;
-  double acc[NR];
+  T acc[NR];
 #pragma unroll
-  for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+  for (int r = 0; r < NR; ++r) acc[r] = T(0);
   if (row < nNo) {
     const int ke = rowptr[row + 1];
     for (int k = rowptr[row] + lane; k < ke; k += kRowLanes) {
       const int j = col[k];
-      double xv[NC];
+      T xv[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) xv[c] = A.W[A.c0 + c + (size_t)A.wdof * j] * x[c + (size_t)NC * j];
 #pragma unroll
@@ -84,7 +87,7 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
   if (row < nNo && lane == 0) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-      double v = A.W[A.r0 + r + (size_t)A.wdof * row] * acc[r];
+      T v = A.W[A.r0 + r + (size_t)A.wdof * row] * acc[r];
       if (A.fix && NR == NC && A.r0 == A.c0) v += A.fix[A.r0 + r + (size_t)A.wdof * row] * x[r + (size_t)NC * row];
       v *= ya;
       if (z) v += za * z[r + (size_t)NR * row];
@@ -94,47 +97,49 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
 }
 
 // Wave-level sum (64 lanes) then across the 4 waves of a 256-thread group.
-__device__ inline double block_sum(double v, double *sh) {
+template <class T>
+__device__ inline T block_sum(T v, T *sh) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   const int w = threadIdx.x / 64;
   if ((threadIdx.x & 63) == 0) sh[w] = v;
   __syncthreads();
-  double t = 0.0;
+  T t = T(0);
   if (threadIdx.x == 0) t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
   __syncthreads();
   return t;
 }
 
 // partial[blk][j] = sum over this block's chunk of U_j . v for j < m, and
-// partial[blk][m] = v . v when self != 0.  U_j = U0 + j * ustride.
-__global__ void __launch_bounds__(256) mdot_kernel(size_t L, const double *__restrict__ v,
-                                                   const double *__restrict__ U0, size_t ustride, int m,
-                                                   int self, double *__restrict__ partial, int pstride,
-                                                   const int *__restrict__ gate) {
+// partial[blk][m] = v . v when self != 0.  U_j = U0 + j * ustride.  Vectors
+// of type T, products and sums in Acc.
+template <class T, class Acc>
+__global__ void __launch_bounds__(256) mdot_kernel(size_t L, const T *__restrict__ v, const T *__restrict__ U0,
+                                                   size_t ustride, int m, int self, Acc *__restrict__ partial,
+                                                   int pstride, const int *__restrict__ gate) {
   if (gate && *gate == 0) return;
-  __shared__ double sh[4];
+  __shared__ Acc sh[4];
   const size_t base = (size_t)blockIdx.x * kChunk;
-  double vv[kChunk / 256];
+  Acc vv[kChunk / 256];
 #pragma unroll
   for (int t = 0; t < kChunk / 256; ++t) {
     const size_t e = base + threadIdx.x + 256 * t;
-    vv[t] = e < L ? v[e] : 0.0;
+    vv[t] = e < L ? (Acc)v[e] : Acc(0);
   }
-  double *out = partial + (size_t)blockIdx.x * pstride;
+  Acc *out = partial + (size_t)blockIdx.x * pstride;
   for (int j = 0; j < m; ++j) {
-    const double *u = U0 + ustride * j;
-    double s = 0.0;
+    const T *u = U0 + ustride * j;
+    Acc s = Acc(0);
 #pragma unroll
     for (int t = 0; t < kChunk / 256; ++t) {
       const size_t e = base + threadIdx.x + 256 * t;
-      if (e < L) s += u[e] * vv[t];
+      if (e < L) s += (Acc)u[e] * vv[t];
     }
     s = block_sum(s, sh);
     if (threadIdx.x == 0) out[j] = s;
   }
   if (self) {
-    double s = 0.0;
+    Acc s = Acc(0);
 #pragma unroll
     for (int t = 0; t < kChunk / 256; ++t) s += vv[t] * vv[t];
     s = block_sum(s, sh);
@@ -142,40 +147,42 @@ __global__ void __launch_bounds__(256) mdot_kernel(size_t L, const double *__res
   }
 }
 
-// out[j] = sum_blk partial[blk][j], j < m, one workgroup per j, fixed order.
-__global__ void __launch_bounds__(256) reduce_kernel(int nblk, const double *__restrict__ partial, int pstride,
+// out[j] = sum_blk partial[blk][j], j < m, one workgroup per j, fixed order
+// (summed in T, stored as double for the host-side FP64 scalars).
+template <class T>
+__global__ void __launch_bounds__(256) reduce_kernel(int nblk, const T *__restrict__ partial, int pstride,
                                                      double *__restrict__ out, const int *__restrict__ gate) {
-  __shared__ double sh[4];
+  __shared__ T sh[4];
   const int j = blockIdx.x;
   if (gate && *gate == 0) {
     if (threadIdx.x == 0) out[j] = 0.0;
     return;
   }
-  double s = 0.0;
+  T s = T(0);
   for (int b = threadIdx.x; b < nblk; b += 256) s += partial[(size_t)b * pstride + j];
   s = block_sum(s, sh);
-  if (threadIdx.x == 0) out[j] = s;
+  if (threadIdx.x == 0) out[j] = (double)s;
 }
 
 // y -= sum_j coef[j] U_j (coef on the device), then the block's partial of
 // y . y into partial[blk * pstride].
-__global__ void __launch_bounds__(256) msub_norm_kernel(size_t L, double *__restrict__ y,
-                                                        const double *__restrict__ U0, size_t ustride, int m,
-                                                        const double *__restrict__ coef,
-                                                        double *__restrict__ partial, int pstride,
+template <class T>
+__global__ void __launch_bounds__(256) msub_norm_kernel(size_t L, T *__restrict__ y, const T *__restrict__ U0,
+                                                        size_t ustride, int m, const double *__restrict__ coef,
+                                                        T *__restrict__ partial, int pstride,
                                                         const int *__restrict__ gate) {
   if (gate && *gate == 0) return;
-  __shared__ double sh[4];
-  __shared__ double cs[256];
-  for (int j = threadIdx.x; j < m && j < 256; j += 256) cs[j] = coef[j];
+  __shared__ T sh[4];
+  __shared__ T cs[256];
+  for (int j = threadIdx.x; j < m && j < 256; j += 256) cs[j] = (T)coef[j];
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * kChunk;
-  double s = 0.0;
+  T s = T(0);
 #pragma unroll
   for (int t = 0; t < kChunk / 256; ++t) {
     const size_t e = base + threadIdx.x + 256 * t;
     if (e < L) {
-      double w = y[e];
+      T w = y[e];
       for (int j = 0; j < m; ++j) w -= cs[j] * U0[ustride * j + e];
       y[e] = w;
       s += w * w;
@@ -186,120 +193,137 @@ __global__ void __launch_bounds__(256) msub_norm_kernel(size_t L, double *__rest
 }
 
 // y += sum_j coef[j] U_j, coef on the device.
-__global__ void maxpy_kernel(size_t L, double *__restrict__ y, const double *__restrict__ U0, size_t ustride,
-                             int m, const double *__restrict__ coef, double sign) {
+template <class T>
+__global__ void maxpy_kernel(size_t L, T *__restrict__ y, const T *__restrict__ U0, size_t ustride, int m,
+                             const double *__restrict__ coef, double sign) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= L) return;
-  double w = y[e];
-  for (int j = 0; j < m; ++j) w += sign * coef[j] * U0[ustride * j + e];
+  T w = y[e];
+  for (int j = 0; j < m; ++j) w += (T)(sign * coef[j]) * U0[ustride * j + e];
   y[e] = w;
 }
 
 // y = a x + b y
-__global__ void axpby_kernel(size_t L, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+template <class T>
+__global__ void axpby_kernel(size_t L, T a, const T *__restrict__ x, T b, T *__restrict__ y) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= L) return;
-  y[e] = a * x[e] + (b == 0.0 ? 0.0 : b * y[e]);
+  y[e] = a * x[e] + (b == T(0) ? T(0) : b * y[e]);
 }
 
-// y = x * W (or x / ... ) elementwise
-__global__ void mul_kernel(size_t L, const double *__restrict__ x, const double *__restrict__ w,
-                           double *__restrict__ y) {
+// y = x * w elementwise (types may differ: the FP64 right-hand side in and
+// the solution out of an FP32 solve)
+template <class TX, class TW, class TY>
+__global__ void mul_kernel(size_t L, const TX *__restrict__ x, const TW *__restrict__ w, TY *__restrict__ y) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= L) return;
-  y[e] = x[e] * w[e];
+  y[e] = (TY)(x[e] * w[e]);
 }
 
 // Strided component extraction/insert between (dof,N) and (n,N) vectors.
-__global__ void extract_kernel(int N, int dof, int c0, int nc, const double *__restrict__ src,
-                               double *__restrict__ dst) {
+template <class T>
+__global__ void extract_kernel(int N, int dof, int c0, int nc, const T *__restrict__ src, T *__restrict__ dst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   for (int c = 0; c < nc; ++c) dst[c + (size_t)nc * i] = src[c0 + c + (size_t)dof * i];
 }
-__global__ void insert_kernel(int N, int dof, int c0, int nc, const double *__restrict__ src,
-                              double *__restrict__ dst) {
+template <class T>
+__global__ void insert_kernel(int N, int dof, int c0, int nc, const T *__restrict__ src, T *__restrict__ dst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   for (int c = 0; c < nc; ++c) dst[c0 + c + (size_t)dof * i] = src[c + (size_t)nc * i];
 }
 
 // PRECOND part 1: W from the diagonal, fix for zero diagonals.
-__global__ void precond_diag_kernel(int N, int dof, int vdof, const double *__restrict__ V, size_t nnz,
-                                    const int *__restrict__ diag, double *__restrict__ W, double *__restrict__ fix) {
+template <class T>
+__global__ void precond_diag_kernel(int N, int dof, int vdof, const T *__restrict__ V, size_t nnz,
+                                    const int *__restrict__ diag, T *__restrict__ W, T *__restrict__ fix) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int d = diag[i];
   for (int r = 0; r < dof; ++r) {
-    const double a = V[(size_t)(r * vdof + r) * nnz + d];
-    W[r + (size_t)dof * i] = (a == 0.0) ? 1.0 : 1.0 / sqrt(fabs(a));
-    fix[r + (size_t)dof * i] = (a == 0.0) ? 1.0 : 0.0;
+    const T a = V[(size_t)(r * vdof + r) * nnz + d];
+    W[r + (size_t)dof * i] = (a == T(0)) ? T(1) : T(1) / sqrt(fabs(a));
+    fix[r + (size_t)dof * i] = (a == T(0)) ? T(1) : T(0);
   }
 }
 // Dirichlet masking of W by a face's val (memLS.f90:1831-1841)
+template <class T>
 __global__ void precond_face_kernel(int n, int dof, int fdof, const int *__restrict__ glob,
-                                    const double *__restrict__ val, double *__restrict__ W) {
+                                    const double *__restrict__ val, T *__restrict__ W) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
   const int nc = fdof < dof ? fdof : dof;
   const int Ac = glob[a];
-  for (int i = 0; i < nc; ++i) W[i + (size_t)dof * Ac] *= val[i + (size_t)fdof * a];
+  for (int i = 0; i < nc; ++i) W[i + (size_t)dof * Ac] *= (T)val[i + (size_t)fdof * a];
 }
 // After masking: diagonal entries that scale to zero are forced to one.
-__global__ void precond_fix_kernel(int N, int dof, int vdof, const double *__restrict__ V, size_t nnz,
-                                   const int *__restrict__ diag, const double *__restrict__ W,
-                                   double *__restrict__ fix) {
+template <class T>
+__global__ void precond_fix_kernel(int N, int dof, int vdof, const T *__restrict__ V, size_t nnz,
+                                   const int *__restrict__ diag, const T *__restrict__ W, T *__restrict__ fix) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int d = diag[i];
   for (int r = 0; r < dof; ++r) {
-    const double a = V[(size_t)(r * vdof + r) * nnz + d];
-    const double w = W[r + (size_t)dof * i];
-    if (a == 0.0 || (a * w) * w == 0.0) fix[r + (size_t)dof * i] = 1.0;
+    const T a = V[(size_t)(r * vdof + r) * nnz + d];
+    const T w = W[r + (size_t)dof * i];
+    if (a == T(0) || (a * w) * w == T(0)) fix[r + (size_t)dof * i] = T(1);
   }
 }
-// valM = val * W for a coupled face
+// valM = val * W for a coupled face (kept in FP64: faces are small)
+template <class T>
 __global__ void face_valm_kernel(int n, int dof, int fdof, const int *__restrict__ glob,
-                                 const double *__restrict__ val, const double *__restrict__ W,
+                                 const double *__restrict__ val, const T *__restrict__ W,
                                  double *__restrict__ valM) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
   const int nc = fdof < dof ? fdof : dof;
-  for (int i = 0; i < nc; ++i) valM[i + (size_t)fdof * a] = val[i + (size_t)fdof * a] * W[i + (size_t)dof * glob[a]];
+  for (int i = 0; i < nc; ++i)
+    valM[i + (size_t)fdof * a] = val[i + (size_t)fdof * a] * (double)W[i + (size_t)dof * glob[a]];
 }
 // S = sum_a sum_i valM(i,a) X(i, glob(a)) in one workgroup (faces are small)
+template <class T>
 __global__ void __launch_bounds__(256) face_dot_kernel(int n, int xdof, int fdof, const int *__restrict__ glob,
-                                                       const double *__restrict__ valM,
-                                                       const double *__restrict__ X, double *__restrict__ out) {
+                                                       const double *__restrict__ valM, const T *__restrict__ X,
+                                                       double *__restrict__ out) {
   __shared__ double sh[4];
   const int nc = fdof < xdof ? fdof : xdof;
   double s = 0.0;
   for (int a = threadIdx.x; a < n; a += 256)
-    for (int i = 0; i < nc; ++i) s += valM[i + (size_t)fdof * a] * X[i + (size_t)xdof * glob[a]];
+    for (int i = 0; i < nc; ++i) s += valM[i + (size_t)fdof * a] * (double)X[i + (size_t)xdof * glob[a]];
   s = block_sum(s, sh);
   if (threadIdx.x == 0) *out = s;
 }
 // Y(i, glob(a)) += valM(i,a) * coef * S
+template <class T>
 __global__ void face_axpy_kernel(int n, int xdof, int fdof, const int *__restrict__ glob,
                                  const double *__restrict__ valM, const double *__restrict__ S, double coef,
-                                 double *__restrict__ Y) {
+                                 T *__restrict__ Y) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
   const int nc = fdof < xdof ? fdof : xdof;
   const double s = coef * *S;
-  for (int i = 0; i < nc; ++i) Y[i + (size_t)xdof * glob[a]] += valM[i + (size_t)fdof * a] * s;
+  for (int i = 0; i < nc; ++i) Y[i + (size_t)xdof * glob[a]] += (T)(valM[i + (size_t)fdof * a] * s);
 }
 // DGKS decision: gate = ||w'||^2 < 0.5 ||w||^2
 __global__ void dgks_kernel(const double *__restrict__ n0, const double *__restrict__ n1, int *__restrict__ gate) {
   *gate = (*n1 < 0.5 * *n0) ? 1 : 0;
 }
 // AoS (vdof*vdof, nnz) -> planes, and Gt = -mG^T via the transpose map
-__global__ void aos_to_planes_kernel(size_t nnz, int m, const double *__restrict__ aos, double *__restrict__ V) {
+template <class TS, class TD>
+__global__ void aos_to_planes_kernel(size_t nnz, int m, const TS *__restrict__ aos, TD *__restrict__ V) {
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
-  for (int p = 0; p < m; ++p) V[(size_t)p * nnz + k] = aos[(size_t)m * k + p];
+  for (int p = 0; p < m; ++p) V[(size_t)p * nnz + k] = (TD)aos[(size_t)m * k + p];
 }
-__global__ void gt_kernel(size_t nnz, const double *__restrict__ V, const int *__restrict__ tr, double *__restrict__ Gt) {
+template <class TS, class TD>
+__global__ void convert_kernel(size_t n, const TS *__restrict__ a, TD *__restrict__ b) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  b[k] = (TD)a[k];
+}
+template <class T>
+__global__ void gt_kernel(size_t nnz, const T *__restrict__ V, const int *__restrict__ tr, T *__restrict__ Gt) {
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
   const int l = tr[k];
@@ -329,21 +353,22 @@ __global__ void cg_alpha_kernel(double *__restrict__ scal) {
   scal[4] = scal[1] / scal[3];
 }
 // X += alpha P ; R -= alpha SP ; partial of R.R
+template <class T>
 __global__ void __launch_bounds__(256) cg_update_kernel(size_t L, const double *__restrict__ scal,
-                                                        double *__restrict__ X, double *__restrict__ R,
-                                                        const double *__restrict__ P, const double *__restrict__ SP,
-                                                        double *__restrict__ partial) {
+                                                        T *__restrict__ X, T *__restrict__ R,
+                                                        const T *__restrict__ P, const T *__restrict__ SP,
+                                                        T *__restrict__ partial) {
   if (scal[5] != 0.0) return;
-  __shared__ double sh[4];
-  const double alpha = scal[4];
+  __shared__ T sh[4];
+  const T alpha = (T)scal[4];
   const size_t base = (size_t)blockIdx.x * kChunk;
-  double s = 0.0;
+  T s = T(0);
 #pragma unroll
   for (int t = 0; t < kChunk / 256; ++t) {
     const size_t e = base + threadIdx.x + 256 * t;
     if (e < L) {
       X[e] = X[e] + alpha * P[e];
-      const double r = R[e] - alpha * SP[e];
+      const T r = R[e] - alpha * SP[e];
       R[e] = r;
       s += r * r;
     }
@@ -357,12 +382,12 @@ __global__ void cg_err_kernel(double *__restrict__ scal, const double *__restric
   scal[0] = n * n;  // err = NORMS(R)**2
 }
 // P = R + err/errO P
-__global__ void cg_p_kernel(size_t L, const double *__restrict__ scal, const double *__restrict__ R,
-                            double *__restrict__ P) {
+template <class T>
+__global__ void cg_p_kernel(size_t L, const double *__restrict__ scal, const T *__restrict__ R, T *__restrict__ P) {
   if (scal[5] != 0.0) return;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= L) return;
-  P[e] = R[e] + scal[0] / scal[1] * P[e];
+  P[e] = R[e] + (T)(scal[0] / scal[1]) * P[e];
 }
 }  // namespace
 
@@ -377,12 +402,21 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// The stored matrix planes and the -mG^T buffer in the solve's type.
+inline const double *planes(const crimson_lhs *l, double) { return l->d_V.p; }
+inline const float *planes(const crimson_lhs *l, float) { return l->d_V32.p; }
+inline crimson::DBuf<double> &gt_buf(crimson_lhs *l, double) { return l->d_Gt; }
+inline crimson::DBuf<float> &gt_buf(crimson_lhs *l, float) { return l->d_Gt32; }
+
 // Per-solve context: device workspaces sized for this lhs and dof.
+template <class T>
 struct Ctx {
   crimson_lhs *lhs;
   hipStream_t s;
   int N, dof;
-  crimson::DBuf<double> W, fix, partial, dsc, hcol;
+  crimson::DBuf<T> W, fix;
+  crimson::DBuf<double> partial;  // per-workgroup partial sums (viewed as T for FP32 accumulation)
+  crimson::DBuf<double> dsc, hcol;
   crimson::DBuf<int> gate;
   std::vector<double> hbuf;
   bool coupled = false;
@@ -405,9 +439,10 @@ struct Ctx {
   }
 };
 
-MatView view(const Ctx &c, const double *base, int vdof, int r0, int c0, int nr, int nc, const double *W,
-             const double *fix, int wdof) {
-  MatView A;
+template <class T>
+MatView<T> view(const Ctx<T> &c, const T *base, int vdof, int r0, int c0, int nr, int nc, const T *W, const T *fix,
+                int wdof) {
+  MatView<T> A;
   A.base = base;
   A.nnz = (size_t)c.lhs->nnz;
   for (int k = 0; k < 16; ++k) A.plane[k] = 0;
@@ -422,12 +457,13 @@ MatView view(const Ctx &c, const double *base, int vdof, int r0, int c0, int nr,
 }
 
 // y = ya * (A x) + za * z
-int spmv(Ctx &c, const MatView &A, int nr, int nc, const double *x, double *y, double ya = 1.0,
-         const double *z = nullptr, double za = 0.0) {
-  const unsigned T = 256;
-  const unsigned G = crimson::grid_for((size_t)c.N * kRowLanes, T);
+template <class T>
+int spmv(Ctx<T> &c, const MatView<T> &A, int nr, int nc, const T *x, T *y, double ya = 1.0, const T *z = nullptr,
+         double za = 0.0) {
+  const unsigned TB = 256;
+  const unsigned G = crimson::grid_for((size_t)c.N * kRowLanes, TB);
   const int *rp = c.lhs->d_rowptr.p, *cl = c.lhs->d_col.p;
-#define CR_SPMV(R_, C_) spmv_kernel<R_, C_><<<G, T, 0, c.s>>>(c.N, rp, cl, A, x, y, ya, z, za)
+#define CR_SPMV(R_, C_) spmv_kernel<R_, C_, T><<<G, TB, 0, c.s>>>(c.N, rp, cl, A, x, y, (T)ya, z, (T)za)
   if (nr == 3 && nc == 3) CR_SPMV(3, 3);
   else if (nr == 4 && nc == 4) CR_SPMV(4, 4);
   else if (nr == 3 && nc == 1) CR_SPMV(3, 1);
@@ -445,35 +481,47 @@ int spmv(Ctx &c, const MatView &A, int nr, int nc, const double *x, double *y, d
   return CRIMSON_OK;
 }
 
-// dots of v with U_j (j < m) [+ v.v] -> dst on device (m [+1] values)
-int mdot(Ctx &c, size_t L, const double *v, const double *U0, size_t ustride, int m, bool self, double *dst,
-         const int *gate = nullptr) {
+// dots of v with U_j (j < m) [+ v.v] -> dst on device (m [+1] values).
+// Accumulation in T, or in FP64 when acc64 (the NS least-squares system and
+// the norms that set the stopping tests of an FP32 solve).
+template <class T>
+int mdot(Ctx<T> &c, size_t L, const T *v, const T *U0, size_t ustride, int m, bool self, double *dst,
+         const int *gate = nullptr, bool acc64 = false) {
   const int nblk = (int)((L + kChunk - 1) / kChunk);
   const int pst = m + (self ? 1 : 0);
   if (pst > 256) {
     crimson::set_error("more than 255 simultaneous projections");
     return CRIMSON_EUNSUPPORTED;
   }
-  mdot_kernel<<<nblk, 256, 0, c.s>>>(L, v, U0, ustride, m, self ? 1 : 0, c.partial.p, pst, gate);
-  reduce_kernel<<<pst, 256, 0, c.s>>>(nblk, c.partial.p, pst, dst, gate);
+  if (acc64 || sizeof(T) == sizeof(double)) {
+    mdot_kernel<T, double><<<nblk, 256, 0, c.s>>>(L, v, U0, ustride, m, self ? 1 : 0, c.partial.p, pst, gate);
+    reduce_kernel<double><<<pst, 256, 0, c.s>>>(nblk, c.partial.p, pst, dst, gate);
+  } else {
+    T *pt = reinterpret_cast<T *>(c.partial.p);
+    mdot_kernel<T, T><<<nblk, 256, 0, c.s>>>(L, v, U0, ustride, m, self ? 1 : 0, pt, pst, gate);
+    reduce_kernel<T><<<pst, 256, 0, c.s>>>(nblk, pt, pst, dst, gate);
+  }
   CR_HIP(hipGetLastError());
   return CRIMSON_OK;
 }
 
-int axpby(Ctx &c, size_t L, double a, const double *x, double b, double *y) {
-  axpby_kernel<<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, a, x, b, y);
+template <class T>
+int axpby(Ctx<T> &c, size_t L, double a, const T *x, double b, T *y) {
+  axpby_kernel<T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, (T)a, x, (T)b, y);
   CR_HIP(hipGetLastError());
   return CRIMSON_OK;
 }
 
-int sync_read(Ctx &c, const double *d, double *h, size_t n) {
+template <class T>
+int sync_read(Ctx<T> &c, const double *d, double *h, size_t n) {
   CR_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c.s));
   CR_HIP(hipStreamSynchronize(c.s));
   return CRIMSON_OK;
 }
 
-double host_norm(Ctx &c, size_t L, const double *v, int *rc) {
-  *rc = mdot(c, L, v, nullptr, 0, 0, true, c.dsc.p + 40);
+template <class T>
+double host_norm(Ctx<T> &c, size_t L, const T *v, int *rc) {
+  *rc = mdot<T>(c, L, v, nullptr, 0, 0, true, c.dsc.p + 40, nullptr, true);
   if (*rc) return 0.0;
   double h = 0;
   *rc = sync_read(c, c.dsc.p + 40, &h, 1);
@@ -481,7 +529,8 @@ double host_norm(Ctx &c, size_t L, const double *v, int *rc) {
 }
 
 // ADDBCMUL (memLS.f90:206-268) over all coupled faces, X (xdof, N) -> Y.
-int addbcmul(Ctx &c, bool pre, int xdof, const double *X, double *Y) {
+template <class T>
+int addbcmul(Ctx<T> &c, bool pre, int xdof, const T *X, T *Y) {
   if (!c.coupled) return CRIMSON_OK;
   auto &faces = c.lhs->face;
   // all face dots first (PRE reads the unmodified vector, as unCondU does)
@@ -503,22 +552,24 @@ int addbcmul(Ctx &c, bool pre, int xdof, const double *X, double *Y) {
 
 // GMRES (memLS.f90:681-788) for the operator x -> A x + ADD(x), left
 // preconditioned by PRE when faces are coupled; R, X are (nd, N) device.
+template <class T>
 struct Workspace {
-  crimson::DBuf<double> U;   // (sD+1) vectors
-  crimson::DBuf<double> tmp;
+  crimson::DBuf<T> U;  // (sD+1) vectors
+  crimson::DBuf<T> tmp;
 };
 
-int gmres_dev(Ctx &c, const MatView &A, int nd, SubLs &ls, const double *R, double *X, Workspace &ws, bool pre_ok,
+template <class T>
+int gmres_dev(Ctx<T> &c, const MatView<T> &A, int nd, SubLs &ls, const T *R, T *X, Workspace<T> &ws, bool pre_ok,
               bool count_first = false) {
   const size_t L = (size_t)nd * c.N;
   const int sD = ls.sD;
   CR_TRY(ws.U.alloc(L * (sD + 1)));
-  double *U = ws.U.p;
+  T *U = ws.U.p;
   std::vector<double> h((size_t)(sD + 1) * sD, 0.0), y(sD), cs(sD), sn(sD), err(sD + 1);
   auto H = [&](int i, int j) -> double & { return h[(size_t)i + (size_t)(sD + 1) * j]; };
   const double t0 = now_s();
   ls.suc = 0;
-  CR_HIP(hipMemsetAsync(X, 0, L * sizeof(double), c.s));
+  CR_HIP(hipMemsetAsync(X, 0, L * sizeof(T), c.s));
   double eps = 0.0;
   int rc = CRIMSON_OK;
   std::vector<double> hc(2 * (sD + 2) + 4);
@@ -526,7 +577,7 @@ int gmres_dev(Ctx &c, const MatView &A, int nd, SubLs &ls, const double *R, doub
     if (l == 1) {
       // X = 0 here, so R - A X is R (GMRESV still counts the product)
       if (count_first) ls.itr = ls.itr + 1;
-      CR_HIP(hipMemcpyAsync(U, R, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+      CR_HIP(hipMemcpyAsync(U, R, L * sizeof(T), hipMemcpyDeviceToDevice, c.s));
     } else {
       ls.itr = ls.itr + 1;
       CR_TRY(spmv(c, A, nd, nd, X, U));
@@ -534,7 +585,7 @@ int gmres_dev(Ctx &c, const MatView &A, int nd, SubLs &ls, const double *R, doub
       CR_TRY(axpby(c, L, 1.0, R, -1.0, U));
     }
     if (c.coupled && pre_ok) {
-      CR_HIP(hipMemcpyAsync(ws.tmp.p, U, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+      CR_HIP(hipMemcpyAsync(ws.tmp.p, U, L * sizeof(T), hipMemcpyDeviceToDevice, c.s));
       CR_TRY(addbcmul(c, true, nd, ws.tmp.p, U));
     }
     err[0] = host_norm(c, L, U, &rc);
@@ -555,25 +606,26 @@ int gmres_dev(Ctx &c, const MatView &A, int nd, SubLs &ls, const double *R, doub
     int i;
     for (i = 1; i <= sD; ++i) {
       ls.itr = ls.itr + 1;
-      double *ui = U + L * (size_t)i;
-      const double *uprev = U + L * (size_t)(i - 1);
+      T *ui = U + L * (size_t)i;
+      const T *uprev = U + L * (size_t)(i - 1);
       CR_TRY(spmv(c, A, nd, nd, uprev, ui));
       CR_TRY(addbcmul(c, false, nd, uprev, ui));
       if (c.coupled && pre_ok) {
-        CR_HIP(hipMemcpyAsync(ws.tmp.p, ui, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+        CR_HIP(hipMemcpyAsync(ws.tmp.p, ui, L * sizeof(T), hipMemcpyDeviceToDevice, c.s));
         CR_TRY(addbcmul(c, true, nd, ws.tmp.p, ui));
       }
       // CGS pass 1: h1 = U^T w, n0 = w.w ; w -= U h1 ; n1 = w.w
       double *h1 = c.hcol.p, *h2 = c.hcol.p + 256, *nrm = c.hcol.p + 512;
       const int nblk = (int)((L + kChunk - 1) / kChunk);
       CR_TRY(mdot(c, L, ui, U, L, i, true, h1));  // h1[i] = n0
-      msub_norm_kernel<<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h1, c.partial.p, 1, nullptr);
-      reduce_kernel<<<1, 256, 0, c.s>>>(nblk, c.partial.p, 1, nrm + 1, nullptr);
+      T *pt = reinterpret_cast<T *>(c.partial.p);
+      msub_norm_kernel<T><<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h1, pt, 1, nullptr);
+      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, nrm + 1, nullptr);
       dgks_kernel<<<1, 1, 0, c.s>>>(h1 + i, nrm + 1, c.gate.p);
       // DGKS pass 2 (gated)
       CR_TRY(mdot(c, L, ui, U, L, i, false, h2, c.gate.p));
-      msub_norm_kernel<<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h2, c.partial.p, 1, c.gate.p);
-      reduce_kernel<<<1, 256, 0, c.s>>>(nblk, c.partial.p, 1, nrm + 2, c.gate.p);
+      msub_norm_kernel<T><<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h2, pt, 1, c.gate.p);
+      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, nrm + 2, c.gate.p);
       CR_HIP(hipGetLastError());
       // read h1[0..i], h2[0..i-1], n1, n2, gate
       CR_HIP(hipMemcpyAsync(hc.data(), h1, (i + 1) * sizeof(double), hipMemcpyDeviceToHost, c.s));
@@ -610,7 +662,7 @@ int gmres_dev(Ctx &c, const MatView &A, int nd, SubLs &ls, const double *R, doub
       y[j - 1] = y[j - 1] / H(j - 1, j - 1);
     }
     CR_HIP(hipMemcpyAsync(c.hcol.p + 768, y.data(), i * sizeof(double), hipMemcpyHostToDevice, c.s));
-    maxpy_kernel<<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, X, U, L, i, c.hcol.p + 768, 1.0);
+    maxpy_kernel<T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, X, U, L, i, c.hcol.p + 768, 1.0);
     CR_HIP(hipGetLastError());
     CR_HIP(hipStreamSynchronize(c.s));  // y is a host buffer read by the copy above
     ls.fNorm = std::fabs(err[i]);
@@ -623,8 +675,8 @@ int gmres_dev(Ctx &c, const MatView &A, int nd, SubLs &ls, const double *R, doub
 
 // Generic CG driver on device vectors (memLS CGRAD / CGRADV / CGRADS).
 // op(P, SP) computes SP for the iteration; R is overwritten with X.
-template <class Op>
-int cg_dev(Ctx &c, size_t L, SubLs &ls, double *R, Op op, double *X, double *P, double *SP, bool schur) {
+template <class T, class Op>
+int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool schur) {
   const double t0 = now_s();
   double *scal = c.dsc.p;  // 0 err 1 errO 2 eps2 3 den 4 alpha 5 done 6 itr 7 mItr 8 suc
   ls.suc = 0;
@@ -636,8 +688,8 @@ int cg_dev(Ctx &c, size_t L, SubLs &ls, double *R, Op op, double *X, double *P, 
   eps = eps * eps;
   double hs[9] = {errO0 * errO0, errO0 * errO0, eps, 0, 0, 0, 0, (double)ls.mItr, 0};
   CR_HIP(hipMemcpyAsync(scal, hs, sizeof(hs), hipMemcpyHostToDevice, c.s));
-  CR_HIP(hipMemsetAsync(X, 0, L * sizeof(double), c.s));
-  CR_HIP(hipMemcpyAsync(P, R, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+  CR_HIP(hipMemsetAsync(X, 0, L * sizeof(T), c.s));
+  CR_HIP(hipMemcpyAsync(P, R, L * sizeof(T), hipMemcpyDeviceToDevice, c.s));
   const int nblk = (int)((L + kChunk - 1) / kChunk);
   const int batch = 8;
   double back[9];
@@ -646,13 +698,14 @@ int cg_dev(Ctx &c, size_t L, SubLs &ls, double *R, Op op, double *X, double *P, 
       cg_check_kernel<<<1, 1, 0, c.s>>>(scal);
       CR_TRY(op(P, SP, scal));
       // den = P . SP
-      mdot_kernel<<<nblk, 256, 0, c.s>>>(L, SP, P, 0, 1, 0, c.partial.p, 1, nullptr);
-      reduce_kernel<<<1, 256, 0, c.s>>>(nblk, c.partial.p, 1, scal + 3, nullptr);
+      T *pt = reinterpret_cast<T *>(c.partial.p);
+      mdot_kernel<T, T><<<nblk, 256, 0, c.s>>>(L, SP, P, 0, 1, 0, pt, 1, nullptr);
+      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, scal + 3, nullptr);
       cg_alpha_kernel<<<1, 1, 0, c.s>>>(scal);
-      cg_update_kernel<<<nblk, 256, 0, c.s>>>(L, scal, X, R, P, SP, c.partial.p);
-      reduce_kernel<<<1, 256, 0, c.s>>>(nblk, c.partial.p, 1, scal + 9, nullptr);
+      cg_update_kernel<T><<<nblk, 256, 0, c.s>>>(L, scal, X, R, P, SP, pt);
+      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, scal + 9, nullptr);
       cg_err_kernel<<<1, 1, 0, c.s>>>(scal, scal + 9);
-      cg_p_kernel<<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, scal, R, P);
+      cg_p_kernel<T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, scal, R, P);
     }
     CR_HIP(hipGetLastError());
     CR_HIP(hipMemcpyAsync(back, scal, sizeof(back), hipMemcpyDeviceToHost, c.s));
@@ -660,7 +713,7 @@ int cg_dev(Ctx &c, size_t L, SubLs &ls, double *R, Op op, double *X, double *P, 
     if (back[5] != 0.0) break;
   }
   // the flag is raised at the top of the iteration after the last one
-  CR_HIP(hipMemcpyAsync(R, X, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+  CR_HIP(hipMemcpyAsync(R, X, L * sizeof(T), hipMemcpyDeviceToDevice, c.s));
   const double err = back[0], errO = back[1];
   ls.suc = back[8] != 0.0;
   ls.fNorm = std::sqrt(err);
@@ -676,13 +729,15 @@ int cg_dev(Ctx &c, size_t L, SubLs &ls, double *R, Op op, double *X, double *P, 
 
 // NSSOLVER (memLS.f90:1460-1759) on device; Ri (4,N) device in/out
 // (preconditioned RHS in, preconditioned solution out).
-int nssolver_dev(Ctx &c, crimson_ls *ls, double *Ri) {
+template <class T>
+int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
   crimson_lhs *l = c.lhs;
   const int N = c.N, nsd = 3;
   const size_t mn = (size_t)nsd * N;
   int iB = ls->RI.mItr;
   const int nB = 2 * iB;
-  crimson::DBuf<double> Rm, Rc, Rmi, Rci, U, P, MU, MP, t1, t2, t3, t4, dxB;
+  crimson::DBuf<T> Rm, Rc, Rmi, Rci, U, P, MU, MP, t1, t2, t3, t4;
+  crimson::DBuf<double> dxB;
   CR_TRY(Rm.alloc(mn));
   CR_TRY(Rc.alloc(N));
   CR_TRY(Rmi.alloc(mn));
@@ -696,13 +751,13 @@ int nssolver_dev(Ctx &c, crimson_ls *ls, double *Ri) {
   CR_TRY(t3.alloc(N));
   CR_TRY(t4.alloc(N));
   CR_TRY(dxB.alloc(nB));
-  Workspace ws;
+  Workspace<T> ws;
   CR_TRY(ws.tmp.alloc(mn));
   const unsigned gN = crimson::grid_for(N, 256);
   extract_kernel<<<gN, 256, 0, c.s>>>(N, 4, 0, 3, Ri, Rmi.p);
   extract_kernel<<<gN, 256, 0, c.s>>>(N, 4, 3, 1, Ri, Rci.p);
-  CR_HIP(hipMemcpyAsync(Rm.p, Rmi.p, mn * sizeof(double), hipMemcpyDeviceToDevice, c.s));
-  CR_HIP(hipMemcpyAsync(Rc.p, Rci.p, N * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+  CR_HIP(hipMemcpyAsync(Rm.p, Rmi.p, mn * sizeof(T), hipMemcpyDeviceToDevice, c.s));
+  CR_HIP(hipMemcpyAsync(Rc.p, Rci.p, N * sizeof(T), hipMemcpyDeviceToDevice, c.s));
   int rc = CRIMSON_OK;
   const double a1 = host_norm(c, mn, Rm.p, &rc);
   if (rc) return rc;
@@ -719,17 +774,18 @@ int nssolver_dev(Ctx &c, crimson_ls *ls, double *Ri) {
   const double t0 = now_s();
   eps = std::max(ls->RI.absTol, ls->RI.relTol * eps);
   // DEPART: views on the 4x4 planes (memLS.f90:1612-1679); Gt = -mG^T
-  const double *V = l->d_V.p;
-  const MatView mK = view(c, V, 4, 0, 0, 3, 3, c.W.p, c.fix.p, 4);
-  const MatView mG = view(c, V, 4, 0, 3, 3, 1, c.W.p, nullptr, 4);
-  const MatView mD = view(c, V, 4, 3, 0, 1, 3, c.W.p, nullptr, 4);
-  const MatView mL = view(c, V, 4, 3, 3, 1, 1, c.W.p, c.fix.p, 4);
-  MatView Gt = mD;
+  const T *V = planes(l, T(0));
+  const MatView<T> mK = view<T>(c, V, 4, 0, 0, 3, 3, c.W.p, c.fix.p, 4);
+  const MatView<T> mG = view<T>(c, V, 4, 0, 3, 3, 1, c.W.p, nullptr, 4);
+  const MatView<T> mD = view<T>(c, V, 4, 3, 0, 1, 3, c.W.p, nullptr, 4);
+  const MatView<T> mL = view<T>(c, V, 4, 3, 3, 1, 1, c.W.p, c.fix.p, 4);
+  MatView<T> Gt = mD;
   if (!l->gt_is_d) {
-    CR_TRY(l->d_Gt.alloc((size_t)3 * l->nnz));
-    gt_kernel<<<crimson::grid_for(l->nnz, 256), 256, 0, c.s>>>(l->nnz, V, l->d_tr.p, l->d_Gt.p);
+    crimson::DBuf<T> &gtb = gt_buf(l, T(0));
+    CR_TRY(gtb.alloc((size_t)3 * l->nnz));
+    gt_kernel<T><<<crimson::grid_for(l->nnz, 256), 256, 0, c.s>>>(l->nnz, V, l->d_tr.p, gtb.p);
     CR_HIP(hipGetLastError());
-    Gt = view(c, l->d_Gt.p, 1, 0, 0, 1, 3, c.W.p, nullptr, 4);
+    Gt = view<T>(c, gtb.p, 1, 0, 0, 1, 3, c.W.p, nullptr, 4);
     Gt.plane[0] = 0;
     Gt.plane[1] = 1;
     Gt.plane[2] = 2;
@@ -748,11 +804,11 @@ int nssolver_dev(Ctx &c, crimson_ls *ls, double *Ri) {
       for (int i = 0; i < nc; ++i) fa.nS = fa.nS + vm[i + (size_t)fa.dof * a] * vm[i + (size_t)fa.dof * a];
   }
   // Schur operator of CGRAD: SP = L P - Gt (G P + PRE)
-  auto schur_op = [&](const double *Pv, double *SPv, const double *scal) -> int {
+  auto schur_op = [&](const T *Pv, T *SPv, const double *scal) -> int {
     (void)scal;
     CR_TRY(spmv(c, mG, 3, 1, Pv, t1.p));
     if (c.coupled) {
-      CR_HIP(hipMemcpyAsync(ws.tmp.p, t1.p, mn * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+      CR_HIP(hipMemcpyAsync(ws.tmp.p, t1.p, mn * sizeof(T), hipMemcpyDeviceToDevice, c.s));
       CR_TRY(addbcmul(c, true, 3, ws.tmp.p, t1.p));
     }
     CR_TRY(spmv(c, Gt, 1, 3, t1.p, t2.p));
@@ -766,9 +822,9 @@ int nssolver_dev(Ctx &c, crimson_ls *ls, double *Ri) {
     iB = 2 * i - 1;
     iBB = 2 * i;
     ls->RI.dB = ls->RI.fNorm;
-    double *Ui = U.p + mn * (i - 1), *Pi = P.p + (size_t)N * (i - 1);
-    double *MUb = MU.p + mn * (iB - 1), *MUbb = MU.p + mn * (iBB - 1);
-    double *MPb = MP.p + (size_t)N * (iB - 1), *MPbb = MP.p + (size_t)N * (iBB - 1);
+    T *Ui = U.p + mn * (i - 1), *Pi = P.p + (size_t)N * (i - 1);
+    T *MUb = MU.p + mn * (iB - 1), *MUbb = MU.p + mn * (iBB - 1);
+    T *MPb = MP.p + (size_t)N * (iB - 1), *MPbb = MP.p + (size_t)N * (iBB - 1);
     CR_TRY(gmres_dev(c, mK, 3, ls->GM, Rm.p, Ui, ws, true));
     CR_TRY(spmv(c, mD, 1, 3, Ui, Pi, -1.0, Rc.p, 1.0));  // P = Rc - D U
     CR_TRY(cg_dev(c, (size_t)N, ls->CG, Pi, schur_op, t3.p, t4.p, MPb, true));
@@ -782,12 +838,12 @@ int nssolver_dev(Ctx &c, crimson_ls *ls, double *Ri) {
     CR_TRY(spmv(c, mD, 1, 3, Ui, MPbb));
     // Gram matrix rows iB, iBB: dots with all previous columns and the RHS
     for (int k = iB; k <= iBB; ++k) {
-      const double *MUk = MU.p + mn * (k - 1), *MPk = MP.p + (size_t)N * (k - 1);
+      const T *MUk = MU.p + mn * (k - 1), *MPk = MP.p + (size_t)N * (k - 1);
       // MU_j . MU_k for j < k, then MU_k.MU_k ; plus MU_k . Rmi
-      CR_TRY(mdot(c, mn, MUk, MU.p, mn, k - 1, true, c.hcol.p));
-      CR_TRY(mdot(c, mn, MUk, Rmi.p, 0, 1, false, c.hcol.p + 256));
-      CR_TRY(mdot(c, N, MPk, MP.p, N, k - 1, true, c.hcol.p + 512));
-      CR_TRY(mdot(c, N, MPk, Rci.p, 0, 1, false, c.hcol.p + 768));
+      CR_TRY(mdot<T>(c, mn, MUk, MU.p, mn, k - 1, true, c.hcol.p, nullptr, true));
+      CR_TRY(mdot<T>(c, mn, MUk, Rmi.p, 0, 1, false, c.hcol.p + 256, nullptr, true));
+      CR_TRY(mdot<T>(c, N, MPk, MP.p, N, k - 1, true, c.hcol.p + 512, nullptr, true));
+      CR_TRY(mdot<T>(c, N, MPk, Rci.p, 0, 1, false, c.hcol.p + 768, nullptr, true));
       std::vector<double> h1(k + 1), h2(k + 1), r1(1), r2(1);
       CR_HIP(hipMemcpyAsync(h1.data(), c.hcol.p, (k) * sizeof(double), hipMemcpyDeviceToHost, c.s));
       CR_HIP(hipMemcpyAsync(r1.data(), c.hcol.p + 256, sizeof(double), hipMemcpyDeviceToHost, c.s));
@@ -885,8 +941,8 @@ int nssolver_dev(Ctx &c, crimson_ls *ls, double *Ri) {
     }
     CR_HIP(hipMemcpyAsync(c.hcol.p, cu.data(), nit * sizeof(double), hipMemcpyHostToDevice, c.s));
     CR_HIP(hipMemcpyAsync(c.hcol.p + 256, cp.data(), nit * sizeof(double), hipMemcpyHostToDevice, c.s));
-    CR_HIP(hipMemsetAsync(Rmi.p, 0, mn * sizeof(double), c.s));
-    CR_HIP(hipMemsetAsync(Rci.p, 0, N * sizeof(double), c.s));
+    CR_HIP(hipMemsetAsync(Rmi.p, 0, mn * sizeof(T), c.s));
+    CR_HIP(hipMemsetAsync(Rci.p, 0, N * sizeof(T), c.s));
     maxpy_kernel<<<crimson::grid_for(mn, 256), 256, 0, c.s>>>(mn, Rmi.p, U.p, mn, nit, c.hcol.p, 1.0);
     maxpy_kernel<<<crimson::grid_for(N, 256), 256, 0, c.s>>>(N, Rci.p, P.p, N, nit, c.hcol.p + 256, 1.0);
     insert_kernel<<<gN, 256, 0, c.s>>>(N, 4, 0, 3, Rmi.p, Ri);
@@ -899,16 +955,11 @@ int nssolver_dev(Ctx &c, crimson_ls *ls, double *Ri) {
   return CRIMSON_OK;
 }
 
-int solve_dev(crimson_lhs *l, crimson_ls *ls, int dof, double *d_Ri, const int *incL, const double *res) {
-  if (l->vdof != dof || !l->d_V.p) {
-    crimson::set_error("lhs holds no matrix of block size %d (stored: %d)", dof, l->vdof);
-    return CRIMSON_ESTATE;
-  }
-  if (ls->LS_type == CRIMSON_LS_NS && dof != 4) {
-    crimson::set_error("LS_TYPE_NS needs dof = 4 (got %d)", dof);
-    return CRIMSON_EUNSUPPORTED;
-  }
-  Ctx c;
+// One memLS_SOLVE in arithmetic type T.  d_Ri is the FP64 (dof,N) right-hand
+// side in and solution out; for T = float the solve runs on an FP32 copy.
+template <class T>
+int solve_impl(crimson_lhs *l, crimson_ls *ls, int dof, double *d_Ri, const int *incL, const double *res) {
+  Ctx<T> c;
   CR_TRY(c.init(l, dof));
   // face flags (memLS.f90:1965-1986)
   for (int f = 0; f < l->nFaces; ++f) {
@@ -926,33 +977,43 @@ int solve_dev(crimson_lhs *l, crimson_ls *ls, int dof, double *d_Ri, const int *
   const int N = l->nNo;
   const size_t L = (size_t)dof * N;
   const unsigned gN = crimson::grid_for(N, 256);
+  const T *V = planes(l, T(0));
   // PRECOND (memLS.f90:1763-1943) as the W/fix vectors
-  precond_diag_kernel<<<gN, 256, 0, c.s>>>(N, dof, l->vdof, l->d_V.p, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
+  precond_diag_kernel<T><<<gN, 256, 0, c.s>>>(N, dof, l->vdof, V, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
   for (auto &fa : l->face)
     if (fa.created && fa.inc && fa.bGrp == CRIMSON_BC_DIR && fa.nNo > 0)
-      precond_face_kernel<<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, dof, fa.dof, fa.d_glob.p,
-                                                                           fa.d_val.p, c.W.p);
-  precond_fix_kernel<<<gN, 256, 0, c.s>>>(N, dof, l->vdof, l->d_V.p, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
+      precond_face_kernel<T><<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, dof, fa.dof, fa.d_glob.p,
+                                                                              fa.d_val.p, c.W.p);
+  precond_fix_kernel<T><<<gN, 256, 0, c.s>>>(N, dof, l->vdof, V, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
   for (auto &fa : l->face)
     if (fa.coupled) {
       CR_TRY(fa.d_valM.alloc((size_t)fa.dof * fa.nNo));
-      face_valm_kernel<<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, dof, fa.dof, fa.d_glob.p,
-                                                                         fa.d_val.p, c.W.p, fa.d_valM.p);
+      face_valm_kernel<T><<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, dof, fa.dof, fa.d_glob.p,
+                                                                            fa.d_val.p, c.W.p, fa.d_valM.p);
     }
-  mul_kernel<<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, d_Ri, c.W.p, d_Ri);  // R = W R
+  // R = W R (into the working vector of type T)
+  crimson::DBuf<T> Rw;
+  T *R = nullptr;
+  if (sizeof(T) == sizeof(double)) {
+    R = reinterpret_cast<T *>(d_Ri);
+  } else {
+    CR_TRY(Rw.alloc(L));
+    R = Rw.p;
+  }
+  mul_kernel<double, T, T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, d_Ri, c.W.p, R);
   CR_HIP(hipGetLastError());
   int rc = CRIMSON_OK;
   if (ls->LS_type == CRIMSON_LS_NS) {
-    rc = nssolver_dev(c, ls, d_Ri);
+    rc = nssolver_dev<T>(c, ls, R);
   } else if (ls->LS_type == CRIMSON_LS_GMRES) {
     // GMRESV (memLS.f90:792-883): no PRE application, restart from X
-    Workspace ws;
+    Workspace<T> ws;
     CR_TRY(ws.tmp.alloc(L));
-    crimson::DBuf<double> X;
+    crimson::DBuf<T> X;
     CR_TRY(X.alloc(L));
-    const MatView A = view(c, l->d_V.p, dof, 0, 0, dof, dof, c.W.p, c.fix.p, dof);
+    const MatView<T> A = view<T>(c, V, dof, 0, 0, dof, dof, c.W.p, c.fix.p, dof);
     ls->RI.itr = 0;
-    const double nr = host_norm(c, L, d_Ri, &rc);
+    const double nr = host_norm(c, L, R, &rc);
     if (rc) return rc;
     if (nr <= ls->RI.absTol) {
       ls->RI.iNorm = nr;
@@ -960,30 +1021,43 @@ int solve_dev(crimson_lhs *l, crimson_ls *ls, int dof, double *d_Ri, const int *
       ls->RI.suc = 0;
       ls->RI.callD = 0.0;
       ls->RI.dB = 0.0;
-      CR_HIP(hipMemsetAsync(d_Ri, 0, L * sizeof(double), c.s));
+      CR_HIP(hipMemsetAsync(R, 0, L * sizeof(T), c.s));
     } else {
       // GMRESV counts the residual product of every cycle (memLS.f90:827-829)
-      rc = gmres_dev(c, A, dof, ls->RI, d_Ri, X.p, ws, false, true);
+      rc = gmres_dev<T>(c, A, dof, ls->RI, R, X.p, ws, false, true);
       if (rc) return rc;
-      CR_HIP(hipMemcpyAsync(d_Ri, X.p, L * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+      CR_HIP(hipMemcpyAsync(R, X.p, L * sizeof(T), hipMemcpyDeviceToDevice, c.s));
     }
   } else if (ls->LS_type == CRIMSON_LS_CG) {
-    const MatView A = view(c, l->d_V.p, dof, 0, 0, dof, dof, c.W.p, c.fix.p, dof);
-    crimson::DBuf<double> X, P, SP;
+    const MatView<T> A = view<T>(c, V, dof, 0, 0, dof, dof, c.W.p, c.fix.p, dof);
+    crimson::DBuf<T> X, P, SP;
     CR_TRY(X.alloc(L));
     CR_TRY(P.alloc(L));
     CR_TRY(SP.alloc(L));
-    auto op = [&](const double *Pv, double *SPv, const double *) -> int { return spmv(c, A, dof, dof, Pv, SPv); };
-    rc = cg_dev(c, L, ls->RI, d_Ri, op, X.p, P.p, SP.p, false);
+    auto op = [&](const T *Pv, T *SPv, const double *) -> int { return spmv<T>(c, A, dof, dof, Pv, SPv); };
+    rc = cg_dev<T>(c, L, ls->RI, R, op, X.p, P.p, SP.p, false);
   } else {
     crimson::set_error("LS_type %d not defined", ls->LS_type);
     return CRIMSON_EARG;
   }
   if (rc) return rc;
-  mul_kernel<<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, d_Ri, c.W.p, d_Ri);  // R = R W
+  mul_kernel<T, T, double><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, R, c.W.p, d_Ri);  // R = R W
   CR_HIP(hipGetLastError());
   CR_HIP(hipStreamSynchronize(c.s));
   return CRIMSON_OK;
+}
+
+int solve_dev(crimson_lhs *l, crimson_ls *ls, int dof, double *d_Ri, const int *incL, const double *res) {
+  if (l->vdof != dof || !(l->prec == 32 ? (const void *)l->d_V32.p : (const void *)l->d_V.p)) {
+    crimson::set_error("lhs holds no matrix of block size %d (stored: %d)", dof, l->vdof);
+    return CRIMSON_ESTATE;
+  }
+  if (ls->LS_type == CRIMSON_LS_NS && dof != 4) {
+    crimson::set_error("LS_TYPE_NS needs dof = 4 (got %d)", dof);
+    return CRIMSON_EUNSUPPORTED;
+  }
+  if (l->prec == 32) return solve_impl<float>(l, ls, dof, d_Ri, incL, res);
+  return solve_impl<double>(l, ls, dof, d_Ri, incL, res);
 }
 
 }  // namespace
@@ -1198,29 +1272,60 @@ extern "C" int crimson_lhs_set_val(crimson_lhs *l, int dof, const double *Val) {
   crimson::DBuf<double> aos;
   CR_TRY(aos.upload(Val, m * nnz, l->stream));
   CR_TRY(l->d_V.alloc(m * nnz));
-  aos_to_planes_kernel<<<crimson::grid_for(nnz, 256), 256, 0, l->stream>>>(nnz, (int)m, aos.p, l->d_V.p);
+  aos_to_planes_kernel<double, double><<<crimson::grid_for(nnz, 256), 256, 0, l->stream>>>(nnz, (int)m, aos.p,
+                                                                                            l->d_V.p);
   CR_HIP(hipGetLastError());
   CR_HIP(hipStreamSynchronize(l->stream));
   l->vdof = dof;
+  l->prec = 64;
   l->gt_is_d = false;
   return CRIMSON_OK;
 }
 
-__global__ void planes_to_aos_kernel(size_t nnz, int m, const double *__restrict__ V, double *__restrict__ aos) {
+extern "C" int crimson_lhs_set_precision(crimson_lhs *l, int bits) {
+  if (!l || (bits != 32 && bits != 64)) {
+    crimson::set_error("crimson_lhs_set_precision: invalid arguments (bits 32 or 64)");
+    return CRIMSON_EARG;
+  }
+  if (bits == l->prec) return CRIMSON_OK;
+  const size_t n = (size_t)l->vdof * l->vdof * l->nnz;
+  if (l->vdof == 0 || n == 0) {
+    crimson::set_error("crimson_lhs_set_precision: no matrix held");
+    return CRIMSON_ESTATE;
+  }
+  const unsigned G = crimson::grid_for(n, 256);
+  if (bits == 32) {
+    CR_TRY(l->d_V32.alloc(n));
+    convert_kernel<double, float><<<G, 256, 0, l->stream>>>(n, l->d_V.p, l->d_V32.p);
+  } else {
+    CR_TRY(l->d_V.alloc(n));
+    convert_kernel<float, double><<<G, 256, 0, l->stream>>>(n, l->d_V32.p, l->d_V.p);
+  }
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipStreamSynchronize(l->stream));
+  l->prec = bits;
+  return CRIMSON_OK;
+}
+
+template <class T>
+__global__ void planes_to_aos_kernel(size_t nnz, int m, const T *__restrict__ V, double *__restrict__ aos) {
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
-  for (int p = 0; p < m; ++p) aos[(size_t)m * k + p] = V[(size_t)p * nnz + k];
+  for (int p = 0; p < m; ++p) aos[(size_t)m * k + p] = (double)V[(size_t)p * nnz + k];
 }
 
 extern "C" int crimson_lhs_get_val(crimson_lhs *l, int dof, double *Val) {
-  if (!l || !Val || dof != l->vdof || !l->d_V.p) {
+  if (!l || !Val || dof != l->vdof || !(l->prec == 32 ? (const void *)l->d_V32.p : (const void *)l->d_V.p)) {
     crimson::set_error("crimson_lhs_get_val: no matrix of block size %d held", dof);
     return CRIMSON_EARG;
   }
   const size_t m = (size_t)dof * dof, nnz = l->nnz;
   crimson::DBuf<double> aos;
   CR_TRY(aos.alloc(m * nnz));
-  planes_to_aos_kernel<<<crimson::grid_for(nnz, 256), 256, 0, l->stream>>>(nnz, (int)m, l->d_V.p, aos.p);
+  if (l->prec == 32)
+    planes_to_aos_kernel<float><<<crimson::grid_for(nnz, 256), 256, 0, l->stream>>>(nnz, (int)m, l->d_V32.p, aos.p);
+  else
+    planes_to_aos_kernel<double><<<crimson::grid_for(nnz, 256), 256, 0, l->stream>>>(nnz, (int)m, l->d_V.p, aos.p);
   CR_HIP(hipGetLastError());
   CR_HIP(hipMemcpyAsync(Val, aos.p, m * nnz * sizeof(double), hipMemcpyDeviceToHost, l->stream));
   CR_HIP(hipStreamSynchronize(l->stream));
@@ -1269,23 +1374,44 @@ extern "C" int crimson_ls_stats(const crimson_ls *ls, int sub, double *o) {
   return CRIMSON_OK;
 }
 
-extern "C" int crimson_lhs_spmv_dev(crimson_lhs *l, int dof, const double *d_x, double *d_y, int reps, double *ms) {
-  if (!l || !d_x || !d_y || reps < 1 || !ms || !l->d_V.p || (dof != 3 && dof != 4) || l->vdof != 4) {
-    crimson::set_error("crimson_lhs_spmv_dev: invalid arguments (needs a 4x4 matrix, dof 3 or 4)");
-    return CRIMSON_EARG;
-  }
-  Ctx c;
+namespace {
+template <class T>
+int spmv_timed(crimson_lhs *l, int dof, const T *x, T *y, int reps, double *ms) {
+  Ctx<T> c;
   CR_TRY(c.init(l, 4));
   const unsigned gN = crimson::grid_for(l->nNo, 256);
-  precond_diag_kernel<<<gN, 256, 0, c.s>>>(l->nNo, 4, 4, l->d_V.p, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
-  const MatView A = view(c, l->d_V.p, 4, 0, 0, dof, dof, c.W.p, c.fix.p, 4);
-  CR_TRY(spmv(c, A, dof, dof, d_x, d_y));  // warm-up
+  const T *V = planes(l, T(0));
+  precond_diag_kernel<T><<<gN, 256, 0, c.s>>>(l->nNo, 4, 4, V, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
+  const MatView<T> A = view<T>(c, V, 4, 0, 0, dof, dof, c.W.p, c.fix.p, 4);
+  CR_TRY(spmv<T>(c, A, dof, dof, x, y));  // warm-up
   CR_HIP(hipEventRecord(l->ev0, c.s));
-  for (int r = 0; r < reps; ++r) CR_TRY(spmv(c, A, dof, dof, d_x, d_y));
+  for (int r = 0; r < reps; ++r) CR_TRY(spmv<T>(c, A, dof, dof, x, y));
   CR_HIP(hipEventRecord(l->ev1, c.s));
   CR_HIP(hipEventSynchronize(l->ev1));
   float t = 0.f;
   CR_HIP(hipEventElapsedTime(&t, l->ev0, l->ev1));
   *ms = t / reps;
+  return CRIMSON_OK;
+}
+}  // namespace
+
+extern "C" int crimson_lhs_spmv_dev(crimson_lhs *l, int dof, const double *d_x, double *d_y, int reps, double *ms) {
+  if (!l || !d_x || !d_y || reps < 1 || !ms || (dof != 3 && dof != 4) || l->vdof != 4 ||
+      !(l->prec == 32 ? (const void *)l->d_V32.p : (const void *)l->d_V.p)) {
+    crimson::set_error("crimson_lhs_spmv_dev: invalid arguments (needs a 4x4 matrix, dof 3 or 4)");
+    return CRIMSON_EARG;
+  }
+  if (l->prec == 64) return spmv_timed<double>(l, dof, d_x, d_y, reps, ms);
+  // FP32 matrix: the product runs on FP32 copies of x and y
+  const size_t L = (size_t)dof * l->nNo;
+  crimson::DBuf<float> xf, yf;
+  CR_TRY(xf.alloc(L));
+  CR_TRY(yf.alloc(L));
+  convert_kernel<double, float><<<crimson::grid_for(L, 256), 256, 0, l->stream>>>(L, d_x, xf.p);
+  CR_HIP(hipGetLastError());
+  CR_TRY(spmv_timed<float>(l, dof, xf.p, yf.p, reps, ms));
+  convert_kernel<float, double><<<crimson::grid_for(L, 256), 256, 0, l->stream>>>(L, yf.p, d_y);
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipStreamSynchronize(l->stream));
   return CRIMSON_OK;
 }

@@ -55,10 +55,13 @@ typedef struct {
   int iconvflow;  /* only 2 (advective form) */
   int idiff;      /* only 1 (global flux reconstruction, row-sum lumping) */
   int ipord;      /* only 1 (P1 tets, 4-point rule) */
-  int precision;  /* 64: FP64 formation; 32: FP32 formation with FP32
-                     accumulation (FP64 kept for tau on elements whose
+  int precision;  /* 64: FP64 formation; 32: FP64 residual with the
+                     tangent formed and assembled in FP32 with FP32
+                     accumulation (metric and tau in FP64 on elements whose
                      aspect ratio exceeds aspect_fp64) */
-  double aspect_fp64; /* aspect-ratio threshold of the FP64 tau island */
+  double aspect_fp64; /* aspect-ratio threshold of the FP64 tau island:
+                         longest edge over shortest altitude, 1 for the
+                         regular tet; used when precision == 32 (>= 1) */
 } crimson_form_params;
 
 typedef struct crimson_form crimson_form;
@@ -108,6 +111,18 @@ int crimson_form_bc3res_dev(crimson_form *f, double *d_res, int res4_layout);
  * on the handle's stream: out[0] flux pass, out[1] element kernel,
  * out[2] assembly, out[3] total. */
 int crimson_form_timing(const crimson_form *f, double *out4);
+
+/* Adjoint of the interior residual (the vector-Jacobian product the
+ * estimation subsystem needs for parameter fitting; no counterpart in the
+ * reference, which has no derivative code): for a cotangent d_w(nshg,4)
+ * laid out like res, d_ybar(nshg,4) = w^T d res/d y with y = (u1,u2,u3,p)
+ * and *mubar = w^T d res/d mu, the exact derivative of the residual that
+ * crimson_form_interior returns (tau, tauBar, ubar and the reconstructed
+ * flux q all differentiated; the acceleration held fixed).  Device arrays,
+ * synchronous. */
+int crimson_form_vjp_dev(crimson_form *f, const double *d_y, const double *d_ac,
+                         const double *d_x, const double *d_w, double *d_ybar,
+                         double *mubar);
 int crimson_form_free(crimson_form *f);
 
 /* Native genadj (genadj.f90:1-84): node-adjacency CSR including self, rows
@@ -197,6 +212,15 @@ int crimson_lhs_set_val(crimson_lhs *lhs, int dof, const double *Val);
 /* Downloads the held matrix as Val(dof*dof,nnz) row-major blocks (the
  * layout solfar.f90:212-235 packs), e.g. after crimson_form_interior_to_lhs. */
 int crimson_lhs_get_val(crimson_lhs *lhs, int dof, double *Val);
+
+/* No counterpart in the reference: the mixed-precision form's switch.
+ * Converts the held matrix to FP32 (bits = 32) or back to FP64 (bits = 64);
+ * with an FP32 matrix every following solve runs its Krylov vectors, SpMV
+ * and dot products in FP32 (FP32 accumulation), keeping FP64 for the
+ * Hessenberg/Givens least squares, the NS Gram system and the FP64 Ri in
+ * and out.  crimson_form_interior_to_lhs with a precision-32 formation
+ * handle leaves an FP32 matrix directly. */
+int crimson_lhs_set_precision(crimson_lhs *lhs, int bits);
 
 /* Stats of the last solve: sub 0 = RI, 1 = GM, 2 = CG; out[0] iNorm,
  * out[1] fNorm, out[2] itr, out[3] suc, out[4] dB, out[5] callD (s),

@@ -168,3 +168,16 @@ def test_no_cpu_fallback_in_product_path():
     for fn in os.listdir(pkg):
         if fn.endswith(".py"):
             assert "oracle" not in open(os.path.join(pkg, fn)).read().replace("oracle)", ""), fn
+
+
+def test_unstructured_mesh_exercises_long_rows(cfa):
+    """The jittered Delaunay mesh of the GPU parity test has rows longer than
+    one 16-block group and nodes with more than two 16-incidence chunks."""
+    from crimsonflowsolver_amd import mesh as M
+    m = M.jittered_delaunay_mesh(7, 7, 12, seed=17)
+    colm, rowp = cfa.genadj(m.ien, m.numnp)
+    assert np.diff(colm).max() > 16
+    assert np.bincount(m.ien.ravel()).max() > 32
+    x = m.x[m.ien - 1]
+    V = np.einsum("ei,ei->e", x[:, 0] - x[:, 3], np.cross(x[:, 1] - x[:, 3], x[:, 2] - x[:, 3]))
+    assert (V > 0).all()

@@ -118,3 +118,38 @@ def test_bc3res_device(cfa, oracle_lib):
     form.bc3res(t)
     want = oracle_lib.bc3res(ibc, res, bc)
     assert np.array_equal(t.cpu().numpy().T, want)
+
+
+def test_formation_unstructured_matches_oracle(cfa, oracle_lib):
+    """A jittered Delaunay mesh: nodes with up to ~50 incident elements and
+    ~32 neighbours, so rows take several incidence chunks and several
+    16-block groups in the assembly (the Kuhn meshes have 24 and 15).  Host
+    layouts, the fused memLS matrix, and the FP32 tangent."""
+    import torch
+    from crimsonflowsolver_amd import mesh as M
+    m = M.jittered_delaunay_mesh(7, 7, 12, seed=17)
+    y, ac = M.poiseuille_state(m, seed=17, noise=0.2)
+    rng = np.random.default_rng(17)
+    ac[:, :3] = rng.uniform(-50, 50, size=(m.numnp, 3))
+    colm, rowp = oracle_lib.genadj(m.ien, m.numnp)
+    assert np.diff(colm).max() > 16 and np.bincount(m.ien.ravel()).max() > 32
+    rres, rK, rP, rq = oracle_lib.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, oracle_lib.make_params(),
+                                                  want_q=True)
+    form = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp)
+    res, K, P, q = form(y, ac, m.x, want_q=True)
+    errs = {n: _rel(a, b) for n, a, b in (("res", res, rres), ("lhsK", K, rK), ("lhsP", P, rP), ("q", q, rq))}
+    print("host", errs)
+    assert all(e <= TOL for e in errs.values()), errs
+    lhs = cfa.memLS_LHS_CREATE(m.numnp, m.numnp, rowp.size, None, colm, rowp, 1)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+    tres4 = torch.zeros((m.numnp, 4), dtype=torch.float64, device="cuda")
+    form.to_lhs(T(y), T(ac), T(m.x), tres4, lhs)
+    rval = oracle_lib.pack_val4(colm, rowp, rK, rP)
+    print("fused", _rel(lhs.get_val(4), rval), _rel(tres4.cpu().numpy(), rres))
+    assert _rel(lhs.get_val(4), rval) <= TOL
+    assert _rel(tres4.cpu().numpy(), rres) <= TOL
+    form32 = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp, params=cfa.form_params(precision=32))
+    res32, K32, P32 = form32(y, ac, m.x)
+    print("fp32 tangent", _rel(K32, rK), _rel(P32, rP))
+    assert _rel(res32, rres) <= TOL
+    assert _rel(K32, rK) <= 3e-6 and _rel(P32, rP) <= 3e-6

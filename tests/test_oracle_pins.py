@@ -480,3 +480,15 @@ def test_memls_ns_solver_and_dirichlet_masking(O):
 
 def test_flop_count_constant(O):
     assert 6000 < O.flops_per_element() < 10000
+
+
+def test_bench_flop_constant_matches_oracle(O):
+    """bench.py prices e3_elem with the oracle's flop count without calling
+    the oracle; the two must agree."""
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(os.path.dirname(__file__), "..", "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    assert bench.FLOPS_PER_ELEMENT == O.flops_per_element()
+
