@@ -397,9 +397,11 @@ __device__ inline float tet_aspect(const double (&xd)[4][3]) {
   return sqrtf(L2) * sqrtf(A2) / six_v * 0.8164965809277260f;
 }
 
-template <class T>
+// T: arithmetic type; TO: record type; S row a at 16 + SSTRIDE a (4 for the
+// FP32 record, 8 for the FP64 record whose rl shares those lines).
+template <class T, class TO = float, int SSTRIDE = 4>
 __device__ inline void tangent_body(const double (&xd)[4][3], const double (&ud)[4][3], const FormConst &c,
-                                    float *__restrict__ r) {
+                                    TO *__restrict__ r) {
   T xl[4][3], ul[4][3];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -462,16 +464,16 @@ __device__ inline void tangent_body(const double (&xd)[4][3], const double (&ud)
       const int lo = a < b ? a : b, hi = a < b ? b : a;
       const int k = lo * 4 - (lo * (lo - 1)) / 2 + (hi - lo);
       const T gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
-      r[R32_S(a) + b] = (float)(tsFct * (T)mm(a, b) + rt * conv + Mu * gab + rt * SUPG[k]);
+      r[16 + SSTRIDE * a + b] = (TO)(tsFct * (T)mm(a, b) + rt * conv + Mu * gab + rt * SUPG[k]);
     }
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) r[R_G + a * 3 + j] = (float)g[a][j];
-  r[R_TLW] = (float)tlW;
-  r[R_TC] = (float)(tlW * TC);
-  r[R_MU] = (float)Mu;
-  r[R_CM] = (float)(tlW * TM / rho);
+    for (int j = 0; j < 3; ++j) r[R_G + a * 3 + j] = (TO)g[a][j];
+  r[R_TLW] = (TO)tlW;
+  r[R_TC] = (TO)(tlW * TC);
+  r[R_MU] = (TO)Mu;
+  r[R_CM] = (TO)(tlW * TM / rho);
 }
 
 __global__ void __launch_bounds__(256) e3_tangent32(int nel, const int *__restrict__ ien,
@@ -499,6 +501,33 @@ __global__ void __launch_bounds__(256) e3_tangent32(int nel, const int *__restri
 #pragma unroll
   for (int k = 0; k < kRec32 / 4; ++k) o[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
 }
+
+// FP64 tangent part of the element record alone (g, scalars, S; the
+// residual kernel e3_elem<false> writes rl): the same tangent as e3_elem<true>
+// split into a second, lighter kernel.
+__global__ void __launch_bounds__(256) e3_tangent64(int nel, const int *__restrict__ ien,
+                                                    const double *__restrict__ node, FormConst c,
+                                                    double *__restrict__ rec) {
+  const int e = xcd_block() * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  double xl[4][3], ul[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const double2 *r = reinterpret_cast<const double2 *>(node + (size_t)ien[a * (size_t)nel + e] * kNRec);
+    const double2 v0 = r[0], v1 = r[1], v2 = r[2];
+    xl[a][0] = v0.x;
+    xl[a][1] = v0.y;
+    xl[a][2] = v1.x;
+    ul[a][0] = v1.y;
+    ul[a][1] = v2.x;
+    ul[a][2] = v2.y;
+  }
+  tangent_body<double, double, 8>(xl, ul, c, rec + (size_t)e * kRec);
+}
+
+// The FP64 element kernel as one kernel (residual and tangent, one wave per
+// SIMD) or as e3_elem<false> + e3_tangent64 (two and four waves per SIMD).
+constexpr bool kSplitE3 = false;
 
 // ---------------------------------------------------------------- assembly
 // Output modes: 0 = reference layout lhsK(9,nnz)/lhsP(4,nnz);
@@ -986,10 +1015,13 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
   // mixed precision: FP64 residual (e3 without tangent, res_node), FP32 tangent
   const bool t32 = lhs && f->prm.precision == 32;
   if (f->nel > 0) {
-    if (lhs && !t32)
+    if (lhs && !t32 && !kSplitE3) {
       e3_elem<true><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, c, f->d_rec.p);
-    else
+    } else {
       e3_elem<false><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, c, f->d_rec.p);
+      if (lhs && !t32)
+        e3_tangent64<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, c, f->d_rec.p);
+    }
     if (t32)
       e3_tangent32<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg, c,
                                                               (float)f->prm.aspect_fp64, f->d_rec32.p);

@@ -56,20 +56,22 @@ struct MatView {
   int wdof, r0, c0;
 };
 
-template <int NR, int NC, class T>
+// LANES lanes per CSR row: 16 for the 3x3 and 4x4 products, 4 for the
+// narrow ones (G, D, L) whose per-entry work is too small to feed 16 lanes.
+template <int NR, int NC, class T, int LANES>
 __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restrict__ rowptr,
                                                    const int *__restrict__ col, MatView<T> A,
                                                    const T *__restrict__ x, T *__restrict__ y, T ya,
                                                    const T *__restrict__ z, T za) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int row = gid / kRowLanes;
-  const int lane = threadIdx.x % kRowLanes;
+  const int row = gid / LANES;
+  const int lane = threadIdx.x % LANES;
   T acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = T(0);
   if (row < nNo) {
     const int ke = rowptr[row + 1];
-    for (int k = rowptr[row] + lane; k < ke; k += kRowLanes) {
+    for (int k = rowptr[row] + lane; k < ke; k += LANES) {
       const int j = col[k];
       T xv[NC];
 #pragma unroll
@@ -81,9 +83,9 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
     }
   }
 #pragma unroll
-  for (int off = kRowLanes / 2; off > 0; off >>= 1)
+  for (int off = LANES / 2; off > 0; off >>= 1)
 #pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] += __shfl_xor(acc[r], off, kRowLanes);
+    for (int r = 0; r < NR; ++r) acc[r] += __shfl_xor(acc[r], off, LANES);
   if (row < nNo && lane == 0) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -93,6 +95,44 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
       if (z) v += za * z[r + (size_t)NR * row];
       y[r + (size_t)NR * row] = v;
     }
+  }
+}
+
+// The Schur product of NSSOLVER's CG (memLS.f90:1719-1735) after G P:
+// SP = L P - Gt t1 with t1 = G P, both scaled views of the same CSR rows, in
+// one pass: SP_i = W_i sum_j L_ij W_j P_j + fix_i P_i - W_i sum_j Gt_ij (W_j t1_j)
+// (the same operations, in the same order, as the two separate products).
+constexpr int kNarrowLanes = 4;
+template <class T>
+__global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restrict__ rowptr,
+                                                    const int *__restrict__ col, MatView<T> Lv, MatView<T> Gv,
+                                                    const T *__restrict__ P, const T *__restrict__ t1,
+                                                    T *__restrict__ SP) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = gid / kNarrowLanes;
+  const int lane = threadIdx.x % kNarrowLanes;
+  T aL = T(0), aG = T(0);
+  if (row < nNo) {
+    const int ke = rowptr[row + 1];
+    for (int k = rowptr[row] + lane; k < ke; k += kNarrowLanes) {
+      const int j = col[k];
+      aL += Lv.base[(size_t)Lv.plane[0] * Lv.nnz + k] * (Lv.W[Lv.c0 + (size_t)Lv.wdof * j] * P[j]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        aG += Gv.base[(size_t)Gv.plane[c] * Gv.nnz + k] * (Gv.W[Gv.c0 + c + (size_t)Gv.wdof * j] * t1[c + (size_t)3 * j]);
+    }
+  }
+#pragma unroll
+  for (int off = kNarrowLanes / 2; off > 0; off >>= 1) {
+    aL += __shfl_xor(aL, off, kNarrowLanes);
+    aG += __shfl_xor(aG, off, kNarrowLanes);
+  }
+  if (row < nNo && lane == 0) {
+    T v = Lv.W[Lv.r0 + (size_t)Lv.wdof * row] * aL;
+    if (Lv.fix) v += Lv.fix[Lv.r0 + (size_t)Lv.wdof * row] * P[row];
+    v *= T(1);
+    const T t2 = Gv.W[Gv.r0 + (size_t)Gv.wdof * row] * aG;
+    SP[row] = v + T(-1) * t2;
   }
 }
 
@@ -333,7 +373,7 @@ __global__ void gt_kernel(size_t nnz, const T *__restrict__ V, const int *__rest
 }
 
 // ----- CG (device-resident iterations with a convergence flag) -----
-// scal layout: 0 err, 1 errO, 2 eps2, 3 den, 4 alpha, 5 done, 6 itr, 7 mItr, 8 suc
+// scal layout: 0 err, 1 errO, 2 eps2, 3 den, 4 (unused), 5 done, 6 itr, 7 mItr, 8 suc
 __global__ void cg_check_kernel(double *__restrict__ scal) {
   if (scal[5] != 0.0) return;
   if (scal[6] >= scal[7]) {
@@ -348,10 +388,6 @@ __global__ void cg_check_kernel(double *__restrict__ scal) {
   scal[1] = scal[0];  // errO = err
   scal[6] += 1.0;
 }
-__global__ void cg_alpha_kernel(double *__restrict__ scal) {
-  if (scal[5] != 0.0) return;
-  scal[4] = scal[1] / scal[3];
-}
 // X += alpha P ; R -= alpha SP ; partial of R.R
 template <class T>
 __global__ void __launch_bounds__(256) cg_update_kernel(size_t L, const double *__restrict__ scal,
@@ -360,7 +396,7 @@ __global__ void __launch_bounds__(256) cg_update_kernel(size_t L, const double *
                                                         T *__restrict__ partial) {
   if (scal[5] != 0.0) return;
   __shared__ T sh[4];
-  const T alpha = (T)scal[4];
+  const T alpha = (T)(scal[1] / scal[3]);  // errO / den (CGRAD's alpha)
   const size_t base = (size_t)blockIdx.x * kChunk;
   T s = T(0);
 #pragma unroll
@@ -376,10 +412,19 @@ __global__ void __launch_bounds__(256) cg_update_kernel(size_t L, const double *
   s = block_sum(s, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
-__global__ void cg_err_kernel(double *__restrict__ scal, const double *__restrict__ rr) {
+// err = NORMS(R)**2 from the per-workgroup partials of R.R (fixed order)
+template <class T>
+__global__ void __launch_bounds__(256) cg_err_kernel(int nblk, const T *__restrict__ partial,
+                                                     double *__restrict__ scal) {
   if (scal[5] != 0.0) return;
-  const double n = sqrt(*rr);
-  scal[0] = n * n;  // err = NORMS(R)**2
+  __shared__ T sh[4];
+  T s = T(0);
+  for (int b = threadIdx.x; b < nblk; b += 256) s += partial[b];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    const double n = sqrt((double)s);
+    scal[0] = n * n;
+  }
 }
 // P = R + err/errO P
 template <class T>
@@ -462,8 +507,15 @@ int spmv(Ctx<T> &c, const MatView<T> &A, int nr, int nc, const T *x, T *y, doubl
          double za = 0.0) {
   const unsigned TB = 256;
   const unsigned G = crimson::grid_for((size_t)c.N * kRowLanes, TB);
+  const unsigned Gn = crimson::grid_for((size_t)c.N * kNarrowLanes, TB);
   const int *rp = c.lhs->d_rowptr.p, *cl = c.lhs->d_col.p;
-#define CR_SPMV(R_, C_) spmv_kernel<R_, C_, T><<<G, TB, 0, c.s>>>(c.N, rp, cl, A, x, y, (T)ya, z, (T)za)
+#define CR_SPMV(R_, C_)                                                                              \
+  do {                                                                                                \
+    if (R_ * C_ >= 9)                                                                                 \
+      spmv_kernel<R_, C_, T, kRowLanes><<<G, TB, 0, c.s>>>(c.N, rp, cl, A, x, y, (T)ya, z, (T)za);    \
+    else                                                                                              \
+      spmv_kernel<R_, C_, T, kNarrowLanes><<<Gn, TB, 0, c.s>>>(c.N, rp, cl, A, x, y, (T)ya, z, (T)za); \
+  } while (0)
   if (nr == 3 && nc == 3) CR_SPMV(3, 3);
   else if (nr == 4 && nc == 4) CR_SPMV(4, 4);
   else if (nr == 3 && nc == 1) CR_SPMV(3, 1);
@@ -529,6 +581,8 @@ double host_norm(Ctx<T> &c, size_t L, const T *v, int *rc) {
 }
 
 // ADDBCMUL (memLS.f90:206-268) over all coupled faces, X (xdof, N) -> Y.
+// X may be Y: every face's dot product is taken (stream order) before any
+// face's update is applied, which is what memLS's copy unCondU provides.
 template <class T>
 int addbcmul(Ctx<T> &c, bool pre, int xdof, const T *X, T *Y) {
   if (!c.coupled) return CRIMSON_OK;
@@ -584,10 +638,7 @@ int gmres_dev(Ctx<T> &c, const MatView<T> &A, int nd, SubLs &ls, const T *R, T *
       CR_TRY(addbcmul(c, false, nd, X, U));
       CR_TRY(axpby(c, L, 1.0, R, -1.0, U));
     }
-    if (c.coupled && pre_ok) {
-      CR_HIP(hipMemcpyAsync(ws.tmp.p, U, L * sizeof(T), hipMemcpyDeviceToDevice, c.s));
-      CR_TRY(addbcmul(c, true, nd, ws.tmp.p, U));
-    }
+    if (c.coupled && pre_ok) CR_TRY(addbcmul(c, true, nd, U, U));
     err[0] = host_norm(c, L, U, &rc);
     if (rc) return rc;
     if (l == 1) {
@@ -610,10 +661,7 @@ int gmres_dev(Ctx<T> &c, const MatView<T> &A, int nd, SubLs &ls, const T *R, T *
       const T *uprev = U + L * (size_t)(i - 1);
       CR_TRY(spmv(c, A, nd, nd, uprev, ui));
       CR_TRY(addbcmul(c, false, nd, uprev, ui));
-      if (c.coupled && pre_ok) {
-        CR_HIP(hipMemcpyAsync(ws.tmp.p, ui, L * sizeof(T), hipMemcpyDeviceToDevice, c.s));
-        CR_TRY(addbcmul(c, true, nd, ws.tmp.p, ui));
-      }
+      if (c.coupled && pre_ok) CR_TRY(addbcmul(c, true, nd, ui, ui));
       // CGS pass 1: h1 = U^T w, n0 = w.w ; w -= U h1 ; n1 = w.w
       double *h1 = c.hcol.p, *h2 = c.hcol.p + 256, *nrm = c.hcol.p + 512;
       const int nblk = (int)((L + kChunk - 1) / kChunk);
@@ -678,7 +726,7 @@ int gmres_dev(Ctx<T> &c, const MatView<T> &A, int nd, SubLs &ls, const T *R, T *
 template <class T, class Op>
 int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool schur) {
   const double t0 = now_s();
-  double *scal = c.dsc.p;  // 0 err 1 errO 2 eps2 3 den 4 alpha 5 done 6 itr 7 mItr 8 suc
+  double *scal = c.dsc.p;  // 0 err 1 errO 2 eps2 3 den 5 done 6 itr 7 mItr 8 suc
   ls.suc = 0;
   int rc = CRIMSON_OK;
   const double errO0 = host_norm(c, L, R, &rc);
@@ -701,10 +749,8 @@ int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool 
       T *pt = reinterpret_cast<T *>(c.partial.p);
       mdot_kernel<T, T><<<nblk, 256, 0, c.s>>>(L, SP, P, 0, 1, 0, pt, 1, nullptr);
       reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, scal + 3, nullptr);
-      cg_alpha_kernel<<<1, 1, 0, c.s>>>(scal);
       cg_update_kernel<T><<<nblk, 256, 0, c.s>>>(L, scal, X, R, P, SP, pt);
-      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, scal + 9, nullptr);
-      cg_err_kernel<<<1, 1, 0, c.s>>>(scal, scal + 9);
+      cg_err_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, scal);
       cg_p_kernel<T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, scal, R, P);
     }
     CR_HIP(hipGetLastError());
@@ -807,12 +853,10 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
   auto schur_op = [&](const T *Pv, T *SPv, const double *scal) -> int {
     (void)scal;
     CR_TRY(spmv(c, mG, 3, 1, Pv, t1.p));
-    if (c.coupled) {
-      CR_HIP(hipMemcpyAsync(ws.tmp.p, t1.p, mn * sizeof(T), hipMemcpyDeviceToDevice, c.s));
-      CR_TRY(addbcmul(c, true, 3, ws.tmp.p, t1.p));
-    }
-    CR_TRY(spmv(c, Gt, 1, 3, t1.p, t2.p));
-    CR_TRY(spmv(c, mL, 1, 1, Pv, SPv, 1.0, t2.p, -1.0));
+    if (c.coupled) CR_TRY(addbcmul(c, true, 3, t1.p, t1.p));
+    schur_kernel<T><<<crimson::grid_for((size_t)N * kNarrowLanes, 256), 256, 0, c.s>>>(
+        N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t1.p, SPv);
+    CR_HIP(hipGetLastError());
     return CRIMSON_OK;
   };
   std::vector<double> A((size_t)nB * nB, 0.0), Bv(nB, 0.0), xB(nB, 0.0);
