@@ -229,6 +229,9 @@ def main():
         y0, ac0 = dy.clone(), dac.clone()
 
         def run_steps(fm, lh, label):
+            """Timesteps of 3 nonlinear iterations; also returns the FP64
+            residual norm (after bc3Res and ElmpvsQ) at each formation and once
+            more after the last update (outside the timed region)."""
             ls = cfa.memLS_LS_CREATE(cfa.LS_TYPE_NS, relTol=1e-8, dimKry=100, maxItr=10,
                                      relTolIn=[0.01, 0.2], maxItrIn=[1, 200])
             dBC = T(np.zeros((N, 7), order="F"))  # (7,N) C == BC(N,7) Fortran
@@ -236,7 +239,7 @@ def main():
             yc, acc = y0.clone(), ac0.clone()
             yold, acold = y0.clone(), ac0.clone()
             ya, aca = torch.empty_like(yc), torch.empty_like(acc)
-            times, its = [], []
+            times, its, rnorms = [], [], []
             for n in range(args.timesteps):
                 t = (n + 1) * prm.Delt
                 prof = np.zeros(N)
@@ -246,6 +249,7 @@ def main():
                 t0 = time.perf_counter()
                 fm.predict(yold, acold, yc, acc)
                 fm.itr_bc(dibc, dBC, yc)
+                rn = []
                 for it in range(3):
                     fm.yalpha(yold, acold, yc, acc, ya, aca)
                     fm.sync()
@@ -253,32 +257,49 @@ def main():
                     fm.bc3res(dres4, res4_layout=True)
                     fm.elmpvsq(dglob, dnabi, R, ya, dres4, dq)
                     fm.sync()
+                    rn.append(torch.linalg.vector_norm(dres4))
                     cfa.memLS_SOLVE_dev(lh, ls, 4, dres4, res=faceres)
                     its.append((ls.RI["itr"], ls.GM["itr"], ls.CG["itr"], ls.RI["fNorm"] / ls.RI["iNorm"]))
                     fm.correct(yc, acc, dres4)
                     fm.itr_bc(dibc, dBC, yc)
                     fm.sync()
-                yold.copy_(yc)
-                acold.copy_(acc)
                 torch.cuda.synchronize()
                 times.append(time.perf_counter() - t0)
+                # the residual the timestep leaves (evaluated in FP64 by the FP64 formation)
+                form.yalpha(yold, acold, yc, acc, ya, aca)
+                form.sync()
+                form.dev(ya, aca, dx, rfin, lhs=0)
+                form.bc3res(rfin)
+                rq = torch.zeros(1, dtype=torch.float64, device=dev)
+                rres4 = rfin.t().contiguous()
+                form.elmpvsq(dglob, dnabi, R, ya, rres4, rq)
+                form.sync()
+                rn.append(torch.linalg.vector_norm(rres4))
+                rnorms.append([float(v) for v in rn])
+                yold.copy_(yc)
+                acold.copy_(acc)
                 log(f"{label} timestep {n + 1}: {times[-1] * 1e3:.1f} ms, solver (RI,GM,CG itr, rel) {its[-3:]}")
-            return times, its, yc
+            return times, its, yc, rnorms
 
-        st64, its64, y64 = run_steps(form, lhs, "fp64")
+        rfin = torch.empty((4, N), dtype=torch.float64, device=dev)  # res(N,4) Fortran
+        st64, its64, y64, rn64 = run_steps(form, lhs, "fp64")
         ts_ms = float(np.mean(st64) * 1e3)
         ts_info = {"timesteps_timed": args.timesteps, "nonlinear_iterations": 3,
                    "solver_iterations": [list(map(float, v)) for v in its64]}
+        def rel_err(a, b):
+            d = (a - b).abs()
+            return {"u": float(d[:, :3].max() / b[:, :3].abs().max()), "p": float(d[:, 3].max() / b[:, 3].abs().max())}
+
         if args.mixed:
-            st32, its32, y32 = run_steps(form32, lhs32, "mixed")
-            d = (y32 - y64).abs()
-            err = {"u": float(d[:, :3].max() / y64[:, :3].abs().max()),
-                   "p": float(d[:, 3].max() / y64[:, 3].abs().max())}
+            st32, its32, y32, rn32 = run_steps(form32, lhs32, "mixed")
+            err = rel_err(y32, y64)
             ms32 = float(np.mean(st32) * 1e3)
             ts_info.update({"timestep_ms_mixed": ms32, "mixed_speedup": ts_ms / ms32,
                             "mixed_rel_err_vs_fp64": err,
+                            "residual_norms_fp64": rn64, "residual_norms_mixed": rn32,
                             "solver_iterations_mixed": [list(map(float, v)) for v in its32]})
             log(f"mixed: {ms32:.1f} ms/timestep vs fp64 {ts_ms:.1f} -> x{ts_ms / ms32:.2f}; rel err {err}")
+            log(f"FP64 residual norms per nonlinear iteration (last: after the step): fp64 {rn64} mixed {rn32}")
 
     # ---- roofline of the dominant kernel (from the live HIP-event timings)
     fpe = FLOPS_PER_ELEMENT

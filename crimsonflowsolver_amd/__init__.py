@@ -40,7 +40,8 @@ EXPORTS = (
     "crimson_ls_solve_dev", "crimson_lhs_set_val", "crimson_lhs_get_val", "crimson_ls_stats",
     "crimson_lhs_spmv_dev", "crimson_itr_predict_dev", "crimson_itr_yalpha_dev", "crimson_itr_correct_dev",
     "crimson_itr_bc_dev", "crimson_elmpvsq_dev", "crimson_form_sync", "crimson_lhs_set_precision",
-    "crimson_form_vjp_dev",
+    "crimson_form_vjp_dev", "crimson_form_set_instances", "crimson_form_graph_create", "crimson_graph_launch",
+    "crimson_graph_free",
 )
 
 
@@ -165,6 +166,23 @@ def genadj(ien, nshg):
     return colm, rowp
 
 
+class FormGraph:
+    """A captured formation (crimson_form_graph_create); keeps its arrays alive."""
+
+    def __init__(self, h, keep):
+        self.h = h
+        self._keep = keep
+
+    def launch(self):
+        check(lib().crimson_graph_launch(self.h), "crimson_graph_launch")
+
+    def __del__(self):
+        try:
+            lib().crimson_graph_free(self.h)
+        except Exception:
+            pass
+
+
 class ElmGMRInterior:
     """Interior ElmGMR on the GPU (elmgmr.f90:82-255).
 
@@ -215,6 +233,20 @@ class ElmGMRInterior:
         check(lib().crimson_form_vjp_dev(self.h, _p(y), _p(ac), _p(x), _p(w), _p(ybar), C.byref(mb)),
               "crimson_form_vjp_dev")
         return mb.value
+
+    def set_instances(self, inst, mu):
+        """Ensemble: inst(nel) 0-based instance of each element, mu(ninst)."""
+        inst = np.ascontiguousarray(inst, dtype=np.uint16)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        check(lib().crimson_form_set_instances(self.h, _p(inst), int(mu.size), _p(mu)), "crimson_form_set_instances")
+
+    def graph_to_lhs(self, y, ac, x, res4, lhs_handle):
+        """Capture to_lhs on these device arrays into a HIP graph; returns a
+        FormGraph whose launch() replays it."""
+        g = C.c_void_p()
+        check(lib().crimson_form_graph_create(self.h, _p(y), _p(ac), _p(x), _p(res4), lhs_handle.h, C.byref(g)),
+              "crimson_form_graph_create")
+        return FormGraph(g, keep=(y, ac, x, res4, lhs_handle))
 
     def bc3res(self, res, res4_layout=False):
         check(lib().crimson_form_bc3res_dev(self.h, _p(res), 1 if res4_layout else 0),

@@ -39,9 +39,11 @@ constexpr int V2_MU = 12;
 
 __global__ void __launch_bounds__(256) vjp_e3_elem(int nel, const int *__restrict__ ien,
                                                    const double *__restrict__ node, const double *__restrict__ w,
-                                                   int nshg, FormConst c, double *__restrict__ vrec) {
+                                                   int nshg, FormConst c0, Ens en, double *__restrict__ vrec) {
   const int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= nel) return;
+  FormConst c = c0;
+  c.mu = ens_mu(en, e, c0.mu);
   const size_t N = nshg;
   int nd[4];
 #pragma unroll
@@ -291,10 +293,11 @@ __global__ void __launch_bounds__(256) vjp_q_node(int nshg, const int *__restric
 __global__ void __launch_bounds__(256) vjp_flux_elem(int nel, const int *__restrict__ ien,
                                                      const double *__restrict__ node,
                                                      const double *__restrict__ frec,
-                                                     const double *__restrict__ qbar, double mu,
+                                                     const double *__restrict__ qbar, double mu0, Ens en,
                                                      double *__restrict__ vrec2) {
   const int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= nel) return;
+  const double mu = ens_mu(en, e, mu0);
   int nd[4];
   double xl[4][3], ul[4][3];
 #pragma unroll
@@ -421,10 +424,12 @@ extern "C" int crimson_form_vjp_dev(crimson_form *f, const double *d_y, const do
   const FormConst c = make_const(f->prm);
   CR_TRY(crimson::form_flux_pass(f, d_y, d_ac, d_x, nullptr, f->d_invm.p));
   const unsigned ge = crimson::grid_for(f->nel, T), gn = crimson::grid_for(nn * kNodeLanes, T);
-  if (f->nel > 0) vjp_e3_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, d_w, f->nshg, c, f->d_vrec.p);
+  if (f->nel > 0)
+    vjp_e3_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, d_w, f->nshg, c, f->ens(), f->d_vrec.p);
   vjp_q_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_vrec.p, f->d_invm.p, f->d_qbar.p);
   if (f->nel > 0)
-    vjp_flux_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, f->d_frec.p, f->d_qbar.p, c.mu, f->d_vrec2.p);
+    vjp_flux_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, f->d_frec.p, f->d_qbar.p, c.mu, f->ens(),
+                                   f->d_vrec2.p);
   vjp_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_vrec.p, f->d_vrec2.p, d_ybar);
   double mb = 0.0;
   if (f->nel > 0) {

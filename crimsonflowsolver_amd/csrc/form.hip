@@ -53,9 +53,10 @@ namespace {
 __global__ void __launch_bounds__(256) flux_elem(int nel, const int *__restrict__ ien,
                                                  const double *__restrict__ x, int numnp,
                                                  const double *__restrict__ y, int nshg,
-                                                 double mu, double *__restrict__ frec) {
+                                                 double mu0, Ens en, double *__restrict__ frec) {
   const int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= nel) return;
+  const double mu = ens_mu(en, e, mu0);
   double xl[4][3], ul[4][3];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -155,15 +156,26 @@ __global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict
 }
 
 // ---------------------------------------------------------------- e3
+// With the tangent, the records of the workgroup's (consecutive) elements are
+// assembled in LDS and stored as one contiguous block with coalesced 16-byte
+// stores (written directly, each thread's 384-byte record reached HBM as
+// partially filled lines).
+constexpr int kE3Threads = 128;
+constexpr bool kE3Stage = true;
 template <bool LHS>
-__global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ ien,
-                                               const double *__restrict__ node, FormConst c,
-                                               double *__restrict__ rec) {
-  const int e = xcd_block() * blockDim.x + threadIdx.x;
-  if (e >= nel) return;
+__global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__restrict__ ien,
+                                                      const double *__restrict__ node, FormConst c0, Ens en,
+                                                      double *__restrict__ rec) {
+  __shared__ double sh_rec[(LHS && kE3Stage) ? kE3Threads * kRec : 1];
+  const int e0 = xcd_block() * kE3Threads;
+  const int e = e0 + threadIdx.x;
+  if (!(LHS && kE3Stage) && e >= nel) return;
+  const int ee = e < nel ? e : nel - 1;  // tail threads recompute a valid element; nothing of theirs is stored
+  FormConst c = c0;
+  c.mu = ens_mu(en, ee, c0.mu);
   int nd[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) nd[a] = ien[a * (size_t)nel + e];
+  for (int a = 0; a < 4; ++a) nd[a] = ien[a * (size_t)nel + ee];
   // the four node records (localx/localy gathers, localy.f90:54-64)
   double nl[4][kNRec];
 #pragma unroll
@@ -306,7 +318,7 @@ __global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ 
     }
   }
 
-  double *r = rec + (size_t)e * kRec;
+  double *r = (LHS && kE3Stage) ? sh_rec + threadIdx.x * kRec : rec + (size_t)e * kRec;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     r[R_A(a) + 4 + 3] = W * (g[a][0] * SU[0] + g[a][1] * SU[1] + g[a][2] * SU[2]);
@@ -343,6 +355,12 @@ __global__ void __launch_bounds__(256) e3_elem(int nel, const int *__restrict__ 
   r[R_TC] = tlW * TC;
   r[R_MU] = Mu;
   r[R_CM] = tlW * TM / rho;
+  if (!kE3Stage) return;
+  __syncthreads();
+  const int cnt = min(kE3Threads, nel - e0);
+  const double2 *src = reinterpret_cast<const double2 *>(sh_rec);
+  double2 *dst = reinterpret_cast<double2 *>(rec + (size_t)e0 * kRec);
+  for (int i = threadIdx.x; i < cnt * (kRec / 2); i += kE3Threads) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------- FP32 tangent
@@ -478,10 +496,12 @@ __device__ inline void tangent_body(const double (&xd)[4][3], const double (&ud)
 
 __global__ void __launch_bounds__(256) e3_tangent32(int nel, const int *__restrict__ ien,
                                                     const double *__restrict__ x, int numnp,
-                                                    const double *__restrict__ y, int nshg, FormConst c,
-                                                    float aspect_fp64, float *__restrict__ rec) {
+                                                    const double *__restrict__ y, int nshg, FormConst c0,
+                                                    Ens en, float aspect_fp64, float *__restrict__ rec) {
   const int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= nel) return;
+  FormConst c = c0;
+  c.mu = ens_mu(en, e, c0.mu);
   double xl[4][3], ul[4][3];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -501,33 +521,6 @@ __global__ void __launch_bounds__(256) e3_tangent32(int nel, const int *__restri
 #pragma unroll
   for (int k = 0; k < kRec32 / 4; ++k) o[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
 }
-
-// FP64 tangent part of the element record alone (g, scalars, S; the
-// residual kernel e3_elem<false> writes rl): the same tangent as e3_elem<true>
-// split into a second, lighter kernel.
-__global__ void __launch_bounds__(256) e3_tangent64(int nel, const int *__restrict__ ien,
-                                                    const double *__restrict__ node, FormConst c,
-                                                    double *__restrict__ rec) {
-  const int e = xcd_block() * blockDim.x + threadIdx.x;
-  if (e >= nel) return;
-  double xl[4][3], ul[4][3];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const double2 *r = reinterpret_cast<const double2 *>(node + (size_t)ien[a * (size_t)nel + e] * kNRec);
-    const double2 v0 = r[0], v1 = r[1], v2 = r[2];
-    xl[a][0] = v0.x;
-    xl[a][1] = v0.y;
-    xl[a][2] = v1.x;
-    ul[a][0] = v1.y;
-    ul[a][1] = v2.x;
-    ul[a][2] = v2.y;
-  }
-  tangent_body<double, double, 8>(xl, ul, c, rec + (size_t)e * kRec);
-}
-
-// The FP64 element kernel as one kernel (residual and tangent, one wave per
-// SIMD) or as e3_elem<false> + e3_tangent64 (two and four waves per SIMD).
-constexpr bool kSplitE3 = false;
 
 // ---------------------------------------------------------------- assembly
 // Output modes: 0 = reference layout lhsK(9,nnz)/lhsP(4,nnz);
@@ -1004,30 +997,29 @@ int ensure_order(crimson_form *f, const double *d_x) {
 }
 
 int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
-                  double *d_res, int lhs, int mode, const AsmOut &o, double *d_qout) {
+                  double *d_res, int lhs, int mode, const AsmOut &o, double *d_qout, bool timed = true) {
   const unsigned T = 256;
   hipStream_t s = f->stream;
   const FormConst c = make_const(f->prm);
   if (lhs) CR_TRY(ensure_order(f, d_x));
-  CR_HIP(hipEventRecord(f->ev[0], s));
+  if (timed) CR_HIP(hipEventRecord(f->ev[0], s));
   CR_TRY(crimson::form_flux_pass(f, d_y, d_ac, d_x, d_qout, nullptr));
-  CR_HIP(hipEventRecord(f->ev[1], s));
+  if (timed) CR_HIP(hipEventRecord(f->ev[1], s));
   // mixed precision: FP64 residual (e3 without tangent, res_node), FP32 tangent
   const bool t32 = lhs && f->prm.precision == 32;
   if (f->nel > 0) {
-    if (lhs && !t32 && !kSplitE3) {
-      e3_elem<true><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, c, f->d_rec.p);
-    } else {
-      e3_elem<false><<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, c, f->d_rec.p);
-      if (lhs && !t32)
-        e3_tangent64<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, c, f->d_rec.p);
-    }
+    if (lhs && !t32)
+      e3_elem<true><<<crimson::grid_for(f->nel, kE3Threads), kE3Threads, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p,
+                                                                                  c, f->ens(), f->d_rec.p);
+    else
+      e3_elem<false><<<crimson::grid_for(f->nel, kE3Threads), kE3Threads, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p,
+                                                                                   c, f->ens(), f->d_rec.p);
     if (t32)
       e3_tangent32<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg, c,
-                                                              (float)f->prm.aspect_fp64, f->d_rec32.p);
+                                                              f->ens(), (float)f->prm.aspect_fp64, f->d_rec32.p);
   }
   CR_HIP(hipGetLastError());
-  CR_HIP(hipEventRecord(f->ev[2], s));
+  if (timed) CR_HIP(hipEventRecord(f->ev[2], s));
   const unsigned gn = crimson::grid_for((size_t)f->nshg * kNodeLanes, T);
   const unsigned gr = crimson::grid_for((size_t)f->nshg * kRowLanes, T);
   if (t32) {
@@ -1053,7 +1045,8 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
                                 f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p, d_res, o, f->d_order.p);
   }
   CR_HIP(hipGetLastError());
-  CR_HIP(hipEventRecord(f->ev[3], s));
+  if (timed) CR_HIP(hipEventRecord(f->ev[3], s));
+  if (!timed) return CRIMSON_OK;  // enqueued only (graph capture)
   CR_HIP(hipStreamSynchronize(s));
   CR_HIP(hipEventElapsedTime(&f->ms[0], f->ev[0], f->ev[1]));
   CR_HIP(hipEventElapsedTime(&f->ms[1], f->ev[1], f->ev[2]));
@@ -1073,7 +1066,7 @@ int form_flux_pass(crimson_form *f, const double *d_y, const double *d_ac, const
   hipStream_t s = f->stream;
   if (f->nel > 0)
     flux_elem<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg,
-                                                         f->prm.mu, f->d_frec.p);
+                                                         f->prm.mu, f->ens(), f->d_frec.p);
   flux_node<<<crimson::grid_for((size_t)f->nshg * kNodeLanes, T), T, 0, s>>>(
       f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_frec.p, d_x, f->numnp, d_y, d_ac, f->d_node.p, d_q, d_invm);
   CR_HIP(hipGetLastError());
@@ -1322,6 +1315,100 @@ extern "C" int crimson_form_interior_to_lhs(crimson_form *f, const double *d_y, 
   lhs->prec = 64;
   AsmOut o{nullptr, nullptr, lhs->d_V.p, nullptr, (size_t)f->nnz};
   return run_formation(f, d_y, d_ac, d_x, d_res, 1, 1, o, nullptr);
+}
+
+extern "C" int crimson_form_set_instances(crimson_form *f, const unsigned short *inst, int ninst, const double *mu) {
+  if (!f || ninst < 0 || (ninst > 0 && (!inst || !mu))) {
+    crimson::set_error("crimson_form_set_instances: invalid arguments");
+    return CRIMSON_EARG;
+  }
+  if (ninst == 0) {
+    f->ninst = 0;
+    return CRIMSON_OK;
+  }
+  for (int k = 0; k < ninst; ++k)
+    if (!(mu[k] >= 0.0)) {
+      crimson::set_error("instance %d: viscosity %g", k, mu[k]);
+      return CRIMSON_EARG;
+    }
+  for (int e = 0; e < f->nel; ++e)
+    if (inst[e] >= ninst) {
+      crimson::set_error("element %d: instance %d of %d", e + 1, (int)inst[e], ninst);
+      return CRIMSON_EARG;
+    }
+  CR_TRY(f->d_inst.upload(inst, (size_t)std::max(f->nel, 1), f->stream));
+  CR_TRY(f->d_mu_inst.upload(mu, (size_t)ninst, f->stream));
+  CR_HIP(hipStreamSynchronize(f->stream));
+  f->ninst = ninst;
+  return CRIMSON_OK;
+}
+
+struct crimson_graph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t s = nullptr;
+};
+
+extern "C" int crimson_form_graph_create(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
+                                         double *d_res, crimson_lhs *lhs, crimson_graph **out) {
+  if (!f || !lhs || !d_y || !d_ac || !d_x || !d_res || !out) {
+    crimson::set_error("crimson_form_graph_create: null argument");
+    return CRIMSON_EARG;
+  }
+  if (lhs->nNo != f->nshg || lhs->nnz != f->nnz) {
+    crimson::set_error("lhs (nNo=%d nnz=%d) does not match the formation graph (nshg=%d nnz=%d)", lhs->nNo,
+                       lhs->nnz, f->nshg, f->nnz);
+    return CRIMSON_EARG;
+  }
+  // everything that allocates or synchronizes happens before the capture
+  lhs->vdof = 4;
+  lhs->gt_is_d = true;
+  AsmOut o{nullptr, nullptr, nullptr, nullptr, (size_t)f->nnz};
+  if (f->prm.precision == 32) {
+    CR_TRY(lhs->d_V32.alloc((size_t)16 * f->nnz));
+    lhs->prec = 32;
+    o.V32 = lhs->d_V32.p;
+  } else {
+    CR_TRY(lhs->d_V.alloc((size_t)16 * f->nnz));
+    lhs->prec = 64;
+    o.V = lhs->d_V.p;
+  }
+  CR_TRY(ensure_order(f, d_x));
+  crimson_graph *gr = new crimson_graph();
+  gr->s = f->stream;
+  if (hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    delete gr;
+    crimson::set_error("hipStreamBeginCapture failed");
+    return CRIMSON_EHIP;
+  }
+  const int rc = run_formation(f, d_y, d_ac, d_x, d_res, 1, 1, o, nullptr, false);
+  const hipError_t ec = hipStreamEndCapture(f->stream, &gr->g);
+  if (rc || ec != hipSuccess || hipGraphInstantiate(&gr->exec, gr->g, nullptr, nullptr, 0) != hipSuccess) {
+    if (gr->g) (void)hipGraphDestroy(gr->g);
+    delete gr;
+    if (!rc) crimson::set_error("capture of the formation into a HIP graph failed");
+    return rc ? rc : CRIMSON_EHIP;
+  }
+  *out = gr;
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_graph_launch(crimson_graph *g) {
+  if (!g) {
+    crimson::set_error("crimson_graph_launch: null graph");
+    return CRIMSON_EARG;
+  }
+  CR_HIP(hipGraphLaunch(g->exec, g->s));
+  CR_HIP(hipStreamSynchronize(g->s));
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_graph_free(crimson_graph *g) {
+  if (!g) return CRIMSON_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->g) (void)hipGraphDestroy(g->g);
+  delete g;
+  return CRIMSON_OK;
 }
 
 extern "C" int crimson_form_bc3res_dev(crimson_form *f, double *d_res, int res4_layout) {

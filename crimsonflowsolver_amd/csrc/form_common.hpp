@@ -44,6 +44,14 @@ __host__ __device__ inline double flux_weight(double W, int a) {
 constexpr int kRowLanes = 16;   // lanes per CSR row in the assembly
 constexpr int kNodeLanes = 4;   // lanes per node in the flux/residual gathers
 
+// Ensemble instances: per-element instance index and per-instance viscosity
+// (both null for a single instance, which uses FormConst::mu).
+struct Ens {
+  const unsigned short *inst;
+  const double *mu;
+};
+__device__ inline double ens_mu(const Ens &en, int e, double mu) { return en.inst ? en.mu[en.inst[e]] : mu; }
+
 struct FormConst {
   double rho, mu, Delt, lhsFct, lhmFct, Dtgl, dtsfct, taucfct;
 };
@@ -154,6 +162,11 @@ struct crimson_form {
   float ms[4] = {0, 0, 0, 0};
   // adjoint (VJP) workspaces, allocated on first use
   crimson::DBuf<double> d_vrec, d_vrec2, d_qbar, d_invm, d_mupart;
+  // ensemble instances (crimson_form_set_instances)
+  crimson::DBuf<unsigned short> d_inst;
+  crimson::DBuf<double> d_mu_inst;
+  int ninst = 0;
+  Ens ens() const { return Ens{ninst ? d_inst.p : nullptr, ninst ? d_mu_inst.p : nullptr}; }
 };
 
 namespace crimson {

@@ -321,18 +321,26 @@ __global__ void face_valm_kernel(int n, int dof, int fdof, const int *__restrict
   for (int i = 0; i < nc; ++i)
     valM[i + (size_t)fdof * a] = val[i + (size_t)fdof * a] * (double)W[i + (size_t)dof * glob[a]];
 }
-// S = sum_a sum_i valM(i,a) X(i, glob(a)) in one workgroup (faces are small)
+// S = sum_a sum_i valM(i,a) X(i, glob(a)) in one 1024-thread workgroup
+// (faces are small): per-wave sums, then the 16 wave sums in a fixed order.
 template <class T>
-__global__ void __launch_bounds__(256) face_dot_kernel(int n, int xdof, int fdof, const int *__restrict__ glob,
-                                                       const double *__restrict__ valM, const T *__restrict__ X,
-                                                       double *__restrict__ out) {
-  __shared__ double sh[4];
+__global__ void __launch_bounds__(1024) face_dot_kernel(int n, int xdof, int fdof, const int *__restrict__ glob,
+                                                        const double *__restrict__ valM, const T *__restrict__ X,
+                                                        double *__restrict__ out) {
+  __shared__ double sh[16];
   const int nc = fdof < xdof ? fdof : xdof;
   double s = 0.0;
-  for (int a = threadIdx.x; a < n; a += 256)
+  for (int a = threadIdx.x; a < n; a += 1024)
     for (int i = 0; i < nc; ++i) s += valM[i + (size_t)fdof * a] * (double)X[i + (size_t)xdof * glob[a]];
-  s = block_sum(s, sh);
-  if (threadIdx.x == 0) *out = s;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x / 64] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += sh[w];
+    *out = t;
+  }
 }
 // Y(i, glob(a)) += valM(i,a) * coef * S
 template <class T>
@@ -591,7 +599,7 @@ int addbcmul(Ctx<T> &c, bool pre, int xdof, const T *X, T *Y) {
   for (size_t f = 0; f < faces.size(); ++f) {
     const CrFace &fa = faces[f];
     if (!fa.coupled) continue;
-    face_dot_kernel<<<1, 256, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p, fa.d_valM.p, X, c.dsc.p + 48 + f % 16);
+    face_dot_kernel<<<1, 1024, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p, fa.d_valM.p, X, c.dsc.p + 48 + f % 16);
   }
   for (size_t f = 0; f < faces.size(); ++f) {
     const CrFace &fa = faces[f];

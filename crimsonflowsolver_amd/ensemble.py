@@ -1,0 +1,85 @@
+"""Ensembles of instances (parameter samples on meshes of uneven size)
+formed and solved together on one GPU.
+
+The instances' meshes are joined into one union mesh with disjoint node
+ranges (a block-diagonal CSR graph), formed by one formation handle whose
+elements carry their instance's viscosity (crimson_form_set_instances), so a
+single launch sequence forms every instance; that sequence is captured once
+in a HIP graph and replayed for each nonlinear iteration.  The memLS matrix
+of the union has one Dirichlet face (every instance's fixed nodes) and one
+resistance face per instance, so one NS solve advances all instances; its
+Krylov space spans all instances, so the instances converge together (the
+stopping test is on the union's residual).  Each instance's blocks and
+residual rows are exactly those of the instance formed alone.
+
+Not in the reference, which runs one case per process.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import ElmGMRInterior, form_params, genadj, memLS_BC_CREATE, memLS_LHS_CREATE, BC_TYPE_Dir, BC_TYPE_Neu
+from . import mesh as M
+
+
+class Ensemble:
+    def __init__(self, meshes, mu, R, precision=64, params=None, outlet_natural=True):
+        self.meshes = list(meshes)
+        K = len(self.meshes)
+        self.K = K
+        self.mu = np.asarray(mu, dtype=np.float64)
+        self.R = np.asarray(R, dtype=np.float64)
+        if self.mu.shape != (K,) or self.R.shape != (K,):
+            raise ValueError("one viscosity and one resistance per instance")
+        self.node_off = np.concatenate([[0], np.cumsum([m.numnp for m in self.meshes])]).astype(np.int64)
+        self.elem_off = np.concatenate([[0], np.cumsum([m.nel for m in self.meshes])]).astype(np.int64)
+        N = int(self.node_off[-1])
+        self.N = N
+        self.ien = np.asfortranarray(np.concatenate([m.ien + int(o) for m, o in zip(self.meshes, self.node_off)])
+                                     .astype(np.int32))
+        self.x = np.asfortranarray(np.concatenate([m.x for m in self.meshes]))
+        inst = np.concatenate([np.full(m.nel, k, np.uint16) for k, m in enumerate(self.meshes)])
+        ibc, fixed, outs, nabis = [], [], [], []
+        for k, m in enumerate(self.meshes):
+            b, _ = M.dirichlet_codes(m, outlet_natural=outlet_natural)
+            ibc.append(b)
+            o = int(self.node_off[k])
+            fixed.append(np.nonzero(b)[0] + o)
+            out = np.nonzero(m.outlet & (b == 0))[0]
+            outs.append(out + o)
+            nabis.append(m.nabi()[out])
+        self.iBC = np.concatenate(ibc).astype(np.int32)
+        self.fixed = np.concatenate(fixed)
+        self.outlets = outs
+        self.nabi = nabis
+        self.colm, self.rowp = genadj(self.ien, N)
+        p = params if params is not None else form_params(precision=precision)
+        p.precision = precision
+        self.params = p
+        self.form = ElmGMRInterior(self.ien, self.colm, self.rowp, N, iBC=self.iBC, params=p)
+        self.form.set_instances(inst, self.mu)
+        self.lhs = memLS_LHS_CREATE(N, N, self.rowp.size, None, self.colm, self.rowp, 1 + K)
+        memLS_BC_CREATE(self.lhs, 1, self.fixed.size, 3, BC_TYPE_Dir, self.fixed + 1,
+                        np.zeros((3, self.fixed.size), order="F"))
+        for k in range(K):
+            memLS_BC_CREATE(self.lhs, 2 + k, self.outlets[k].size, 3, BC_TYPE_Neu, self.outlets[k] + 1,
+                            np.asfortranarray(self.nabi[k].T))
+        self.graph = None
+
+    def face_res(self):
+        """memLS face coupling coefficients: 0 on the Dirichlet face, R_k
+        alpha_f gamma Delt on instance k's outlet (elmgmr.f90:1806)."""
+        p = self.params
+        return np.concatenate([[0.0], self.R * p.alfi * p.gami * p.Delt])
+
+    def capture(self, y, ac, x, res4):
+        """Capture the union formation on these device arrays (torch
+        tensors in the C ABI layouts) into a HIP graph."""
+        self.graph = self.form.graph_to_lhs(y, ac, x, res4, self.lhs)
+        return self.graph
+
+    def rows(self, k):
+        return slice(int(self.node_off[k]), int(self.node_off[k + 1]))
+
+    def nnz_range(self, k):
+        return slice(int(self.colm[self.node_off[k]] - 1), int(self.colm[self.node_off[k + 1]] - 1))

@@ -112,6 +112,27 @@ int crimson_form_bc3res_dev(crimson_form *f, double *d_res, int res4_layout);
  * out[2] assembly, out[3] total. */
 int crimson_form_timing(const crimson_form *f, double *out4);
 
+/* Ensembles (no counterpart in the reference, which runs one case per job):
+ * the handle's mesh is the union of several instances' meshes (disjoint
+ * node ranges), inst(nel) gives each element's 0-based instance and mu(ninst)
+ * each instance's viscosity, which replaces crimson_form_params.mu for its
+ * elements.  ninst = 0 returns to the single viscosity. */
+int crimson_form_set_instances(crimson_form *f, const unsigned short *inst,
+                               int ninst, const double *mu);
+
+/* The formation into the memLS matrix (crimson_form_interior_to_lhs) captured
+ * once into a HIP graph over fixed device arrays, then replayed by
+ * crimson_graph_launch (synchronous) for each nonlinear iteration whose
+ * inputs were written into the same arrays.  No counterpart in the
+ * reference.  No per-phase timing is recorded for replays. */
+typedef struct crimson_graph crimson_graph;
+int crimson_form_graph_create(crimson_form *f, const double *d_y,
+                              const double *d_ac, const double *d_x,
+                              double *d_Res4, struct crimson_lhs *lhs,
+                              crimson_graph **out);
+int crimson_graph_launch(crimson_graph *g);
+int crimson_graph_free(crimson_graph *g);
+
 /* Adjoint of the interior residual (the vector-Jacobian product the
  * estimation subsystem needs for parameter fitting; no counterpart in the
  * reference, which has no derivative code): for a cotangent d_w(nshg,4)
