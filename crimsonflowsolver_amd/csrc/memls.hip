@@ -100,14 +100,68 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
 
 // The Schur product of NSSOLVER's CG (memLS.f90:1719-1735) after G P:
 // SP = L P - Gt t1 with t1 = G P, both scaled views of the same CSR rows, in
-// one pass: SP_i = W_i sum_j L_ij W_j P_j + fix_i P_i - W_i sum_j Gt_ij (W_j t1_j)
-// (the same operations, in the same order, as the two separate products).
+// one pass: SP_i = W_i sum_j L_ij (W_j P_j) + fix_i P_i - W_i sum_j Gt_ij (W_j t1_j)
+// with Ps = W o P and t1s = W o t1 precomputed (the same operations, in the
+// same order, as the two separate products).
 constexpr int kNarrowLanes = 4;
+
+// The Schur operator's vectors pre-multiplied by their column scaling, so
+// that its two row kernels gather no W: Ps = W_p o P before the G product;
+// the G product writes t1 = W_u o (G Ps) and t1s = W_u o t1 (the operand of
+// Gt).  Same products in the same order as the unscaled kernels.
+template <class T>
+__global__ void scale_p_kernel(int n, const T *__restrict__ W, int wdof, int off, const T *__restrict__ x,
+                               T *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = W[off + (size_t)wdof * i] * x[i];
+}
+template <class T>
+__global__ void __launch_bounds__(256) g_product_kernel(int nNo, const int *__restrict__ rowptr,
+                                                        const int *__restrict__ col, MatView<T> G,
+                                                        const T *__restrict__ Ps, T *__restrict__ t1,
+                                                        T *__restrict__ t1s) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = gid / kNarrowLanes;
+  const int lane = threadIdx.x % kNarrowLanes;
+  T acc[3] = {T(0), T(0), T(0)};
+  if (row < nNo) {
+    const int ke = rowptr[row + 1];
+    for (int k = rowptr[row] + lane; k < ke; k += kNarrowLanes) {
+      const T xv = Ps[col[k]];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) acc[r] += G.base[(size_t)G.plane[r * 4] * G.nnz + k] * xv;
+    }
+  }
+#pragma unroll
+  for (int off = kNarrowLanes / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) acc[r] += __shfl_xor(acc[r], off, kNarrowLanes);
+  if (row < nNo && lane == 0) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const T w = G.W[G.r0 + r + (size_t)G.wdof * row];
+      T v = w * acc[r];
+      v *= T(1);
+      t1[r + (size_t)3 * row] = v;
+      t1s[r + (size_t)3 * row] = w * v;
+    }
+  }
+}
+// t1s = W_u o t1 again at a face's nodes after ADDBCMUL's PRE changed t1 there
+template <class T>
+__global__ void face_rescale_kernel(int n, const int *__restrict__ glob, const T *__restrict__ W, int wdof,
+                                    const T *__restrict__ t1, T *__restrict__ t1s) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int g = glob[a];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) t1s[c + (size_t)3 * g] = W[c + (size_t)wdof * g] * t1[c + (size_t)3 * g];
+}
 template <class T>
 __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restrict__ rowptr,
                                                     const int *__restrict__ col, MatView<T> Lv, MatView<T> Gv,
-                                                    const T *__restrict__ P, const T *__restrict__ t1,
-                                                    T *__restrict__ SP) {
+                                                    const T *__restrict__ P, const T *__restrict__ Ps,
+                                                    const T *__restrict__ t1s, T *__restrict__ SP) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int row = gid / kNarrowLanes;
   const int lane = threadIdx.x % kNarrowLanes;
@@ -116,10 +170,9 @@ __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restri
     const int ke = rowptr[row + 1];
     for (int k = rowptr[row] + lane; k < ke; k += kNarrowLanes) {
       const int j = col[k];
-      aL += Lv.base[(size_t)Lv.plane[0] * Lv.nnz + k] * (Lv.W[Lv.c0 + (size_t)Lv.wdof * j] * P[j]);
+      aL += Lv.base[(size_t)Lv.plane[0] * Lv.nnz + k] * Ps[j];
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-        aG += Gv.base[(size_t)Gv.plane[c] * Gv.nnz + k] * (Gv.W[Gv.c0 + c + (size_t)Gv.wdof * j] * t1[c + (size_t)3 * j]);
+      for (int c = 0; c < 3; ++c) aG += Gv.base[(size_t)Gv.plane[c] * Gv.nnz + k] * t1s[c + (size_t)3 * j];
     }
   }
 #pragma unroll
@@ -860,10 +913,17 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
   // Schur operator of CGRAD: SP = L P - Gt (G P + PRE)
   auto schur_op = [&](const T *Pv, T *SPv, const double *scal) -> int {
     (void)scal;
-    CR_TRY(spmv(c, mG, 3, 1, Pv, t1.p));
-    if (c.coupled) CR_TRY(addbcmul(c, true, 3, t1.p, t1.p));
-    schur_kernel<T><<<crimson::grid_for((size_t)N * kNarrowLanes, 256), 256, 0, c.s>>>(
-        N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t1.p, SPv);
+    const unsigned gr = crimson::grid_for((size_t)N * kNarrowLanes, 256);
+    scale_p_kernel<T><<<crimson::grid_for(N, 256), 256, 0, c.s>>>(N, c.W.p, 4, mG.c0, Pv, t2.p);  // Ps
+    g_product_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2.p, t1.p, ws.tmp.p);
+    if (c.coupled) {
+      CR_TRY(addbcmul(c, true, 3, t1.p, t1.p));
+      for (auto &fa : l->face)
+        if (fa.coupled)
+          face_rescale_kernel<T><<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, fa.d_glob.p, c.W.p, 4,
+                                                                                 t1.p, ws.tmp.p);
+    }
+    schur_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2.p, ws.tmp.p, SPv);
     CR_HIP(hipGetLastError());
     return CRIMSON_OK;
   };
