@@ -81,6 +81,16 @@ struct DBuf {
   }
 };
 
+// Workgroup index remapped so that each XCD (blocks are dealt round-robin
+// over the 8 XCDs) walks a contiguous range of rows or elements and keeps
+// the vector entries or nodes it gathers in its own L2.
+__device__ inline unsigned xcd_block() {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned per = nb / 8u;
+  if (per == 0 || b >= per * 8u) return b;
+  return (b % 8u) * per + b / 8u;
+}
+
 // Number of workgroups of `threads` covering n items.
 inline unsigned grid_for(size_t n, unsigned threads) {
   return (unsigned)((n + threads - 1) / threads);

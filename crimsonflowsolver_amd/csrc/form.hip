@@ -691,31 +691,62 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A lane's contribution-list entries (16 bits each), eight at a time: two
+// 64-bit words filled by four independent dword loads, consumed from the low
+// end.  The list of one block is contiguous in lst and sorted by incidence,
+// so the loads of a whole block are issued at once instead of one dependent
+// load per contribution.  Only dwords holding an entry of [q, q1) are read.
+struct LstBuf {
+  unsigned long long w0, w1;
+  int n;  // entries left in the buffer
+};
+__device__ inline void lst_fill(LstBuf &b, const unsigned short *__restrict__ lst, int q, int q1) {
+  const unsigned *lw = reinterpret_cast<const unsigned *>(lst);
+  const int d = q >> 1, dend = (q1 + 1) >> 1;  // dwords [d, dend) hold the entries
+  unsigned v[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) v[m] = d + m < dend ? lw[d + m] : 0u;
+  b.w0 = (unsigned long long)v[0] | (unsigned long long)v[1] << 32;
+  b.w1 = (unsigned long long)v[2] | (unsigned long long)v[3] << 32;
+  if (q & 1) {
+    b.w0 = b.w0 >> 16 | b.w1 << 48;
+    b.w1 >>= 16;
+  }
+  b.n = min(q1 - q, 8 - (q & 1));
+}
+__device__ inline void lst_pop(LstBuf &b) {
+  b.w0 = b.w0 >> 16 | b.w1 << 48;
+  b.w1 >>= 16;
+  --b.n;
+}
+
 // Row-parallel assembly, kRowLanes lanes per CSR row; rows are taken in the
 // handle's schedule order (neighbouring rows together, so that the element
-// records they share are read from L2).  The row's element
-// incidences are taken in chunks of kRowLanes, in element order; each chunk's
-// record data (line 0 plus the row node's S and residual half line) is staged
-// in LDS with whole-line loads, 8 lanes per record line, so every record line
-// is read once per row.  Then:
+// records they share are read from L2).  The row's element incidences are
+// taken in chunks of CH, in element order; each chunk's record data (line 0
+// plus the row node's S and residual half line) is staged in LDS with
+// whole-line loads, 8 lanes per record line, so every record line is read
+// once per row.  Then:
 //  - lane s accumulates off-diagonal block s (and s + 16, ... for long rows)
 //    from its contribution list (lst: local incidence << 4 | a << 2 | b, in
 //    element order, built once from the sparseloc positions): fillsparseI's
 //    accumulation (fillsparse.f90:29-63) for that block;
-//  - lane t adds incidence t's diagonal contribution (b = a) and residual row
-//    (asigmr.f90:116); the lanes' partial sums are combined by a fixed
-//    shuffle tree.
+//  - lane t < CH adds incidence t's diagonal contribution (b = a) and
+//    residual row (asigmr.f90:116); the lanes' partial sums are combined by a
+//    fixed shuffle tree.
 // Every block of the row is written, so blocks no element touches come out
 // zero (the reference zeroes lhsK/lhsP first, elmgmr.f90:127-130).
 constexpr int kRowsPerBlock = 256 / kRowLanes;
-template <int MODE>
+constexpr int kAsmChunk = 16;  // incidences staged per chunk (a multiple of 2, at most kRowLanes)
+template <int MODE, int CH = kAsmChunk>
 __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__ rowptr,
                                                const int *__restrict__ diagk, const int *__restrict__ inc_ptr,
                                                const int *__restrict__ inc, const int *__restrict__ lst_ptr,
                                                const unsigned short *__restrict__ lst,
                                                const double *__restrict__ rec, double *__restrict__ res,
                                                AsmOut o, const int *__restrict__ order) {
-  __shared__ double st_all[kRowsPerBlock][kRowLanes][kStage];
+  static_assert(CH % 2 == 0 && CH <= kRowLanes, "chunk of 2..16 incidences");
+  __shared__ double st_all[kRowsPerBlock][CH][kStage];
   const int gid = xcd_block() * blockDim.x + threadIdx.x;
   const int slot = gid / kRowLanes, lane = gid % kRowLanes;
   const bool live = slot < nshg;
@@ -729,7 +760,7 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
     p0 = inc_ptr[row];
     ninc = inc_ptr[row + 1] - p0;
   }
-  const int nchunk = (ninc + kRowLanes - 1) / kRowLanes;
+  const int nchunk = (ninc + CH - 1) / CH;
   const int ngroup = (nslot + kRowLanes - 1) / kRowLanes;
   DiagAcc<double> D;
   diag_zero(D);
@@ -741,13 +772,15 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
     block_zero(A);
     int q = own ? lst_ptr[k0 + s] : 0;
     const int q1 = own ? lst_ptr[k0 + s + 1] : 0;
+    LstBuf lb;
+    lst_fill(lb, lst, q, q1);
     for (int c = 0; c < nchunk; ++c) {
-      const int tb = c * kRowLanes;
-      const int nt = min(kRowLanes, ninc - tb);
+      const int tb = c * CH;
+      const int nt = min(CH, ninc - tb);
       const int ea = lane < nt ? inc[p0 + tb + lane] : 0;
       wave_lds_sync();  // the previous chunk's readers are done
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {  // line 0: 8 pieces of 16 bytes per record
+      for (int j = 0; j < CH * 8 / kRowLanes; ++j) {  // line 0: 8 pieces of 16 bytes per record
         const int piece = j * kRowLanes + lane;
         const int t = piece >> 3, part = piece & 7;
         const int et = __shfl(ea, t, kRowLanes);
@@ -757,8 +790,7 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
           st[t][2 * part + 1] = v.y;
         }
       }
-      // node a's S row (lanes 0..7 of each pair of incidences) and, for the
-      // incidence's own lane, its residual row
+      // node a's S row and, for the incidence's own lane, its residual row
       double2 rl01 = make_double2(0.0, 0.0), rl23 = rl01;
       if (lane < nt) {
         const double *ra = rec + (size_t)(ea >> 2) * kRec + R_A(ea & 3);
@@ -782,11 +814,14 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
         rs[2] += rl23.x;
         rs[3] += rl23.y;
       }
-      for (; q < q1; ++q) {
-        const unsigned v = lst[q];
+      while (q < q1) {
+        if (lb.n == 0) lst_fill(lb, lst, q, q1);  // blocks with more than 8 contributions
+        const unsigned v = (unsigned)(lb.w0 & 0xffffu);
         const int t = (int)(v >> 4) - tb;
-        if (t >= kRowLanes) break;
+        if (t >= CH) break;
         block_add(A, st[t], (int)((v >> 2) & 3u), (int)(v & 3u));
+        lst_pop(lb);
+        ++q;
       }
     }
     if (own) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
@@ -842,6 +877,8 @@ __global__ void __launch_bounds__(256) asm_row32(int nshg, const int *__restrict
     block_zero(A);
     int q = own ? lst_ptr[k0 + s] : 0;
     const int q1 = own ? lst_ptr[k0 + s + 1] : 0;
+    LstBuf lb;
+    lst_fill(lb, lst, q, q1);
     for (int c = 0; c < nchunk; ++c) {
       const int tb = c * kRowLanes;
       const int nt = min(kRowLanes, ninc - tb);
@@ -869,11 +906,14 @@ __global__ void __launch_bounds__(256) asm_row32(int nshg, const int *__restrict
       }
       wave_lds_sync();
       if (g == 0 && lane < nt) diag_add(D, st[lane], ea & 3);
-      for (; q < q1; ++q) {
-        const unsigned v = lst[q];
+      while (q < q1) {
+        if (lb.n == 0) lst_fill(lb, lst, q, q1);
+        const unsigned v = (unsigned)(lb.w0 & 0xffffu);
         const int t = (int)(v >> 4) - tb;
         if (t >= kRowLanes) break;
         block_add(A, st[t], (int)((v >> 2) & 3u), (int)(v & 3u));
+        lst_pop(lb);
+        ++q;
       }
     }
     if (own) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
@@ -1175,7 +1215,8 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
     }
   }
   for (int k = 0; k < nnz; ++k) lst_ptr[k + 1] += lst_ptr[k];
-  std::vector<unsigned short> lst((size_t)lst_ptr[nnz]);
+  // two spare entries: the assembly reads the list in whole dwords
+  std::vector<unsigned short> lst((size_t)lst_ptr[nnz] + 2, 0);
   {
     std::vector<int> fill(lst_ptr.begin(), lst_ptr.end() - 1);
     for (int i = 0; i < nshg; ++i)

@@ -52,19 +52,11 @@ struct Ens {
 };
 __device__ inline double ens_mu(const Ens &en, int e, double mu) { return en.inst ? en.mu[en.inst[e]] : mu; }
 
+using crimson::xcd_block;
+
 struct FormConst {
   double rho, mu, Delt, lhsFct, lhmFct, Dtgl, dtsfct, taucfct;
 };
-
-// Workgroup index remapped so that each XCD (blocks are dealt round-robin
-// over the 8 XCDs) walks a contiguous range of elements and keeps the nodes
-// it gathers in its own L2.
-__device__ inline unsigned xcd_block() {
-  const unsigned nb = gridDim.x, b = blockIdx.x;
-  const unsigned per = nb / 8u;
-  if (per == 0 || b >= per * 8u) return b;
-  return (b % 8u) * per + b / 8u;
-}
 
 // e3metric (e3metric.f90:28-81) specialised to P1: dxdxi_ij =
 // (x_j,i - x_4,i)/2 since shgl = dN/dxi / 2 (genshp.f90:35-36).

@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
                                                    const int *__restrict__ col, MatView<T> A,
                                                    const T *__restrict__ x, T *__restrict__ y, T ya,
                                                    const T *__restrict__ z, T za) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = crimson::xcd_block() * blockDim.x + threadIdx.x;
   const int row = gid / LANES;
   const int lane = threadIdx.x % LANES;
   T acc[NR];
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256) g_product_kernel(int nNo, const int *__re
                                                         const int *__restrict__ col, MatView<T> G,
                                                         const T *__restrict__ Ps, T *__restrict__ t1,
                                                         T *__restrict__ t1s) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = crimson::xcd_block() * blockDim.x + threadIdx.x;
   const int row = gid / kNarrowLanes;
   const int lane = threadIdx.x % kNarrowLanes;
   T acc[3] = {T(0), T(0), T(0)};
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restri
                                                     const int *__restrict__ col, MatView<T> Lv, MatView<T> Gv,
                                                     const T *__restrict__ P, const T *__restrict__ Ps,
                                                     const T *__restrict__ t1s, T *__restrict__ SP) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = crimson::xcd_block() * blockDim.x + threadIdx.x;
   const int row = gid / kNarrowLanes;
   const int lane = threadIdx.x % kNarrowLanes;
   T aL = T(0), aG = T(0);
