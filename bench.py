@@ -59,7 +59,7 @@ def pmc_traffic(grid, kernel):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) == 3 and parts[0] == "traffic_bytes_per_dispatch" and parts[1] == kernel:
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_dispatch" and parts[1] == kernel.replace(" ", ""):
             return float(parts[2])
     return None
 
@@ -317,9 +317,9 @@ def main():
         # off-diagonal contribution, 12 per element) indices
         byt = nel * 48 * 8 + nnz * 16 * 8 + N * 4 * 8 + nel * 4 * 4 + nel * 12 * 2
         ach = byt / (tm[2] * 1e-3) / 1e9
-        roof = {"kernel": "asm_row<1>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+        roof = {"kernel": "asm_row<1,16>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": byt,
-                "traffic": pmc_traffic(grid, "asm_row<1>")}
+                "traffic": pmc_traffic(grid, "asm_row<1,16>")}
 
     cpu = None
     if args.cpu_seconds > 0:

@@ -162,11 +162,14 @@ __global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict
 // partially filled lines).
 constexpr int kE3Threads = 128;
 constexpr bool kE3Stage = true;
+// LDS stride of a staged record: one double more than the record, so that
+// the threads' stores at the same offset fall in different banks.
+constexpr int kE3Pad = kRec + 1;
 template <bool LHS>
 __global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__restrict__ ien,
                                                       const double *__restrict__ node, FormConst c0, Ens en,
                                                       double *__restrict__ rec) {
-  __shared__ double sh_rec[(LHS && kE3Stage) ? kE3Threads * kRec : 1];
+  __shared__ double sh_rec[(LHS && kE3Stage) ? kE3Threads * kE3Pad : 1];
   const int e0 = xcd_block() * kE3Threads;
   const int e = e0 + threadIdx.x;
   if (!(LHS && kE3Stage) && e >= nel) return;
@@ -318,7 +321,7 @@ __global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__rest
     }
   }
 
-  double *r = (LHS && kE3Stage) ? sh_rec + threadIdx.x * kRec : rec + (size_t)e * kRec;
+  double *r = (LHS && kE3Stage) ? sh_rec + threadIdx.x * kE3Pad : rec + (size_t)e * kRec;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     r[R_A(a) + 4 + 3] = W * (g[a][0] * SU[0] + g[a][1] * SU[1] + g[a][2] * SU[2]);
@@ -358,9 +361,11 @@ __global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__rest
   if (!kE3Stage) return;
   __syncthreads();
   const int cnt = min(kE3Threads, nel - e0);
-  const double2 *src = reinterpret_cast<const double2 *>(sh_rec);
   double2 *dst = reinterpret_cast<double2 *>(rec + (size_t)e0 * kRec);
-  for (int i = threadIdx.x; i < cnt * (kRec / 2); i += kE3Threads) dst[i] = src[i];
+  for (int i = threadIdx.x; i < cnt * (kRec / 2); i += kE3Threads) {
+    const double *src = sh_rec + (i / (kRec / 2)) * kE3Pad + 2 * (i % (kRec / 2));
+    dst[i] = make_double2(src[0], src[1]);
+  }
 }
 
 // ---------------------------------------------------------------- FP32 tangent
@@ -657,15 +662,21 @@ __device__ inline void block_add(BlockAcc<T> &A, const T *r, int a, int b) {
 }
 
 // Diagonal contribution (b = a): the transposed block is the block itself,
-// so only K, lhsP(1:3) and C are accumulated.
+// so only K, lhsP(1:3) and C are accumulated.  K^{ii,jj} = delta S_aa +
+// Tc g_a,ii g_a,jj + Mu g_a,jj g_a,ii is symmetric: its upper triangle (ii <=
+// jj, each entry formed as the full-block loop forms it) is kept, in the
+// order 00 01 02 11 12 22, and mirrored when the block is written.
 template <class T>
 struct DiagAcc {
-  T K[9], G[3], C;
+  T K[6], G[3], C;
 };
+__host__ __device__ constexpr int sym3(int ii, int jj) {
+  return ii <= jj ? ii * 3 - (ii * (ii - 1)) / 2 + (jj - ii) : jj * 3 - (jj * (jj - 1)) / 2 + (ii - jj);
+}
 template <class T>
 __device__ inline void diag_zero(DiagAcc<T> &A) {
 #pragma unroll
-  for (int m = 0; m < 9; ++m) A.K[m] = T(0);
+  for (int m = 0; m < 6; ++m) A.K[m] = T(0);
   A.G[0] = A.G[1] = A.G[2] = A.C = T(0);
 }
 template <class T>
@@ -675,12 +686,32 @@ __device__ inline void diag_add(DiagAcc<T> &A, const T *r, int a) {
 #pragma unroll
   for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
-    for (int ii = 0; ii < 3; ++ii)
-      A.K[ii + 3 * jj] += (ii == jj ? Saa : T(0)) + (Tc * ga[ii] * ga[jj] + Mu * ga[jj] * ga[ii]);
+    for (int ii = 0; ii <= jj; ++ii)
+      A.K[sym3(ii, jj)] += (ii == jj ? Saa : T(0)) + (Tc * ga[ii] * ga[jj] + Mu * ga[jj] * ga[ii]);
   const T ta = tl * T(sum_n_rt(a));
 #pragma unroll
   for (int ii = 0; ii < 3; ++ii) A.G[ii] += ta * ga[ii];
   A.C += Cm * (ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2]);
+}
+// The lanes' diagonal partial sums combined by a fixed shuffle tree over
+// the kRowLanes lanes of a row.
+template <class T>
+__device__ inline void diag_reduce(DiagAcc<T> &D) {
+#pragma unroll
+  for (int off = kRowLanes / 2; off > 0; off >>= 1) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) D.K[m] += __shfl_xor(D.K[m], off, kRowLanes);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) D.G[m] += __shfl_xor(D.G[m], off, kRowLanes);
+    D.C += __shfl_xor(D.C, off, kRowLanes);
+  }
+}
+template <class T>
+__device__ inline void diag_full(const DiagAcc<T> &D, T (&K)[9]) {
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) K[ii + 3 * jj] = D.K[sym3(ii, jj)];
 }
 
 // Orders LDS accesses of the lanes of one wavefront (the lanes of a CSR row
@@ -779,40 +810,42 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
       const int nt = min(CH, ninc - tb);
       const int ea = lane < nt ? inc[p0 + tb + lane] : 0;
       wave_lds_sync();  // the previous chunk's readers are done
+      // line 0 (8 pieces of 16 bytes per record) and node a's S row.  No
+      // guard on the loads, so that they are all in flight together: lanes
+      // and pieces past the chunk's end read incidence 0 (ea = 0, always a
+      // valid record) into staging slots nothing reads.
+      double2 pc[CH * 8 / kRowLanes];
 #pragma unroll
-      for (int j = 0; j < CH * 8 / kRowLanes; ++j) {  // line 0: 8 pieces of 16 bytes per record
-        const int piece = j * kRowLanes + lane;
-        const int t = piece >> 3, part = piece & 7;
+      for (int j = 0; j < CH * 8 / kRowLanes; ++j) {
+        const int t = (j * kRowLanes + lane) >> 3, part = lane & 7;
         const int et = __shfl(ea, t, kRowLanes);
-        if (t < nt) {
-          const double2 v = *reinterpret_cast<const double2 *>(rec + (size_t)(et >> 2) * kRec + 2 * part);
-          st[t][2 * part] = v.x;
-          st[t][2 * part + 1] = v.y;
-        }
+        pc[j] = *reinterpret_cast<const double2 *>(rec + (size_t)(et >> 2) * kRec + 2 * part);
       }
-      // node a's S row and, for the incidence's own lane, its residual row
-      double2 rl01 = make_double2(0.0, 0.0), rl23 = rl01;
-      if (lane < nt) {
-        const double *ra = rec + (size_t)(ea >> 2) * kRec + R_A(ea & 3);
-        const double2 s01 = *reinterpret_cast<const double2 *>(ra);
-        const double2 s23 = *reinterpret_cast<const double2 *>(ra + 2);
-        if (g == 0) {
-          rl01 = *reinterpret_cast<const double2 *>(ra + 4);
-          rl23 = *reinterpret_cast<const double2 *>(ra + 6);
-        }
+      const double *ra = rec + (size_t)(ea >> 2) * kRec + R_A(ea & 3);
+      const double2 s01 = *reinterpret_cast<const double2 *>(ra);
+      const double2 s23 = *reinterpret_cast<const double2 *>(ra + 2);
+#pragma unroll
+      for (int j = 0; j < CH * 8 / kRowLanes; ++j) {
+        const int t = (j * kRowLanes + lane) >> 3, part = lane & 7;
+        st[t][2 * part] = pc[j].x;
+        st[t][2 * part + 1] = pc[j].y;
+      }
+      if (lane < CH) {
         st[lane][ST_S + 0] = s01.x;
         st[lane][ST_S + 1] = s01.y;
         st[lane][ST_S + 2] = s23.x;
         st[lane][ST_S + 3] = s23.y;
       }
       wave_lds_sync();
-      if (g == 0 && lane < nt) {
-        const int a = ea & 3;
-        diag_add(D, st[lane], a);
-        rs[0] += rl01.x;
-        rs[1] += rl01.y;
-        rs[2] += rl23.x;
-        rs[3] += rl23.y;
+      // the incidence's own lane: diagonal contribution, and its residual
+      // row, loaded here and added after the off-diagonal blocks so that the
+      // load is not held in registers while the chunk is staged
+      const bool dl = g == 0 && lane < nt;
+      double2 rl01 = make_double2(0.0, 0.0), rl23 = rl01;
+      if (dl) {
+        rl01 = *reinterpret_cast<const double2 *>(ra + 4);
+        rl23 = *reinterpret_cast<const double2 *>(ra + 6);
+        diag_add(D, st[lane], ea & 3);
       }
       while (q < q1) {
         if (lb.n == 0) lst_fill(lb, lst, q, q1);  // blocks with more than 8 contributions
@@ -823,21 +856,24 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
         lst_pop(lb);
         ++q;
       }
+      if (dl) {
+        rs[0] += rl01.x;
+        rs[1] += rl01.y;
+        rs[2] += rl23.x;
+        rs[3] += rl23.y;
+      }
     }
     if (own) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
   }
+  diag_reduce(D);
 #pragma unroll
-  for (int off = kRowLanes / 2; off > 0; off >>= 1) {
-#pragma unroll
-    for (int m = 0; m < 9; ++m) D.K[m] += __shfl_xor(D.K[m], off, kRowLanes);
-#pragma unroll
-    for (int m = 0; m < 3; ++m) D.G[m] += __shfl_xor(D.G[m], off, kRowLanes);
-    D.C += __shfl_xor(D.C, off, kRowLanes);
+  for (int off = kRowLanes / 2; off > 0; off >>= 1)
 #pragma unroll
     for (int k = 0; k < 4; ++k) rs[k] += __shfl_xor(rs[k], off, kRowLanes);
-  }
   if (live && lane == 0) {
-    put_block<MODE>(o, k0 + sdiag, D.K, D.G, D.C, D.G);
+    double K[9];
+    diag_full(D, K);
+    put_block<MODE>(o, k0 + sdiag, K, D.G, D.C, D.G);
     put_res<MODE>(res, row, nshg, rs);
   }
 }
@@ -884,26 +920,28 @@ __global__ void __launch_bounds__(256) asm_row32(int nshg, const int *__restrict
       const int nt = min(kRowLanes, ninc - tb);
       const int ea = lane < nt ? inc[p0 + tb + lane] : 0;
       wave_lds_sync();
+      // line 0 of the record (4 pieces of 16 bytes) and node a's S row,
+      // unguarded as in asm_row (past the chunk's end: incidence 0)
+      float4 pc[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {  // line 0 of the record: 4 pieces of 16 bytes
-        const int piece = j * kRowLanes + lane;
-        const int t = piece >> 2, part = piece & 3;
+      for (int j = 0; j < 4; ++j) {
+        const int t = (j * kRowLanes + lane) >> 2, part = lane & 3;
         const int et = __shfl(ea, t, kRowLanes);
-        if (t < nt) {
-          const float4 v = *reinterpret_cast<const float4 *>(rec + (size_t)(et >> 2) * kRec32 + 4 * part);
-          st[t][4 * part + 0] = v.x;
-          st[t][4 * part + 1] = v.y;
-          st[t][4 * part + 2] = v.z;
-          st[t][4 * part + 3] = v.w;
-        }
+        pc[j] = *reinterpret_cast<const float4 *>(rec + (size_t)(et >> 2) * kRec32 + 4 * part);
       }
-      if (lane < nt) {
-        const float4 v = *reinterpret_cast<const float4 *>(rec + (size_t)(ea >> 2) * kRec32 + R32_S(ea & 3));
-        st[lane][ST_S + 0] = v.x;
-        st[lane][ST_S + 1] = v.y;
-        st[lane][ST_S + 2] = v.z;
-        st[lane][ST_S + 3] = v.w;
+      const float4 sv = *reinterpret_cast<const float4 *>(rec + (size_t)(ea >> 2) * kRec32 + R32_S(ea & 3));
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int t = (j * kRowLanes + lane) >> 2, part = lane & 3;
+        st[t][4 * part + 0] = pc[j].x;
+        st[t][4 * part + 1] = pc[j].y;
+        st[t][4 * part + 2] = pc[j].z;
+        st[t][4 * part + 3] = pc[j].w;
       }
+      st[lane][ST_S + 0] = sv.x;
+      st[lane][ST_S + 1] = sv.y;
+      st[lane][ST_S + 2] = sv.z;
+      st[lane][ST_S + 3] = sv.w;
       wave_lds_sync();
       if (g == 0 && lane < nt) diag_add(D, st[lane], ea & 3);
       while (q < q1) {
@@ -918,15 +956,12 @@ __global__ void __launch_bounds__(256) asm_row32(int nshg, const int *__restrict
     }
     if (own) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
   }
-#pragma unroll
-  for (int off = kRowLanes / 2; off > 0; off >>= 1) {
-#pragma unroll
-    for (int m = 0; m < 9; ++m) D.K[m] += __shfl_xor(D.K[m], off, kRowLanes);
-#pragma unroll
-    for (int m = 0; m < 3; ++m) D.G[m] += __shfl_xor(D.G[m], off, kRowLanes);
-    D.C += __shfl_xor(D.C, off, kRowLanes);
+  diag_reduce(D);
+  if (live && lane == 0) {
+    float K[9];
+    diag_full(D, K);
+    put_block<MODE>(o, k0 + sdiag, K, D.G, D.C, D.G);
   }
-  if (live && lane == 0) put_block<MODE>(o, k0 + sdiag, D.K, D.G, D.C, D.G);
 }
 
 // bc3Res for codes 0/7 on the velocity bits, bit 2 (pressure) and bit 11

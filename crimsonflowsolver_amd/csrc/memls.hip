@@ -109,27 +109,59 @@ constexpr int kNarrowLanes = 4;
 // that its two row kernels gather no W: Ps = W_p o P before the G product;
 // the G product writes t1 = W_u o (G Ps) and t1s = W_u o t1 (the operand of
 // Gt).  Same products in the same order as the unscaled kernels.
+// Entries per lane loaded ahead in the narrow row kernels: the column
+// indices of kNarrowUnroll entries (k, k + 4, ...) are loaded together, then
+// the gathered vector entries and matrix values, so a lane waits for one
+// index load and one gather per kNarrowUnroll entries instead of per entry.
+// Indices past the row end are clamped to its last entry (in bounds; the
+// product is not added), so the accumulation order is the plain loop's.
+constexpr int kNarrowUnroll = 4;
+
+// `done` (may be null): the CG's converged flag; the remaining iterations of
+// a batch launched after convergence return at once.
 template <class T>
 __global__ void scale_p_kernel(int n, const T *__restrict__ W, int wdof, int off, const T *__restrict__ x,
-                               T *__restrict__ y) {
+                               T *__restrict__ y, const double *__restrict__ done) {
+  if (done && *done != 0.0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = W[off + (size_t)wdof * i] * x[i];
 }
-template <class T>
+template <class T, int U = kNarrowUnroll>
 __global__ void __launch_bounds__(256) g_product_kernel(int nNo, const int *__restrict__ rowptr,
                                                         const int *__restrict__ col, MatView<T> G,
                                                         const T *__restrict__ Ps, T *__restrict__ t1,
-                                                        T *__restrict__ t1s) {
+                                                        T *__restrict__ t1s, const double *__restrict__ done) {
+  if (done && *done != 0.0) return;
   const int gid = crimson::xcd_block() * blockDim.x + threadIdx.x;
   const int row = gid / kNarrowLanes;
   const int lane = threadIdx.x % kNarrowLanes;
   T acc[3] = {T(0), T(0), T(0)};
   if (row < nNo) {
     const int ke = rowptr[row + 1];
-    for (int k = rowptr[row] + lane; k < ke; k += kNarrowLanes) {
-      const T xv = Ps[col[k]];
+    const T *g0 = G.base + (size_t)G.plane[0] * G.nnz, *g1 = G.base + (size_t)G.plane[4] * G.nnz,
+            *g2 = G.base + (size_t)G.plane[8] * G.nnz;
+    for (int k = rowptr[row] + lane; k < ke; k += kNarrowLanes * U) {
+      int j[U], kc[U];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) acc[r] += G.base[(size_t)G.plane[r * 4] * G.nnz + k] * xv;
+      for (int u = 0; u < U; ++u) {
+        kc[u] = min(k + kNarrowLanes * u, ke - 1);
+        j[u] = col[kc[u]];
+      }
+      T xv[U], a0[U], a1[U], a2[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        xv[u] = Ps[j[u]];
+        a0[u] = g0[kc[u]];
+        a1[u] = g1[kc[u]];
+        a2[u] = g2[kc[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (k + kNarrowLanes * u < ke) {
+          acc[0] += a0[u] * xv[u];
+          acc[1] += a1[u] * xv[u];
+          acc[2] += a2[u] * xv[u];
+        }
     }
   }
 #pragma unroll
@@ -157,22 +189,49 @@ __global__ void face_rescale_kernel(int n, const int *__restrict__ glob, const T
 #pragma unroll
   for (int c = 0; c < 3; ++c) t1s[c + (size_t)3 * g] = W[c + (size_t)wdof * g] * t1[c + (size_t)3 * g];
 }
-template <class T>
+template <class T, int U = kNarrowUnroll>
 __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restrict__ rowptr,
                                                     const int *__restrict__ col, MatView<T> Lv, MatView<T> Gv,
                                                     const T *__restrict__ P, const T *__restrict__ Ps,
-                                                    const T *__restrict__ t1s, T *__restrict__ SP) {
+                                                    const T *__restrict__ t1s, T *__restrict__ SP,
+                                                    const double *__restrict__ done) {
+  if (done && *done != 0.0) return;
   const int gid = crimson::xcd_block() * blockDim.x + threadIdx.x;
   const int row = gid / kNarrowLanes;
   const int lane = threadIdx.x % kNarrowLanes;
   T aL = T(0), aG = T(0);
   if (row < nNo) {
     const int ke = rowptr[row + 1];
-    for (int k = rowptr[row] + lane; k < ke; k += kNarrowLanes) {
-      const int j = col[k];
-      aL += Lv.base[(size_t)Lv.plane[0] * Lv.nnz + k] * Ps[j];
+    const T *lp = Lv.base + (size_t)Lv.plane[0] * Lv.nnz;
+    const T *d0 = Gv.base + (size_t)Gv.plane[0] * Gv.nnz, *d1 = Gv.base + (size_t)Gv.plane[1] * Gv.nnz,
+            *d2 = Gv.base + (size_t)Gv.plane[2] * Gv.nnz;
+    for (int k = rowptr[row] + lane; k < ke; k += kNarrowLanes * U) {
+      int j[U], kc[U];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) aG += Gv.base[(size_t)Gv.plane[c] * Gv.nnz + k] * t1s[c + (size_t)3 * j];
+      for (int u = 0; u < U; ++u) {
+        kc[u] = min(k + kNarrowLanes * u, ke - 1);
+        j[u] = col[kc[u]];
+      }
+      T l[U], ps[U], b0[U], b1[U], b2[U], x0[U], x1[U], x2[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        l[u] = lp[kc[u]];
+        ps[u] = Ps[j[u]];
+        b0[u] = d0[kc[u]];
+        b1[u] = d1[kc[u]];
+        b2[u] = d2[kc[u]];
+        x0[u] = t1s[(size_t)3 * j[u]];
+        x1[u] = t1s[(size_t)3 * j[u] + 1];
+        x2[u] = t1s[(size_t)3 * j[u] + 2];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (k + kNarrowLanes * u < ke) {
+          aL += l[u] * ps[u];
+          aG += b0[u] * x0[u];
+          aG += b1[u] * x1[u];
+          aG += b2[u] * x2[u];
+        }
     }
   }
 #pragma unroll
@@ -912,10 +971,10 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
   }
   // Schur operator of CGRAD: SP = L P - Gt (G P + PRE)
   auto schur_op = [&](const T *Pv, T *SPv, const double *scal) -> int {
-    (void)scal;
     const unsigned gr = crimson::grid_for((size_t)N * kNarrowLanes, 256);
-    scale_p_kernel<T><<<crimson::grid_for(N, 256), 256, 0, c.s>>>(N, c.W.p, 4, mG.c0, Pv, t2.p);  // Ps
-    g_product_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2.p, t1.p, ws.tmp.p);
+    const double *done = scal + 5;
+    scale_p_kernel<T><<<crimson::grid_for(N, 256), 256, 0, c.s>>>(N, c.W.p, 4, mG.c0, Pv, t2.p, done);  // Ps
+    g_product_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2.p, t1.p, ws.tmp.p, done);
     if (c.coupled) {
       CR_TRY(addbcmul(c, true, 3, t1.p, t1.p));
       for (auto &fa : l->face)
@@ -923,7 +982,7 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
           face_rescale_kernel<T><<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, fa.d_glob.p, c.W.p, 4,
                                                                                  t1.p, ws.tmp.p);
     }
-    schur_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2.p, ws.tmp.p, SPv);
+    schur_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2.p, ws.tmp.p, SPv, done);
     CR_HIP(hipGetLastError());
     return CRIMSON_OK;
   };
