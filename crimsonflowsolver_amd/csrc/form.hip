@@ -729,7 +729,8 @@ __device__ inline void wave_lds_sync() {
 // load per contribution.  Only dwords holding an entry of [q, q1) are read.
 struct LstBuf {
   unsigned long long w0, w1;
-  int n;  // entries left in the buffer
+  int n;     // entries left in the buffer
+  int skip;  // 1: the first 16 bits precede q (q odd); dropped at first use
 };
 __device__ inline void lst_fill(LstBuf &b, const unsigned short *__restrict__ lst, int q, int q1) {
   const unsigned *lw = reinterpret_cast<const unsigned *>(lst);
@@ -739,16 +740,38 @@ __device__ inline void lst_fill(LstBuf &b, const unsigned short *__restrict__ ls
   for (int m = 0; m < 4; ++m) v[m] = d + m < dend ? lw[d + m] : 0u;
   b.w0 = (unsigned long long)v[0] | (unsigned long long)v[1] << 32;
   b.w1 = (unsigned long long)v[2] | (unsigned long long)v[3] << 32;
-  if (q & 1) {
-    b.w0 = b.w0 >> 16 | b.w1 << 48;
-    b.w1 >>= 16;
-  }
+  b.skip = q & 1;
   b.n = min(q1 - q, 8 - (q & 1));
 }
-__device__ inline void lst_pop(LstBuf &b) {
+__device__ inline void lst_shift(LstBuf &b) {
   b.w0 = b.w0 >> 16 | b.w1 << 48;
   b.w1 >>= 16;
+}
+// the next entry (the buffer holds at least one)
+__device__ inline unsigned lst_front(LstBuf &b) {
+  if (b.skip) {
+    lst_shift(b);
+    b.skip = 0;
+  }
+  return (unsigned)(b.w0 & 0xffffu);
+}
+__device__ inline void lst_pop(LstBuf &b) {
+  lst_shift(b);
   --b.n;
+}
+
+// Per row of the schedule: row, first CSR position, slots, diagonal slot
+// (int4), first incidence and incidence count (int4, two unused), built once
+// with the schedule so that a row's assembly starts from one load.
+__global__ void slot_table_kernel(int n, const int *__restrict__ order, const int *__restrict__ rowptr,
+                                  const int *__restrict__ diagk, const int *__restrict__ inc_ptr,
+                                  int4 *__restrict__ tab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int row = order[i];
+  const int k0 = rowptr[row], p0 = inc_ptr[row];
+  tab[2 * i] = make_int4(row, k0, rowptr[row + 1] - k0, diagk[row] - k0);
+  tab[2 * i + 1] = make_int4(p0, inc_ptr[row + 1] - p0, 0, 0);
 }
 
 // Row-parallel assembly, kRowLanes lanes per CSR row; rows are taken in the
@@ -770,26 +793,26 @@ __device__ inline void lst_pop(LstBuf &b) {
 constexpr int kRowsPerBlock = 256 / kRowLanes;
 constexpr int kAsmChunk = 16;  // incidences staged per chunk (a multiple of 2, at most kRowLanes)
 template <int MODE, int CH = kAsmChunk>
-__global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__ rowptr,
-                                               const int *__restrict__ diagk, const int *__restrict__ inc_ptr,
-                                               const int *__restrict__ inc, const int *__restrict__ lst_ptr,
+__global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict__ tab, const int *__restrict__ inc,
+                                               const int *__restrict__ lst_ptr,
                                                const unsigned short *__restrict__ lst,
                                                const double *__restrict__ rec, double *__restrict__ res,
-                                               AsmOut o, const int *__restrict__ order) {
+                                               AsmOut o) {
   static_assert(CH % 2 == 0 && CH <= kRowLanes, "chunk of 2..16 incidences");
   __shared__ double st_all[kRowsPerBlock][CH][kStage];
   const int gid = xcd_block() * blockDim.x + threadIdx.x;
   const int slot = gid / kRowLanes, lane = gid % kRowLanes;
   const bool live = slot < nshg;
-  const int row = live ? order[slot] : slot;
   double(*st)[kStage] = st_all[threadIdx.x / kRowLanes];
-  int k0 = 0, nslot = 0, sdiag = -1, p0 = 0, ninc = 0;
+  int row = 0, k0 = 0, nslot = 0, sdiag = -1, p0 = 0, ninc = 0;
   if (live) {
-    k0 = rowptr[row];
-    nslot = rowptr[row + 1] - k0;
-    sdiag = diagk[row] - k0;
-    p0 = inc_ptr[row];
-    ninc = inc_ptr[row + 1] - p0;
+    const int4 t0 = tab[2 * slot], t1 = tab[2 * slot + 1];
+    row = t0.x;
+    k0 = t0.y;
+    nslot = t0.z;
+    sdiag = t0.w;
+    p0 = t1.x;
+    ninc = t1.y;
   }
   const int nchunk = (ninc + CH - 1) / CH;
   const int ngroup = (nslot + kRowLanes - 1) / kRowLanes;
@@ -801,14 +824,21 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
     const bool own = s < nslot && s != sdiag;
     BlockAcc<double> A;
     block_zero(A);
-    int q = own ? lst_ptr[k0 + s] : 0;
-    const int q1 = own ? lst_ptr[k0 + s + 1] : 0;
+    // the block's list range and the first chunk's incidences, loaded
+    // together (lanes without a block read lst_ptr[0..1], an in-bounds pair)
+    const int kq = own ? k0 + s : 0;
+    int q = lst_ptr[kq];
+    const int q1r = lst_ptr[kq + 1];
+    int nt = min(CH, ninc);
+    int ea = lane < nt ? inc[p0 + lane] : 0;
+    const int q1 = own ? q1r : q;
     LstBuf lb;
     lst_fill(lb, lst, q, q1);
     for (int c = 0; c < nchunk; ++c) {
       const int tb = c * CH;
-      const int nt = min(CH, ninc - tb);
-      const int ea = lane < nt ? inc[p0 + tb + lane] : 0;
+      // the next chunk's incidences, in flight while this chunk is staged
+      const int ntn = min(CH, ninc - tb - CH);
+      const int ean = lane < ntn ? inc[p0 + tb + CH + lane] : 0;
       wave_lds_sync();  // the previous chunk's readers are done
       // line 0 (8 pieces of 16 bytes per record) and node a's S row.  No
       // guard on the loads, so that they are all in flight together: lanes
@@ -849,7 +879,7 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
       }
       while (q < q1) {
         if (lb.n == 0) lst_fill(lb, lst, q, q1);  // blocks with more than 8 contributions
-        const unsigned v = (unsigned)(lb.w0 & 0xffffu);
+        const unsigned v = lst_front(lb);
         const int t = (int)(v >> 4) - tb;
         if (t >= CH) break;
         block_add(A, st[t], (int)((v >> 2) & 3u), (int)(v & 3u));
@@ -862,6 +892,8 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
         rs[2] += rl23.x;
         rs[3] += rl23.y;
       }
+      nt = ntn;
+      ea = ean;
     }
     if (own) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
   }
@@ -882,25 +914,23 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int *__restrict__
 // FP32 records (one line per element: g, scalars, S), FP32 accumulation and
 // FP32 planes (MODE 1) or FP64 lhsK/lhsP (MODE 0).
 template <int MODE>
-__global__ void __launch_bounds__(256) asm_row32(int nshg, const int *__restrict__ rowptr,
-                                                 const int *__restrict__ diagk, const int *__restrict__ inc_ptr,
+__global__ void __launch_bounds__(256) asm_row32(int nshg, const int4 *__restrict__ tab,
                                                  const int *__restrict__ inc, const int *__restrict__ lst_ptr,
                                                  const unsigned short *__restrict__ lst,
-                                                 const float *__restrict__ rec, AsmOut o,
-                                                 const int *__restrict__ order) {
+                                                 const float *__restrict__ rec, AsmOut o) {
   __shared__ float st_all[kRowsPerBlock][kRowLanes][kStage];
   const int gid = xcd_block() * blockDim.x + threadIdx.x;
   const int slot = gid / kRowLanes, lane = gid % kRowLanes;
   const bool live = slot < nshg;
-  const int row = live ? order[slot] : slot;
   float(*st)[kStage] = st_all[threadIdx.x / kRowLanes];
   int k0 = 0, nslot = 0, sdiag = -1, p0 = 0, ninc = 0;
   if (live) {
-    k0 = rowptr[row];
-    nslot = rowptr[row + 1] - k0;
-    sdiag = diagk[row] - k0;
-    p0 = inc_ptr[row];
-    ninc = inc_ptr[row + 1] - p0;
+    const int4 t0 = tab[2 * slot], t1 = tab[2 * slot + 1];
+    k0 = t0.y;
+    nslot = t0.z;
+    sdiag = t0.w;
+    p0 = t1.x;
+    ninc = t1.y;
   }
   const int nchunk = (ninc + kRowLanes - 1) / kRowLanes;
   const int ngroup = (nslot + kRowLanes - 1) / kRowLanes;
@@ -911,14 +941,18 @@ __global__ void __launch_bounds__(256) asm_row32(int nshg, const int *__restrict
     const bool own = s < nslot && s != sdiag;
     BlockAcc<float> A;
     block_zero(A);
-    int q = own ? lst_ptr[k0 + s] : 0;
-    const int q1 = own ? lst_ptr[k0 + s + 1] : 0;
+    const int kq = own ? k0 + s : 0;
+    int q = lst_ptr[kq];
+    const int q1r = lst_ptr[kq + 1];
+    int nt = min(kRowLanes, ninc);
+    int ea = lane < nt ? inc[p0 + lane] : 0;
+    const int q1 = own ? q1r : q;
     LstBuf lb;
     lst_fill(lb, lst, q, q1);
     for (int c = 0; c < nchunk; ++c) {
       const int tb = c * kRowLanes;
-      const int nt = min(kRowLanes, ninc - tb);
-      const int ea = lane < nt ? inc[p0 + tb + lane] : 0;
+      const int ntn = min(kRowLanes, ninc - tb - kRowLanes);
+      const int ean = lane < ntn ? inc[p0 + tb + kRowLanes + lane] : 0;
       wave_lds_sync();
       // line 0 of the record (4 pieces of 16 bytes) and node a's S row,
       // unguarded as in asm_row (past the chunk's end: incidence 0)
@@ -946,13 +980,15 @@ __global__ void __launch_bounds__(256) asm_row32(int nshg, const int *__restrict
       if (g == 0 && lane < nt) diag_add(D, st[lane], ea & 3);
       while (q < q1) {
         if (lb.n == 0) lst_fill(lb, lst, q, q1);
-        const unsigned v = (unsigned)(lb.w0 & 0xffffu);
+        const unsigned v = lst_front(lb);
         const int t = (int)(v >> 4) - tb;
         if (t >= kRowLanes) break;
         block_add(A, st[t], (int)((v >> 2) & 3u), (int)(v & 3u));
         lst_pop(lb);
         ++q;
       }
+      nt = ntn;
+      ea = ean;
     }
     if (own) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
   }
@@ -1067,6 +1103,10 @@ int ensure_order(crimson_form *f, const double *d_x) {
   std::vector<int> order(n);
   for (int i = 0; i < n; ++i) order[i] = key[i].second;
   CR_TRY(f->d_order.upload(order.data(), n, f->stream));
+  CR_TRY(f->d_slot.alloc((size_t)2 * n));
+  slot_table_kernel<<<crimson::grid_for(n, 256), 256, 0, f->stream>>>(n, f->d_order.p, f->d_rowptr.p, f->d_diagk.p,
+                                                                      f->d_inc_ptr.p, f->d_slot.p);
+  CR_HIP(hipGetLastError());
   CR_HIP(hipStreamSynchronize(f->stream));
   return CRIMSON_OK;
 }
@@ -1100,12 +1140,12 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
   if (t32) {
     if (mode == 0) {
       res_node<0><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
-      asm_row32<0><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
-                                    f->d_lst_ptr.p, f->d_lst.p, f->d_rec32.p, o, f->d_order.p);
+      asm_row32<0><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p,
+                                    f->d_rec32.p, o);
     } else {
       res_node<1><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
-      asm_row32<1><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
-                                    f->d_lst_ptr.p, f->d_lst.p, f->d_rec32.p, o, f->d_order.p);
+      asm_row32<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p,
+                                    f->d_rec32.p, o);
     }
   } else if (!lhs) {
     if (mode == 0)
@@ -1113,11 +1153,11 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
     else
       res_node<1><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
   } else if (mode == 0) {
-    asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
-                                f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p, d_res, o, f->d_order.p);
+    asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p,
+                                f->d_rec.p, d_res, o);
   } else {
-    asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_rowptr.p, f->d_diagk.p, f->d_inc_ptr.p, f->d_inc.p,
-                                f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p, d_res, o, f->d_order.p);
+    asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p,
+                                f->d_rec.p, d_res, o);
   }
   CR_HIP(hipGetLastError());
   if (timed) CR_HIP(hipEventRecord(f->ev[3], s));

@@ -106,9 +106,11 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
 constexpr int kNarrowLanes = 4;
 
 // The Schur operator's vectors pre-multiplied by their column scaling, so
-// that its two row kernels gather no W: Ps = W_p o P before the G product;
-// the G product writes t1 = W_u o (G Ps) and t1s = W_u o t1 (the operand of
-// Gt).  Same products in the same order as the unscaled kernels.
+// that its two row kernels gather no W: Ps = W_p o P (written by the CG's
+// direction update, scale_p_kernel for the first direction); the G product
+// writes t1 = W_u o (G Ps) and t1s = W_u o t1 (the operand of Gt).  Same
+// products in the same order as the unscaled kernels.
+
 // Entries per lane loaded ahead in the narrow row kernels: the column
 // indices of kNarrowUnroll entries (k, k + 4, ...) are loaded together, then
 // the gathered vector entries and matrix values, so a lane waits for one
@@ -178,16 +180,6 @@ __global__ void __launch_bounds__(256) g_product_kernel(int nNo, const int *__re
       t1s[r + (size_t)3 * row] = w * v;
     }
   }
-}
-// t1s = W_u o t1 again at a face's nodes after ADDBCMUL's PRE changed t1 there
-template <class T>
-__global__ void face_rescale_kernel(int n, const int *__restrict__ glob, const T *__restrict__ W, int wdof,
-                                    const T *__restrict__ t1, T *__restrict__ t1s) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
-  const int g = glob[a];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) t1s[c + (size_t)3 * g] = W[c + (size_t)wdof * g] * t1[c + (size_t)3 * g];
 }
 template <class T, int U = kNarrowUnroll>
 __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restrict__ rowptr,
@@ -465,6 +457,58 @@ __global__ void face_axpy_kernel(int n, int xdof, int fdof, const int *__restric
   const double s = coef * *S;
   for (int i = 0; i < nc; ++i) Y[i + (size_t)xdof * glob[a]] += (T)(valM[i + (size_t)fdof * a] * s);
 }
+// A coupled face on the device, with its PRE coefficient -res/(1 + res nS)
+// (memLS.f90:1096-1118 with BCPRE's nS).
+struct FaceDev {
+  const int *glob;
+  const double *valM;
+  double coef;
+  int nNo, dof;
+};
+// ADDBCMUL's PRE on the Schur CG's G product t1 (3, N), for every coupled
+// face, in one 1024-thread workgroup: all face dots first (face_dot_kernel's
+// sums, in its order), then t1 += coef S valM face by face (face_axpy_kernel),
+// then t1s = W_u o t1 at the face nodes (the operand of Gt).
+template <class T>
+__global__ void __launch_bounds__(1024) face_pre_kernel(int nf, const FaceDev *__restrict__ fd, T *__restrict__ t1,
+                                                        T *__restrict__ t1s, const T *__restrict__ W, int wdof,
+                                                        double *__restrict__ fsum, const double *__restrict__ done) {
+  if (done && *done != 0.0) return;
+  __shared__ double sh[16];
+  for (int f = 0; f < nf; ++f) {
+    const FaceDev F = fd[f];
+    const int nc = F.dof < 3 ? F.dof : 3;
+    double s = 0.0;
+    for (int a = threadIdx.x; a < F.nNo; a += 1024)
+      for (int i = 0; i < nc; ++i) s += F.valM[i + (size_t)F.dof * a] * (double)t1[i + (size_t)3 * F.glob[a]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < 16; ++w) t += sh[w];
+      fsum[f] = t;
+    }
+    __syncthreads();
+  }
+  for (int f = 0; f < nf; ++f) {  // faces in turn: they may share nodes
+    const FaceDev F = fd[f];
+    const int nc = F.dof < 3 ? F.dof : 3;
+    const double sc = F.coef * fsum[f];
+    for (int a = threadIdx.x; a < F.nNo; a += 1024)
+      for (int i = 0; i < nc; ++i) t1[i + (size_t)3 * F.glob[a]] += (T)(F.valM[i + (size_t)F.dof * a] * sc);
+    __syncthreads();
+  }
+  for (int f = 0; f < nf; ++f) {
+    const FaceDev F = fd[f];
+    for (int a = threadIdx.x; a < F.nNo; a += 1024) {
+      const size_t g = F.glob[a];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) t1s[c + 3 * g] = W[c + (size_t)wdof * g] * t1[c + 3 * g];
+    }
+  }
+}
 // DGKS decision: gate = ||w'||^2 < 0.5 ||w||^2
 __global__ void dgks_kernel(const double *__restrict__ n0, const double *__restrict__ n1, int *__restrict__ gate) {
   *gate = (*n1 < 0.5 * *n0) ? 1 : 0;
@@ -494,7 +538,9 @@ __global__ void gt_kernel(size_t nnz, const T *__restrict__ V, const int *__rest
 
 // ----- CG (device-resident iterations with a convergence flag) -----
 // scal layout: 0 err, 1 errO, 2 eps2, 3 den, 4 (unused), 5 done, 6 itr, 7 mItr, 8 suc
-__global__ void cg_check_kernel(double *__restrict__ scal) {
+// Top-of-iteration test of CGRAD (memLS.f90:413-427): stop at mItr or when
+// err < eps^2; otherwise errO = err and count the iteration.
+__device__ inline void check_next(double *scal) {
   if (scal[5] != 0.0) return;
   if (scal[6] >= scal[7]) {
     scal[5] = 1.0;
@@ -508,6 +554,7 @@ __global__ void cg_check_kernel(double *__restrict__ scal) {
   scal[1] = scal[0];  // errO = err
   scal[6] += 1.0;
 }
+__global__ void cg_check_kernel(double *__restrict__ scal) { check_next(scal); }
 // X += alpha P ; R -= alpha SP ; partial of R.R
 template <class T>
 __global__ void __launch_bounds__(256) cg_update_kernel(size_t L, const double *__restrict__ scal,
@@ -532,7 +579,9 @@ __global__ void __launch_bounds__(256) cg_update_kernel(size_t L, const double *
   s = block_sum(s, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
-// err = NORMS(R)**2 from the per-workgroup partials of R.R (fixed order)
+// err = NORMS(R)**2 from the per-workgroup partials of R.R (fixed order);
+// then beta = err/errO for the direction update, and the convergence test
+// of the next iteration (cg_check_kernel's, with the same err).
 template <class T>
 __global__ void __launch_bounds__(256) cg_err_kernel(int nblk, const T *__restrict__ partial,
                                                      double *__restrict__ scal) {
@@ -544,15 +593,21 @@ __global__ void __launch_bounds__(256) cg_err_kernel(int nblk, const T *__restri
   if (threadIdx.x == 0) {
     const double n = sqrt((double)s);
     scal[0] = n * n;
+    scal[4] = scal[0] / scal[1];
+    check_next(scal);
   }
 }
-// P = R + err/errO P
+// P = R + beta P, and optionally Ps = W(off + wdof i) P (the operator's
+// scaled copy of the new direction)
 template <class T>
-__global__ void cg_p_kernel(size_t L, const double *__restrict__ scal, const T *__restrict__ R, T *__restrict__ P) {
+__global__ void cg_p_kernel(size_t L, const double *__restrict__ scal, const T *__restrict__ R, T *__restrict__ P,
+                            const T *__restrict__ W, int wdof, int off, T *__restrict__ Ps) {
   if (scal[5] != 0.0) return;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= L) return;
-  P[e] = R[e] + (T)(scal[0] / scal[1]) * P[e];
+  const T p = R[e] + (T)scal[4] * P[e];
+  P[e] = p;
+  if (Ps) Ps[e] = W[off + (size_t)wdof * e] * p;
 }
 }  // namespace
 
@@ -582,6 +637,9 @@ struct Ctx {
   crimson::DBuf<T> W, fix;
   crimson::DBuf<double> partial;  // per-workgroup partial sums (viewed as T for FP32 accumulation)
   crimson::DBuf<double> dsc, hcol;
+  crimson::DBuf<double> fsum;  // one face dot per face (ADDBCMUL)
+  crimson::DBuf<FaceDev> fdev;  // coupled faces with their PRE coefficients (Schur CG)
+  std::vector<FaceDev> hfdev;
   crimson::DBuf<int> gate;
   std::vector<double> hbuf;
   bool coupled = false;
@@ -599,6 +657,7 @@ struct Ctx {
     CR_TRY(partial.alloc((size_t)nblk_max * 256));
     CR_TRY(dsc.alloc(64));
     CR_TRY(hcol.alloc(4 * 256));
+    CR_TRY(fsum.alloc(std::max<size_t>(1, l->face.size())));
     CR_TRY(gate.alloc(4));
     return CRIMSON_OK;
   }
@@ -711,14 +770,14 @@ int addbcmul(Ctx<T> &c, bool pre, int xdof, const T *X, T *Y) {
   for (size_t f = 0; f < faces.size(); ++f) {
     const CrFace &fa = faces[f];
     if (!fa.coupled) continue;
-    face_dot_kernel<<<1, 1024, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p, fa.d_valM.p, X, c.dsc.p + 48 + f % 16);
+    face_dot_kernel<<<1, 1024, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p, fa.d_valM.p, X, c.fsum.p + f);
   }
   for (size_t f = 0; f < faces.size(); ++f) {
     const CrFace &fa = faces[f];
     if (!fa.coupled) continue;
     const double coef = pre ? -fa.res / (1.0 + fa.res * fa.nS) : fa.res;
     face_axpy_kernel<<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p,
-                                                                        fa.d_valM.p, c.dsc.p + 48 + f % 16, coef, Y);
+                                                                        fa.d_valM.p, c.fsum.p + f, coef, Y);
   }
   CR_HIP(hipGetLastError());
   return CRIMSON_OK;
@@ -841,12 +900,22 @@ int gmres_dev(Ctx<T> &c, const MatView<T> &A, int nd, SubLs &ls, const T *R, T *
   return CRIMSON_OK;
 }
 
+// The operator's column-scaled copy of the CG direction, Ps(i) = W(off +
+// wdof i) P(i), kept up to date by the direction update (Ps null: none).
+template <class T>
+struct Prescale {
+  const T *W = nullptr;
+  int wdof = 0, off = 0;
+  T *Ps = nullptr;
+};
+
 // Generic CG driver on device vectors (memLS CGRAD / CGRADV / CGRADS).
 // op(P, SP) computes SP for the iteration; R is overwritten with X.
 template <class T, class Op>
-int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool schur) {
+int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool schur,
+           Prescale<T> ps = Prescale<T>()) {
   const double t0 = now_s();
-  double *scal = c.dsc.p;  // 0 err 1 errO 2 eps2 3 den 5 done 6 itr 7 mItr 8 suc
+  double *scal = c.dsc.p;  // 0 err 1 errO 2 eps2 3 den 4 beta 5 done 6 itr 7 mItr 8 suc
   ls.suc = 0;
   int rc = CRIMSON_OK;
   const double errO0 = host_norm(c, L, R, &rc);
@@ -858,12 +927,15 @@ int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool 
   CR_HIP(hipMemcpyAsync(scal, hs, sizeof(hs), hipMemcpyHostToDevice, c.s));
   CR_HIP(hipMemsetAsync(X, 0, L * sizeof(T), c.s));
   CR_HIP(hipMemcpyAsync(P, R, L * sizeof(T), hipMemcpyDeviceToDevice, c.s));
+  if (ps.Ps)
+    scale_p_kernel<T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>((int)L, ps.W, ps.wdof, ps.off, P, ps.Ps, nullptr);
   const int nblk = (int)((L + kChunk - 1) / kChunk);
   const int batch = 8;
   double back[9];
+  // the first iteration's test; cg_err_kernel makes each later one
+  cg_check_kernel<<<1, 1, 0, c.s>>>(scal);
   for (;;) {
     for (int b = 0; b < batch; ++b) {
-      cg_check_kernel<<<1, 1, 0, c.s>>>(scal);
       CR_TRY(op(P, SP, scal));
       // den = P . SP
       T *pt = reinterpret_cast<T *>(c.partial.p);
@@ -871,7 +943,7 @@ int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool 
       reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, scal + 3, nullptr);
       cg_update_kernel<T><<<nblk, 256, 0, c.s>>>(L, scal, X, R, P, SP, pt);
       cg_err_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, scal);
-      cg_p_kernel<T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, scal, R, P);
+      cg_p_kernel<T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, scal, R, P, ps.W, ps.wdof, ps.off, ps.Ps);
     }
     CR_HIP(hipGetLastError());
     CR_HIP(hipMemcpyAsync(back, scal, sizeof(back), hipMemcpyDeviceToHost, c.s));
@@ -969,23 +1041,34 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
     for (int a = 0; a < fa.nNo; ++a)
       for (int i = 0; i < nc; ++i) fa.nS = fa.nS + vm[i + (size_t)fa.dof * a] * vm[i + (size_t)fa.dof * a];
   }
-  // Schur operator of CGRAD: SP = L P - Gt (G P + PRE)
+  // the coupled faces with their PRE coefficients, for face_pre_kernel
+  c.hfdev.clear();
+  for (auto &fa : l->face)
+    if (fa.coupled)
+      c.hfdev.push_back(FaceDev{fa.d_glob.p, fa.d_valM.p, -fa.res / (1.0 + fa.res * fa.nS), fa.nNo, fa.dof});
+  const int nfc = (int)c.hfdev.size();
+  if (nfc > 0) {
+    CR_TRY(c.fdev.alloc(c.hfdev.size()));
+    CR_HIP(hipMemcpyAsync(c.fdev.p, c.hfdev.data(), c.hfdev.size() * sizeof(FaceDev), hipMemcpyHostToDevice, c.s));
+    CR_HIP(hipStreamSynchronize(c.s));
+  }
+  // Schur operator of CGRAD: SP = L P - Gt (G P + PRE), on P and its
+  // scaled copy Ps = W_p o P in t2 (kept by the CG's direction update)
   auto schur_op = [&](const T *Pv, T *SPv, const double *scal) -> int {
-    const unsigned gr = crimson::grid_for((size_t)N * kNarrowLanes, 256);
     const double *done = scal + 5;
-    scale_p_kernel<T><<<crimson::grid_for(N, 256), 256, 0, c.s>>>(N, c.W.p, 4, mG.c0, Pv, t2.p, done);  // Ps
+    const unsigned gr = crimson::grid_for((size_t)N * kNarrowLanes, 256);
     g_product_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2.p, t1.p, ws.tmp.p, done);
-    if (c.coupled) {
-      CR_TRY(addbcmul(c, true, 3, t1.p, t1.p));
-      for (auto &fa : l->face)
-        if (fa.coupled)
-          face_rescale_kernel<T><<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, fa.d_glob.p, c.W.p, 4,
-                                                                                 t1.p, ws.tmp.p);
-    }
+    if (nfc > 0)
+      face_pre_kernel<T><<<1, 1024, 0, c.s>>>(nfc, c.fdev.p, t1.p, ws.tmp.p, c.W.p, 4, c.fsum.p, done);
     schur_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2.p, ws.tmp.p, SPv, done);
     CR_HIP(hipGetLastError());
     return CRIMSON_OK;
   };
+  Prescale<T> pre_p;
+  pre_p.W = c.W.p;
+  pre_p.wdof = 4;
+  pre_p.off = mG.c0;
+  pre_p.Ps = t2.p;
   std::vector<double> A((size_t)nB * nB, 0.0), Bv(nB, 0.0), xB(nB, 0.0);
   std::vector<double> dots(2 * nB + 4);
   int i, iBB = 0;
@@ -998,7 +1081,7 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
     T *MPb = MP.p + (size_t)N * (iB - 1), *MPbb = MP.p + (size_t)N * (iBB - 1);
     CR_TRY(gmres_dev(c, mK, 3, ls->GM, Rm.p, Ui, ws, true));
     CR_TRY(spmv(c, mD, 1, 3, Ui, Pi, -1.0, Rc.p, 1.0));  // P = Rc - D U
-    CR_TRY(cg_dev(c, (size_t)N, ls->CG, Pi, schur_op, t3.p, t4.p, MPb, true));
+    CR_TRY(cg_dev(c, (size_t)N, ls->CG, Pi, schur_op, t3.p, t4.p, MPb, true, pre_p));
     CR_TRY(spmv(c, mG, 3, 1, Pi, MUb));
     CR_TRY(axpby(c, mn, 1.0, Rm.p, 0.0, MUbb));
     CR_TRY(axpby(c, mn, -1.0, MUb, 1.0, MUbb));  // MU(iBB) = Rm - MU(iB)
