@@ -41,7 +41,7 @@ EXPORTS = (
     "crimson_lhs_spmv_dev", "crimson_itr_predict_dev", "crimson_itr_yalpha_dev", "crimson_itr_correct_dev",
     "crimson_itr_bc_dev", "crimson_elmpvsq_dev", "crimson_form_sync", "crimson_lhs_set_precision",
     "crimson_form_vjp_dev", "crimson_form_set_instances", "crimson_form_graph_create", "crimson_graph_launch",
-    "crimson_graph_free",
+    "crimson_graph_free", "crimson_form_set_boundary",
 )
 
 
@@ -111,6 +111,7 @@ def lib():
         L.crimson_itr_bc_dev.argtypes = [vp, vp, vp, C.c_int, vp]
         L.crimson_elmpvsq_dev.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp, vp, vp]
         L.crimson_form_sync.argtypes = [vp]
+        L.crimson_form_set_boundary.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_int]
         _lib = L
     return _lib
 
@@ -239,6 +240,23 @@ class ElmGMRInterior:
         inst = np.ascontiguousarray(inst, dtype=np.uint16)
         mu = np.ascontiguousarray(mu, dtype=np.float64)
         check(lib().crimson_form_set_instances(self.h, _p(inst), int(mu.size), _p(mu)), "crimson_form_set_instances")
+
+    def set_boundary(self, ienb, iBCB, BCB, nqb=3, influx=0.5, permit=None):
+        """Boundary elements (the AsBMFG loop): ienb(nelb,4) 1-based with the
+        face nodes first, iBCB(nelb,2), BCB(nelb,3,ndBCB); permit(0..nsrf)
+        flow-permitted flags by srfID or None.  Every later formation adds
+        their residual rows; pass nelb = 0 rows to remove them."""
+        ienb = np.asfortranarray(ienb, dtype=np.int32)
+        iBCB = np.asfortranarray(iBCB, dtype=np.int32)
+        BCB = np.asfortranarray(BCB, dtype=np.float64)
+        if BCB.ndim != 3 or BCB.shape[:2] != (ienb.shape[0], 3) or iBCB.shape != (ienb.shape[0], 2):
+            raise ValueError("ienb(nelb,4), iBCB(nelb,2) and BCB(nelb,3,ndBCB) disagree")
+        perm = None if permit is None else np.ascontiguousarray(permit, dtype=np.int32)
+        self._bnd = (ienb, iBCB, BCB, perm)  # kept alive with the handle
+        check(lib().crimson_form_set_boundary(self.h, int(ienb.shape[0]), _p(ienb), _p(iBCB), _p(BCB),
+                                              int(BCB.shape[2]), int(nqb), float(influx), _p(perm),
+                                              -1 if perm is None else int(perm.size - 1)),
+              "crimson_form_set_boundary")
 
     def graph_to_lhs(self, y, ac, x, res4, lhs_handle):
         """Capture to_lhs on these device arrays into a HIP graph; returns a

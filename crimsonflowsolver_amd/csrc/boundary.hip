@@ -1,0 +1,297 @@
+// Boundary elements of ElmGMR's boundary loop (elmgmr.f90:325-460) for P1
+// tets: AsBMFG (asbmfg.f90:19-80) gathers each boundary element, e3b
+// (e3b.f90:103-560) integrates over its face with e3bvar's quantities
+// (e3bvar.f90), and local(..., 'scatter ') adds rl into res.
+//
+// Supported: rigid walls (ideformwall 0), so the boundary xKebe stays zero
+// and the boundary loop adds nothing to lhsK/lhsP; iBCB code bits 0 (mass
+// flux), 1 (pressure), 2 (traction) and 4 (vessel wall: no traction term);
+// no heart model, no Lagrange multipliers; iconvflow 2; Newtonian mu.
+// Boundary rule: symtri with 1 or 3 points (symtri.c:56-60), weights doubled
+// (genint.f90:86); on the face N = (r, s, t) of nodes 1-3 and N4 = 0.
+//
+//   bnd_elem   one thread per boundary element: the four node records of
+//              the current formation (x, u, p), the exact P1 gradient of u,
+//              the face normal and Jacobian, and rl of the three face nodes
+//              (12 doubles) into a boundary record
+//   bnd_node   one thread per boundary node: res(node) += rl of its
+//              incidences in boundary-element order, after the interior rows
+//              are written (the reference adds the boundary loop after the
+//              interior loop)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+#include "form_common.hpp"
+
+namespace {
+
+constexpr double kTriA = 0.3333333333333330000000000000000000;  // symtri.c:20
+constexpr double kTriB = 0.666666666666667;                     // symtri.c:21
+constexpr double kTriC = 0.166666666666667;                     // symtri.c:22
+
+__device__ inline double tri_pt(int nqb, int q, int a) {
+  if (nqb == 1) return kTriA;
+  return q == a ? kTriB : kTriC;
+}
+
+__global__ void __launch_bounds__(256) bnd_elem(int nelb, const int *__restrict__ ienb,
+                                                const int *__restrict__ bcode, const double *__restrict__ bcb,
+                                                const int *__restrict__ permit, int nsrf, int nqb, double influx,
+                                                double rho, double mu, const double *__restrict__ node,
+                                                double *__restrict__ brec) {
+  const int e = xcd_block() * blockDim.x + threadIdx.x;
+  if (e >= nelb) return;
+  double xl[4][3], yl[4][4];  // yl(a, (p, u1, u2, u3)) as localy orders it
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const double *r = node + (size_t)ienb[a * (size_t)nelb + e] * kNRec;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) xl[a][i] = r[N_X + i];
+    yl[a][0] = r[N_P];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) yl[a][i + 1] = r[N_U + i];
+  }
+  // e3bvar.f90: dxdxib from shglb = dN/dxi / 2, its inverse by cofactors,
+  // the global gradients gy[j][c] = d(yl c)/dx_j
+  constexpr double sg[3][4] = {{0.5, 0.0, 0.0, -0.5}, {0.0, 0.5, 0.0, -0.5}, {0.0, 0.0, 0.5, -0.5}};
+  double J[3][3], D[3][3], gl[3][4], gy[3][4];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) s = s + xl[a][i] * sg[j][a];
+      J[i][j] = s;
+    }
+  D[0][0] = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+  D[0][1] = J[2][1] * J[0][2] - J[0][1] * J[2][2];
+  D[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  const double tdet = 1.0 / (D[0][0] * J[0][0] + D[0][1] * J[1][0] + D[0][2] * J[2][0]);
+  D[0][0] *= tdet;
+  D[0][1] *= tdet;
+  D[0][2] *= tdet;
+  D[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * tdet;
+  D[1][1] = (J[0][0] * J[2][2] - J[2][0] * J[0][2]) * tdet;
+  D[1][2] = (J[1][0] * J[0][2] - J[0][0] * J[1][2]) * tdet;
+  D[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * tdet;
+  D[2][1] = (J[2][0] * J[0][1] - J[0][0] * J[2][1]) * tdet;
+  D[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * tdet;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 1; c < 4; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) s = s + sg[k][a] * yl[a][c];
+      gl[k][c] = s;
+    }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int c = 1; c < 4; ++c) gy[j][c] = D[0][j] * gl[0][c] + D[1][j] * gl[1][c] + D[2][j] * gl[2][c];
+  // normal from nodes 1-3 and the face Jacobian (e3bvar.f90, lcsyst 1)
+  double v1[3], v2[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    v1[i] = xl[1][i] - xl[0][i];
+    v2[i] = xl[2][i] - xl[0][i];
+  }
+  const double t1 = v1[1] * v2[2] - v2[1] * v1[2];
+  const double t2 = v2[0] * v1[2] - v1[0] * v2[2];
+  const double t3 = v1[0] * v2[1] - v2[0] * v1[1];
+  const double temp = 1.0 / sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+  const double bn[3] = {t1 * temp, t2 * temp, t3 * temp};
+  const int code = bcode[e], srf = bcode[(size_t)nelb + e];
+  const bool flow_ok = !permit || srf < 0 || srf > nsrf || permit[srf] != 0;
+  // the viscous traction of the element's constant gradient (e3bvar.f90)
+  double tau0[3];
+  tau0[0] = bn[0] * 2.0 * mu * gy[0][1] + bn[1] * (mu * (gy[1][1] + gy[0][2])) + bn[2] * (mu * (gy[2][1] + gy[0][3]));
+  tau0[1] = bn[0] * (mu * (gy[1][1] + gy[0][2])) + bn[1] * 2.0 * mu * gy[1][2] + bn[2] * (mu * (gy[2][2] + gy[1][3]));
+  tau0[2] = bn[0] * (mu * (gy[2][1] + gy[0][3])) + bn[1] * (mu * (gy[2][2] + gy[1][3])) + bn[2] * 2.0 * mu * gy[2][3];
+  const double tn = bn[0] * tau0[0] + bn[1] * tau0[1] + bn[2] * tau0[2];
+  double rl[3][4];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rl[a][k] = 0.0;
+  const double qw = nqb == 1 ? 2.0 * 1.0 : 2.0 * kTriA;
+  for (int q = 0; q < nqb; ++q) {
+    const double N[3] = {tri_pt(nqb, q, 0), tri_pt(nqb, q, 1), tri_pt(nqb, q, 2)};
+    const double WdetJb = qw / (4.0 * temp);
+    double pres = 0.0, u[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      pres = pres + N[a] * yl[a][0];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) u[i] = u[i] + N[a] * yl[a][i + 1];
+    }
+    double unm = bn[0] * u[0] + bn[1] * u[1] + bn[2] * u[2];
+    pres = pres - tn;
+    double tau[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) tau[i] = tau0[i] - bn[i] * tn;
+    // e3b.f90: influx stabilisation, then the prescribed fluxes
+    double unm_copy = 0.0;
+    if ((code & 2) && unm < 0.0) unm_copy = unm;
+    if (code & 1) {
+      unm = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) unm = unm + N[a] * bcb[e + (size_t)nelb * (a + 3 * 0)];
+    }
+    if ((code & 2) && flow_ok) {
+      pres = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) pres = pres + N[a] * bcb[e + (size_t)nelb * (a + 3 * 1)];
+    }
+    if (code & 4) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        tau[i] = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) tau[i] = tau[i] + N[a] * bcb[e + (size_t)nelb * (a + 3 * (2 + i))];
+      }
+    }
+    if (code & 16) {
+      pres = 0.0;
+      tau[0] = tau[1] = tau[2] = 0.0;
+    }
+    double rNa[4];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rNa[i] = -tau[i] + bn[i] * pres;
+    rNa[3] = unm;
+    const double rou_copy = rho * unm_copy;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rNa[i] = rNa[i] - influx * rou_copy * u[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rNa[k] = rNa[k] * WdetJb;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) rl[a][k] = rl[a][k] - N[a] * rNa[k];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) brec[(size_t)e * 12 + 4 * a + k] = rl[a][k];
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) bnd_node(int nbnode, int nshg, const int *__restrict__ bnode,
+                                                const int *__restrict__ binc_ptr, const int *__restrict__ binc,
+                                                const double *__restrict__ brec, double *__restrict__ res) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbnode) return;
+  const int i = bnode[b];
+  double r[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r[k] = MODE == 0 ? res[i + (size_t)nshg * k] : res[4 * (size_t)i + k];
+  for (int p = binc_ptr[b]; p < binc_ptr[b + 1]; ++p) {
+    const int ea = binc[p];
+    const double *rl = brec + (size_t)(ea >> 2) * 12 + 4 * (ea & 3);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = r[k] + rl[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (MODE == 0)
+      res[i + (size_t)nshg * k] = r[k];
+    else
+      res[4 * (size_t)i + k] = r[k];
+  }
+}
+
+}  // namespace
+
+namespace crimson {
+int form_boundary_pass(crimson_form *f, double *d_res, int mode) {
+  if (f->nelb == 0) return CRIMSON_OK;
+  hipStream_t s = f->stream;
+  bnd_elem<<<grid_for(f->nelb, 256), 256, 0, s>>>(f->nelb, f->d_ienb.p, f->d_bcode.p, f->d_bcb.p,
+                                                  f->nsrf >= 0 ? f->d_permit.p : nullptr, f->nsrf, f->nqb,
+                                                  f->influx, f->prm.rho, f->prm.mu, f->d_node.p, f->d_brec.p);
+  if (mode == 0)
+    bnd_node<0><<<grid_for(f->nbnode, 256), 256, 0, s>>>(f->nbnode, f->nshg, f->d_bnode.p, f->d_binc_ptr.p,
+                                                         f->d_binc.p, f->d_brec.p, d_res);
+  else
+    bnd_node<1><<<grid_for(f->nbnode, 256), 256, 0, s>>>(f->nbnode, f->nshg, f->d_bnode.p, f->d_binc_ptr.p,
+                                                         f->d_binc.p, f->d_brec.p, d_res);
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+}  // namespace crimson
+
+extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *ienb, const int *iBCB,
+                                         const double *BCB, int ndBCB, int nqb, double influx,
+                                         const int *permit, int nsrf) {
+  if (!f) return crimson::set_error("crimson_form_set_boundary: null handle"), CRIMSON_EARG;
+  if (nelb == 0) {
+    f->nelb = f->nbnode = 0;
+    return CRIMSON_OK;
+  }
+  if (nelb < 0 || !ienb || !iBCB || !BCB || ndBCB < 2 || (nqb != 1 && nqb != 3) || (permit && nsrf < 0))
+    return crimson::set_error("crimson_form_set_boundary: bad arguments"), CRIMSON_EARG;
+  if (f->ninst > 0)
+    return crimson::set_error("crimson_form_set_boundary: boundary elements with ensemble instances are not "
+                              "supported"),
+           CRIMSON_EUNSUPPORTED;
+  std::vector<int> ib((size_t)4 * nelb), code((size_t)2 * nelb);
+  std::vector<double> bcb((size_t)nelb * 15, 0.0);
+  for (int e = 0; e < nelb; ++e) {
+    for (int a = 0; a < 4; ++a) {
+      const int g = ienb[e + (size_t)nelb * a] - 1;
+      if (g < 0 || g >= f->nshg || g >= f->numnp)
+        return crimson::set_error("crimson_form_set_boundary: ienb(%d,%d) out of range", e + 1, a + 1), CRIMSON_EARG;
+      ib[(size_t)a * nelb + e] = g;
+    }
+    const int c = iBCB[e];
+    if ((c & 4) && ndBCB < 5)
+      return crimson::set_error("crimson_form_set_boundary: traction bit set with ndBCB < 5"), CRIMSON_EARG;
+    code[e] = c;
+    code[(size_t)nelb + e] = iBCB[e + (size_t)nelb];
+    for (int a = 0; a < 3; ++a)
+      for (int k = 0; k < std::min(ndBCB, 5); ++k)
+        bcb[e + (size_t)nelb * (a + 3 * k)] = BCB[e + (size_t)nelb * (a + 3 * k)];
+  }
+  // boundary nodes and their incidences (element, face node) in element order
+  std::vector<int> cnt(f->nshg, 0);
+  for (int e = 0; e < nelb; ++e)
+    for (int a = 0; a < 3; ++a) cnt[ib[(size_t)a * nelb + e]]++;
+  std::vector<int> bnode, bptr(1, 0), pos(f->nshg, -1);
+  for (int i = 0; i < f->nshg; ++i)
+    if (cnt[i]) {
+      pos[i] = (int)bnode.size();
+      bnode.push_back(i);
+      bptr.push_back(bptr.back() + cnt[i]);
+    }
+  std::vector<int> binc(bptr.back()), fill(bptr.begin(), bptr.end() - 1);
+  for (int e = 0; e < nelb; ++e)
+    for (int a = 0; a < 3; ++a) {
+      const int b = pos[ib[(size_t)a * nelb + e]];
+      binc[fill[b]++] = e << 2 | a;
+    }
+  if (nelb > (1 << 29)) return crimson::set_error("crimson_form_set_boundary: too many elements"), CRIMSON_EARG;
+  hipStream_t s = f->stream;
+  int rc;
+  if ((rc = f->d_ienb.upload(ib.data(), ib.size(), s)) || (rc = f->d_bcode.upload(code.data(), code.size(), s)) ||
+      (rc = f->d_bcb.upload(bcb.data(), bcb.size(), s)) || (rc = f->d_brec.alloc((size_t)12 * nelb)) ||
+      (rc = f->d_bnode.upload(bnode.data(), bnode.size(), s)) ||
+      (rc = f->d_binc_ptr.upload(bptr.data(), bptr.size(), s)) || (rc = f->d_binc.upload(binc.data(), binc.size(), s)))
+    return rc;
+  if (permit) {
+    if ((rc = f->d_permit.upload(permit, (size_t)nsrf + 1, s))) return rc;
+  }
+  CR_HIP(hipStreamSynchronize(s));
+  f->nelb = nelb;
+  f->nbnode = (int)bnode.size();
+  f->nqb = nqb;
+  f->influx = influx;
+  f->nsrf = permit ? nsrf : -1;
+  return CRIMSON_OK;
+}

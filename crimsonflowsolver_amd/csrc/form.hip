@@ -1160,6 +1160,7 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
                                 f->d_rec.p, d_res, o);
   }
   CR_HIP(hipGetLastError());
+  CR_TRY(crimson::form_boundary_pass(f, d_res, mode));  // the boundary loop after the interior (elmgmr.f90:325)
   if (timed) CR_HIP(hipEventRecord(f->ev[3], s));
   if (!timed) return CRIMSON_OK;  // enqueued only (graph capture)
   CR_HIP(hipStreamSynchronize(s));
@@ -1434,6 +1435,10 @@ extern "C" int crimson_form_interior_to_lhs(crimson_form *f, const double *d_y, 
 }
 
 extern "C" int crimson_form_set_instances(crimson_form *f, const unsigned short *inst, int ninst, const double *mu) {
+  if (f && f->nelb > 0 && ninst > 0)
+    return crimson::set_error("crimson_form_set_instances: ensemble instances with boundary elements are not "
+                              "supported"),
+           CRIMSON_EUNSUPPORTED;
   if (!f || ninst < 0 || (ninst > 0 && (!inst || !mu))) {
     crimson::set_error("crimson_form_set_instances: invalid arguments");
     return CRIMSON_EARG;

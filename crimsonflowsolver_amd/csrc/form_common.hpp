@@ -160,9 +160,22 @@ struct crimson_form {
   crimson::DBuf<double> d_mu_inst;
   int ninst = 0;
   Ens ens() const { return Ens{ninst ? d_inst.p : nullptr, ninst ? d_mu_inst.p : nullptr}; }
+  // boundary elements (crimson_form_set_boundary)
+  int nelb = 0, nbnode = 0, nqb = 3, nsrf = -1;
+  double influx = 0.5;
+  crimson::DBuf<int> d_ienb;      // (4, nelb) 0-based, face nodes first
+  crimson::DBuf<int> d_bcode;     // (2, nelb): iBCB code bits, srfID
+  crimson::DBuf<double> d_bcb;    // BCB(nelb, 3, 5): mass flux, pressure, traction (zero past ndBCB)
+  crimson::DBuf<int> d_permit;    // flow-permitted flag by srfID, 0..nsrf
+  crimson::DBuf<double> d_brec;   // (12, nelb): rl of the three face nodes
+  crimson::DBuf<int> d_bnode, d_binc_ptr, d_binc;  // boundary nodes; their incidences e << 2 | a
 };
 
 namespace crimson {
 int form_flux_pass(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, double *d_q,
                    double *d_invm);
+// Boundary elements' residual rows added to res (mode 0: res(nshg,4); mode
+// 1: Res4(4,nshg)) from the node records of the current formation; a no-op
+// without boundary elements.  Enqueued on the handle's stream.
+int form_boundary_pass(crimson_form *f, double *d_res, int mode);
 }  // namespace crimson

@@ -112,6 +112,27 @@ int crimson_form_bc3res_dev(crimson_form *f, double *d_res, int res4_layout);
  * out[2] assembly, out[3] total. */
 int crimson_form_timing(const crimson_form *f, double *out4);
 
+/* Boundary elements: replaces the boundary-element loop of ElmGMR
+ * (elmgmr.f90:325-460: AsBMFG asbmfg.f90:1-80 -> e3b e3b.f90:1-560 -> e3bvar
+ * e3bvar.f90).  After this call every formation of the handle adds the
+ * boundary elements' residual rows to res / Res4 after the interior rows, as
+ * the reference's boundary loop does; the matrix is unchanged (rigid walls:
+ * the boundary xKebe is zero).  ienb(nelb,4) 1-based with the three face
+ * nodes first (the mienb blocks concatenated), iBCB(nelb,2) (code bits: 1
+ * mass flux, 2 pressure, 4 traction, 16 vessel wall; srfID), BCB(nelb,3,
+ * ndBCB) (mass flux, pressure, traction x3; ndBCB >= 5 when bit 4 is used),
+ * nqb = 1 or 3 boundary points ("Quadrature Rule on Boundary" 1 or 2), influx
+ * = "Influx Coefficient", permit(0..nsrf) the flow-permitted flag of each
+ * srfID (callCPPDiscoverWhetherFlowPermittedAcrossSurface, e3b.f90; NULL: all
+ * permitted).  nelb = 0 removes the boundary.  Deformable walls, the heart
+ * model, Lagrange multipliers and the surface flux integrals (flxID) are not
+ * formed.  CRIMSON_EUNSUPPORTED with ensemble instances; the VJP
+ * (crimson_form_vjp_dev) rejects a handle with boundary elements. */
+int crimson_form_set_boundary(crimson_form *f, int nelb, const int *ienb,
+                              const int *iBCB, const double *BCB, int ndBCB,
+                              int nqb, double influx, const int *permit,
+                              int nsrf);
+
 /* Ensembles (no counterpart in the reference, which runs one case per job):
  * the handle's mesh is the union of several instances' meshes (disjoint
  * node ranges), inst(nel) gives each element's 0-based instance and mu(ninst)

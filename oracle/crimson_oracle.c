@@ -1609,3 +1609,159 @@ double cfo_flops_per_element(void) {
    */
   return 1228.0 + 46.0 + 5540.0 + 224.0;
 }
+
+/* ------------------------------------------------------------ boundary
+ * Boundary-element residual of ElmGMR's boundary loop (elmgmr.f90:325-460):
+ * AsBMFG (asbmfg.f90:19-80) gathers each boundary element (localy, localx),
+ * e3b (e3b.f90:103-560) integrates over its face with e3bvar's quantities
+ * (e3bvar.f90), and local(res, rl, ienb, nflow, 'scatter ') adds rl.
+ * P1 tets (lcsyst 1, nshlb 3, lnode = 1,2,3: the face nodes come first in
+ * ienb), rigid walls (ideformwall 0, so xKebe stays zero and the boundary
+ * loop adds nothing to lhsK/lhsP), no heart model (iheart 0), no Lagrange
+ * multipliers, iconvflow 2, ires 1, Newtonian rmu = mu.
+ * Boundary rule: symtri (symtri.c:56-60) with 1 or 3 points, weights doubled
+ * (genint.f90:86); N = (r, s, t, 1-r-s-t) at the points (shpTet), so N4 = 0
+ * on the face; shglb = dN/dxi / 2 (genshpb.f90:19-20).
+ */
+static const double TRI_A = 0.3333333333333330000000000000000000; /* symtri.c:20 */
+static const double TRI_B = 0.666666666666667;                    /* symtri.c:21 */
+static const double TRI_C = 0.166666666666667;                    /* symtri.c:22 */
+
+int cfo_boundary_res(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
+                     const double *BCB, int ndBCB, const double *y, const double *x,
+                     double rho, double mu, double influx, int nqb, const int *permit,
+                     int nsrf, int ibksiz, double *res) {
+  double qp[3][3], qw[3];
+  if (nqb == 1) {
+    qp[0][0] = qp[0][1] = qp[0][2] = TRI_A;
+    qw[0] = 2.0 * 1.0;
+  } else if (nqb == 3) {
+    const double r[3][3] = {{TRI_B, TRI_C, TRI_C}, {TRI_C, TRI_B, TRI_C}, {TRI_C, TRI_C, TRI_B}};
+    for (int q = 0; q < 3; ++q) {
+      for (int j = 0; j < 3; ++j) qp[q][j] = r[q][j];
+      qw[q] = 2.0 * TRI_A;
+    }
+  } else {
+    return CFO_EARG;
+  }
+  if (nelb < 0 || ndBCB < 2 || ibksiz < 1) return CFO_EARG;
+  for (int e = 0; e < nelb; ++e)
+    if ((iBCB[e] & 4) && ndBCB < 5) return CFO_EARG;
+  /* shglb for tets: d(r,s,t,1-r-s-t)/d(r,s,t), halved */
+  const double shglb[3][4] = {{0.5, 0.0, 0.0, -0.5}, {0.0, 0.5, 0.0, -0.5}, {0.0, 0.0, 0.5, -0.5}};
+  double *rl = (double *)malloc(sizeof(double) * (size_t)ibksiz * 16);
+  if (!rl) return CFO_EARG;
+  for (int e0 = 0; e0 < nelb; e0 += ibksiz) {
+    const int n = (nelb - e0) < ibksiz ? (nelb - e0) : ibksiz;
+    for (int i = 0; i < n * 16; ++i) rl[i] = 0.0;
+    for (int el = 0; el < n; ++el) {
+      const int e = e0 + el;
+      int g[4];
+      double yl[4][4], xl[4][3]; /* yl(a, (p,u1,u2,u3)) as localy orders it */
+      for (int a = 0; a < 4; ++a) {
+        g[a] = ienb[e + (size_t)nelb * a] - 1;
+        yl[a][0] = y[g[a] + (size_t)nshg * 3];
+        for (int i = 0; i < 3; ++i) yl[a][i + 1] = y[g[a] + (size_t)nshg * i];
+        for (int i = 0; i < 3; ++i) xl[a][i] = x[g[a] + (size_t)numnp * i];
+      }
+      /* e3bvar.f90: dxdxib, its inverse by cofactors, global gradients */
+      double J[3][3] = {{0}}, D[3][3], gl[3][4] = {{0}}, gy[3][4];
+      for (int a = 0; a < 4; ++a)
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) J[i][j] = J[i][j] + xl[a][i] * shglb[j][a];
+      D[0][0] = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+      D[0][1] = J[2][1] * J[0][2] - J[0][1] * J[2][2];
+      D[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+      const double tdet = 1.0 / (D[0][0] * J[0][0] + D[0][1] * J[1][0] + D[0][2] * J[2][0]);
+      D[0][0] = D[0][0] * tdet;
+      D[0][1] = D[0][1] * tdet;
+      D[0][2] = D[0][2] * tdet;
+      D[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * tdet;
+      D[1][1] = (J[0][0] * J[2][2] - J[2][0] * J[0][2]) * tdet;
+      D[1][2] = (J[1][0] * J[0][2] - J[0][0] * J[1][2]) * tdet;
+      D[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * tdet;
+      D[2][1] = (J[2][0] * J[0][1] - J[0][0] * J[2][1]) * tdet;
+      D[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * tdet;
+      for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 3; ++k)
+          for (int c = 0; c < 4; ++c) gl[k][c] = gl[k][c] + shglb[k][a] * yl[a][c];
+      /* gy[j][c] = d(yl c)/dx_j = sum_k dxidxb(k,j) gl_k */
+      for (int j = 0; j < 3; ++j)
+        for (int c = 1; c < 4; ++c) gy[j][c] = D[0][j] * gl[0][c] + D[1][j] * gl[1][c] + D[2][j] * gl[2][c];
+      /* outward normal and the face Jacobian (e3bvar.f90, lcsyst 1) */
+      double v1[3], v2[3];
+      for (int i = 0; i < 3; ++i) {
+        v1[i] = xl[1][i] - xl[0][i];
+        v2[i] = xl[2][i] - xl[0][i];
+      }
+      const double t1 = v1[1] * v2[2] - v2[1] * v1[2];
+      const double t2 = v2[0] * v1[2] - v1[0] * v2[2];
+      const double t3 = v1[0] * v2[1] - v2[0] * v1[1];
+      const double temp = 1.0 / sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+      const double bn[3] = {t1 * temp, t2 * temp, t3 * temp};
+      const int code = iBCB[e], srf = iBCB[e + (size_t)nelb];
+      for (int q = 0; q < nqb; ++q) {
+        const double N[3] = {qp[q][0], qp[q][1], qp[q][2]};
+        const double WdetJb = qw[q] / (4.0 * temp);
+        double pres = 0.0, u[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 3; ++a) {
+          pres = pres + N[a] * yl[a][0];
+          for (int i = 0; i < 3; ++i) u[i] = u[i] + N[a] * yl[a][i + 1];
+        }
+        double unm = bn[0] * u[0] + bn[1] * u[1] + bn[2] * u[2];
+        /* gy[j][c]: g(j+1)yi(:,c+1) */
+        double tau[3];
+        tau[0] = bn[0] * 2.0 * mu * gy[0][1] + bn[1] * (mu * (gy[1][1] + gy[0][2])) +
+                 bn[2] * (mu * (gy[2][1] + gy[0][3]));
+        tau[1] = bn[0] * (mu * (gy[1][1] + gy[0][2])) + bn[1] * 2.0 * mu * gy[1][2] +
+                 bn[2] * (mu * (gy[2][2] + gy[1][3]));
+        tau[2] = bn[0] * (mu * (gy[2][1] + gy[0][3])) + bn[1] * (mu * (gy[2][2] + gy[1][3])) +
+                 bn[2] * 2.0 * mu * gy[2][3];
+        const double tn = bn[0] * tau[0] + bn[1] * tau[1] + bn[2] * tau[2];
+        pres = pres - tn;
+        for (int i = 0; i < 3; ++i) tau[i] = tau[i] - bn[i] * tn;
+        /* e3b.f90: influx stabilisation, then the prescribed fluxes */
+        double unm_copy = 0.0;
+        if ((code & 2) && unm < 0.0) unm_copy = unm;
+        if (code & 1) {
+          unm = 0.0;
+          for (int a = 0; a < 3; ++a) unm = unm + N[a] * BCB[e + (size_t)nelb * (a + 3 * 0)];
+        }
+        if (code & 2) {
+          const int ok = (permit == NULL || srf < 0 || srf > nsrf) ? 1 : permit[srf];
+          if (ok) {
+            pres = 0.0;
+            for (int a = 0; a < 3; ++a) pres = pres + N[a] * BCB[e + (size_t)nelb * (a + 3 * 1)];
+          }
+        }
+        if (code & 4) {
+          for (int i = 0; i < 3; ++i) {
+            tau[i] = 0.0;
+            for (int a = 0; a < 3; ++a) tau[i] = tau[i] + N[a] * BCB[e + (size_t)nelb * (a + 3 * (2 + i))];
+          }
+        }
+        if (code & 16) {
+          pres = 0.0;
+          tau[0] = tau[1] = tau[2] = 0.0;
+        }
+        double rNa[4];
+        for (int i = 0; i < 3; ++i) rNa[i] = -tau[i] + bn[i] * pres;
+        rNa[3] = unm;
+        const double rou_copy = rho * unm_copy;
+        for (int i = 0; i < 3; ++i) rNa[i] = rNa[i] - influx * rou_copy * u[i];
+        for (int k = 0; k < 4; ++k) rNa[k] = rNa[k] * WdetJb;
+        for (int a = 0; a < 3; ++a)
+          for (int k = 0; k < 4; ++k) rl[el + n * (a + 4 * k)] = rl[el + n * (a + 4 * k)] - N[a] * rNa[k];
+      }
+    }
+    /* local(res, rl, ienb, nflow, 'scatter ') (local.f90:68-75) */
+    for (int k = 0; k < 4; ++k)
+      for (int a = 0; a < 4; ++a)
+        for (int el = 0; el < n; ++el) {
+          const int gg = ienb[e0 + el + (size_t)nelb * a] - 1;
+          res[gg + (size_t)nshg * k] = res[gg + (size_t)nshg * k] + rl[el + n * (a + 4 * k)];
+        }
+  }
+  free(rl);
+  return CFO_OK;
+}

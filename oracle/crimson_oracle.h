@@ -141,6 +141,18 @@ int cfo_ls_solve(cfo_lhs *lhs, cfo_ls *ls, int dof, double *Ri, double *Val,
  * out[2]=itr out[3]=suc out[4]=dB out[5]=callD; plus Resm/Resc in out[6..7] */
 void cfo_ls_stats(const cfo_ls *ls, int sub, double *out);
 
+/* Boundary-element residual of ElmGMR's boundary loop (AsBMFG -> e3b ->
+ * e3bvar, elmgmr.f90:325-460): res(nshg,4) += the boundary elements' rl.
+ * ienb(nelb,4) 1-based, face nodes first; iBCB(nelb,2) (code bits, srfID);
+ * BCB(nelb,3,ndBCB) (mass flux, pressure, traction x3); y(nshg,4) =
+ * (u1,u2,u3,p); nqb = 1 or 3 boundary points; permit(0..nsrf) flow-permitted
+ * flags by srfID (NULL: all); influx = the Influx Coefficient (0.5).
+ * P1 tets, rigid walls, no heart model: the boundary tangent is zero. */
+int cfo_boundary_res(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
+                     const double *BCB, int ndBCB, const double *y, const double *x,
+                     double rho, double mu, double influx, int nqb, const int *permit,
+                     int nsrf, int ibksiz, double *res);
+
 /* Flop count of one element formation in the reference formulation
  * (flux pass + e3 with lhs=1 + scatter adds), counted stage by stage from
  * the restatement above (+,-,*,/,sqrt each 1). */

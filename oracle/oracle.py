@@ -70,6 +70,9 @@ def lib():
         L.cfo_time_params.argtypes = [C.c_double] + [C.POINTER(C.c_double)] * 3
         L.cfo_time_params.restype = None
         L.cfo_flops_per_element.restype = C.c_double
+        L.cfo_boundary_res.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, C.c_int, dp, dp,
+                                       C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p,
+                                       C.c_int, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -157,6 +160,24 @@ def bc3res(ibc, res, bc=None):
     bca = None if bc is None else _f(bc)
     _check(lib().cfo_bc3res(res.shape[0], _f(ibc, np.int32), _ptr(bca),
                             0 if bc is None else bc.shape[1], res), "bc3res")
+    return res
+
+
+def boundary_res(ienb, iBCB, BCB, y, x, res, rho=1.06, mu=0.04, influx=0.5, nqb=3, permit=None,
+                 ibksiz=255):
+    """res(nshg,4) plus the boundary elements' residual (AsBMFG/e3b/e3bvar):
+    returns a new array.  ienb(nelb,4) 1-based with the face nodes first,
+    iBCB(nelb,2), BCB(nelb,3,ndBCB), y(nshg,4) = (u1,u2,u3,p), x(numnp,3)."""
+    res = _f(res).copy(order="F")
+    ienb = _f(ienb, np.int32)
+    iBCB = _f(iBCB, np.int32)
+    BCB = _f(BCB)
+    y = _f(y)
+    x = _f(x)
+    perm = None if permit is None else _f(permit, np.int32)
+    nsrf = -1 if permit is None else perm.size - 1
+    _check(lib().cfo_boundary_res(res.shape[0], x.shape[0], ienb.shape[0], ienb, iBCB, BCB, BCB.shape[2],
+                                  y, x, rho, mu, influx, nqb, _ptr(perm), nsrf, ibksiz, res), "boundary_res")
     return res
 
 

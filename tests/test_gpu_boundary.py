@@ -1,0 +1,152 @@
+"""GPU parity of the boundary-element residual (crimson_form_set_boundary: the
+AsBMFG -> e3b -> e3bvar loop of ElmGMR, elmgmr.f90:325-460) against the
+oracle: interior plus boundary residual, FP64, normwise relative 1e-12; the
+matrix is the interior one (rigid walls add no boundary tangent).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def cfa():
+    import crimsonflowsolver_amd as cfa
+    if cfa.device_count() < 1:
+        pytest.fail("no GPU visible: the HIP path must run here")
+    return cfa
+
+
+def _oracle(O, ien, y, ac, x, colm, rowp, ienb, iBCB, BCB, **kw):
+    rres, rK, rP = O.elmgmr_interior(ien, y, ac, x, colm, rowp, O.make_params())
+    return O.boundary_res(ienb, iBCB, BCB, y, x, rres, **kw), rK, rP
+
+
+def _cylinder_state(m, seed):
+    from crimsonflowsolver_amd import mesh as M
+    y, ac = M.poiseuille_state(m, seed=seed)
+    rng = np.random.default_rng(seed)
+    ac[:, :3] = rng.uniform(-50, 50, size=(m.numnp, 3))
+    # back-flow on part of the outlet, so that the influx term is exercised
+    back = m.outlet & (m.x[:, 0] > 0.0)
+    y[back, 2] = -np.abs(y[back, 2]) - 1.0
+    return y, ac
+
+
+def test_cylinder_reference_codes(cfa, oracle_lib):
+    """Inlet code 0, wall 16, outlet 2 (the reference cases' codes), with
+    back-flow at part of the outlet; host entry point."""
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(6, 6, 10)
+    y, ac = _cylinder_state(m, 31)
+    ienb, iBCB, BCB = M.cylinder_boundary(m)
+    colm, rowp = cfa.genadj(m.ien, m.numnp)
+    form = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp)
+    res0, K0, P0 = form(y, ac, m.x)
+    form.set_boundary(ienb, iBCB, BCB)
+    res, K, P = form(y, ac, m.x)
+    rres, rK, rP = _oracle(oracle_lib, m.ien, y, ac, m.x, colm, rowp, ienb, iBCB, BCB)
+    errs = {"res": _rel(res, rres), "lhsK": _rel(K, rK), "lhsP": _rel(P, rP)}
+    print("cylinder boundary", errs, "boundary share", _rel(res, res0))
+    assert all(e <= TOL for e in errs.values()), errs
+    assert np.array_equal(K, K0) and np.array_equal(P, P0)
+    assert _rel(res, res0) > 1e-6  # the boundary rows changed the residual
+    form.set_boundary(np.zeros((0, 4)), np.zeros((0, 2)), np.zeros((0, 3, 2)))
+    res1, _, _ = form(y, ac, m.x)
+    assert np.array_equal(res1, res0)
+
+
+@pytest.mark.parametrize("nqb", [1, 3])
+def test_every_code_into_memls(cfa, oracle_lib, nqb):
+    """Random codes over bits 0, 1, 2, 4 with random BCB (ndBCB 5), the
+    outlet's flow flag closed, formed straight into the memLS matrix
+    (Res4 layout)."""
+    import torch
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(5, 5, 8)
+    y, ac = _cylinder_state(m, 37)
+    ienb, iBCB, _ = M.cylinder_boundary(m)
+    rng = np.random.default_rng(nqb)
+    nelb = ienb.shape[0]
+    codes = np.array([0, 1, 2, 4, 16, 3, 5, 6, 7, 18], np.int32)
+    iBCB = np.asfortranarray(np.stack([codes[rng.integers(0, codes.size, nelb)], iBCB[:, 1]], 1).astype(np.int32))
+    BCB = np.asfortranarray(rng.uniform(-5, 5, (nelb, 3, 5)))
+    permit = np.array([1, 1, 1, 0], np.int32)
+    colm, rowp = cfa.genadj(m.ien, m.numnp)
+    form = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp)
+    form.set_boundary(ienb, iBCB, BCB, nqb=nqb, permit=permit)
+    lhs = cfa.memLS_LHS_CREATE(m.numnp, m.numnp, rowp.size, None, colm, rowp, 1)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(dev)
+    tres4 = torch.zeros((m.numnp, 4), dtype=torch.float64, device=dev)
+    form.to_lhs(T(y), T(ac), T(m.x), tres4, lhs)
+    torch.cuda.synchronize()
+    rres, rK, rP = _oracle(oracle_lib, m.ien, y, ac, m.x, colm, rowp, ienb, iBCB, BCB, nqb=nqb, permit=permit)
+    err = _rel(tres4.cpu().numpy(), rres)
+    print("every code, nqb", nqb, err)
+    assert err <= TOL
+    assert _rel(lhs.get_val(4), oracle_lib.pack_val4(colm, rowp, rK, rP)) <= TOL
+
+
+def test_real_geometry_boundary(cfa, oracle_lib):
+    """The reference's own test cylinder (mainTests/basic) with its own
+    boundary elements, codes and BCB (tests/golden/basic_case.npz)."""
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "basic_case.npz"))
+    x, ien = np.asfortranarray(z["x"]), np.asfortranarray(z["ien"])
+    ienb, iBCB, BCB = (np.asfortranarray(z[k]) for k in ("ienb", "iBCB", "BCB"))
+    n = x.shape[0]
+    rng = np.random.default_rng(41)
+    r2 = (x[:, 0] ** 2 + x[:, 1] ** 2) / 9.0
+    y = np.zeros((n, 4), order="F")
+    y[:, :3] = rng.uniform(-1.5, 1.5, size=(n, 3))
+    y[:, 2] += 30.0 * np.clip(1.0 - r2, 0.0, None)
+    y[:, 3] = 1330.0 * (20.0 - x[:, 2]) / 40.0
+    ac = np.asfortranarray(rng.uniform(-50, 50, size=(n, 4)))
+    colm, rowp = oracle_lib.genadj(ien, n)
+    form = cfa.ElmGMRInterior(ien, colm, rowp, n)
+    form.set_boundary(ienb, iBCB, BCB)
+    res, K, P = form(y, ac, x)
+    rres, rK, rP = _oracle(oracle_lib, ien, y, ac, x, colm, rowp, ienb, iBCB, BCB)
+    errs = {"res": _rel(res, rres), "lhsK": _rel(K, rK), "lhsP": _rel(P, rP)}
+    print("real geometry boundary", errs)
+    assert all(e <= TOL for e in errs.values()), errs
+
+
+def test_mixed_precision_residual_with_boundary(cfa, oracle_lib):
+    """The mixed form keeps the residual (interior and boundary) in FP64."""
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(5, 5, 8)
+    y, ac = _cylinder_state(m, 43)
+    ienb, iBCB, BCB = M.cylinder_boundary(m)
+    colm, rowp = cfa.genadj(m.ien, m.numnp)
+    form = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp, params=cfa.form_params(precision=32))
+    form.set_boundary(ienb, iBCB, BCB)
+    res, _, _ = form(y, ac, m.x)
+    rres, _, _ = _oracle(oracle_lib, m.ien, y, ac, m.x, colm, rowp, ienb, iBCB, BCB)
+    assert _rel(res, rres) <= TOL
+
+
+def test_boundary_rejects_ensembles_and_vjp(cfa):
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(4, 4, 4)
+    ienb, iBCB, BCB = M.cylinder_boundary(m)
+    colm, rowp = cfa.genadj(m.ien, m.numnp)
+    form = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp)
+    form.set_boundary(ienb, iBCB, BCB)
+    with pytest.raises(cfa.CrimsonError):
+        form.set_instances(np.zeros(m.nel, np.uint16), np.array([0.04]))
+    import torch
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(dev)
+    y, ac = M.poiseuille_state(m)
+    with pytest.raises(cfa.CrimsonError):
+        form.vjp(T(y), T(ac), T(m.x), torch.ones((4, m.numnp), dtype=torch.float64, device=dev),
+                 torch.zeros((4, m.numnp), dtype=torch.float64, device=dev))

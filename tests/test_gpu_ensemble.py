@@ -101,3 +101,49 @@ def test_union_solve_matches_instances(env):
         err = np.abs(xu[ens.rows(k)] - xk).max() / np.abs(xk).max()
         print("instance", k, "union vs own solve", err)
         assert err <= 1e-4
+
+
+def test_union_solve_many_coupled_faces(env):
+    """18 instances: 18 coupled resistance faces in one memLS matrix (more
+    than any fixed per-solve slot count), each face's ADD/PRE term must use
+    its own face dot."""
+    cfa, torch = env
+    from crimsonflowsolver_amd.ensemble import Ensemble
+    from crimsonflowsolver_amd import mesh as M
+    K = 18
+    meshes = [M.cylinder_mesh(4, 4, 4 + k % 5) for k in range(K)]
+    mu = 0.03 + 0.002 * np.arange(K)
+    R = np.linspace(800.0, 2000.0, K)
+    states = [M.poiseuille_state(m, seed=90 + k) for k, m in enumerate(meshes)]
+    ens = Ensemble(meshes, mu, R)
+    y = np.concatenate([s[0] for s in states])
+    ac = np.concatenate([s[1] for s in states])
+    r4 = torch.zeros((ens.N, 4), dtype=torch.float64, device="cuda")
+    ens.form.to_lhs(_T(torch, y), _T(torch, ac), _T(torch, ens.x), r4, ens.lhs)
+    ens.form.bc3res(r4, res4_layout=True)
+    kw = dict(relTol=1e-6, dimKry=60, maxItr=10, relTolIn=[1e-4, 1e-4], maxItrIn=[10, 500])
+    ls = cfa.memLS_LS_CREATE(cfa.LS_TYPE_NS, **kw)
+    cfa.memLS_SOLVE_dev(ens.lhs, ls, 4, r4, res=ens.face_res())
+    assert ls.RI["suc"] == 1
+    xu = r4.cpu().numpy()
+    p = ens.params
+    worst = 0.0
+    for k, m in enumerate(meshes):
+        colm, rowp = cfa.genadj(m.ien, m.numnp)
+        ibc, _ = M.dirichlet_codes(m)
+        f = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp, iBC=ibc, params=cfa.form_params(mu=float(mu[k])))
+        lhs = cfa.memLS_LHS_CREATE(m.numnp, m.numnp, rowp.size, None, colm, rowp, 2)
+        fixed = np.nonzero(ibc)[0]
+        out = np.nonzero(m.outlet & (ibc == 0))[0]
+        cfa.memLS_BC_CREATE(lhs, 1, fixed.size, 3, cfa.BC_TYPE_Dir, fixed + 1, np.zeros((3, fixed.size), order="F"))
+        cfa.memLS_BC_CREATE(lhs, 2, out.size, 3, cfa.BC_TYPE_Neu, out + 1, np.asfortranarray(m.nabi()[out].T))
+        rk = torch.zeros((m.numnp, 4), dtype=torch.float64, device="cuda")
+        f.to_lhs(_T(torch, states[k][0]), _T(torch, states[k][1]), _T(torch, m.x), rk, lhs)
+        f.bc3res(rk, res4_layout=True)
+        lk = cfa.memLS_LS_CREATE(cfa.LS_TYPE_NS, **kw)
+        cfa.memLS_SOLVE_dev(lhs, lk, 4, rk, res=np.array([0.0, R[k] * p.alfi * p.gami * p.Delt]))
+        xk = rk.cpu().numpy()
+        err = np.abs(xu[ens.rows(k)] - xk).max() / np.abs(xk).max()
+        worst = max(worst, err)
+    print("18 coupled faces: worst union vs own solve", worst)
+    assert worst <= 1e-4
