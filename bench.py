@@ -74,19 +74,27 @@ def inflow_mean(t, seed=7, period=1.0):
     return u
 
 
-def cpu_baseline(sample_grid, target_s=15.0):
-    """Oracle (CPU restatement, one core) formation on a bounded slab of the
-    same cross-section; returns Melem/s."""
+def cpu_baseline(sample_grid, target_s=15.0, boundary=True):
+    """Oracle (CPU restatement, one core) formation (interior and, with
+    boundary, the boundary-element loop) on a bounded slab of the same
+    cross-section; returns Melem/s of interior elements."""
     from crimsonflowsolver_amd import mesh as M
     from oracle import oracle as O
     m = M.cylinder_mesh(*sample_grid)
     colm, rowp = O.genadj(m.ien, m.numnp)
     y, ac = M.poiseuille_state(m)
+    ienb, iBCB, BCB = M.cylinder_boundary(m) if boundary else (None, None, None)
     p = O.make_params()
-    O.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, p)  # warm-up
+
+    def once():
+        res, _, _ = O.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, p)
+        if boundary:
+            O.boundary_res(ienb, iBCB, BCB, y, m.x, res)
+
+    once()  # warm-up
     n, t0 = 0, time.perf_counter()
     while True:
-        O.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, p)
+        once()
         n += 1
         if time.perf_counter() - t0 >= target_s:
             break
@@ -104,6 +112,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0: skip)")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision timesteps (0: skip)")
+    ap.add_argument("--boundary", type=int, default=1,
+                    help="form the boundary elements too (inlet code 0, wall 16, outlet 2), as ElmGMR does")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MB first (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -144,6 +154,16 @@ def main():
         lhs32 = cfa.memLS_LHS_CREATE(N, N, nnz, None, colm, rowp, 2)
         cfa.memLS_BC_CREATE(lhs32, 1, fixed.size, 3, cfa.BC_TYPE_Dir, fixed + 1, np.zeros((3, fixed.size), order="F"))
         cfa.memLS_BC_CREATE(lhs32, 2, out.size, 3, cfa.BC_TYPE_Neu, out + 1, np.asfortranarray(nabi[out].T))
+
+    nelb = 0
+    if args.boundary:
+        t0 = time.time()
+        ienb, iBCB, BCB = M.cylinder_boundary(m)
+        nelb = ienb.shape[0]
+        for fm in (form, form32):
+            if fm is not None:
+                fm.set_boundary(ienb, iBCB, BCB)
+        log(f"boundary elements: {nelb} ({time.time() - t0:.1f}s)")
 
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)  # (n,k) Fortran -> (k,n) C
     dy, dac, dx = T(y), T(ac), T(m.x)
@@ -324,9 +344,10 @@ def main():
     cpu = None
     if args.cpu_seconds > 0:
         sample = (grid[0], grid[1], max(4, grid[2] // 16))
-        v, sn, nrep, dt = cpu_baseline(sample, args.cpu_seconds)
+        v, sn, nrep, dt = cpu_baseline(sample, args.cpu_seconds, bool(args.boundary))
+        what = "flux+e3+fillsparseI" + (" + boundary elements" if args.boundary else "")
         cpu = {"value": v, "unit": "Melem/s", "cores": 1, "kind": "port",
-               "sample": f"oracle interior formation (flux+e3+fillsparseI) on the {sample[0]}x{sample[1]}x{sample[2]} "
+               "sample": f"oracle formation ({what}) on the {sample[0]}x{sample[1]}x{sample[2]} "
                          f"slab ({sn} tets), {nrep} calls in {dt:.1f}s, one core"}
         log(f"cpu baseline: {v:.3f} Melem/s")
 
@@ -334,11 +355,13 @@ def main():
         "metric": METRIC, "value": melems, "unit": "Melem/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"FP64 interior formation (flux pass + e3 + assembly into the memLS matrix) on the "
+        "config": {"workload": f"FP64 formation (flux pass + e3 + assembly into the memLS matrix"
+                               f"{' + boundary elements' if nelb else ''}) on the "
                                f"cylinder r=1 L=10 from a {grid[0]}x{grid[1]}x{grid[2]} hex grid clipped by hex-centroid "
                                f"radius, 6 Kuhn tets per hex",
-                   "nel": nel, "nshg": N, "nnzb": nnz, "rho": 1.06, "mu": 0.04, "dt": 1e-3, "rho_inf": 0.5},
-        "phase_ms": {"flux": tm[0], "e3_elem": tm[1], "assembly": tm[2], "events_total": tm[3]},
+                   "nel": nel, "nelb": nelb, "nshg": N, "nnzb": nnz, "rho": 1.06, "mu": 0.04, "dt": 1e-3, "rho_inf": 0.5},
+        "phase_ms": {"flux": tm[0], "e3_elem": tm[1], "assembly": tm[2], "boundary": tm[3] - tm[0] - tm[1] - tm[2],
+                     "events_total": tm[3]},
         "krylov_spmv_gbs": {"bsr3x3": gbs3, "bsr4x4": gbs4, "ms3": ms3, "ms4": ms4},
         "krylov_spmv_gbs_fp32": spmv32, "mixed_formation": mixed_form,
         "timestep_ms_fp64": ts_ms, **ts_info,

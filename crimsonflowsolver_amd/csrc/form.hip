@@ -1160,14 +1160,15 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
                                 f->d_rec.p, d_res, o);
   }
   CR_HIP(hipGetLastError());
-  CR_TRY(crimson::form_boundary_pass(f, d_res, mode));  // the boundary loop after the interior (elmgmr.f90:325)
   if (timed) CR_HIP(hipEventRecord(f->ev[3], s));
+  CR_TRY(crimson::form_boundary_pass(f, d_res, mode));  // the boundary loop after the interior (elmgmr.f90:325)
+  if (timed) CR_HIP(hipEventRecord(f->ev[4], s));
   if (!timed) return CRIMSON_OK;  // enqueued only (graph capture)
   CR_HIP(hipStreamSynchronize(s));
   CR_HIP(hipEventElapsedTime(&f->ms[0], f->ev[0], f->ev[1]));
   CR_HIP(hipEventElapsedTime(&f->ms[1], f->ev[1], f->ev[2]));
   CR_HIP(hipEventElapsedTime(&f->ms[2], f->ev[2], f->ev[3]));
-  CR_HIP(hipEventElapsedTime(&f->ms[3], f->ev[0], f->ev[3]));
+  CR_HIP(hipEventElapsedTime(&f->ms[3], f->ev[0], f->ev[4]));
   return CRIMSON_OK;
 }
 

@@ -151,7 +151,7 @@ struct crimson_form {
   crimson::DBuf<float> d_rec32;  // FP32 tangent records (precision 32)
   crimson::DBuf<double> s_y, s_ac, s_x, s_res, s_K, s_P, s_q;  // staging of the host API
   bool has_ibc = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   float ms[4] = {0, 0, 0, 0};
   // adjoint (VJP) workspaces, allocated on first use
   crimson::DBuf<double> d_vrec, d_vrec2, d_qbar, d_invm, d_mupart;

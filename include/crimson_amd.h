@@ -109,7 +109,8 @@ int crimson_form_bc3res_dev(crimson_form *f, double *d_res, int res4_layout);
 
 /* Wall time (ms) of the last formation's kernels, measured with HIP events
  * on the handle's stream: out[0] flux pass, out[1] element kernel,
- * out[2] assembly, out[3] total. */
+ * out[2] assembly of the interior rows, out[3] total (with the boundary
+ * elements, crimson_form_set_boundary). */
 int crimson_form_timing(const crimson_form *f, double *out4);
 
 /* Boundary elements: replaces the boundary-element loop of ElmGMR
