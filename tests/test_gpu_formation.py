@@ -177,3 +177,37 @@ def test_formation_real_geometry(cfa, oracle_lib):
     errs = {k: _rel(a, b) for k, a, b in (("res", res, rres), ("lhsK", K, rK), ("lhsP", P, rP), ("q", q, rq))}
     print("real geometry", errs)
     assert all(e <= TOL for e in errs.values()), errs
+
+
+def test_formation_scattered_numbering(cfa, oracle_lib):
+    """A cylinder whose node numbers are randomly permuted (elements span
+    far more than 32K numbers, so the assembly schedules rows in Morton
+    order of the coordinates instead of the numbering): residual, tangent
+    and the memLS planes against the oracle."""
+    import torch
+    from crimsonflowsolver_amd import mesh as M
+    m = M.cylinder_mesh(32, 32, 80)
+    n = m.numnp
+    perm = np.random.default_rng(77).permutation(n)  # new number of old node i
+    inv = np.argsort(perm)
+    ien = np.asfortranarray((perm[m.ien - 1] + 1).astype(np.int32))
+    x = np.asfortranarray(m.x[inv])
+    y0, ac0 = M.poiseuille_state(m, seed=78)
+    y, ac = np.asfortranarray(y0[inv]), np.asfortranarray(ac0[inv])
+    span = ien.max(axis=1) - ien.min(axis=1)
+    assert (span > 32768).mean() > 0.01  # the numbering is not local
+    colm, rowp = cfa.genadj(ien, n)
+    form = cfa.ElmGMRInterior(ien, colm, rowp, n)
+    res, K, P = form(y, ac, x)
+    rres, rK, rP = oracle_lib.elmgmr_interior(ien, y, ac, x, colm, rowp, oracle_lib.make_params())
+    errs = {"res": _rel(res, rres), "lhsK": _rel(K, rK), "lhsP": _rel(P, rP)}
+    print("scattered numbering", n, "nodes", errs)
+    assert all(e <= TOL for e in errs.values()), errs
+    lhs = cfa.memLS_LHS_CREATE(n, n, rowp.size, None, colm, rowp, 1)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+    tres4 = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+    form.to_lhs(T(y), T(ac), T(x), tres4, lhs)
+    torch.cuda.synchronize()
+    assert _rel(lhs.get_val(4), oracle_lib.pack_val4(colm, rowp, rK, rP)) <= TOL
+    assert _rel(tres4.cpu().numpy(), rres) <= TOL
