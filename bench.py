@@ -102,6 +102,112 @@ def cpu_baseline(sample_grid, target_s=15.0, boundary=True):
     return m.nel * n / dt / 1e6, m.nel, n, dt
 
 
+def cpu_timestep(grid, R=1333.0):
+    """One timestep (3 nonlinear iterations: yAlpha, formation with the
+    boundary elements, bc3Res, ElmpvsQ's resistance term, Val4 packing, memLS
+    NS solve, itrCorrect, itrBC) of the oracle on one core, on the slab
+    `grid`; the same steps and solver settings as the GPU timestep.
+    Returns (seconds, solver iterations per solve)."""
+    from crimsonflowsolver_amd import mesh as M
+    from oracle import oracle as O
+    import numpy as np
+    m = M.cylinder_mesh(*grid)
+    N = m.numnp
+    colm, rowp = O.genadj(m.ien, N)
+    y0, ac0 = M.poiseuille_state(m)
+    ibc, _ = M.dirichlet_codes(m)
+    ienb, iBCB, BCB = M.cylinder_boundary(m)
+    nabi = m.nabi()
+    out = np.nonzero(m.outlet & (ibc == 0))[0]
+    p = O.make_params()
+    faceres = np.array([0.0, R * p.alfi * p.gami * p.Delt])
+    ls = O.MemLS(colm, rowp, 2, relTol=1e-8, dimKry=100, maxItr=10, relTolIn=[0.01, 0.2], maxItrIn=[1, 200])
+    fixed = np.nonzero(ibc)[0]
+    ls.bc_create(1, fixed + 1, 3, O.BC_TYPE_Dir, np.zeros((3, fixed.size), order="F"))
+    ls.bc_create(2, out + 1, 3, O.BC_TYPE_Neu, np.asfortranarray(nabi[out].T))
+    rr = m.x[:, 0] ** 2 + m.x[:, 1] ** 2
+    inl = m.inlet & ~m.wall
+    bc = np.zeros((N, 7), order="F")
+    bc[inl, 4] = 2.0 * inflow_mean(p.Delt) * np.clip(1.0 - rr[inl], 0.0, None)
+    its = []
+    t0 = time.perf_counter()
+    y, ac = O.itr_predict(y0, ac0, p.gami)
+    y = O.itr_bc(ibc, bc, y)
+    for _ in range(3):
+        ya, aca = O.itr_yalpha(y0, ac0, y, ac, p.alfi, p.almi)
+        res, K, P = O.elmgmr_interior(m.ien, ya, aca, m.x, colm, rowp, p)
+        res = O.boundary_res(ienb, iBCB, BCB, ya, m.x, res)
+        res = O.bc3res(ibc, res)
+        res, _ = O.elmpvsq_resistance(res, ya, out, nabi[out], R)
+        sol = ls.solve(np.asfortranarray(res.T), O.pack_val4(colm, rowp, K, P), res=faceres)
+        its.append((ls.stats(0)["itr"], ls.stats(1)["itr"], ls.stats(2)["itr"]))
+        y, ac = O.itr_correct(y, ac, sol, p.gami, p.alfi, p.Delt)
+        y = O.itr_bc(ibc, bc, y)
+    return time.perf_counter() - t0, its
+
+
+def gpu_timestep(cfa, M, torch, grid, R=1333.0, reps=2):
+    """The GPU timestep of the main bench on the slab `grid` (the same steps
+    as cpu_timestep, through the device entry points); returns the mean
+    milliseconds of `reps` timesteps from the same initial state and the
+    solver iterations of the last."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    m = M.cylinder_mesh(*grid)
+    N = m.numnp
+    colm, rowp = cfa.genadj(m.ien, N)
+    y0, ac0 = M.poiseuille_state(m)
+    ibc, _ = M.dirichlet_codes(m)
+    prm = cfa.form_params()
+    form = cfa.ElmGMRInterior(m.ien, colm, rowp, N, iBC=ibc, params=prm)
+    form.set_boundary(*M.cylinder_boundary(m))
+    nabi = m.nabi()
+    out = np.nonzero(m.outlet & (ibc == 0))[0]
+    fixed = np.nonzero(ibc)[0]
+    lhs = cfa.memLS_LHS_CREATE(N, N, rowp.size, None, colm, rowp, 2)
+    cfa.memLS_BC_CREATE(lhs, 1, fixed.size, 3, cfa.BC_TYPE_Dir, fixed + 1, np.zeros((3, fixed.size), order="F"))
+    cfa.memLS_BC_CREATE(lhs, 2, out.size, 3, cfa.BC_TYPE_Neu, out + 1, np.asfortranarray(nabi[out].T))
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+    dx = T(m.x)
+    faceres = np.array([0.0, R * prm.alfi * prm.gami * prm.Delt])
+    rr = m.x[:, 0] ** 2 + m.x[:, 1] ** 2
+    inl = m.inlet & ~m.wall
+    bc = np.zeros((N, 7), order="F")
+    bc[inl, 4] = 2.0 * inflow_mean(prm.Delt) * np.clip(1.0 - rr[inl], 0.0, None)
+    dBC, dibc = T(bc), torch.from_numpy(ibc.astype(np.int32)).to(dev)
+    dglob = torch.from_numpy(out.astype(np.int32)).to(dev)
+    dnabi = torch.from_numpy(np.ascontiguousarray(nabi[out])).to(dev)
+    dq = torch.zeros(1, dtype=torch.float64, device=dev)
+    dres4 = torch.zeros((N, 4), dtype=torch.float64, device=dev)
+    yold, acold = T(y0), T(ac0)
+    times = []
+    for _ in range(reps):
+        ls = cfa.memLS_LS_CREATE(cfa.LS_TYPE_NS, relTol=1e-8, dimKry=100, maxItr=10,
+                                 relTolIn=[0.01, 0.2], maxItrIn=[1, 200])
+        yc, acc = torch.empty_like(yold), torch.empty_like(acold)
+        ya, aca = torch.empty_like(yold), torch.empty_like(acold)
+        its = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        form.predict(yold, acold, yc, acc)
+        form.itr_bc(dibc, dBC, yc)
+        for _ in range(3):
+            form.yalpha(yold, acold, yc, acc, ya, aca)
+            form.sync()
+            form.to_lhs(ya, aca, dx, dres4, lhs)
+            form.bc3res(dres4, res4_layout=True)
+            form.elmpvsq(dglob, dnabi, R, ya, dres4, dq)
+            form.sync()
+            cfa.memLS_SOLVE_dev(lhs, ls, 4, dres4, res=faceres)
+            its.append((ls.RI["itr"], ls.GM["itr"], ls.CG["itr"]))
+            form.correct(yc, acc, dres4)
+            form.itr_bc(dibc, dBC, yc)
+            form.sync()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    return float(np.mean(times) * 1e3), its
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +218,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0: skip)")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision timesteps (0: skip)")
+    ap.add_argument("--slab", default="32,32,20",
+                    help="slab on which one timestep is timed on the GPU and with the oracle on one core "
+                         "(empty: skip; needs --cpu-seconds > 0)")
     ap.add_argument("--boundary", type=int, default=1,
                     help="form the boundary elements too (inlet code 0, wall 16, outlet 2), as ElmGMR does")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -342,6 +451,7 @@ def main():
                 "traffic": pmc_traffic(grid, "asm_row<1,16>")}
 
     cpu = None
+    slab = None
     if args.cpu_seconds > 0:
         sample = (grid[0], grid[1], max(4, grid[2] // 16))
         v, sn, nrep, dt = cpu_baseline(sample, args.cpu_seconds, bool(args.boundary))
@@ -350,6 +460,15 @@ def main():
                "sample": f"oracle formation ({what}) on the {sample[0]}x{sample[1]}x{sample[2]} "
                          f"slab ({sn} tets), {nrep} calls in {dt:.1f}s, one core"}
         log(f"cpu baseline: {v:.3f} Melem/s")
+        if args.slab:
+            sg = tuple(int(v) for v in args.slab.split(","))
+            gms, gits = gpu_timestep(cfa, M, torch, sg)
+            cs, cits = cpu_timestep(sg)
+            slab = {"grid": list(sg), "gpu_ms": gms, "cpu_ms": cs * 1e3, "speedup": cs * 1e3 / gms,
+                    "solver_iterations_gpu": [list(map(float, v)) for v in gits],
+                    "solver_iterations_cpu": [list(map(float, v)) for v in cits],
+                    "cpu": "oracle (C restatement of the formation and memLS), numpy step glue, one core"}
+            log(f"timestep on the {sg} slab: GPU {gms:.1f} ms, CPU {cs * 1e3:.0f} ms -> x{cs * 1e3 / gms:.0f}")
 
     line = {
         "metric": METRIC, "value": melems, "unit": "Melem/s", "n_gpus": 1, "steps": args.steps,
@@ -365,7 +484,7 @@ def main():
         "krylov_spmv_gbs": {"bsr3x3": gbs3, "bsr4x4": gbs4, "ms3": ms3, "ms4": ms4},
         "krylov_spmv_gbs_fp32": spmv32, "mixed_formation": mixed_form,
         "timestep_ms_fp64": ts_ms, **ts_info,
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "timestep_slab": slab,
     }
     print(json.dumps(line), flush=True)
 

@@ -258,3 +258,54 @@ class MemLS:
 
 def flops_per_element():
     return lib().cfo_flops_per_element()
+
+
+# ------------------------------------------------------------ step glue
+# numpy restatements of the updates either side of the formation/solve in
+# one nonlinear iteration (itrdrv.f90:1284-1528); y, ac (nshg,4) with
+# columns (u1,u2,u3,p) as the solver arrays hold them.
+
+def itr_predict(yold, acold, gami):
+    """itrPredict, ipred = 1 (itrPC.f90:104-108): y = yold, ac = acold (gami-1)/gami."""
+    return np.asfortranarray(yold.copy()), np.asfortranarray(acold * ((gami - 1.0) / gami))
+
+
+def itr_yalpha(yold, acold, y, ac, alfi, almi):
+    """itrYAlpha (itrPC.f90:326-334): the state at n+alpha; pressure at n+1,
+    the pressure column of acAlpha zero."""
+    ya = np.asfortranarray(y.copy())
+    aca = np.zeros_like(ac, order="F")
+    ya[:, :3] = yold[:, :3] + alfi * (y[:, :3] - yold[:, :3])
+    aca[:, :3] = acold[:, :3] + almi * (ac[:, :3] - acold[:, :3])
+    return ya, aca
+
+
+def itr_correct(y, ac, sol4, gami, alfi, delt):
+    """itrCorrect (itrPC.f90:176-181) with solinc(i,1:4) = Ri(1:4,i)."""
+    y = np.asfortranarray(y.copy())
+    ac = np.asfortranarray(ac.copy())
+    d = np.asarray(sol4).T
+    y[:, :3] += (gami * delt) * d[:, :3]
+    ac[:, :3] += d[:, :3]
+    y[:, 3] += (gami * alfi * delt) * d[:, 3]
+    return y, ac
+
+
+def itr_bc(ibc, bc, y):
+    """itrBC for velocity codes 0/7 and the pressure bit (itrbc.f90:83-98):
+    BC(:,3:5) = the prescribed velocity, BC(:,1) = the pressure."""
+    y = np.asfortranarray(y.copy())
+    v = ((ibc >> 3) & 7) == 7
+    y[v, :3] = bc[v, 2:5]
+    p = ((ibc >> 2) & 1) == 1
+    y[p, 3] = bc[p, 0]
+    return y
+
+
+def elmpvsq_resistance(res, y, glob, nabi, R):
+    """ElmpvsQ's resistance term (elmgmr.f90:1152-1176): Q = sum NABI.u over
+    the face's nodes, res(glob,1:3) += -R Q NABI."""
+    res = np.asfortranarray(res.copy())
+    q = float(np.sum(nabi * y[glob, :3]))
+    res[glob, :3] += (-q * R) * nabi
+    return res, q
