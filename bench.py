@@ -218,7 +218,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0: skip)")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision timesteps (0: skip)")
-    ap.add_argument("--slab", default="32,32,20",
+    ap.add_argument("--slab", default="64,64,20",
                     help="slab on which one timestep is timed on the GPU and with the oracle on one core "
                          "(empty: skip; needs --cpu-seconds > 0)")
     ap.add_argument("--boundary", type=int, default=1,
