@@ -412,11 +412,6 @@ extern "C" int crimson_form_vjp_dev(crimson_form *f, const double *d_y, const do
     crimson::set_error("crimson_form_vjp_dev: null argument");
     return CRIMSON_EARG;
   }
-  if (f->nelb > 0) {
-    crimson::set_error("crimson_form_vjp_dev: the boundary elements' residual is not differentiated; "
-                       "use a handle without crimson_form_set_boundary");
-    return CRIMSON_EUNSUPPORTED;
-  }
   const size_t ne = (size_t)std::max(f->nel, 1), nn = (size_t)f->nshg;
   const int nmu = (int)((ne + kMuChunk - 1) / kMuChunk);
   CR_TRY(f->d_vrec.alloc(ne * kVRec));
@@ -445,6 +440,8 @@ extern "C" int crimson_form_vjp_dev(crimson_form *f, const double *d_y, const do
   }
   CR_HIP(hipGetLastError());
   CR_HIP(hipStreamSynchronize(s));
-  if (mubar) *mubar = mb;
+  double mbb = 0.0;
+  CR_TRY(crimson::form_boundary_vjp(f, d_w, d_ybar, &mbb));  // the boundary rows (crimson_form_set_boundary)
+  if (mubar) *mubar = mb + mbb;
   return CRIMSON_OK;
 }

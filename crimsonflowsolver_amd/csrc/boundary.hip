@@ -18,6 +18,7 @@
 //              incidences in boundary-element order, after the interior rows
 //              are written (the reference adds the boundary loop after the
 //              interior loop)
+//   bnd_vjp_*  the adjoint of the boundary rows for crimson_form_vjp_dev
 
 #include <hip/hip_runtime.h>
 
@@ -34,6 +35,7 @@ namespace {
 constexpr double kTriA = 0.3333333333333330000000000000000000;  // symtri.c:20
 constexpr double kTriB = 0.666666666666667;                     // symtri.c:21
 constexpr double kTriC = 0.166666666666667;                     // symtri.c:22
+constexpr int kBV = 17;  // boundary VJP record: (u1,u2,u3,p) cotangents of the 4 nodes, mu cotangent
 
 __device__ inline double tri_pt(int nqb, int q, int a) {
   if (nqb == 1) return kTriA;
@@ -207,6 +209,203 @@ __global__ void __launch_bounds__(256) bnd_node(int nbnode, int nshg, const int 
   }
 }
 
+// Adjoint of bnd_elem: w^T d rl/d (u, p) of the four tet nodes and
+// w^T d rl/d mu, for the cotangent w(nshg,4) laid out like res.  Reverse
+// mode by hand through the same quantities: the face-point values (p, u,
+// unm), the traction of the constant gradient, the code-bit overrides
+// (constants carry no derivative) and the influx term's branch.
+__global__ void __launch_bounds__(256) bnd_vjp_elem(int nelb, int nshg, const int *__restrict__ ienb,
+                                                    const int *__restrict__ bcode, const int *__restrict__ permit,
+                                                    int nsrf, int nqb, double influx, double rho, double mu,
+                                                    const double *__restrict__ node, const double *__restrict__ w,
+                                                    double *__restrict__ vrec) {
+  const int e = xcd_block() * blockDim.x + threadIdx.x;
+  if (e >= nelb) return;
+  double xl[4][3], yl[4][4], wl[3][4];
+  int g[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    g[a] = ienb[a * (size_t)nelb + e];
+    const double *r = node + (size_t)g[a] * kNRec;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) xl[a][i] = r[N_X + i];
+    yl[a][0] = r[N_P];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) yl[a][i + 1] = r[N_U + i];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wl[a][k] = w[g[a] + (size_t)nshg * k];
+  constexpr double sg[3][4] = {{0.5, 0.0, 0.0, -0.5}, {0.0, 0.5, 0.0, -0.5}, {0.0, 0.0, 0.5, -0.5}};
+  double J[3][3], D[3][3], gl[3][4], gy[3][4], Gn[4][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) s = s + xl[a][i] * sg[j][a];
+      J[i][j] = s;
+    }
+  D[0][0] = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+  D[0][1] = J[2][1] * J[0][2] - J[0][1] * J[2][2];
+  D[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  const double tdet = 1.0 / (D[0][0] * J[0][0] + D[0][1] * J[1][0] + D[0][2] * J[2][0]);
+  D[0][0] *= tdet;
+  D[0][1] *= tdet;
+  D[0][2] *= tdet;
+  D[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * tdet;
+  D[1][1] = (J[0][0] * J[2][2] - J[2][0] * J[0][2]) * tdet;
+  D[1][2] = (J[1][0] * J[0][2] - J[0][0] * J[1][2]) * tdet;
+  D[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * tdet;
+  D[2][1] = (J[2][0] * J[0][1] - J[0][0] * J[2][1]) * tdet;
+  D[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * tdet;
+  // Gn[a][j] = d(gy[j][c]) / d(yl[a][c]): the P1 gradient of node a
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Gn[a][j] = D[0][j] * sg[0][a] + D[1][j] * sg[1][a] + D[2][j] * sg[2][a];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 1; c < 4; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) s = s + sg[k][a] * yl[a][c];
+      gl[k][c] = s;
+    }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int c = 1; c < 4; ++c) gy[j][c] = D[0][j] * gl[0][c] + D[1][j] * gl[1][c] + D[2][j] * gl[2][c];
+  double v1[3], v2[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    v1[i] = xl[1][i] - xl[0][i];
+    v2[i] = xl[2][i] - xl[0][i];
+  }
+  const double t1 = v1[1] * v2[2] - v2[1] * v1[2];
+  const double t2 = v2[0] * v1[2] - v1[0] * v2[2];
+  const double t3 = v1[0] * v2[1] - v2[0] * v1[1];
+  const double temp = 1.0 / sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+  const double bn[3] = {t1 * temp, t2 * temp, t3 * temp};
+  const int code = bcode[e], srf = bcode[(size_t)nelb + e];
+  const bool flow_ok = !permit || srf < 0 || srf > nsrf || permit[srf] != 0;
+  // (grad u + grad u^T) n, so that tau0 = mu sym_n
+  double symn[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s = s + (gy[j][k + 1] + gy[k][j + 1]) * bn[j];
+    symn[k] = s;
+  }
+  const bool p_free = !(code & 16) && !((code & 2) && flow_ok);  // pres = p(q) - tn
+  const bool t_free = !(code & 16) && !(code & 4);               // tau = tau0 - n tn
+  double ub[4][3], pb[3], tnb = 0.0, t0b[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 4; ++a) ub[a][0] = ub[a][1] = ub[a][2] = 0.0;
+  pb[0] = pb[1] = pb[2] = 0.0;
+  const double qw = nqb == 1 ? 2.0 * 1.0 : 2.0 * kTriA;
+  for (int q = 0; q < nqb; ++q) {
+    const double N[3] = {tri_pt(nqb, q, 0), tri_pt(nqb, q, 1), tri_pt(nqb, q, 2)};
+    const double W = qw / (4.0 * temp);
+    double u[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) u[i] = u[i] + N[a] * yl[a][i + 1];
+    const double unm = bn[0] * u[0] + bn[1] * u[1] + bn[2] * u[2];
+    double gk[4];  // d J / d raw_k at this point
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gk[k] = -W * (N[0] * wl[0][k] + N[1] * wl[1][k] + N[2] * wl[2][k]);
+    double uq[3] = {0.0, 0.0, 0.0}, pq = 0.0;
+    if (!(code & 1))
+#pragma unroll
+      for (int j = 0; j < 3; ++j) uq[j] += gk[3] * bn[j];
+    const double gn = gk[0] * bn[0] + gk[1] * bn[1] + gk[2] * bn[2];
+    if (p_free) {
+      pq += gn;
+      tnb -= gn;
+    }
+    if (t_free) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) t0b[i] -= gk[i];
+      tnb += gn;
+    }
+    if ((code & 2) && unm < 0.0) {
+      const double cr = influx * rho;
+      const double gu = gk[0] * u[0] + gk[1] * u[1] + gk[2] * u[2];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) uq[j] -= cr * (bn[j] * gu + unm * gk[j]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      pb[a] += N[a] * pq;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ub[a][j] += N[a] * uq[j];
+    }
+  }
+  // tn = n . tau0, tau0 = mu (grad u + grad u^T) n
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t0b[i] += tnb * bn[i];
+  const double mub = t0b[0] * symn[0] + t0b[1] * symn[1] + t0b[2] * symn[2];
+  // d tau0_k / d (du_k'/dx_j): mu (delta_kk' n_j + delta_jk' n_k)
+#pragma unroll
+  for (int kp = 0; kp < 3; ++kp)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double db = mu * (t0b[kp] * bn[j] + t0b[j] * bn[kp]);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) ub[a][kp] += db * Gn[a][j];
+    }
+  double *o = vrec + (size_t)e * kBV;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o[4 * a + i] = ub[a][i];
+    o[4 * a + 3] = a < 3 ? pb[a] : 0.0;
+  }
+  o[16] = mub;
+}
+
+// ybar(node, :) += the boundary records of its incidences (all four tet
+// nodes), in boundary-element order
+__global__ void __launch_bounds__(256) bnd_vjp_node(int nvn, int nshg, const int *__restrict__ vnode,
+                                                    const int *__restrict__ vptr, const int *__restrict__ vinc,
+                                                    const double *__restrict__ vrec, double *__restrict__ ybar) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nvn) return;
+  const int i = vnode[b];
+  double r[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r[k] = ybar[i + (size_t)nshg * k];
+  for (int p = vptr[b]; p < vptr[b + 1]; ++p) {
+    const int ea = vinc[p];
+    const double *v = vrec + (size_t)(ea >> 2) * kBV + 4 * (ea & 3);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = r[k] + v[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) ybar[i + (size_t)nshg * k] = r[k];
+}
+
+// sum of the boundary elements' mu cotangents, one workgroup, fixed order
+__global__ void __launch_bounds__(256) bnd_vjp_mu(int nelb, const double *__restrict__ vrec,
+                                                  double *__restrict__ out) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int e = threadIdx.x; e < nelb; e += 256) s += vrec[(size_t)e * kBV + 16];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
 }  // namespace
 
 namespace crimson {
@@ -223,6 +422,24 @@ int form_boundary_pass(crimson_form *f, double *d_res, int mode) {
     bnd_node<1><<<grid_for(f->nbnode, 256), 256, 0, s>>>(f->nbnode, f->nshg, f->d_bnode.p, f->d_binc_ptr.p,
                                                          f->d_binc.p, f->d_brec.p, d_res);
   CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+int form_boundary_vjp(crimson_form *f, const double *d_w, double *d_ybar, double *mubar) {
+  *mubar = 0.0;
+  if (f->nelb == 0) return CRIMSON_OK;
+  hipStream_t s = f->stream;
+  CR_TRY(f->d_bvrec.alloc((size_t)kBV * f->nelb + 1));
+  bnd_vjp_elem<<<grid_for(f->nelb, 256), 256, 0, s>>>(f->nelb, f->nshg, f->d_ienb.p, f->d_bcode.p,
+                                                      f->nsrf >= 0 ? f->d_permit.p : nullptr, f->nsrf, f->nqb,
+                                                      f->influx, f->prm.rho, f->prm.mu, f->d_node.p, d_w,
+                                                      f->d_bvrec.p);
+  bnd_vjp_node<<<grid_for(f->nvnode, 256), 256, 0, s>>>(f->nvnode, f->nshg, f->d_vnode.p, f->d_vptr.p,
+                                                        f->d_vinc.p, f->d_bvrec.p, d_ybar);
+  double *d_mu = f->d_bvrec.p + (size_t)kBV * f->nelb;
+  bnd_vjp_mu<<<1, 256, 0, s>>>(f->nelb, f->d_bvrec.p, d_mu);
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipMemcpyAsync(mubar, d_mu, sizeof(double), hipMemcpyDeviceToHost, s));
+  CR_HIP(hipStreamSynchronize(s));
   return CRIMSON_OK;
 }
 }  // namespace crimson
@@ -276,13 +493,32 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
       const int b = pos[ib[(size_t)a * nelb + e]];
       binc[fill[b]++] = e << 2 | a;
     }
+  // the adjoint's incidences: all four nodes of each boundary element
+  std::vector<int> vcnt(f->nshg, 0);
+  for (int e = 0; e < nelb; ++e)
+    for (int a = 0; a < 4; ++a) vcnt[ib[(size_t)a * nelb + e]]++;
+  std::vector<int> vnode, vptr(1, 0), vpos(f->nshg, -1);
+  for (int i = 0; i < f->nshg; ++i)
+    if (vcnt[i]) {
+      vpos[i] = (int)vnode.size();
+      vnode.push_back(i);
+      vptr.push_back(vptr.back() + vcnt[i]);
+    }
+  std::vector<int> vinc(vptr.back()), vfill(vptr.begin(), vptr.end() - 1);
+  for (int e = 0; e < nelb; ++e)
+    for (int a = 0; a < 4; ++a) {
+      const int b = vpos[ib[(size_t)a * nelb + e]];
+      vinc[vfill[b]++] = e << 2 | a;
+    }
   if (nelb > (1 << 29)) return crimson::set_error("crimson_form_set_boundary: too many elements"), CRIMSON_EARG;
   hipStream_t s = f->stream;
   int rc;
   if ((rc = f->d_ienb.upload(ib.data(), ib.size(), s)) || (rc = f->d_bcode.upload(code.data(), code.size(), s)) ||
       (rc = f->d_bcb.upload(bcb.data(), bcb.size(), s)) || (rc = f->d_brec.alloc((size_t)12 * nelb)) ||
       (rc = f->d_bnode.upload(bnode.data(), bnode.size(), s)) ||
-      (rc = f->d_binc_ptr.upload(bptr.data(), bptr.size(), s)) || (rc = f->d_binc.upload(binc.data(), binc.size(), s)))
+      (rc = f->d_binc_ptr.upload(bptr.data(), bptr.size(), s)) || (rc = f->d_binc.upload(binc.data(), binc.size(), s)) ||
+      (rc = f->d_vnode.upload(vnode.data(), vnode.size(), s)) || (rc = f->d_vptr.upload(vptr.data(), vptr.size(), s)) ||
+      (rc = f->d_vinc.upload(vinc.data(), vinc.size(), s)))
     return rc;
   if (permit) {
     if ((rc = f->d_permit.upload(permit, (size_t)nsrf + 1, s))) return rc;
@@ -290,6 +526,7 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
   CR_HIP(hipStreamSynchronize(s));
   f->nelb = nelb;
   f->nbnode = (int)bnode.size();
+  f->nvnode = (int)vnode.size();
   f->nqb = nqb;
   f->influx = influx;
   f->nsrf = permit ? nsrf : -1;

@@ -170,6 +170,9 @@ struct crimson_form {
   crimson::DBuf<int> d_permit;    // flow-permitted flag by srfID, 0..nsrf
   crimson::DBuf<double> d_brec;   // (12, nelb): rl of the three face nodes
   crimson::DBuf<int> d_bnode, d_binc_ptr, d_binc;  // boundary nodes; their incidences e << 2 | a
+  int nvnode = 0;                                  // the adjoint's: all four nodes of each element
+  crimson::DBuf<int> d_vnode, d_vptr, d_vinc;
+  crimson::DBuf<double> d_bvrec;                   // boundary VJP records
 };
 
 namespace crimson {
@@ -179,4 +182,7 @@ int form_flux_pass(crimson_form *f, const double *d_y, const double *d_ac, const
 // 1: Res4(4,nshg)) from the node records of the current formation; a no-op
 // without boundary elements.  Enqueued on the handle's stream.
 int form_boundary_pass(crimson_form *f, double *d_res, int mode);
+// Adjoint of the boundary rows: ybar(nshg,4) += w^T d rl_bnd/d y from the
+// node records of the last flux pass; *mubar = w^T d rl_bnd/d mu.
+int form_boundary_vjp(crimson_form *f, const double *d_w, double *d_ybar, double *mubar);
 }  // namespace crimson

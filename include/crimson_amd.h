@@ -127,8 +127,9 @@ int crimson_form_timing(const crimson_form *f, double *out4);
  * srfID (callCPPDiscoverWhetherFlowPermittedAcrossSurface, e3b.f90; NULL: all
  * permitted).  nelb = 0 removes the boundary.  Deformable walls, the heart
  * model, Lagrange multipliers and the surface flux integrals (flxID) are not
- * formed.  CRIMSON_EUNSUPPORTED with ensemble instances; the VJP
- * (crimson_form_vjp_dev) rejects a handle with boundary elements. */
+ * formed.  CRIMSON_EUNSUPPORTED with ensemble instances.  The VJP
+ * (crimson_form_vjp_dev) of a handle with boundary elements includes their
+ * rows. */
 int crimson_form_set_boundary(crimson_form *f, int nelb, const int *ienb,
                               const int *iBCB, const double *BCB, int ndBCB,
                               int nqb, double influx, const int *permit,
