@@ -111,6 +111,20 @@ constexpr int kNarrowLanes = 4;
 // writes t1 = W_u o (G Ps) and t1s = W_u o t1 (the operand of Gt).  Same
 // products in the same order as the unscaled kernels.
 
+// Wave-level sum (64 lanes) then across the 4 waves of a 256-thread group.
+template <class T>
+__device__ inline T block_sum(T v, T *sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  T t = T(0);
+  if (threadIdx.x == 0) t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return t;
+}
+
 // Entries per lane loaded ahead in the narrow row kernels: the column
 // indices of kNarrowUnroll entries (k, k + 4, ...) are loaded together, then
 // the gathered vector entries and matrix values, so a lane waits for one
@@ -240,19 +254,6 @@ __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restri
   }
 }
 
-// Wave-level sum (64 lanes) then across the 4 waves of a 256-thread group.
-template <class T>
-__device__ inline T block_sum(T v, T *sh) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  const int w = threadIdx.x / 64;
-  if ((threadIdx.x & 63) == 0) sh[w] = v;
-  __syncthreads();
-  T t = T(0);
-  if (threadIdx.x == 0) t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-  __syncthreads();
-  return t;
-}
 
 // partial[blk][j] = sum over this block's chunk of U_j . v for j < m, and
 // partial[blk][m] = v . v when self != 0.  U_j = U0 + j * ustride.  Vectors
@@ -465,14 +466,16 @@ struct FaceDev {
   double coef;
   int nNo, dof;
 };
-// ADDBCMUL's PRE on the Schur CG's G product t1 (3, N), for every coupled
-// face, in one 1024-thread workgroup: all face dots first (face_dot_kernel's
-// sums, in its order), then t1 += coef S valM face by face (face_axpy_kernel),
-// then t1s = W_u o t1 at the face nodes (the operand of Gt).
+// ADDBCMUL's PRE on the Schur CG's G product t1 (3, N): first every coupled
+// face's dot S_f = valM_f . t1 (face_dot_kernel's sums, in its order) in one
+// 1024-thread workgroup, all before any update (PRE reads the unmodified
+// vector), then face by face t1 += coef_f S_f valM_f with t1s = W_u o t1
+// rewritten at the same nodes (face_pre_apply_kernel, one launch per face
+// since faces may share nodes).
 template <class T>
-__global__ void __launch_bounds__(1024) face_pre_kernel(int nf, const FaceDev *__restrict__ fd, T *__restrict__ t1,
-                                                        T *__restrict__ t1s, const T *__restrict__ W, int wdof,
-                                                        double *__restrict__ fsum, const double *__restrict__ done) {
+__global__ void __launch_bounds__(1024) face_pre_dot_kernel(int nf, const FaceDev *__restrict__ fd,
+                                                            const T *__restrict__ t1, double *__restrict__ fsum,
+                                                            const double *__restrict__ done) {
   if (done && *done != 0.0) return;
   __shared__ double sh[16];
   for (int f = 0; f < nf; ++f) {
@@ -492,22 +495,21 @@ __global__ void __launch_bounds__(1024) face_pre_kernel(int nf, const FaceDev *_
     }
     __syncthreads();
   }
-  for (int f = 0; f < nf; ++f) {  // faces in turn: they may share nodes
-    const FaceDev F = fd[f];
-    const int nc = F.dof < 3 ? F.dof : 3;
-    const double sc = F.coef * fsum[f];
-    for (int a = threadIdx.x; a < F.nNo; a += 1024)
-      for (int i = 0; i < nc; ++i) t1[i + (size_t)3 * F.glob[a]] += (T)(F.valM[i + (size_t)F.dof * a] * sc);
-    __syncthreads();
-  }
-  for (int f = 0; f < nf; ++f) {
-    const FaceDev F = fd[f];
-    for (int a = threadIdx.x; a < F.nNo; a += 1024) {
-      const size_t g = F.glob[a];
+}
+template <class T>
+__global__ void __launch_bounds__(256) face_pre_apply_kernel(FaceDev F, const double *__restrict__ S,
+                                                             T *__restrict__ t1, T *__restrict__ t1s,
+                                                             const T *__restrict__ W, int wdof,
+                                                             const double *__restrict__ done) {
+  if (done && *done != 0.0) return;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= F.nNo) return;
+  const int nc = F.dof < 3 ? F.dof : 3;
+  const double sc = F.coef * *S;
+  const size_t g = F.glob[a];
+  for (int i = 0; i < nc; ++i) t1[i + 3 * g] += (T)(F.valM[i + (size_t)F.dof * a] * sc);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) t1s[c + 3 * g] = W[c + (size_t)wdof * g] * t1[c + 3 * g];
-    }
-  }
+  for (int c = 0; c < 3; ++c) t1s[c + 3 * g] = W[c + (size_t)wdof * g] * t1[c + 3 * g];
 }
 // DGKS decision: gate = ||w'||^2 < 0.5 ||w||^2
 __global__ void dgks_kernel(const double *__restrict__ n0, const double *__restrict__ n1, int *__restrict__ gate) {
@@ -1041,7 +1043,7 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
     for (int a = 0; a < fa.nNo; ++a)
       for (int i = 0; i < nc; ++i) fa.nS = fa.nS + vm[i + (size_t)fa.dof * a] * vm[i + (size_t)fa.dof * a];
   }
-  // the coupled faces with their PRE coefficients, for face_pre_kernel
+  // the coupled faces with their PRE coefficients, for the face_pre kernels
   c.hfdev.clear();
   for (auto &fa : l->face)
     if (fa.coupled)
@@ -1058,8 +1060,12 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
     const double *done = scal + 5;
     const unsigned gr = crimson::grid_for((size_t)N * kNarrowLanes, 256);
     g_product_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2.p, t1.p, ws.tmp.p, done);
-    if (nfc > 0)
-      face_pre_kernel<T><<<1, 1024, 0, c.s>>>(nfc, c.fdev.p, t1.p, ws.tmp.p, c.W.p, 4, c.fsum.p, done);
+    if (nfc > 0) {
+      face_pre_dot_kernel<T><<<1, 1024, 0, c.s>>>(nfc, c.fdev.p, t1.p, c.fsum.p, done);
+      for (int f = 0; f < nfc; ++f)
+        face_pre_apply_kernel<T><<<crimson::grid_for(c.hfdev[f].nNo, 256), 256, 0, c.s>>>(
+            c.hfdev[f], c.fsum.p + f, t1.p, ws.tmp.p, c.W.p, 4, done);
+    }
     schur_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2.p, ws.tmp.p, SPv, done);
     CR_HIP(hipGetLastError());
     return CRIMSON_OK;
