@@ -41,7 +41,7 @@ EXPORTS = (
     "crimson_lhs_spmv_dev", "crimson_itr_predict_dev", "crimson_itr_yalpha_dev", "crimson_itr_correct_dev",
     "crimson_itr_bc_dev", "crimson_elmpvsq_dev", "crimson_form_sync", "crimson_lhs_set_precision",
     "crimson_form_vjp_dev", "crimson_form_set_instances", "crimson_form_graph_create", "crimson_graph_launch",
-    "crimson_graph_free", "crimson_form_set_boundary",
+    "crimson_graph_free", "crimson_form_set_boundary", "crimson_form_autotune",
 )
 
 
@@ -112,6 +112,7 @@ def lib():
         L.crimson_elmpvsq_dev.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp, vp, vp]
         L.crimson_form_sync.argtypes = [vp]
         L.crimson_form_set_boundary.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_int]
+        L.crimson_form_autotune.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         _lib = L
     return _lib
 
@@ -257,6 +258,21 @@ class ElmGMRInterior:
                                               int(BCB.shape[2]), int(nqb), float(influx), _p(perm),
                                               -1 if perm is None else int(perm.size - 1)),
               "crimson_form_set_boundary")
+
+    AUTOTUNE_CANDIDATES = ("natural/16", "morton/16", "natural/8", "morton/8")
+
+    def autotune(self, y, ac, x, res4, lhs_handle, reps=3, only=None):
+        """Time each launch configuration of to_lhs on these device arrays
+        and keep the fastest (row schedule x assembly chunk); returns
+        (chosen name, {name: mean assembly ms}).  ``only`` restricts the
+        candidates to these names.  Results do not depend on the choice."""
+        mask = 0 if only is None else sum(1 << self.AUTOTUNE_CANDIDATES.index(n) for n in only)
+        choice = C.c_int()
+        ms = (C.c_double * 4)(*([float("nan")] * 4))
+        check(lib().crimson_form_autotune(self.h, _p(y), _p(ac), _p(x), _p(res4), lhs_handle.h, int(reps),
+                                          int(mask), C.byref(choice), ms), "crimson_form_autotune")
+        times = {n: ms[i] for i, n in enumerate(self.AUTOTUNE_CANDIDATES) if ms[i] == ms[i]}
+        return self.AUTOTUNE_CANDIDATES[choice.value], times
 
     def graph_to_lhs(self, y, ac, x, res4, lhs_handle):
         """Capture to_lhs on these device arrays into a HIP graph; returns a

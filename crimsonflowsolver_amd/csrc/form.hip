@@ -791,14 +791,14 @@ __global__ void slot_table_kernel(int n, const int *__restrict__ order, const in
 // Every block of the row is written, so blocks no element touches come out
 // zero (the reference zeroes lhsK/lhsP first, elmgmr.f90:127-130).
 constexpr int kRowsPerBlock = 256 / kRowLanes;
-constexpr int kAsmChunk = 16;  // incidences staged per chunk (a multiple of 2, at most kRowLanes)
+constexpr int kAsmChunk = 16;  // incidences staged per chunk (even, dividing kRowLanes)
 template <int MODE, int CH = kAsmChunk>
 __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict__ tab, const int *__restrict__ inc,
                                                const int *__restrict__ lst_ptr,
                                                const unsigned short *__restrict__ lst,
                                                const double *__restrict__ rec, double *__restrict__ res,
                                                AsmOut o) {
-  static_assert(CH % 2 == 0 && CH <= kRowLanes, "chunk of 2..16 incidences");
+  static_assert(CH % 2 == 0 && kRowLanes % CH == 0, "chunk of 2, 4, 8 or 16 incidences");
   __shared__ double st_all[kRowsPerBlock][CH][kStage];
   const int gid = xcd_block() * blockDim.x + threadIdx.x;
   const int slot = gid / kRowLanes, lane = gid % kRowLanes;
@@ -836,9 +836,14 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict_
     lst_fill(lb, lst, q, q1);
     for (int c = 0; c < nchunk; ++c) {
       const int tb = c * CH;
+      // incidence t of the row is held by lane t % 16 whatever the chunk
+      // size, so that every chunk size sums the diagonal block and the
+      // residual row in the same order: chunk c occupies lanes lo..lo+CH-1
+      const int lo = CH == kRowLanes ? 0 : tb % kRowLanes;
+      const int lon = CH == kRowLanes ? 0 : (tb + CH) % kRowLanes;
       // the next chunk's incidences, in flight while this chunk is staged
       const int ntn = min(CH, ninc - tb - CH);
-      const int ean = lane < ntn ? inc[p0 + tb + CH + lane] : 0;
+      const int ean = lane >= lon && lane - lon < ntn ? inc[p0 + tb + CH + lane - lon] : 0;
       wave_lds_sync();  // the previous chunk's readers are done
       // line 0 (8 pieces of 16 bytes per record) and node a's S row.  No
       // guard on the loads, so that they are all in flight together: lanes
@@ -848,7 +853,7 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict_
 #pragma unroll
       for (int j = 0; j < CH * 8 / kRowLanes; ++j) {
         const int t = (j * kRowLanes + lane) >> 3, part = lane & 7;
-        const int et = __shfl(ea, t, kRowLanes);
+        const int et = __shfl(ea, lo + t, kRowLanes);
         pc[j] = *reinterpret_cast<const double2 *>(rec + (size_t)(et >> 2) * kRec + 2 * part);
       }
       const double *ra = rec + (size_t)(ea >> 2) * kRec + R_A(ea & 3);
@@ -860,22 +865,22 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict_
         st[t][2 * part] = pc[j].x;
         st[t][2 * part + 1] = pc[j].y;
       }
-      if (lane < CH) {
-        st[lane][ST_S + 0] = s01.x;
-        st[lane][ST_S + 1] = s01.y;
-        st[lane][ST_S + 2] = s23.x;
-        st[lane][ST_S + 3] = s23.y;
+      if (lane >= lo && lane < lo + CH) {
+        st[lane - lo][ST_S + 0] = s01.x;
+        st[lane - lo][ST_S + 1] = s01.y;
+        st[lane - lo][ST_S + 2] = s23.x;
+        st[lane - lo][ST_S + 3] = s23.y;
       }
       wave_lds_sync();
       // the incidence's own lane: diagonal contribution, and its residual
       // row, loaded here and added after the off-diagonal blocks so that the
       // load is not held in registers while the chunk is staged
-      const bool dl = g == 0 && lane < nt;
+      const bool dl = g == 0 && lane >= lo && lane - lo < nt;
       double2 rl01 = make_double2(0.0, 0.0), rl23 = rl01;
       if (dl) {
         rl01 = *reinterpret_cast<const double2 *>(ra + 4);
         rl23 = *reinterpret_cast<const double2 *>(ra + 6);
-        diag_add(D, st[lane], ea & 3);
+        diag_add(D, st[lane - lo], ea & 3);
       }
       while (q < q1) {
         if (lb.n == 0) lst_fill(lb, lst, q, q1);  // blocks with more than 8 contributions
@@ -1171,11 +1176,19 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
     else
       res_node<1><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
   } else if (mode == 0) {
-    asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p,
-                                f->d_rec.p, d_res, o);
+    if (f->asm_chunk == 8)
+      asm_row<0, 8><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
+                                     d_res, o);
+    else
+      asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
+                                  d_res, o);
   } else {
-    asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p,
-                                f->d_rec.p, d_res, o);
+    if (f->asm_chunk == 8)
+      asm_row<1, 8><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
+                                     d_res, o);
+    else
+      asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
+                                  d_res, o);
   }
   CR_HIP(hipGetLastError());
   if (timed) CR_HIP(hipEventRecord(f->ev[3], s));
@@ -1466,6 +1479,58 @@ extern "C" int crimson_form_interior_to_lhs(crimson_form *f, const double *d_y, 
   lhs->prec = 64;
   AsmOut o{nullptr, nullptr, lhs->d_V.p, nullptr, (size_t)f->nnz};
   return run_formation(f, d_y, d_ac, d_x, d_res, 1, 1, o, nullptr);
+}
+
+// Launch configurations of the formation chosen by timing them on the
+// caller's arrays: the assembly's row schedule (the caller's numbering or
+// Morton order) and, for the FP64 assembly, its incidence chunk (16 or 8).
+// No candidate changes a result (the schedule orders rows, never sums).
+extern "C" int crimson_form_autotune(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
+                                     double *d_res, crimson_lhs *lhs, int reps, int mask, int *choice,
+                                     double *ms_out) {
+  if (!f || !lhs || !d_y || !d_ac || !d_x || !d_res || reps < 1) {
+    crimson::set_error("crimson_form_autotune: invalid arguments");
+    return CRIMSON_EARG;
+  }
+  const bool fp32 = f->prm.precision == 32;
+  const int ncand = fp32 ? 2 : 4;
+  if (mask == 0) mask = (1 << ncand) - 1;
+  if (mask & ~((1 << ncand) - 1)) {
+    crimson::set_error("crimson_form_autotune: candidate mask names a configuration this handle lacks");
+    return CRIMSON_EARG;
+  }
+  double best = 1e300;
+  int bestc = -1;
+  for (int cand = 0; cand < ncand; ++cand) {
+    if (!(mask & (1 << cand))) continue;
+    const bool natural = (cand & 1) == 0;
+    f->asm_chunk = (cand & 2) ? 8 : 16;
+    if (f->natural_sched != natural || !f->d_order.p) {
+      f->natural_sched = natural;
+      f->d_order.release();
+      CR_TRY(ensure_order(f, d_x));
+    }
+    double t = 0.0;
+    for (int r = 0; r <= reps; ++r) {  // the first call is a warm-up
+      CR_TRY(crimson_form_interior_to_lhs(f, d_y, d_ac, d_x, d_res, lhs));
+      if (r > 0) t += f->ms[2];
+    }
+    t /= reps;
+    if (ms_out) ms_out[cand] = t;
+    if (bestc < 0 || t < best) {
+      best = t;
+      bestc = cand;
+    }
+  }
+  f->asm_chunk = (bestc & 2) ? 8 : 16;
+  const bool natural = (bestc & 1) == 0;
+  if (f->natural_sched != natural) {
+    f->natural_sched = natural;
+    f->d_order.release();
+    CR_TRY(ensure_order(f, d_x));
+  }
+  if (choice) *choice = bestc;
+  return CRIMSON_OK;
 }
 
 extern "C" int crimson_form_set_instances(crimson_form *f, const unsigned short *inst, int ninst, const double *mu) {

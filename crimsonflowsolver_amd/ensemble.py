@@ -72,9 +72,12 @@ class Ensemble:
         p = self.params
         return np.concatenate([[0.0], self.R * p.alfi * p.gami * p.Delt])
 
-    def capture(self, y, ac, x, res4):
+    def capture(self, y, ac, x, res4, autotune=True):
         """Capture the union formation on these device arrays (torch
-        tensors in the C ABI layouts) into a HIP graph."""
+        tensors in the C ABI layouts) into a HIP graph, after timing its
+        launch configurations on them and keeping the fastest
+        (ElmGMRInterior.autotune; self.tuned holds the choice and times)."""
+        self.tuned = self.form.autotune(y, ac, x, res4, self.lhs) if autotune else None
         self.graph = self.form.graph_to_lhs(y, ac, x, res4, self.lhs)
         return self.graph
 

@@ -135,6 +135,20 @@ int crimson_form_set_boundary(crimson_form *f, int nelb, const int *ienb,
                               int nqb, double influx, const int *permit,
                               int nsrf);
 
+/* Autotuner (no counterpart in the reference): times the formation into
+ * the memLS handle `lhs` (crimson_form_interior_to_lhs, reps calls after one
+ * warm-up per candidate, assembly phase from HIP events) for each launch
+ * configuration and keeps the fastest for the handle's later calls:
+ * candidate bit 0 = Morton row schedule (else the caller's numbering), bit
+ * 1 = 8-incidence assembly chunks (else 16; FP64 handles only).  mask
+ * selects the candidates tried (bit c = candidate c; 0 = all).  *choice
+ * receives the winner, ms_out (4 doubles, may be NULL) the mean assembly
+ * milliseconds of each candidate tried.  No configuration changes a
+ * result; the matrix and res hold the last candidate's formation. */
+int crimson_form_autotune(crimson_form *f, const double *d_y, const double *d_ac,
+                          const double *d_x, double *d_res, crimson_lhs *lhs,
+                          int reps, int mask, int *choice, double *ms_out);
+
 /* Ensembles (no counterpart in the reference, which runs one case per job):
  * the handle's mesh is the union of several instances' meshes (disjoint
  * node ranges), inst(nel) gives each element's 0-based instance and mu(ninst)

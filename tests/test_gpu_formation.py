@@ -211,3 +211,48 @@ def test_formation_scattered_numbering(cfa, oracle_lib):
     torch.cuda.synchronize()
     assert _rel(lhs.get_val(4), oracle_lib.pack_val4(colm, rowp, rK, rP)) <= TOL
     assert _rel(tres4.cpu().numpy(), rres) <= TOL
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_autotune_candidates_bitwise(cfa, oracle_lib, precision):
+    """Every launch configuration the autotuner may pick (row schedule x
+    assembly chunk) forms bitwise the same residual and memLS matrix, on a
+    jittered Delaunay mesh whose rows take 1-3 chunks of 16 and 1-6 of 8
+    (ragged last chunks); the FP64 ones match the oracle.  A full run
+    returns a candidate and a time for each one tried."""
+    import torch
+    from crimsonflowsolver_amd import mesh as M
+    m = M.jittered_delaunay_mesh(7, 7, 12, seed=19)
+    y, ac = M.poiseuille_state(m, seed=19, noise=0.2)
+    rng = np.random.default_rng(19)
+    ac[:, :3] = rng.uniform(-50, 50, size=(m.numnp, 3))
+    nin = np.bincount(m.ien.ravel())[1:]
+    assert nin.max() > 32 and (nin % 8).any() and (nin % 16).any()
+    colm, rowp = cfa.genadj(m.ien, m.numnp)
+    form = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp, params=cfa.form_params(precision=precision))
+    lhs = cfa.memLS_LHS_CREATE(m.numnp, m.numnp, rowp.size, None, colm, rowp, 1)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+    ty, tac, tx = T(y), T(ac), T(m.x)
+    tres4 = torch.zeros((m.numnp, 4), dtype=torch.float64, device="cuda")
+    names = cfa.ElmGMRInterior.AUTOTUNE_CANDIDATES[:4 if precision == 64 else 2]
+    outs = []
+    for name in names:
+        chosen, times = form.autotune(ty, tac, tx, tres4, lhs, reps=1, only=[name])
+        assert chosen == name and set(times) == {name}
+        tres4.zero_()
+        form.to_lhs(ty, tac, tx, tres4, lhs)
+        torch.cuda.synchronize()
+        outs.append((name, tres4.cpu().numpy(), lhs.get_val(4)))
+    for name, r, v in outs[1:]:
+        assert np.array_equal(r, outs[0][1]), name
+        assert np.array_equal(v, outs[0][2]), name
+    rres, rK, rP = oracle_lib.elmgmr_interior(m.ien, y, ac, m.x, colm, rowp, oracle_lib.make_params())
+    assert _rel(outs[0][1], rres) <= TOL
+    if precision == 64:
+        assert _rel(outs[0][2], oracle_lib.pack_val4(colm, rowp, rK, rP)) <= TOL
+    chosen, times = form.autotune(ty, tac, tx, tres4, lhs, reps=2)
+    print("autotune", precision, chosen, times)
+    assert chosen in names and set(times) == set(names) and all(t > 0 for t in times.values())
+    if precision == 32:  # the FP32 assembly has no chunk choice
+        with pytest.raises(cfa.CrimsonError):
+            form.autotune(ty, tac, tx, tres4, lhs, only=["natural/8"])
