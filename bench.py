@@ -457,7 +457,7 @@ def main():
         # off-diagonal contribution, 12 per element) indices
         byt = nel * 48 * 8 + nnz * 16 * 8 + N * 4 * 8 + nel * 4 * 4 + nel * 12 * 2
         ach = byt / (tm[2] * 1e-3) / 1e9
-        kname = "asm_row<1,8>" if tuned is not None and tuned["fp64"][0].endswith("/8") else "asm_row<1,16>"
+        kname = "asm_row<1,8>" if tuned is not None and tuned["fp64"][0].split("/")[1] == "8" else "asm_row<1,16>"
         roof = {"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": byt,
                 "traffic": pmc_traffic(grid, kname)}

@@ -259,7 +259,9 @@ class ElmGMRInterior:
                                               -1 if perm is None else int(perm.size - 1)),
               "crimson_form_set_boundary")
 
-    AUTOTUNE_CANDIDATES = ("natural/16", "morton/16", "natural/8", "morton/8")
+    # row schedule / FP64 incidence chunk / XCD dealing (one range per XCD, or turns of 64 workgroups)
+    AUTOTUNE_CANDIDATES = tuple(f"{s}/{c}/{d}" for d in ("range", "turns64") for c in (16, 8)
+                                for s in ("natural", "morton"))
 
     def autotune(self, y, ac, x, res4, lhs_handle, reps=3, only=None):
         """Time each launch configuration of to_lhs on these device arrays
@@ -268,7 +270,7 @@ class ElmGMRInterior:
         candidates to these names.  Results do not depend on the choice."""
         mask = 0 if only is None else sum(1 << self.AUTOTUNE_CANDIDATES.index(n) for n in only)
         choice = C.c_int()
-        ms = (C.c_double * 4)(*([float("nan")] * 4))
+        ms = (C.c_double * 8)(*([float("nan")] * 8))
         check(lib().crimson_form_autotune(self.h, _p(y), _p(ac), _p(x), _p(res4), lhs_handle.h, int(reps),
                                           int(mask), C.byref(choice), ms), "crimson_form_autotune")
         times = {n: ms[i] for i, n in enumerate(self.AUTOTUNE_CANDIDATES) if ms[i] == ms[i]}

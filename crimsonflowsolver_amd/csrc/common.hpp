@@ -91,6 +91,20 @@ __device__ inline unsigned xcd_block() {
   return (b % 8u) * per + b / 8u;
 }
 
+// The same with the workgroups dealt to the XCDs in turns of `chunk`
+// consecutive ones (XCD x takes chunks x, x + 8, x + 16, ...), so that the
+// XCDs work on neighbouring ranges at the same time and data shared by
+// items far apart in the order (a layer of a structured numbering) is found
+// again in the shared MALL instead of HBM.  chunk 0: xcd_block().
+__device__ inline unsigned xcd_block(unsigned chunk) {
+  if (chunk == 0) return xcd_block();
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned full = nb / (8u * chunk) * (8u * chunk);
+  if (b >= full) return b;
+  const unsigned x = b % 8u, k = b / 8u;  // the k-th workgroup dealt to XCD x
+  return ((k / chunk) * 8u + x) * chunk + k % chunk;
+}
+
 // Number of workgroups of `threads` covering n items.
 inline unsigned grid_for(size_t n, unsigned threads) {
   return (unsigned)((n + threads - 1) / threads);

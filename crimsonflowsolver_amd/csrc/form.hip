@@ -797,10 +797,10 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict_
                                                const int *__restrict__ lst_ptr,
                                                const unsigned short *__restrict__ lst,
                                                const double *__restrict__ rec, double *__restrict__ res,
-                                               AsmOut o) {
+                                               AsmOut o, unsigned xc) {
   static_assert(CH % 2 == 0 && kRowLanes % CH == 0, "chunk of 2, 4, 8 or 16 incidences");
   __shared__ double st_all[kRowsPerBlock][CH][kStage];
-  const int gid = xcd_block() * blockDim.x + threadIdx.x;
+  const int gid = xcd_block(xc) * blockDim.x + threadIdx.x;
   const int slot = gid / kRowLanes, lane = gid % kRowLanes;
   const bool live = slot < nshg;
   double(*st)[kStage] = st_all[threadIdx.x / kRowLanes];
@@ -922,9 +922,9 @@ template <int MODE>
 __global__ void __launch_bounds__(256) asm_row32(int nshg, const int4 *__restrict__ tab,
                                                  const int *__restrict__ inc, const int *__restrict__ lst_ptr,
                                                  const unsigned short *__restrict__ lst,
-                                                 const float *__restrict__ rec, AsmOut o) {
+                                                 const float *__restrict__ rec, AsmOut o, unsigned xc) {
   __shared__ float st_all[kRowsPerBlock][kRowLanes][kStage];
-  const int gid = xcd_block() * blockDim.x + threadIdx.x;
+  const int gid = xcd_block(xc) * blockDim.x + threadIdx.x;
   const int slot = gid / kRowLanes, lane = gid % kRowLanes;
   const bool live = slot < nshg;
   float(*st)[kStage] = st_all[threadIdx.x / kRowLanes];
@@ -1164,11 +1164,11 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
     if (mode == 0) {
       res_node<0><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
       asm_row32<0><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p,
-                                    f->d_rec32.p, o);
+                                    f->d_rec32.p, o, f->asm_xcd);
     } else {
       res_node<1><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
       asm_row32<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p,
-                                    f->d_rec32.p, o);
+                                    f->d_rec32.p, o, f->asm_xcd);
     }
   } else if (!lhs) {
     if (mode == 0)
@@ -1178,17 +1178,17 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
   } else if (mode == 0) {
     if (f->asm_chunk == 8)
       asm_row<0, 8><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
-                                     d_res, o);
+                                     d_res, o, f->asm_xcd);
     else
       asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
-                                  d_res, o);
+                                  d_res, o, f->asm_xcd);
   } else {
     if (f->asm_chunk == 8)
       asm_row<1, 8><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
-                                     d_res, o);
+                                     d_res, o, f->asm_xcd);
     else
       asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
-                                  d_res, o);
+                                  d_res, o, f->asm_xcd);
   }
   CR_HIP(hipGetLastError());
   if (timed) CR_HIP(hipEventRecord(f->ev[3], s));
@@ -1483,8 +1483,12 @@ extern "C" int crimson_form_interior_to_lhs(crimson_form *f, const double *d_y, 
 
 // Launch configurations of the formation chosen by timing them on the
 // caller's arrays: the assembly's row schedule (the caller's numbering or
-// Morton order) and, for the FP64 assembly, its incidence chunk (16 or 8).
-// No candidate changes a result (the schedule orders rows, never sums).
+// Morton order), its workgroups' dealing to the XCDs (a contiguous range
+// each, or turns of kAsmXcdChunk) and, for the FP64 assembly, its incidence
+// chunk (16 or 8).  No candidate changes a result: schedule and dealing
+// order rows, never sums, and both chunk sizes give incidence t to lane
+// t % 16.
+constexpr unsigned kAsmXcdChunk = 64;
 extern "C" int crimson_form_autotune(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
                                      double *d_res, crimson_lhs *lhs, int reps, int mask, int *choice,
                                      double *ms_out) {
@@ -1493,9 +1497,9 @@ extern "C" int crimson_form_autotune(crimson_form *f, const double *d_y, const d
     return CRIMSON_EARG;
   }
   const bool fp32 = f->prm.precision == 32;
-  const int ncand = fp32 ? 2 : 4;
-  if (mask == 0) mask = (1 << ncand) - 1;
-  if (mask & ~((1 << ncand) - 1)) {
+  const int ncand = 8, valid = fp32 ? 0x33 : 0xff;  // the FP32 assembly has no chunk choice
+  if (mask == 0) mask = valid;
+  if (mask & ~valid) {
     crimson::set_error("crimson_form_autotune: candidate mask names a configuration this handle lacks");
     return CRIMSON_EARG;
   }
@@ -1505,6 +1509,7 @@ extern "C" int crimson_form_autotune(crimson_form *f, const double *d_y, const d
     if (!(mask & (1 << cand))) continue;
     const bool natural = (cand & 1) == 0;
     f->asm_chunk = (cand & 2) ? 8 : 16;
+    f->asm_xcd = (cand & 4) ? kAsmXcdChunk : 0;
     if (f->natural_sched != natural || !f->d_order.p) {
       f->natural_sched = natural;
       f->d_order.release();
@@ -1523,6 +1528,7 @@ extern "C" int crimson_form_autotune(crimson_form *f, const double *d_y, const d
     }
   }
   f->asm_chunk = (bestc & 2) ? 8 : 16;
+  f->asm_xcd = (bestc & 4) ? kAsmXcdChunk : 0;
   const bool natural = (bestc & 1) == 0;
   if (f->natural_sched != natural) {
     f->natural_sched = natural;

@@ -148,6 +148,7 @@ struct crimson_form {
   crimson::DBuf<int4> d_slot;              // per schedule slot: the row's offsets (slot_table_kernel)
   bool natural_sched = false;               // schedule = the caller's numbering (set at setup)
   int asm_chunk = 16;                       // incidences per staged chunk of the FP64 assembly
+  unsigned asm_xcd = 0;                     // assembly workgroups dealt per XCD turn (0: one range each)
   crimson::DBuf<unsigned short> d_lst;
   crimson::DBuf<double> d_frec, d_node, d_q, d_rec;
   crimson::DBuf<float> d_rec32;  // FP32 tangent records (precision 32)

@@ -216,10 +216,12 @@ def test_formation_scattered_numbering(cfa, oracle_lib):
 @pytest.mark.parametrize("precision", [64, 32])
 def test_autotune_candidates_bitwise(cfa, oracle_lib, precision):
     """Every launch configuration the autotuner may pick (row schedule x
-    assembly chunk) forms bitwise the same residual and memLS matrix, on a
-    jittered Delaunay mesh whose rows take 1-3 chunks of 16 and 1-6 of 8
-    (ragged last chunks); the FP64 ones match the oracle.  A full run
-    returns a candidate and a time for each one tried."""
+    assembly chunk x XCD dealing) forms bitwise the same residual and memLS
+    matrix, on a jittered Delaunay mesh whose rows take 1-3 chunks of 16 and
+    1-6 of 8 (ragged last chunks), and on a cylinder of 1,100+ assembly
+    workgroups (so that dealing in turns of 64 remaps all but the last
+    partial round); the FP64 ones match the oracle.  A full run returns a
+    candidate and a time for each one tried."""
     import torch
     from crimsonflowsolver_amd import mesh as M
     m = M.jittered_delaunay_mesh(7, 7, 12, seed=19)
@@ -234,7 +236,7 @@ def test_autotune_candidates_bitwise(cfa, oracle_lib, precision):
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
     ty, tac, tx = T(y), T(ac), T(m.x)
     tres4 = torch.zeros((m.numnp, 4), dtype=torch.float64, device="cuda")
-    names = cfa.ElmGMRInterior.AUTOTUNE_CANDIDATES[:4 if precision == 64 else 2]
+    names = [n for i, n in enumerate(cfa.ElmGMRInterior.AUTOTUNE_CANDIDATES) if precision == 64 or not i & 2]
     outs = []
     for name in names:
         chosen, times = form.autotune(ty, tac, tx, tres4, lhs, reps=1, only=[name])
@@ -250,9 +252,27 @@ def test_autotune_candidates_bitwise(cfa, oracle_lib, precision):
     assert _rel(outs[0][1], rres) <= TOL
     if precision == 64:
         assert _rel(outs[0][2], oracle_lib.pack_val4(colm, rowp, rK, rP)) <= TOL
+    mc = M.cylinder_mesh(16, 16, 80)
+    assert mc.numnp * 16 // 256 > 8 * 64 * 2 and (mc.numnp * 16 // 256) % (8 * 64)
+    yc, acc = M.poiseuille_state(mc, seed=20)
+    cc, rc = cfa.genadj(mc.ien, mc.numnp)
+    fc = cfa.ElmGMRInterior(mc.ien, cc, rc, mc.numnp, params=cfa.form_params(precision=precision))
+    lc = cfa.memLS_LHS_CREATE(mc.numnp, mc.numnp, rc.size, None, cc, rc, 1)
+    tc = [T(a) for a in (yc, acc, mc.x)]
+    rc4 = torch.zeros((mc.numnp, 4), dtype=torch.float64, device="cuda")
+    cyl = []
+    for name in names:
+        fc.autotune(*tc, rc4, lc, reps=1, only=[name])
+        rc4.zero_()
+        fc.to_lhs(*tc, rc4, lc)
+        torch.cuda.synchronize()
+        cyl.append((name, rc4.cpu().numpy(), lc.get_val(4)))
+    for name, r, v in cyl[1:]:
+        assert np.array_equal(r, cyl[0][1]), name
+        assert np.array_equal(v, cyl[0][2]), name
     chosen, times = form.autotune(ty, tac, tx, tres4, lhs, reps=2)
     print("autotune", precision, chosen, times)
     assert chosen in names and set(times) == set(names) and all(t > 0 for t in times.values())
     if precision == 32:  # the FP32 assembly has no chunk choice
         with pytest.raises(cfa.CrimsonError):
-            form.autotune(ty, tac, tx, tres4, lhs, only=["natural/8"])
+            form.autotune(ty, tac, tx, tres4, lhs, only=["natural/8/range"])
