@@ -37,6 +37,13 @@ namespace {
 constexpr int kRowLanes = 16;  // lanes per CSR row in the SpMV kernels
 constexpr int kRedThreads = 256;
 constexpr int kChunk = 1024;   // vector entries per workgroup in the dot sweeps
+// Row kernels deal their workgroups to the XCDs in turns of this many
+// (crimson::xcd_block(chunk)): rows gather the vector at neighbours a mesh
+// layer away, which the shared MALL then holds for all XCDs.  One FP64
+// timestep of the 64x64x320 cylinder: 2036 ms with one contiguous range per
+// XCD, 1966 / 1918 / 1916 / 1915 / 1929 ms with turns of 4 / 8 / 16 / 32 /
+// 64 for the Schur CG's two row kernels, 1912 ms with the SpMV too.
+constexpr unsigned kRowXcdChunk = 16;
 
 struct SubLs {
   double absTol = 0, relTol = 0, iNorm = 0, fNorm = 0, dB = 0, callD = 0;
@@ -63,7 +70,7 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
                                                    const int *__restrict__ col, MatView<T> A,
                                                    const T *__restrict__ x, T *__restrict__ y, T ya,
                                                    const T *__restrict__ z, T za) {
-  const int gid = crimson::xcd_block() * blockDim.x + threadIdx.x;
+  const int gid = crimson::xcd_block(kRowXcdChunk) * blockDim.x + threadIdx.x;
   const int row = gid / LANES;
   const int lane = threadIdx.x % LANES;
   T acc[NR];
@@ -125,6 +132,19 @@ __device__ inline T block_sum(T v, T *sh) {
   return t;
 }
 
+// block_sum's value in every thread of the workgroup.
+template <class T>
+__device__ inline T block_sum_all(T v, T *sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  const T t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return t;
+}
+
 // Entries per lane loaded ahead in the narrow row kernels: the column
 // indices of kNarrowUnroll entries (k, k + 4, ...) are loaded together, then
 // the gathered vector entries and matrix values, so a lane waits for one
@@ -148,7 +168,7 @@ __global__ void __launch_bounds__(256) g_product_kernel(int nNo, const int *__re
                                                         const T *__restrict__ Ps, T *__restrict__ t1,
                                                         T *__restrict__ t1s, const double *__restrict__ done) {
   if (done && *done != 0.0) return;
-  const int gid = crimson::xcd_block() * blockDim.x + threadIdx.x;
+  const int gid = crimson::xcd_block(kRowXcdChunk) * blockDim.x + threadIdx.x;
   const int row = gid / kNarrowLanes;
   const int lane = threadIdx.x % kNarrowLanes;
   T acc[3] = {T(0), T(0), T(0)};
@@ -202,7 +222,7 @@ __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restri
                                                     const T *__restrict__ t1s, T *__restrict__ SP,
                                                     const double *__restrict__ done) {
   if (done && *done != 0.0) return;
-  const int gid = crimson::xcd_block() * blockDim.x + threadIdx.x;
+  const int gid = crimson::xcd_block(kRowXcdChunk) * blockDim.x + threadIdx.x;
   const int row = gid / kNarrowLanes;
   const int lane = threadIdx.x % kNarrowLanes;
   T aL = T(0), aG = T(0);
@@ -557,15 +577,21 @@ __device__ inline void check_next(double *scal) {
   scal[6] += 1.0;
 }
 __global__ void cg_check_kernel(double *__restrict__ scal) { check_next(scal); }
-// X += alpha P ; R -= alpha SP ; partial of R.R
+// X += alpha P ; R -= alpha SP ; partial of R.R.  den = P . SP is summed
+// from its nblk workgroup partials by every workgroup (reduce_kernel's
+// order, so the same bits in each), which saves a launch per iteration.
 template <class T>
 __global__ void __launch_bounds__(256) cg_update_kernel(size_t L, const double *__restrict__ scal,
+                                                        const T *__restrict__ dpart, int nblk,
                                                         T *__restrict__ X, T *__restrict__ R,
                                                         const T *__restrict__ P, const T *__restrict__ SP,
                                                         T *__restrict__ partial) {
   if (scal[5] != 0.0) return;
   __shared__ T sh[4];
-  const T alpha = (T)(scal[1] / scal[3]);  // errO / den (CGRAD's alpha)
+  T d = T(0);
+  for (int b = threadIdx.x; b < nblk; b += 256) d += dpart[b];
+  const T den = block_sum_all(d, sh);
+  const T alpha = (T)(scal[1] / (double)den);  // errO / den (CGRAD's alpha)
   const size_t base = (size_t)blockIdx.x * kChunk;
   T s = T(0);
 #pragma unroll
@@ -942,9 +968,8 @@ int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool 
       // den = P . SP
       T *pt = reinterpret_cast<T *>(c.partial.p);
       mdot_kernel<T, T><<<nblk, 256, 0, c.s>>>(L, SP, P, 0, 1, 0, pt, 1, nullptr);
-      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, scal + 3, nullptr);
-      cg_update_kernel<T><<<nblk, 256, 0, c.s>>>(L, scal, X, R, P, SP, pt);
-      cg_err_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, scal);
+      cg_update_kernel<T><<<nblk, 256, 0, c.s>>>(L, scal, pt, nblk, X, R, P, SP, pt + nblk);
+      cg_err_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt + nblk, scal);
       cg_p_kernel<T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, scal, R, P, ps.W, ps.wdof, ps.off, ps.Ps);
     }
     CR_HIP(hipGetLastError());
