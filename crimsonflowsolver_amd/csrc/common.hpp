@@ -81,29 +81,27 @@ struct DBuf {
   }
 };
 
-// Workgroup index remapped so that each XCD (blocks are dealt round-robin
-// over the 8 XCDs) walks a contiguous range of rows or elements and keeps
-// the vector entries or nodes it gathers in its own L2.
-__device__ inline unsigned xcd_block() {
-  const unsigned nb = gridDim.x, b = blockIdx.x;
-  const unsigned per = nb / 8u;
-  if (per == 0 || b >= per * 8u) return b;
-  return (b % 8u) * per + b / 8u;
-}
-
-// The same with the workgroups dealt to the XCDs in turns of `chunk`
-// consecutive ones (XCD x takes chunks x, x + 8, x + 16, ...), so that the
-// XCDs work on neighbouring ranges at the same time and data shared by
-// items far apart in the order (a layer of a structured numbering) is found
-// again in the shared MALL instead of HBM.  chunk 0: xcd_block().
-__device__ inline unsigned xcd_block(unsigned chunk) {
-  if (chunk == 0) return xcd_block();
-  const unsigned nb = gridDim.x, b = blockIdx.x;
+// Workgroup index b of nb remapped so that each XCD (blocks are dealt
+// round-robin over the 8 XCDs) walks a contiguous range of rows or elements
+// (chunk 0) and keeps the vector entries or nodes it gathers in its own L2;
+// or with the workgroups dealt to the XCDs in turns of `chunk` consecutive
+// ones (XCD x takes chunks x, x + 8, x + 16, ...), so that the XCDs work on
+// neighbouring ranges at the same time and data shared by items far apart
+// in the order (a layer of a structured numbering) is found again in the
+// shared MALL instead of HBM.
+__device__ inline unsigned xcd_block(unsigned chunk, unsigned b, unsigned nb) {
+  if (chunk == 0) {
+    const unsigned per = nb / 8u;
+    if (per == 0 || b >= per * 8u) return b;
+    return (b % 8u) * per + b / 8u;
+  }
   const unsigned full = nb / (8u * chunk) * (8u * chunk);
   if (b >= full) return b;
   const unsigned x = b % 8u, k = b / 8u;  // the k-th workgroup dealt to XCD x
   return ((k / chunk) * 8u + x) * chunk + k % chunk;
 }
+__device__ inline unsigned xcd_block(unsigned chunk) { return xcd_block(chunk, blockIdx.x, gridDim.x); }
+__device__ inline unsigned xcd_block() { return xcd_block(0u); }
 
 // Number of workgroups of `threads` covering n items.
 inline unsigned grid_for(size_t n, unsigned threads) {
@@ -128,6 +126,14 @@ struct CrFace {
   crimson::DBuf<int> d_glob;
   crimson::DBuf<double> d_val;   // (dof, nNo)
   crimson::DBuf<double> d_valM;  // (dof, nNo) = val * W
+  // the columns j of the rows of the face's nodes (sorted) and, for each,
+  // its (face node a, CSR entry k) pairs in (a, k) order: the support of
+  // the face dot as a functional of the G product's operand (built once
+  // per face)
+  bool qpat = false;
+  int nsup = 0;
+  crimson::DBuf<int> d_sup, d_qptr, d_qa, d_qk;
+  crimson::DBuf<double> d_q;  // the functional's coefficients, per solve
 };
 
 // memLS matrix handle.  The matrix of block size vdof (1..4) is held as

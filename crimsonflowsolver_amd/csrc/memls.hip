@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -112,6 +113,24 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
 // same order, as the two separate products).
 constexpr int kNarrowLanes = 4;
 
+// A coupled face of the Schur CG (ADDBCMUL's PRE, memLS.f90:1719-1735):
+// its nodes, valM, coefficient -res/(1 + res nS), and the face dot
+// S_f = valM_f . t1 restated as a functional of the G product's operand,
+// S_f = sum_j q_j Ps_j over the support sup (the columns of the face nodes'
+// rows), q_j = sum over the pairs (a, k) of column j of
+// valM(i, a) W_u(i, glob(a)) G_i(k), i < min(dof, 3).
+struct FaceDev {
+  const int *glob;
+  const double *valM;
+  double coef;
+  int nNo, dof;
+  const int *sup;
+  const double *q;
+  int nsup;
+};
+// Workgroups per face computing S_f's partial sums beside the G product.
+constexpr int kFaceDotBlocks = 4;
+
 // The Schur operator's vectors pre-multiplied by their column scaling, so
 // that its two row kernels gather no W: Ps = W_p o P (written by the CG's
 // direction update, scale_p_kernel for the first direction); the G product
@@ -162,13 +181,49 @@ __global__ void scale_p_kernel(int n, const T *__restrict__ W, int wdof, int off
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = W[off + (size_t)wdof * i] * x[i];
 }
+// The first nfc * kFaceDotBlocks workgroups form the coupled faces' dots
+// from Ps (fpart[f * kFaceDotBlocks + b]: the b-th workgroup's share, entries
+// s = b * 256 + t + 1024 u), so that no launch between the two row kernels
+// waits for t1; the rest form t1.
 template <class T, int U = kNarrowUnroll>
 __global__ void __launch_bounds__(256) g_product_kernel(int nNo, const int *__restrict__ rowptr,
                                                         const int *__restrict__ col, MatView<T> G,
                                                         const T *__restrict__ Ps, T *__restrict__ t1,
-                                                        T *__restrict__ t1s, const double *__restrict__ done) {
+                                                        T *__restrict__ t1s, int nfc,
+                                                        const FaceDev *__restrict__ fd,
+                                                        double *__restrict__ fpart,
+                                                        const double *__restrict__ done) {
   if (done && *done != 0.0) return;
-  const int gid = crimson::xcd_block(kRowXcdChunk) * blockDim.x + threadIdx.x;
+  const unsigned nx = (unsigned)nfc * kFaceDotBlocks;
+  if (blockIdx.x < nx) {
+    __shared__ double sh[4];
+    const int f = blockIdx.x / kFaceDotBlocks, b = blockIdx.x % kFaceDotBlocks;
+    const int *sup = fd[f].sup;
+    const double *q = fd[f].q;
+    const int ns = fd[f].nsup;
+    constexpr int kStride = 256 * kFaceDotBlocks;
+    double s = 0.0;
+    for (int s0 = b * 256 + threadIdx.x; s0 < ns; s0 += kStride * 4) {
+      int j[4];
+      double qv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = min(s0 + kStride * u, ns - 1);
+        j[u] = sup[e];
+        qv[u] = q[e];
+      }
+      T pv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pv[u] = Ps[j[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (s0 + kStride * u < ns) s += qv[u] * (double)pv[u];
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) fpart[blockIdx.x] = s;
+    return;
+  }
+  const int gid = crimson::xcd_block(kRowXcdChunk, blockIdx.x - nx, gridDim.x - nx) * blockDim.x + threadIdx.x;
   const int row = gid / kNarrowLanes;
   const int lane = threadIdx.x % kNarrowLanes;
   T acc[3] = {T(0), T(0), T(0)};
@@ -480,44 +535,13 @@ __global__ void face_axpy_kernel(int n, int xdof, int fdof, const int *__restric
 }
 // A coupled face on the device, with its PRE coefficient -res/(1 + res nS)
 // (memLS.f90:1096-1118 with BCPRE's nS).
-struct FaceDev {
-  const int *glob;
-  const double *valM;
-  double coef;
-  int nNo, dof;
-};
-// ADDBCMUL's PRE on the Schur CG's G product t1 (3, N): first every coupled
-// face's dot S_f = valM_f . t1 (face_dot_kernel's sums, in its order) in one
-// 1024-thread workgroup, all before any update (PRE reads the unmodified
-// vector), then face by face t1 += coef_f S_f valM_f with t1s = W_u o t1
-// rewritten at the same nodes (face_pre_apply_kernel, one launch per face
-// since faces may share nodes).
+// ADDBCMUL's PRE on the Schur CG's G product t1 (3, N): face by face
+// t1 += coef_f S_f valM_f, with t1s = W_u o t1 rewritten at the same nodes
+// (one launch per face, since faces may share nodes).  S_f, every face's dot
+// of the unmodified t1, is the sum of its kFaceDotBlocks partials from the G
+// product (summed here in a fixed order).
 template <class T>
-__global__ void __launch_bounds__(1024) face_pre_dot_kernel(int nf, const FaceDev *__restrict__ fd,
-                                                            const T *__restrict__ t1, double *__restrict__ fsum,
-                                                            const double *__restrict__ done) {
-  if (done && *done != 0.0) return;
-  __shared__ double sh[16];
-  for (int f = 0; f < nf; ++f) {
-    const FaceDev F = fd[f];
-    const int nc = F.dof < 3 ? F.dof : 3;
-    double s = 0.0;
-    for (int a = threadIdx.x; a < F.nNo; a += 1024)
-      for (int i = 0; i < nc; ++i) s += F.valM[i + (size_t)F.dof * a] * (double)t1[i + (size_t)3 * F.glob[a]];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x / 64] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int w = 0; w < 16; ++w) t += sh[w];
-      fsum[f] = t;
-    }
-    __syncthreads();
-  }
-}
-template <class T>
-__global__ void __launch_bounds__(256) face_pre_apply_kernel(FaceDev F, const double *__restrict__ S,
+__global__ void __launch_bounds__(256) face_pre_apply_kernel(FaceDev F, const double *__restrict__ part,
                                                              T *__restrict__ t1, T *__restrict__ t1s,
                                                              const T *__restrict__ W, int wdof,
                                                              const double *__restrict__ done) {
@@ -525,11 +549,33 @@ __global__ void __launch_bounds__(256) face_pre_apply_kernel(FaceDev F, const do
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= F.nNo) return;
   const int nc = F.dof < 3 ? F.dof : 3;
-  const double sc = F.coef * *S;
+  double S = 0.0;
+#pragma unroll
+  for (int b = 0; b < kFaceDotBlocks; ++b) S += part[b];
+  const double sc = F.coef * S;
   const size_t g = F.glob[a];
   for (int i = 0; i < nc; ++i) t1[i + 3 * g] += (T)(F.valM[i + (size_t)F.dof * a] * sc);
 #pragma unroll
   for (int c = 0; c < 3; ++c) t1s[c + 3 * g] = W[c + (size_t)wdof * g] * t1[c + 3 * g];
+}
+// The face functional's coefficients q (FaceDev), one thread per support
+// column, its pairs in their stored order, in FP64.
+template <class T>
+__global__ void face_q_kernel(int nsup, const int *__restrict__ qptr, const int *__restrict__ qa,
+                              const int *__restrict__ qk, const int *__restrict__ glob,
+                              const double *__restrict__ valM, int dof, MatView<T> G, double *__restrict__ q) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsup) return;
+  const int nc = dof < 3 ? dof : 3;
+  double v = 0.0;
+  for (int p = qptr[s]; p < qptr[s + 1]; ++p) {
+    const int a = qa[p], k = qk[p];
+    const size_t g = glob[a];
+    for (int i = 0; i < nc; ++i)
+      v += valM[i + (size_t)dof * a] * (double)G.W[G.r0 + i + (size_t)G.wdof * g] *
+           (double)G.base[(size_t)G.plane[4 * i] * G.nnz + k];
+  }
+  q[s] = v;
 }
 // DGKS decision: gate = ||w'||^2 < 0.5 ||w||^2
 __global__ void dgks_kernel(const double *__restrict__ n0, const double *__restrict__ n1, int *__restrict__ gate) {
@@ -666,6 +712,7 @@ struct Ctx {
   crimson::DBuf<double> partial;  // per-workgroup partial sums (viewed as T for FP32 accumulation)
   crimson::DBuf<double> dsc, hcol;
   crimson::DBuf<double> fsum;  // one face dot per face (ADDBCMUL)
+  crimson::DBuf<double> fpart;  // the Schur CG's face dot partials (kFaceDotBlocks per coupled face)
   crimson::DBuf<FaceDev> fdev;  // coupled faces with their PRE coefficients (Schur CG)
   std::vector<FaceDev> hfdev;
   crimson::DBuf<int> gate;
@@ -686,6 +733,7 @@ struct Ctx {
     CR_TRY(dsc.alloc(64));
     CR_TRY(hcol.alloc(4 * 256));
     CR_TRY(fsum.alloc(std::max<size_t>(1, l->face.size())));
+    CR_TRY(fpart.alloc(std::max<size_t>(1, l->face.size()) * kFaceDotBlocks));
     CR_TRY(gate.alloc(4));
     return CRIMSON_OK;
   }
@@ -994,6 +1042,40 @@ int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool 
 
 // NSSOLVER (memLS.f90:1460-1759) on device; Ri (4,N) device in/out
 // (preconditioned RHS in, preconditioned solution out).
+// The support of a coupled face's dot functional (FaceDev): the columns of
+// the face nodes' rows, each with its (face node, CSR entry) pairs, from the
+// host copy of the pattern; built once per face (the pattern is fixed).
+int face_support(crimson_lhs *l, CrFace &fa) {
+  if (fa.qpat) return CRIMSON_OK;
+  std::vector<std::array<int, 3>> t;  // (column j, face node a, entry k)
+  for (int a = 0; a < fa.nNo; ++a) {
+    const int g = fa.glob[a];
+    for (int k = l->h_rowptr[g]; k < l->h_rowptr[g + 1]; ++k) t.push_back({l->h_col[k], a, k});
+  }
+  std::sort(t.begin(), t.end());
+  std::vector<int> sup, ptr, qa, qk;
+  for (size_t p = 0; p < t.size(); ++p) {
+    if (p == 0 || t[p][0] != t[p - 1][0]) {
+      sup.push_back(t[p][0]);
+      ptr.push_back((int)p);
+    }
+    qa.push_back(t[p][1]);
+    qk.push_back(t[p][2]);
+  }
+  ptr.push_back((int)t.size());
+  fa.nsup = (int)sup.size();
+  if (fa.nsup > 0) {
+    CR_TRY(fa.d_sup.upload(sup.data(), sup.size(), l->stream));
+    CR_TRY(fa.d_qptr.upload(ptr.data(), ptr.size(), l->stream));
+    CR_TRY(fa.d_qa.upload(qa.data(), qa.size(), l->stream));
+    CR_TRY(fa.d_qk.upload(qk.data(), qk.size(), l->stream));
+    CR_TRY(fa.d_q.alloc(sup.size()));
+    CR_HIP(hipStreamSynchronize(l->stream));
+  }
+  fa.qpat = true;
+  return CRIMSON_OK;
+}
+
 template <class T>
 int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
   crimson_lhs *l = c.lhs;
@@ -1068,11 +1150,19 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
     for (int a = 0; a < fa.nNo; ++a)
       for (int i = 0; i < nc; ++i) fa.nS = fa.nS + vm[i + (size_t)fa.dof * a] * vm[i + (size_t)fa.dof * a];
   }
-  // the coupled faces with their PRE coefficients, for the face_pre kernels
+  // the coupled faces with their PRE coefficients and dot functionals
   c.hfdev.clear();
-  for (auto &fa : l->face)
-    if (fa.coupled)
-      c.hfdev.push_back(FaceDev{fa.d_glob.p, fa.d_valM.p, -fa.res / (1.0 + fa.res * fa.nS), fa.nNo, fa.dof});
+  for (auto &fa : l->face) {
+    if (!fa.coupled) continue;
+    CR_TRY(face_support(l, fa));
+    if (fa.nsup > 0) {
+      face_q_kernel<T><<<crimson::grid_for(fa.nsup, 256), 256, 0, c.s>>>(
+          fa.nsup, fa.d_qptr.p, fa.d_qa.p, fa.d_qk.p, fa.d_glob.p, fa.d_valM.p, fa.dof, mG, fa.d_q.p);
+      CR_HIP(hipGetLastError());
+    }
+    c.hfdev.push_back(FaceDev{fa.d_glob.p, fa.d_valM.p, -fa.res / (1.0 + fa.res * fa.nS), fa.nNo, fa.dof,
+                              fa.d_sup.p, fa.d_q.p, fa.nsup});
+  }
   const int nfc = (int)c.hfdev.size();
   if (nfc > 0) {
     CR_TRY(c.fdev.alloc(c.hfdev.size()));
@@ -1084,13 +1174,11 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
   auto schur_op = [&](const T *Pv, T *SPv, const double *scal) -> int {
     const double *done = scal + 5;
     const unsigned gr = crimson::grid_for((size_t)N * kNarrowLanes, 256);
-    g_product_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2.p, t1.p, ws.tmp.p, done);
-    if (nfc > 0) {
-      face_pre_dot_kernel<T><<<1, 1024, 0, c.s>>>(nfc, c.fdev.p, t1.p, c.fsum.p, done);
-      for (int f = 0; f < nfc; ++f)
-        face_pre_apply_kernel<T><<<crimson::grid_for(c.hfdev[f].nNo, 256), 256, 0, c.s>>>(
-            c.hfdev[f], c.fsum.p + f, t1.p, ws.tmp.p, c.W.p, 4, done);
-    }
+    g_product_kernel<T><<<gr + nfc * kFaceDotBlocks, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2.p, t1.p,
+                                                                     ws.tmp.p, nfc, c.fdev.p, c.fpart.p, done);
+    for (int f = 0; f < nfc; ++f)
+      face_pre_apply_kernel<T><<<crimson::grid_for(c.hfdev[f].nNo, 256), 256, 0, c.s>>>(
+          c.hfdev[f], c.fpart.p + f * kFaceDotBlocks, t1.p, ws.tmp.p, c.W.p, 4, done);
     schur_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2.p, ws.tmp.p, SPv, done);
     CR_HIP(hipGetLastError());
     return CRIMSON_OK;
@@ -1528,6 +1616,7 @@ extern "C" int crimson_bc_create(crimson_lhs *l, int faIn, int nNo, int dof, int
   CrFace &fa = l->face[faIn - 1];
   fa.nNo = nNo;
   fa.dof = dof;
+  fa.qpat = false;
   fa.bGrp = bcType;
   fa.res = 0.0;
   fa.created = true;
