@@ -719,6 +719,29 @@ struct Ctx {
   std::vector<double> hbuf;
   bool coupled = false;
   int nblk_max = 0;
+  // the CG's flag read-back: two pinned slots and their events
+  double *hback = nullptr;
+  hipEvent_t evb[2] = {nullptr, nullptr};
+
+  Ctx() = default;
+  Ctx(const Ctx &) = delete;
+  Ctx &operator=(const Ctx &) = delete;
+  ~Ctx() {
+    if (hback || evb[0] || evb[1]) (void)hipStreamSynchronize(s);  // a read-back may still be in flight
+    if (hback) (void)hipHostFree(hback);
+    for (hipEvent_t e : evb)
+      if (e) (void)hipEventDestroy(e);
+  }
+  int readback_slots() {
+    if (!hback && hipHostMalloc((void **)&hback, 2 * 16 * sizeof(double)) != hipSuccess) {
+      hback = nullptr;
+      crimson::set_error("hipHostMalloc of the CG read-back slots failed");
+      return CRIMSON_ENOMEM;
+    }
+    for (hipEvent_t &e : evb)
+      if (!e) CR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return CRIMSON_OK;
+  }
 
   int init(crimson_lhs *l, int d) {
     lhs = l;
@@ -1008,9 +1031,9 @@ int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool 
   const int nblk = (int)((L + kChunk - 1) / kChunk);
   const int batch = 8;
   double back[9];
-  // the first iteration's test; cg_err_kernel makes each later one
-  cg_check_kernel<<<1, 1, 0, c.s>>>(scal);
-  for (;;) {
+  CR_TRY(c.readback_slots());
+  // a batch of iterations, then the flags' copy into read-back slot `slot`
+  auto enqueue_batch = [&](int slot) -> int {
     for (int b = 0; b < batch; ++b) {
       CR_TRY(op(P, SP, scal));
       // den = P . SP
@@ -1021,8 +1044,21 @@ int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool 
       cg_p_kernel<T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, scal, R, P, ps.W, ps.wdof, ps.off, ps.Ps);
     }
     CR_HIP(hipGetLastError());
-    CR_HIP(hipMemcpyAsync(back, scal, sizeof(back), hipMemcpyDeviceToHost, c.s));
-    CR_HIP(hipStreamSynchronize(c.s));
+    CR_HIP(hipMemcpyAsync(c.hback + 16 * slot, scal, sizeof(back), hipMemcpyDeviceToHost, c.s));
+    CR_HIP(hipEventRecord(c.evb[slot], c.s));
+    return CRIMSON_OK;
+  };
+  // the first iteration's test; cg_err_kernel makes each later one
+  cg_check_kernel<<<1, 1, 0, c.s>>>(scal);
+  CR_TRY(enqueue_batch(0));
+  for (int k = 0;; ++k) {
+    // the next batch is queued before this one's flags are read, so that the
+    // GPU does not idle while the host waits and launches; once the flag is
+    // raised every kernel that changes the state returns at once, so the
+    // extra batch leaves the state as this one did
+    CR_TRY(enqueue_batch((k + 1) & 1));
+    CR_HIP(hipEventSynchronize(c.evb[k & 1]));
+    std::memcpy(back, c.hback + 16 * (k & 1), sizeof(back));
     if (back[5] != 0.0) break;
   }
   // the flag is raised at the top of the iteration after the last one
