@@ -41,7 +41,7 @@ EXPORTS = (
     "crimson_lhs_spmv_dev", "crimson_itr_predict_dev", "crimson_itr_yalpha_dev", "crimson_itr_correct_dev",
     "crimson_itr_bc_dev", "crimson_elmpvsq_dev", "crimson_form_sync", "crimson_lhs_set_precision",
     "crimson_form_vjp_dev", "crimson_form_set_instances", "crimson_form_graph_create", "crimson_graph_launch",
-    "crimson_graph_free", "crimson_form_set_boundary", "crimson_form_autotune",
+    "crimson_graph_free", "crimson_form_set_boundary", "crimson_form_autotune", "crimson_form_set_bc",
 )
 
 
@@ -112,6 +112,7 @@ def lib():
         L.crimson_elmpvsq_dev.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp, vp, vp]
         L.crimson_form_sync.argtypes = [vp]
         L.crimson_form_set_boundary.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_int]
+        L.crimson_form_set_bc.argtypes = [vp, vp, C.c_int]
         L.crimson_form_autotune.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         _lib = L
     return _lib
@@ -275,6 +276,18 @@ class ElmGMRInterior:
                                           int(mask), C.byref(choice), ms), "crimson_form_autotune")
         times = {n: ms[i] for i, n in enumerate(self.AUTOTUNE_CANDIDATES) if ms[i] == ms[i]}
         return self.AUTOTUNE_CANDIDATES[choice.value], times
+
+    def set_bc(self, BC):
+        """BC(nshg, ndofBC) of the handle's iBC (ndofBC >= 6 for 1- and
+        2-component velocity codes): bc3lhs on every later tangent, bc3res's
+        projections; None removes it."""
+        if BC is None:
+            self._bc = None
+            check(lib().crimson_form_set_bc(self.h, None, 0), "crimson_form_set_bc")
+            return
+        bc = np.asfortranarray(BC, dtype=np.float64)
+        self._bc = bc
+        check(lib().crimson_form_set_bc(self.h, _p(bc), int(bc.shape[1])), "crimson_form_set_bc")
 
     def graph_to_lhs(self, y, ac, x, res4, lhs_handle):
         """Capture to_lhs on these device arrays into a HIP graph; returns a

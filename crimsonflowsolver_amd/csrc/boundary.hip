@@ -450,6 +450,10 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
   if (!f) return crimson::set_error("crimson_form_set_boundary: null handle"), CRIMSON_EARG;
   if (nelb == 0) {
     f->nelb = f->nbnode = 0;
+    if (f->npartial > 0) {
+      f->h_nbnd.assign(f->nshg, 0);
+      return crimson::form_bc3lhs_table(f);
+    }
     return CRIMSON_OK;
   }
   if (nelb < 0 || !ienb || !iBCB || !BCB || ndBCB < 2 || (nqb != 1 && nqb != 3) || (permit && nsrf < 0))
@@ -530,5 +534,11 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
   f->nqb = nqb;
   f->influx = influx;
   f->nsrf = permit ? nsrf : -1;
+  if (f->npartial > 0) {
+    // the boundary loop's bc3lhs sets the constrained diagonal entries of
+    // each boundary element's partial-BC nodes to one (elmgmr.f90:364-372)
+    f->h_nbnd = vcnt;
+    return crimson::form_bc3lhs_table(f);
+  }
   return CRIMSON_OK;
 }

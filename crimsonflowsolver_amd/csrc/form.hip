@@ -1007,7 +1007,12 @@ __global__ void __launch_bounds__(256) asm_row32(int nshg, const int4 *__restric
 
 // bc3Res for codes 0/7 on the velocity bits, bit 2 (pressure) and bit 11
 // (bc3res.f90:34-80).
-__global__ void bc3res_kernel(int nshg, const int *__restrict__ ibc, double *__restrict__ res, int res4) {
+// bc3Res (bc3res.f90:34-120) without periodic nodes: the pressure row of
+// bit 2; the velocity rows of 1- and 2-component codes projected with
+// BC(:,4:6) (bc(nshg, ndofbc); null when no node has such a code), of
+// 3-component codes and of bit 11 zeroed.
+__global__ void bc3res_kernel(int nshg, const int *__restrict__ ibc, const double *__restrict__ bc,
+                              double *__restrict__ res, int res4) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nshg) return;
   const int c = ibc[i];
@@ -1015,10 +1020,143 @@ __global__ void bc3res_kernel(int nshg, const int *__restrict__ ibc, double *__r
   // component k of node i: res(i,k) or Res4(k,i)
   const size_t s = res4 ? 1 : N, b = res4 ? 4 * (size_t)i : (size_t)i;
   if ((c >> 2) & 1) res[b + 3 * s] = 0.0;
-  if (((c >> 3) & 7) == 7 || ((c >> 11) & 1)) {
+  const int vb = (c >> 3) & 7;
+  if (vb != 0 && vb != 7) {
+    const double b4 = bc[i + 3 * N], b5 = bc[i + 4 * N], b6 = bc[i + 5 * N];
+    double r1 = res[b], r2 = res[b + s], r3 = res[b + 2 * s];
+    switch (vb) {
+      case 1:
+        r2 = r2 - b4 * r1;
+        r3 = r3 - b5 * r1;
+        r1 = 0.0;
+        break;
+      case 2:
+        r1 = r1 - b4 * r2;
+        r3 = r3 - b5 * r2;
+        r2 = 0.0;
+        break;
+      case 3:
+        r3 = r3 - b4 * r1 - b6 * r2;
+        r1 = r2 = 0.0;
+        break;
+      case 4:
+        r1 = r1 - b4 * r3;
+        r2 = r2 - b5 * r3;
+        r3 = 0.0;
+        break;
+      case 5:
+        r2 = r2 - b4 * r1 - b6 * r3;
+        r1 = r3 = 0.0;
+        break;
+      default:  // 6
+        r1 = r1 - b4 * r2 - b6 * r3;
+        r2 = r3 = 0.0;
+        break;
+    }
+    res[b] = r1;
+    res[b + s] = r2;
+    res[b + 2 * s] = r3;
+  }
+  if (vb == 7 || ((c >> 11) & 1)) {
     res[b] = 0.0;
     res[b + s] = 0.0;
     res[b + 2 * s] = 0.0;
+  }
+}
+
+// bc3lhs (bc3lhs.f90:41-690) after the assembly.  For P1 elements each of
+// the reference's per-element operations is linear in the element block:
+// the column operations of node j (M -> M T_j) and the row operations of
+// node i on block (i, j) summed over the elements are those operations on
+// the assembled block, and the constrained diagonal entries it sets to one
+// per element sum to the node's element count.  The operations of a row and
+// a column commute, except the code-6 row operations (which the setup
+// rejects where the order would matter).  Entries e = 0..8 of a block are
+// the reference's 1..9 (1 2 3 / 4 5 6 / 7 8 9, lhsK(1:9)).  Each operation:
+// entry t -= BC(4 + b1) K(s1) [- BC(4 + b2) K(s2)].
+struct BclOp {
+  signed char t, s1, s2, b1, b2;  // s2 < 0: none
+};
+struct BclCode {
+  BclOp col[6], row[6];
+  signed char ncol, nrow, one0, one1;  // one1 < 0: one entry only
+  unsigned short colz, rowz;           // zeroed entries (bit e)
+};
+__constant__ BclCode kBclCodes[7] = {
+    {{}, {}, 0, 0, -1, -1, 0, 0},
+    // 1: x1 (bc3lhs.f90:56-200)
+    {{{1, 0, -1, 0, 0}, {4, 3, -1, 0, 0}, {7, 6, -1, 0, 0}, {2, 0, -1, 1, 0}, {5, 3, -1, 1, 0}, {8, 6, -1, 1, 0}},
+     {{3, 0, -1, 0, 0}, {4, 1, -1, 0, 0}, {5, 2, -1, 0, 0}, {6, 0, -1, 1, 0}, {7, 1, -1, 1, 0}, {8, 2, -1, 1, 0}},
+     6, 6, 0, -1, 0x049, 0x007},
+    // 2: x2 (:202-297)
+    {{{0, 1, -1, 0, 0}, {3, 4, -1, 0, 0}, {6, 7, -1, 0, 0}, {2, 1, -1, 1, 0}, {5, 4, -1, 1, 0}, {8, 7, -1, 1, 0}},
+     {{0, 3, -1, 0, 0}, {1, 4, -1, 0, 0}, {2, 5, -1, 0, 0}, {6, 3, -1, 1, 0}, {7, 4, -1, 1, 0}, {8, 5, -1, 1, 0}},
+     6, 6, 4, -1, 0x092, 0x038},
+    // 3: x1 and x2 (:394-501)
+    {{{2, 0, 1, 0, 2}, {5, 3, 4, 0, 2}, {8, 6, 7, 0, 2}},
+     {{6, 0, 3, 0, 2}, {7, 1, 4, 0, 2}, {8, 2, 5, 0, 2}},
+     3, 3, 0, 4, 0x0db, 0x03f},
+    // 4: x3 (:299-392)
+    {{{0, 2, -1, 0, 0}, {3, 5, -1, 0, 0}, {6, 8, -1, 0, 0}, {1, 2, -1, 1, 0}, {4, 5, -1, 1, 0}, {7, 8, -1, 1, 0}},
+     {{0, 6, -1, 0, 0}, {1, 7, -1, 0, 0}, {2, 8, -1, 0, 0}, {3, 6, -1, 1, 0}, {4, 7, -1, 1, 0}, {5, 8, -1, 1, 0}},
+     6, 6, 8, -1, 0x124, 0x1c0},
+    // 5: x1 and x3 (:503-589)
+    {{{1, 0, 2, 0, 2}, {4, 3, 5, 0, 2}, {7, 6, 8, 0, 2}},
+     {{3, 0, 6, 0, 2}, {4, 1, 7, 0, 2}, {5, 2, 8, 0, 2}},
+     3, 3, 0, 8, 0x16d, 0x1c7},
+    // 6: x2 and x3 (:591-690), the row operations as the reference writes them
+    {{{0, 1, 2, 0, 2}, {3, 4, 5, 0, 2}, {6, 7, 8, 0, 2}},
+     {{6, 0, -1, 0, 0}, {7, 1, -1, 0, 0}, {8, 2, 5, 0, 2}},
+     3, 3, 4, 8, 0x1b6, 0x03f},
+};
+template <class T>
+__device__ inline void bcl_apply(T (&K)[9], const BclOp *ops, int n, const double (&bc)[3]) {
+  for (int o = 0; o < n; ++o) {
+    const BclOp op = ops[o];
+    T v = K[op.t] - (T)bc[op.b1] * K[op.s1];
+    if (op.s2 >= 0) v = v - (T)bc[op.b2] * K[op.s2];
+    K[op.t] = v;
+  }
+}
+// One thread per touched block {k, partial index of the row node, of the
+// column node (-1: none), diagonal}.  MODE 0: lhsK(9, nnz) (double); MODE 1:
+// the memLS planes of T (lhsK(1:9) in planes 0-2, 4-6, 8-10, put_block's).
+template <int MODE, class T>
+__global__ void bc3lhs_kernel(int n, const int4 *__restrict__ blk, const BclNode *__restrict__ nodes, T *out,
+                              size_t nnz) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int4 b = blk[t];
+  const size_t k = (size_t)b.x;
+  T K[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) K[e] = MODE == 0 ? out[9 * k + e] : out[(size_t)(e + e / 3) * nnz + k];
+  if (b.z >= 0) {
+    const BclNode nd = nodes[b.z];
+    const BclCode &c = kBclCodes[nd.code];
+    bcl_apply(K, c.col, c.ncol, nd.b);
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+      if ((c.colz >> e) & 1) K[e] = T(0);
+  }
+  if (b.y >= 0) {
+    const BclNode nd = nodes[b.y];
+    const BclCode &c = kBclCodes[nd.code];
+    bcl_apply(K, c.row, c.nrow, nd.b);
+#pragma unroll
+    for (int e = 0; e < 9; ++e)
+      if ((c.rowz >> e) & 1) K[e] = T(0);
+    if (b.w) {
+      K[c.one0] = (T)nd.count;
+      if (c.one1 >= 0) K[c.one1] = (T)nd.count;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    if (MODE == 0)
+      out[9 * k + e] = K[e];
+    else
+      out[(size_t)(e + e / 3) * nnz + k] = K[e];
   }
 }
 
@@ -1139,6 +1277,10 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
   const unsigned T = 256;
   hipStream_t s = f->stream;
   const FormConst c = make_const(f->prm);
+  if (lhs && f->npartial > 0 && !f->d_bc.p) {
+    crimson::set_error("iBC holds 1- or 2-component velocity codes: BC is needed first (crimson_form_set_bc)");
+    return CRIMSON_ESTATE;
+  }
   if (lhs) CR_TRY(ensure_order(f, d_x));
   if (timed) CR_HIP(hipEventRecord(f->ev[0], s));
   CR_TRY(crimson::form_flux_pass(f, d_y, d_ac, d_x, d_qout, nullptr));
@@ -1189,6 +1331,17 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
     else
       asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
                                   d_res, o, f->asm_xcd);
+  }
+  if (lhs && f->nbcl > 0) {
+    {
+      const unsigned gb = crimson::grid_for(f->nbcl, 256);
+      if (mode == 0)
+        bc3lhs_kernel<0, double><<<gb, 256, 0, s>>>(f->nbcl, f->d_bcl.p, f->d_bcn.p, o.lhsK, o.nnz);
+      else if (t32)
+        bc3lhs_kernel<1, float><<<gb, 256, 0, s>>>(f->nbcl, f->d_bcl.p, f->d_bcn.p, o.V32, o.nnz);
+      else
+        bc3lhs_kernel<1, double><<<gb, 256, 0, s>>>(f->nbcl, f->d_bcl.p, f->d_bcn.p, o.V, o.nnz);
+    }
   }
   CR_HIP(hipGetLastError());
   if (timed) CR_HIP(hipEventRecord(f->ev[3], s));
@@ -1265,15 +1418,34 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
       return CRIMSON_ENODIAG;
     }
   }
+  int npartial = 0;
   if (iBC) {
     for (int i = 0; i < nshg; ++i) {
       const int vb = (iBC[i] >> 3) & 7;
-      if (vb != 0 && vb != 7) {
-        crimson::set_error("node %d: partial velocity BC code %d (bc3lhs projection) is not supported", i + 1, vb);
-        return CRIMSON_EUNSUPPORTED;
-      }
+      npartial += vb != 0 && vb != 7;
       if ((iBC[i] >> 10) & 1) {
         crimson::set_error("node %d: periodic BC (iBC bit 10) is not supported", i + 1);
+        return CRIMSON_EUNSUPPORTED;
+      }
+    }
+    // The reference's code-6 row operations (bc3lhs.f90:648-680) do not
+    // commute with another node's column operations, so their sum over the
+    // elements depends on each element's local node order: not supported
+    // where a code-6 node shares an element with another partial node.
+    for (int e = 0; e < nel && npartial; ++e) {
+      int n6 = 0, np = 0;
+      for (int a = 0; a < 4; ++a) {
+        const int g = ien[(size_t)a * nel + e] - 1;
+        if (g < 0 || g >= nshg) continue;  // reported below
+        const int vb = (iBC[g] >> 3) & 7;
+        n6 += vb == 6;
+        np += vb != 0 && vb != 7;
+      }
+      if (n6 > 0 && np > 1) {
+        crimson::set_error("element %d: a node with velocity code 6 shares the element with another 1- or "
+                           "2-component node; the reference's code-6 projection then depends on the local "
+                           "node order (not supported)",
+                           e + 1);
         return CRIMSON_EUNSUPPORTED;
       }
     }
@@ -1399,6 +1571,15 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
   if (iBC) {
     f->has_ibc = true;
     if ((rc = f->d_ibc.upload(iBC, nshg, s))) return fail(rc);
+  }
+  f->npartial = npartial;
+  if (npartial > 0) {
+    f->h_ibc.assign(iBC, iBC + nshg);
+    f->h_rp = rp;
+    f->h_cols = cols;
+    f->h_ninc.resize(nshg);
+    for (int i = 0; i < nshg; ++i) f->h_ninc[i] = inc_ptr[i + 1] - inc_ptr[i];
+    f->h_nbnd.assign(nshg, 0);
   }
   if (hipStreamSynchronize(s) != hipSuccess) {
     crimson::set_error("upload of the formation maps failed");
@@ -1637,13 +1818,75 @@ extern "C" int crimson_graph_free(crimson_graph *g) {
   return CRIMSON_OK;
 }
 
+namespace crimson {
+int form_bc3lhs_table(crimson_form *f) {
+  if (f->npartial == 0 || !f->d_bc.p) return CRIMSON_OK;
+  const int n = f->nshg;
+  std::vector<int> pidx(n, -1);
+  std::vector<BclNode> nodes;
+  const std::vector<double> &hbc = f->h_bc;
+  for (int i = 0; i < n; ++i) {
+    const int vb = (f->h_ibc[i] >> 3) & 7;
+    if (vb == 0 || vb == 7) continue;
+    pidx[i] = (int)nodes.size();
+    BclNode nd;
+    nd.code = vb;
+    nd.count = f->h_ninc[i] + f->h_nbnd[i];
+    for (int k = 0; k < 3; ++k) nd.b[k] = hbc[i + (size_t)n * (3 + k)];
+    nodes.push_back(nd);
+  }
+  std::vector<int4> blk;
+  for (int i = 0; i < n; ++i)
+    for (int k = f->h_rp[i]; k < f->h_rp[i + 1]; ++k) {
+      const int j = f->h_cols[k];
+      if (pidx[i] < 0 && pidx[j] < 0) continue;
+      blk.push_back(make_int4(k, pidx[i], pidx[j], i == j ? 1 : 0));
+    }
+  CR_TRY(f->d_bcn.upload(nodes.data(), nodes.size(), f->stream));
+  CR_TRY(f->d_bcl.upload(blk.data(), blk.size(), f->stream));
+  CR_HIP(hipStreamSynchronize(f->stream));
+  f->nbcl = (int)blk.size();
+  return CRIMSON_OK;
+}
+}  // namespace crimson
+
+extern "C" int crimson_form_set_bc(crimson_form *f, const double *BC, int ndofBC) {
+  if (!f || (!BC && ndofBC > 0) || ndofBC < 0) {
+    crimson::set_error("crimson_form_set_bc: invalid arguments");
+    return CRIMSON_EARG;
+  }
+  if (ndofBC == 0) {
+    f->d_bc.release();
+    f->d_bcl.release();
+    f->d_bcn.release();
+    f->nbcl = 0;
+    f->ndofbc = 0;
+    f->h_bc.clear();
+    return CRIMSON_OK;
+  }
+  if (f->npartial > 0 && ndofBC < 6) {
+    crimson::set_error("crimson_form_set_bc: %d nodes have 1- or 2-component velocity codes, which read "
+                       "BC(:,4:6); ndofBC = %d",
+                       f->npartial, ndofBC);
+    return CRIMSON_EARG;
+  }
+  CR_TRY(f->d_bc.upload(BC, (size_t)f->nshg * ndofBC, f->stream));
+  f->ndofbc = ndofBC;
+  if (f->npartial > 0) f->h_bc.assign(BC, BC + (size_t)f->nshg * ndofBC);
+  return crimson::form_bc3lhs_table(f);
+}
+
 extern "C" int crimson_form_bc3res_dev(crimson_form *f, double *d_res, int res4_layout) {
   if (!f || !d_res) {
     crimson::set_error("crimson_form_bc3res_dev: null argument");
     return CRIMSON_EARG;
   }
   if (!f->has_ibc) return CRIMSON_OK;
-  bc3res_kernel<<<crimson::grid_for(f->nshg, 256), 256, 0, f->stream>>>(f->nshg, f->d_ibc.p, d_res,
+  if (f->npartial > 0 && !f->d_bc.p) {
+    crimson::set_error("iBC holds 1- or 2-component velocity codes: BC is needed first (crimson_form_set_bc)");
+    return CRIMSON_ESTATE;
+  }
+  bc3res_kernel<<<crimson::grid_for(f->nshg, 256), 256, 0, f->stream>>>(f->nshg, f->d_ibc.p, f->d_bc.p, d_res,
                                                                          res4_layout ? 1 : 0);
   CR_HIP(hipGetLastError());
   CR_HIP(hipStreamSynchronize(f->stream));

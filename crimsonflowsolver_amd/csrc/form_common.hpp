@@ -138,6 +138,13 @@ inline FormConst make_const(const crimson_form_params &p) {
 
 }  // namespace
 
+// A node with a 1- or 2-component velocity code (bc3lhs): its code, the
+// number of elements (interior and boundary) that hold it, and BC(4:6).
+struct BclNode {
+  int code, count;
+  double b[3];
+};
+
 // ---------------------------------------------------------------- handle
 struct crimson_form {
   int nshg = 0, numnp = 0, nel = 0, nnz = 0;
@@ -154,6 +161,15 @@ struct crimson_form {
   crimson::DBuf<float> d_rec32;  // FP32 tangent records (precision 32)
   crimson::DBuf<double> s_y, s_ac, s_x, s_res, s_K, s_P, s_q;  // staging of the host API
   bool has_ibc = false;
+  // partial velocity BCs (crimson_form_set_bc): bc3lhs on the tangent after
+  // the assembly, bc3res on the residual; host copies kept when npartial > 0
+  int npartial = 0, ndofbc = 0;
+  std::vector<int> h_ibc, h_rp, h_cols, h_ninc, h_nbnd;
+  std::vector<double> h_bc;
+  crimson::DBuf<double> d_bc;      // BC(nshg, ndofbc)
+  crimson::DBuf<int4> d_bcl;       // blocks touched: k, row's / column's partial index (-1: none), diagonal
+  crimson::DBuf<BclNode> d_bcn;    // the partial nodes
+  int nbcl = 0;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   float ms[4] = {0, 0, 0, 0};
   // adjoint (VJP) workspaces, allocated on first use
@@ -178,6 +194,9 @@ struct crimson_form {
 };
 
 namespace crimson {
+// (Re)builds the bc3lhs block table from the handle's host copies and BC;
+// a no-op until BC is set.
+int form_bc3lhs_table(crimson_form *f);
 int form_flux_pass(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, double *d_q,
                    double *d_invm);
 // Boundary elements' residual rows added to res (mode 0: res(nshg,4); mode

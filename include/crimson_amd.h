@@ -102,10 +102,24 @@ int crimson_form_interior_to_lhs(crimson_form *f, const double *d_y,
                                  const double *d_ac, const double *d_x,
                                  double *d_Res4, struct crimson_lhs *lhs);
 
-/* bc3Res for 0- and 3-component velocity codes plus the pressure bit
- * (bc3res.f90:34-80) on a device residual: res(nshg,4) when res4_layout is
- * 0, Res4(4,nshg) when it is 1.  A no-op for a handle set up without iBC. */
+/* bc3Res (bc3res.f90:34-120, no periodic nodes) on a device residual:
+ * res(nshg,4) when res4_layout is 0, Res4(4,nshg) when it is 1.  A no-op
+ * for a handle set up without iBC; 1- and 2-component velocity codes read
+ * BC (crimson_form_set_bc, else CRIMSON_ESTATE). */
 int crimson_form_bc3res_dev(crimson_form *f, double *d_res, int res4_layout);
+
+/* BC(nshg, ndofBC) of the handle's iBC (column-major, the reference's
+ * BC array; ndofBC >= 6 when iBC has 1- or 2-component velocity codes,
+ * whose projections read BC(:,4:6)).  With it every formation with the
+ * tangent applies bc3lhs (bc3lhs.f90:41-690, called by ElmGMR for the
+ * interior elements at elmgmr.f90:223-226 and for the boundary elements at
+ * :364-372) to the matrix, and crimson_form_bc3res_dev the partial codes'
+ * residual projection.  Formations with the tangent of a handle whose iBC
+ * has such codes fail with CRIMSON_ESTATE until BC is set.  ndofBC = 0
+ * removes it.  (The setup rejects a code-6 node that shares an element with
+ * another such node: the reference's code-6 row operations then depend on
+ * the element's local node order.) */
+int crimson_form_set_bc(crimson_form *f, const double *BC, int ndofBC);
 
 /* Wall time (ms) of the last formation's kernels, measured with HIP events
  * on the handle's stream: out[0] flux pass, out[1] element kernel,

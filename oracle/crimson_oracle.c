@@ -425,14 +425,117 @@ static int sparseloc(const int *list, int n, int target) {
   return lo;
 }
 
-static int check_bc_supported(int nshg, const int *iBC) {
+static int check_bc_supported(int nshg, const int *iBC, const double *BC, int ndofBC) {
   if (!iBC) return CFO_OK;
   for (int i = 0; i < nshg; ++i) {
     const int vb = (iBC[i] >> 3) & 7;
-    if (vb != 0 && vb != 7) return CFO_EUNSUPPORTED; /* bc3lhs projection */
+    /* bc3lhs projection: needs BC(:,4:6) */
+    if (vb != 0 && vb != 7 && (!BC || ndofBC < 6)) return CFO_EUNSUPPORTED;
     if ((iBC[i] >> 10) & 1) return CFO_EUNSUPPORTED; /* periodic (qpbc) */
   }
   return CFO_OK;
+}
+
+/* bc3lhs (bc3lhs.f90:41-690) on one element's tangent: for each local node
+ * inod with a 1- or 2-component velocity code, in local order, first the
+ * column operations on the blocks (i, inod), then the row operations on the
+ * blocks (inod, i), i = 1..4, then the entries of the constrained
+ * components on the diagonal block (inod, inod) set to one.  Entries are
+ * numbered m = 1..9 as in the reference's block-9 drawing (1 2 3 / 4 5 6 /
+ * 7 8 9); each operation is entry t -= BC(in,4) * entry s1 [- BC(in,6 or 5)
+ * * entry s2], exactly as the reference writes it for each code (its code-6
+ * row operations included).  K(m, a, b) = xKebe[off + stride * ((m-1) + 9 *
+ * (a + 4 b))]. */
+typedef struct {
+  signed char t, s1, s2; /* target, first source, second source (0: none), 1-based */
+  signed char b1, b2;    /* BC columns (1-based) of the two sources */
+} bc_op;
+static void bc_apply(double *K, const bc_op *ops, int nops, const double *bcv) {
+  for (int o = 0; o < nops; ++o) {
+    double v = K[ops[o].t - 1] - bcv[ops[o].b1 - 1] * K[ops[o].s1 - 1];
+    if (ops[o].s2) v = v - bcv[ops[o].b2 - 1] * K[ops[o].s2 - 1];
+    K[ops[o].t - 1] = v;
+  }
+}
+/* per code: column operations, zeroed entries of the columns, row
+ * operations, zeroed entries of the rows, entries set to one */
+typedef struct {
+  bc_op col[6];
+  int ncol;
+  signed char colz[6];
+  bc_op row[6];
+  int nrow;
+  signed char rowz[6];
+  signed char one[2];
+} bc_code;
+static const bc_code BC_CODES[7] = {
+    {{{0}}, 0, {0}, {{0}}, 0, {0}, {0}},
+    /* 1: x1 (bc3lhs.f90:56-200) */
+    {{{2, 1, 0, 4, 0}, {5, 4, 0, 4, 0}, {8, 7, 0, 4, 0}, {3, 1, 0, 5, 0}, {6, 4, 0, 5, 0}, {9, 7, 0, 5, 0}},
+     6, {1, 4, 7, 0, 0, 0},
+     {{4, 1, 0, 4, 0}, {5, 2, 0, 4, 0}, {6, 3, 0, 4, 0}, {7, 1, 0, 5, 0}, {8, 2, 0, 5, 0}, {9, 3, 0, 5, 0}},
+     6, {1, 2, 3, 0, 0, 0}, {1, 0}},
+    /* 2: x2 (:202-297) */
+    {{{1, 2, 0, 4, 0}, {4, 5, 0, 4, 0}, {7, 8, 0, 4, 0}, {3, 2, 0, 5, 0}, {6, 5, 0, 5, 0}, {9, 8, 0, 5, 0}},
+     6, {2, 5, 8, 0, 0, 0},
+     {{1, 4, 0, 4, 0}, {2, 5, 0, 4, 0}, {3, 6, 0, 4, 0}, {7, 4, 0, 5, 0}, {8, 5, 0, 5, 0}, {9, 6, 0, 5, 0}},
+     6, {4, 5, 6, 0, 0, 0}, {5, 0}},
+    /* 3: x1 and x2 (:394-501) */
+    {{{3, 1, 2, 4, 6}, {6, 4, 5, 4, 6}, {9, 7, 8, 4, 6}},
+     3, {1, 4, 7, 2, 5, 8},
+     {{7, 1, 4, 4, 6}, {8, 2, 5, 4, 6}, {9, 3, 6, 4, 6}},
+     3, {1, 2, 3, 4, 5, 6}, {1, 5}},
+    /* 4: x3 (:299-392) */
+    {{{1, 3, 0, 4, 0}, {4, 6, 0, 4, 0}, {7, 9, 0, 4, 0}, {2, 3, 0, 5, 0}, {5, 6, 0, 5, 0}, {8, 9, 0, 5, 0}},
+     6, {3, 6, 9, 0, 0, 0},
+     {{1, 7, 0, 4, 0}, {2, 8, 0, 4, 0}, {3, 9, 0, 4, 0}, {4, 7, 0, 5, 0}, {5, 8, 0, 5, 0}, {6, 9, 0, 5, 0}},
+     6, {7, 8, 9, 0, 0, 0}, {9, 0}},
+    /* 5: x1 and x3 (:503-589) */
+    {{{2, 1, 3, 4, 6}, {5, 4, 6, 4, 6}, {8, 7, 9, 4, 6}},
+     3, {1, 4, 7, 3, 6, 9},
+     {{4, 1, 7, 4, 6}, {5, 2, 8, 4, 6}, {6, 3, 9, 4, 6}},
+     3, {1, 2, 3, 7, 8, 9}, {1, 9}},
+    /* 6: x2 and x3 (:591-690; the row operations as the reference writes
+     * them: row 3 from row 1, and entry 9 also from entry 6) */
+    {{{1, 2, 3, 4, 6}, {4, 5, 6, 4, 6}, {7, 8, 9, 4, 6}},
+     3, {2, 5, 8, 3, 6, 9},
+     {{7, 1, 0, 4, 0}, {8, 2, 0, 4, 0}, {9, 3, 6, 4, 6}},
+     3, {1, 2, 3, 4, 5, 6}, {5, 9}},
+};
+static void bc3lhs_elem(int nshg, const int *iBC, const double *BC, const int nodes[4], double *xK,
+                        size_t off, size_t stride) {
+  for (int inod = 0; inod < 4; ++inod) {
+    const int in = nodes[inod] - 1;
+    const int vb = (iBC[in] >> 3) & 7;
+    if (vb == 0 || vb == 7) continue; /* bc3lhs.f90:48-49 */
+    const bc_code *c = &BC_CODES[vb];
+    double bcv[6];
+    for (int k = 0; k < 6; ++k) bcv[k] = BC[in + (size_t)nshg * k];
+    double K[9];
+    for (int i = 0; i < 4; ++i) { /* columns: blocks (i, inod) */
+      double *b = xK + off + stride * (size_t)(9 * (i + 4 * inod));
+      for (int m = 0; m < 9; ++m) K[m] = b[stride * m];
+      bc_apply(K, c->col, c->ncol, bcv);
+      for (int z = 0; z < 6; ++z)
+        if (c->colz[z]) K[c->colz[z] - 1] = 0.0;
+      for (int m = 0; m < 9; ++m) b[stride * m] = K[m];
+    }
+    for (int i = 0; i < 4; ++i) { /* rows: blocks (inod, i) */
+      double *b = xK + off + stride * (size_t)(9 * (inod + 4 * i));
+      for (int m = 0; m < 9; ++m) K[m] = b[stride * m];
+      bc_apply(K, c->row, c->nrow, bcv);
+      for (int z = 0; z < 6; ++z)
+        if (c->rowz[z]) K[c->rowz[z] - 1] = 0.0;
+      for (int m = 0; m < 9; ++m) b[stride * m] = K[m];
+    }
+    double *d = xK + off + stride * (size_t)(9 * (inod + 4 * inod));
+    for (int o = 0; o < 2; ++o)
+      if (c->one[o]) d[stride * (c->one[o] - 1)] = 1.0;
+  }
+}
+
+void cfo_bc3lhs_elem(int nshg, const int *iBC, const double *BC, const int *nodes, double *xK) {
+  bc3lhs_elem(nshg, iBC, BC, nodes, xK, 0, 1);
 }
 
 typedef struct {
@@ -465,12 +568,13 @@ static void gather_block(const mesh_view *m, int iel, int n, const int *const *i
 static int interior_core(int nshg, int numnp, int nel, const int *ien,
                          const double *y, const double *ac, const double *x,
                          const double *qin, int do_flux, const int *iBC,
+                         const double *BC, int ndofBC,
                          const int *colm, const int *rowp, int nnz,
                          double *res, double *lhsK, double *lhsP,
                          double *qres_out, const cfo_params *p) {
   if (nshg <= 0 || nel < 0 || !ien || !y || !ac || !x || !res) return CFO_EARG;
   if (p->lhs == 1 && (!colm || !rowp || !lhsK || !lhsP)) return CFO_EARG;
-  int rc = check_bc_supported(nshg, iBC);
+  int rc = check_bc_supported(nshg, iBC, BC, ndofBC);
   if (rc) return rc;
   const int bs = p->ibksiz > 0 ? p->ibksiz : 255;
   mesh_view m = {nshg, numnp, nel, ien, y, ac, x};
@@ -528,8 +632,12 @@ static int interior_core(int nshg, int numnp, int nel, const int *ien,
     cfo_e3_block(n, yl, acl, xl, ql, rl, xKebe, xGoC, p);
     scatter(n, ienb, 4, rl, res, nshg);
     if (p->lhs != 1) continue;
-    /* bc3lhs is a no-op for 0- and 3-component velocity BCs
-     * (bc3lhs.f90:48-49); fillsparseI (fillsparse.f90:29-63) */
+    /* bc3lhs (elmgmr.f90:223-226), then fillsparseI (fillsparse.f90:29-63) */
+    if (iBC && BC)
+      for (int e = 0; e < n; ++e) {
+        const int nodes[4] = {ienb[0][e], ienb[1][e], ienb[2][e], ienb[3][e]};
+        bc3lhs_elem(nshg, iBC, BC, nodes, xKebe, e, n);
+      }
     for (int e = 0; e < n && rc == CFO_OK; ++e)
       for (int a = 0; a < 4; ++a) {
         const int i = ienb[a][e];
@@ -567,8 +675,42 @@ int cfo_elmgmr_interior(int nshg, int numnp, int nel, const int *ien,
                         const int *iBC, const int *colm, const int *rowp,
                         int nnz, double *res, double *lhsK, double *lhsP,
                         double *qres, const cfo_params *p) {
-  return interior_core(nshg, numnp, nel, ien, y, ac, x, NULL, 1, iBC, colm,
+  return interior_core(nshg, numnp, nel, ien, y, ac, x, NULL, 1, iBC, NULL, 0, colm,
                        rowp, nnz, res, lhsK, lhsP, qres, p);
+}
+
+int cfo_elmgmr_interior_bc(int nshg, int numnp, int nel, const int *ien,
+                           const double *y, const double *ac, const double *x,
+                           const int *iBC, const double *BC, int ndofBC,
+                           const int *colm, const int *rowp, int nnz, double *res,
+                           double *lhsK, double *lhsP, const cfo_params *p) {
+  return interior_core(nshg, numnp, nel, ien, y, ac, x, NULL, 1, iBC, BC, ndofBC, colm,
+                       rowp, nnz, res, lhsK, lhsP, NULL, p);
+}
+
+int cfo_boundary_lhs(int nshg, int nelb, const int *ienb, const int *iBC, const double *BC,
+                     int ndofBC, const int *colm, const int *rowp, double *lhsK) {
+  /* the boundary loop's tangent (elmgmr.f90:364-372) for rigid walls: AsBMFG
+   * leaves xKebe zero (asbmfg.f90:71), bc3lhs sets the constrained diagonal
+   * entries of its partial-BC nodes to one, fillsparseI adds it */
+  if (!iBC || !BC || ndofBC < 6) return CFO_EARG;
+  double xK[144];
+  for (int e = 0; e < nelb; ++e) {
+    const int nodes[4] = {ienb[e], ienb[e + (size_t)nelb], ienb[e + 2 * (size_t)nelb], ienb[e + 3 * (size_t)nelb]};
+    memset(xK, 0, sizeof(xK));
+    bc3lhs_elem(nshg, iBC, BC, nodes, xK, 0, 1);
+    for (int a = 0; a < 4; ++a) {
+      const int i = nodes[a];
+      const int c = colm[i - 1];
+      const int len = colm[i] - c;
+      for (int b = 0; b < 4; ++b) {
+        const int k = sparseloc(rowp + (c - 1), len, nodes[b]) + c - 1;
+        if (rowp[k - 1] != nodes[b]) return CFO_ESEARCH;
+        for (int m = 0; m < 9; ++m) lhsK[m + 9 * (size_t)(k - 1)] += xK[m + 9 * (a + 4 * b)];
+      }
+    }
+  }
+  return CFO_OK;
 }
 
 int cfo_elmgmr_interior_q(int nshg, int numnp, int nel, const int *ien,
@@ -577,7 +719,7 @@ int cfo_elmgmr_interior_q(int nshg, int numnp, int nel, const int *ien,
                           const int *rowp, int nnz, double *res, double *lhsK,
                           double *lhsP, const cfo_params *p) {
   if (!q) return CFO_EARG;
-  return interior_core(nshg, numnp, nel, ien, y, ac, x, q, 0, iBC, colm, rowp,
+  return interior_core(nshg, numnp, nel, ien, y, ac, x, q, 0, iBC, NULL, 0, colm, rowp,
                        nnz, res, lhsK, lhsP, NULL, p);
 }
 

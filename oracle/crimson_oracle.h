@@ -98,6 +98,25 @@ int cfo_elmgmr_interior_q(int nshg, int numnp, int nel, const int *ien,
                           const int *rowp, int nnz, double *res, double *lhsK,
                           double *lhsP, const cfo_params *p);
 
+/* The same with BC(nshg,ndofBC) (ndofBC >= 6): 1- and 2-component velocity
+ * codes are projected out of each element's tangent by bc3lhs before the
+ * assembly (elmgmr.f90:223-226, bc3lhs.f90:41-690). */
+int cfo_elmgmr_interior_bc(int nshg, int numnp, int nel, const int *ien,
+                           const double *y, const double *ac, const double *x,
+                           const int *iBC, const double *BC, int ndofBC,
+                           const int *colm, const int *rowp, int nnz, double *res,
+                           double *lhsK, double *lhsP, const cfo_params *p);
+
+/* bc3lhs on one element's xKebe(9,4,4) (xK[(m-1) + 9 (a + 4 b)]); nodes
+ * 1-based. */
+void cfo_bc3lhs_elem(int nshg, const int *iBC, const double *BC, const int *nodes, double *xK);
+
+/* The boundary loop's tangent for rigid walls (elmgmr.f90:364-372): a zero
+ * xKebe per boundary element through bc3lhs and fillsparseI, added to
+ * lhsK(9,nnz).  ienb(nelb,4) 1-based. */
+int cfo_boundary_lhs(int nshg, int nelb, const int *ienb, const int *iBC, const double *BC,
+                     int ndofBC, const int *colm, const int *rowp, double *lhsK);
+
 /* bc3Res without periodicity (bc3res.f90:1-80). BC(nshg,ndofBC). */
 int cfo_bc3res(int nshg, const int *iBC, const double *BC, int ndofBC,
                double *res);

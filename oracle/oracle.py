@@ -53,6 +53,11 @@ def lib():
         L.cfo_e3q_block.argtypes = [C.c_int, dp, dp, dp, dp, C.POINTER(Params)]
         L.cfo_e3q_block.restype = None
         L.cfo_bc3res.argtypes = [C.c_int, ip, C.c_void_p, C.c_int, dp]
+        L.cfo_elmgmr_interior_bc.argtypes = [C.c_int, C.c_int, C.c_int, ip, dp, dp, dp, ip, dp, C.c_int,
+                                             ip, ip, C.c_int, dp, C.c_void_p, C.c_void_p, C.POINTER(Params)]
+        L.cfo_bc3lhs_elem.argtypes = [C.c_int, ip, dp, ip, dp]
+        L.cfo_bc3lhs_elem.restype = None
+        L.cfo_boundary_lhs.argtypes = [C.c_int, C.c_int, ip, ip, dp, C.c_int, ip, ip, dp]
         L.cfo_genadj.argtypes = [C.c_int, C.c_int, ip, ip, ip, C.c_int, C.POINTER(C.c_int)]
         L.cfo_pack_val4.argtypes = [C.c_int, ip, ip, dp, dp, dp]
         L.cfo_pack_val4.restype = None
@@ -113,8 +118,10 @@ def genadj(ien, nshg):
     return colm, rowp[: nnz.value].copy()
 
 
-def elmgmr_interior(ien, y, ac, x, colm, rowp, params, ibc=None, want_q=False, q=None):
-    """Interior ElmGMR: returns res(nshg,4), lhsK(9,nnz), lhsP(4,nnz)[, qres]."""
+def elmgmr_interior(ien, y, ac, x, colm, rowp, params, ibc=None, want_q=False, q=None, bc=None):
+    """Interior ElmGMR: returns res(nshg,4), lhsK(9,nnz), lhsP(4,nnz)[, qres].
+    With bc = BC(nshg,ndofBC >= 6), partial velocity codes in ibc are
+    projected out of the element tangents (bc3lhs)."""
     ien = _f(ien, np.int32)
     y, ac, x = _f(y), _f(ac), _f(x)
     nshg = y.shape[0]
@@ -123,6 +130,15 @@ def elmgmr_interior(ien, y, ac, x, colm, rowp, params, ibc=None, want_q=False, q
     lhsK = np.zeros((9, nnz), order="F") if params.lhs == 1 else None
     lhsP = np.zeros((4, nnz), order="F") if params.lhs == 1 else None
     ibc_a = None if ibc is None else _f(ibc, np.int32)
+    if bc is not None:
+        if ibc is None or want_q or q is not None:
+            raise ValueError("bc needs ibc, and neither want_q nor q")
+        bc_a = _f(bc)
+        rc = lib().cfo_elmgmr_interior_bc(nshg, x.shape[0], ien.shape[0], ien, y, ac, x, ibc_a, bc_a,
+                                          bc_a.shape[1], _f(colm, np.int32), _f(rowp, np.int32), nnz, res,
+                                          _ptr(lhsK), _ptr(lhsP), C.byref(params))
+        _check(rc, "elmgmr_interior_bc")
+        return res, lhsK, lhsP
     if q is not None:
         rc = lib().cfo_elmgmr_interior_q(nshg, x.shape[0], ien.shape[0], ien, y, ac, x, _f(q),
                                          _ptr(ibc_a), _f(colm, np.int32), _f(rowp, np.int32), nnz,
@@ -135,6 +151,25 @@ def elmgmr_interior(ien, y, ac, x, colm, rowp, params, ibc=None, want_q=False, q
                                    _ptr(lhsK), _ptr(lhsP), _ptr(qres), C.byref(params))
     _check(rc, "elmgmr_interior")
     return (res, lhsK, lhsP, qres) if want_q else (res, lhsK, lhsP)
+
+
+def bc3lhs_elem(ibc, bc, nodes, xK):
+    """bc3lhs on one element: xK(9,4,4) Fortran (K(m,a,b), a the row node),
+    nodes 1-based; returns a new array."""
+    out = _f(xK).copy(order="F")
+    lib().cfo_bc3lhs_elem(int(np.asarray(ibc).shape[0]), _f(ibc, np.int32), _f(bc), _f(nodes, np.int32), out)
+    return out
+
+
+def boundary_lhs(ienb, ibc, bc, colm, rowp, lhsK):
+    """lhsK(9,nnz) plus the boundary loop's tangent for rigid walls (bc3lhs's
+    diagonal ones of partial-BC nodes, one per boundary element)."""
+    out = _f(lhsK).copy(order="F")
+    bc_a = _f(bc)
+    _check(lib().cfo_boundary_lhs(bc_a.shape[0], int(np.asarray(ienb).shape[0]), _f(ienb, np.int32),
+                                  _f(ibc, np.int32), bc_a, bc_a.shape[1], _f(colm, np.int32),
+                                  _f(rowp, np.int32), out), "boundary_lhs")
+    return out
 
 
 def e3_block(yl, acl, xl, ql, params):

@@ -107,14 +107,23 @@ def test_form_setup_rejects_unsupported_flags(cfa, field, value, code):
     assert e.value.code == code
 
 
-def test_form_setup_rejects_partial_and_periodic_bc(cfa):
+def test_form_setup_rejects_periodic_and_order_dependent_code6(cfa):
+    """Periodic nodes, and a code-6 node sharing an element with another
+    1- or 2-component node (the reference's code-6 row operations then depend
+    on the element's local node order), are rejected before any GPU work."""
     m, colm, rowp = _small_graph()
-    for code in (8, 16 | 32, 1 << 10):
+    ibc = np.zeros(m.numnp, np.int32)
+    ibc[3] = 1 << 10
+    with pytest.raises(cfa.CrimsonError) as e:
+        cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp, iBC=ibc)
+    assert e.value.code == 2
+    for other in (8, 16, 8 | 32, 16 | 32):
         ibc = np.zeros(m.numnp, np.int32)
-        ibc[3] = code
+        ibc[m.ien[0, 0] - 1] = 16 | 32
+        ibc[m.ien[0, 2] - 1] = other
         with pytest.raises(cfa.CrimsonError) as e:
             cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp, iBC=ibc)
-        assert e.value.code == 2
+        assert e.value.code == 2 and "code 6" in str(e.value)
 
 
 def test_form_setup_rejects_bad_graph_and_connectivity(cfa):
