@@ -260,9 +260,10 @@ class ElmGMRInterior:
                                               -1 if perm is None else int(perm.size - 1)),
               "crimson_form_set_boundary")
 
-    # row schedule / FP64 incidence chunk / XCD dealing (one range per XCD, or turns of 64 workgroups)
+    # row schedule (numbering, Morton, Morton-ordered runs of 64 rows) / FP64 incidence chunk /
+    # XCD dealing (one range per XCD, or turns of 64 workgroups)
     AUTOTUNE_CANDIDATES = tuple(f"{s}/{c}/{d}" for d in ("range", "turns64") for c in (16, 8)
-                                for s in ("natural", "morton"))
+                                for s in ("natural", "morton", "tiles"))
 
     def autotune(self, y, ac, x, res4, lhs_handle, reps=3, only=None):
         """Time each launch configuration of to_lhs on these device arrays
@@ -271,7 +272,7 @@ class ElmGMRInterior:
         candidates to these names.  Results do not depend on the choice."""
         mask = 0 if only is None else sum(1 << self.AUTOTUNE_CANDIDATES.index(n) for n in only)
         choice = C.c_int()
-        ms = (C.c_double * 8)(*([float("nan")] * 8))
+        ms = (C.c_double * 12)(*([float("nan")] * 12))
         check(lib().crimson_form_autotune(self.h, _p(y), _p(ac), _p(x), _p(res4), lhs_handle.h, int(reps),
                                           int(mask), C.byref(choice), ms), "crimson_form_autotune")
         times = {n: ms[i] for i, n in enumerate(self.AUTOTUNE_CANDIDATES) if ms[i] == ms[i]}

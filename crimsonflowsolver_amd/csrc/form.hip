@@ -1217,52 +1217,59 @@ constexpr int kLocalSpan = 32768;
 int ensure_order(crimson_form *f, const double *d_x) {
   if (f->d_order.p) return CRIMSON_OK;
   const int n = f->nshg;
-  if (f->natural_sched) {
-    std::vector<int> order(n);
-    for (int i = 0; i < n; ++i) order[i] = i;
-    CR_TRY(f->d_order.upload(order.data(), n, f->stream));
-    CR_TRY(f->d_slot.alloc((size_t)2 * n));
-    slot_table_kernel<<<crimson::grid_for(n, 256), 256, 0, f->stream>>>(n, f->d_order.p, f->d_rowptr.p,
-                                                                        f->d_diagk.p, f->d_inc_ptr.p, f->d_slot.p);
-    CR_HIP(hipGetLastError());
-    CR_HIP(hipStreamSynchronize(f->stream));
-    return CRIMSON_OK;
-  }
-  const size_t np = f->numnp;
-  std::vector<double> x(np * 3);
-  CR_HIP(hipMemcpyAsync(x.data(), d_x, x.size() * sizeof(double), hipMemcpyDeviceToHost, f->stream));
-  CR_HIP(hipStreamSynchronize(f->stream));
-  double lo[3], hi[3];
-  for (int k = 0; k < 3; ++k) {
-    lo[k] = hi[k] = x[k * np];
-    for (int i = 0; i < n; ++i) {
-      lo[k] = std::min(lo[k], x[i + k * np]);
-      hi[k] = std::max(hi[k], x[i + k * np]);
-    }
-  }
-  auto spread = [](unsigned long long v) {  // 21 bits -> every third bit
-    v &= 0x1fffffULL;
-    v = (v | v << 32) & 0x1f00000000ffffULL;
-    v = (v | v << 16) & 0x1f0000ff0000ffULL;
-    v = (v | v << 8) & 0x100f00f00f00f00fULL;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
-    v = (v | v << 2) & 0x1249249249249249ULL;
-    return v;
-  };
-  std::vector<std::pair<unsigned long long, int>> key(n);
-  for (int i = 0; i < n; ++i) {
-    unsigned long long m = 0;
-    for (int k = 0; k < 3; ++k) {
-      const double span = hi[k] - lo[k];
-      const double t = span > 0 ? (x[i + k * np] - lo[k]) / span : 0.0;
-      const unsigned long long q = (unsigned long long)(t * 2097151.0);
-      m |= spread(q) << k;
-    }
-    key[i] = {m, i};
-  }
-  std::sort(key.begin(), key.end());
   std::vector<int> order(n);
-  for (int i = 0; i < n; ++i) order[i] = key[i].second;
+  if (f->sched == kSchedNumbering) {
+    for (int i = 0; i < n; ++i) order[i] = i;
+  } else {
+    const size_t np = f->numnp;
+    std::vector<double> x(np * 3);
+    CR_HIP(hipMemcpyAsync(x.data(), d_x, x.size() * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    CR_HIP(hipStreamSynchronize(f->stream));
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = hi[k] = x[k * np];
+      for (int i = 0; i < n; ++i) {
+        lo[k] = std::min(lo[k], x[i + k * np]);
+        hi[k] = std::max(hi[k], x[i + k * np]);
+      }
+    }
+    auto spread = [](unsigned long long v) {  // 21 bits -> every third bit
+      v &= 0x1fffffULL;
+      v = (v | v << 32) & 0x1f00000000ffffULL;
+      v = (v | v << 16) & 0x1f0000ff0000ffULL;
+      v = (v | v << 8) & 0x100f00f00f00f00fULL;
+      v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+      v = (v | v << 2) & 0x1249249249249249ULL;
+      return v;
+    };
+    auto morton = [&](const double (&p)[3]) {
+      unsigned long long m = 0;
+      for (int k = 0; k < 3; ++k) {
+        const double span = hi[k] - lo[k];
+        const double t = span > 0 ? (p[k] - lo[k]) / span : 0.0;
+        m |= spread((unsigned long long)(t * 2097151.0)) << k;
+      }
+      return m;
+    };
+    // Morton: every row by its node's code; tiles: runs of kSchedTile
+    // consecutive rows (whose plane segments are contiguous) by the code of
+    // their centroid, rows in order within a run
+    const int unit = f->sched == kSchedMorton ? 1 : kSchedTile;
+    const int nu = (n + unit - 1) / unit;
+    std::vector<std::pair<unsigned long long, int>> key(nu);
+    for (int u = 0; u < nu; ++u) {
+      const int i0 = u * unit, i1 = std::min(n, i0 + unit);
+      double c[3] = {0, 0, 0};
+      for (int i = i0; i < i1; ++i)
+        for (int k = 0; k < 3; ++k) c[k] += x[i + k * np];
+      for (int k = 0; k < 3; ++k) c[k] /= (i1 - i0);
+      key[u] = {morton(c), u};
+    }
+    std::sort(key.begin(), key.end());
+    int p = 0;
+    for (int u = 0; u < nu; ++u)
+      for (int i = key[u].second * unit; i < std::min(n, (key[u].second + 1) * unit); ++i) order[p++] = i;
+  }
   CR_TRY(f->d_order.upload(order.data(), n, f->stream));
   CR_TRY(f->d_slot.alloc((size_t)2 * n));
   slot_table_kernel<<<crimson::grid_for(n, 256), 256, 0, f->stream>>>(n, f->d_order.p, f->d_rowptr.p, f->d_diagk.p,
@@ -1532,7 +1539,7 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
         }
 
   crimson_form *f = new crimson_form();
-  f->natural_sched = natural_sched;
+  f->sched = natural_sched ? kSchedNumbering : kSchedMorton;
   f->nshg = nshg;
   f->numnp = numnp;
   f->nel = nel;
@@ -1663,8 +1670,9 @@ extern "C" int crimson_form_interior_to_lhs(crimson_form *f, const double *d_y, 
 }
 
 // Launch configurations of the formation chosen by timing them on the
-// caller's arrays: the assembly's row schedule (the caller's numbering or
-// Morton order), its workgroups' dealing to the XCDs (a contiguous range
+// caller's arrays: the assembly's row schedule (the caller's numbering,
+// Morton order of the nodes, or runs of consecutive rows in Morton order of
+// their centroids), its workgroups' dealing to the XCDs (a contiguous range
 // each, or turns of kAsmXcdChunk) and, for the FP64 assembly, its incidence
 // chunk (16 or 8).  No candidate changes a result: schedule and dealing
 // order rows, never sums, and both chunk sizes give incidence t to lane
@@ -1678,7 +1686,9 @@ extern "C" int crimson_form_autotune(crimson_form *f, const double *d_y, const d
     return CRIMSON_EARG;
   }
   const bool fp32 = f->prm.precision == 32;
-  const int ncand = 8, valid = fp32 ? 0x33 : 0xff;  // the FP32 assembly has no chunk choice
+  // candidate c: schedule c % 3, 8-incidence chunks (c / 3) % 2, XCD turns c / 6;
+  // the FP32 assembly has no chunk choice
+  const int ncand = 12, valid = fp32 ? 0x1c7 : 0xfff;
   if (mask == 0) mask = valid;
   if (mask & ~valid) {
     crimson::set_error("crimson_form_autotune: candidate mask names a configuration this handle lacks");
@@ -1688,11 +1698,11 @@ extern "C" int crimson_form_autotune(crimson_form *f, const double *d_y, const d
   int bestc = -1;
   for (int cand = 0; cand < ncand; ++cand) {
     if (!(mask & (1 << cand))) continue;
-    const bool natural = (cand & 1) == 0;
-    f->asm_chunk = (cand & 2) ? 8 : 16;
-    f->asm_xcd = (cand & 4) ? kAsmXcdChunk : 0;
-    if (f->natural_sched != natural || !f->d_order.p) {
-      f->natural_sched = natural;
+    const int sched = cand % 3;
+    f->asm_chunk = (cand / 3) % 2 ? 8 : 16;
+    f->asm_xcd = cand / 6 ? kAsmXcdChunk : 0;
+    if (f->sched != sched || !f->d_order.p) {
+      f->sched = sched;
       f->d_order.release();
       CR_TRY(ensure_order(f, d_x));
     }
@@ -1708,11 +1718,10 @@ extern "C" int crimson_form_autotune(crimson_form *f, const double *d_y, const d
       bestc = cand;
     }
   }
-  f->asm_chunk = (bestc & 2) ? 8 : 16;
-  f->asm_xcd = (bestc & 4) ? kAsmXcdChunk : 0;
-  const bool natural = (bestc & 1) == 0;
-  if (f->natural_sched != natural) {
-    f->natural_sched = natural;
+  f->asm_chunk = (bestc / 3) % 2 ? 8 : 16;
+  f->asm_xcd = bestc / 6 ? kAsmXcdChunk : 0;
+  if (f->sched != bestc % 3) {
+    f->sched = bestc % 3;
     f->d_order.release();
     CR_TRY(ensure_order(f, d_x));
   }

@@ -145,6 +145,11 @@ struct BclNode {
   double b[3];
 };
 
+// Row schedules of the assembly: the caller's numbering, Morton order of the
+// node coordinates, or runs of kSchedTile consecutive rows ordered by the
+// Morton code of their centroids.
+constexpr int kSchedNumbering = 0, kSchedMorton = 1, kSchedTiles = 2, kSchedTile = 64;
+
 // ---------------------------------------------------------------- handle
 struct crimson_form {
   int nshg = 0, numnp = 0, nel = 0, nnz = 0;
@@ -153,7 +158,7 @@ struct crimson_form {
   crimson::DBuf<int> d_ien, d_inc_ptr, d_inc, d_diagk, d_rowptr, d_ibc;
   crimson::DBuf<int> d_lst_ptr, d_order;  // d_order: assembly row schedule
   crimson::DBuf<int4> d_slot;              // per schedule slot: the row's offsets (slot_table_kernel)
-  bool natural_sched = false;               // schedule = the caller's numbering (set at setup)
+  int sched = 1;                            // assembly row schedule (kSched*; set at setup)
   int asm_chunk = 16;                       // incidences per staged chunk of the FP64 assembly
   unsigned asm_xcd = 0;                     // assembly workgroups dealt per XCD turn (0: one range each)
   crimson::DBuf<unsigned short> d_lst;

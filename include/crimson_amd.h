@@ -153,12 +153,14 @@ int crimson_form_set_boundary(crimson_form *f, int nelb, const int *ienb,
  * the memLS handle `lhs` (crimson_form_interior_to_lhs, reps calls after one
  * warm-up per candidate, assembly phase from HIP events) for each launch
  * configuration and keeps the fastest for the handle's later calls:
- * candidate bit 0 = Morton row schedule (else the caller's numbering), bit
- * 1 = 8-incidence assembly chunks (else 16; FP64 handles only), bit 2 =
- * workgroups dealt to the XCDs in turns of 64 (else one contiguous range
- * per XCD).  mask selects the candidates tried (bit c = candidate c; 0 =
- * all valid ones).  *choice receives the winner, ms_out (8 doubles, may be
- * NULL) the mean assembly milliseconds of each candidate tried.  No configuration changes a
+ * candidate c = 0..11: row schedule c % 3 (0 the caller's numbering, 1
+ * Morton order of the nodes, 2 runs of 64 consecutive rows in Morton order
+ * of their centroids), 8-incidence assembly chunks if (c / 3) % 2 (else
+ * 16; FP64 handles only), workgroups dealt to the XCDs in turns of 64 if
+ * c / 6 (else one contiguous range per XCD).  mask selects the candidates
+ * tried (bit c = candidate c; 0 = all valid ones).  *choice receives the
+ * winner, ms_out (12 doubles, may be NULL) the mean assembly milliseconds
+ * of each candidate tried.  No configuration changes a
  * result; the matrix and res hold the last candidate's formation. */
 int crimson_form_autotune(crimson_form *f, const double *d_y, const double *d_ac,
                           const double *d_x, double *d_res, crimson_lhs *lhs,

@@ -215,8 +215,8 @@ def test_formation_scattered_numbering(cfa, oracle_lib):
 
 @pytest.mark.parametrize("precision", [64, 32])
 def test_autotune_candidates_bitwise(cfa, oracle_lib, precision):
-    """Every launch configuration the autotuner may pick (row schedule x
-    assembly chunk x XCD dealing) forms bitwise the same residual and memLS
+    """Every launch configuration the autotuner may pick (row schedule:
+    numbering, Morton, Morton-ordered runs; assembly chunk; XCD dealing) forms bitwise the same residual and memLS
     matrix, on a jittered Delaunay mesh whose rows take 1-3 chunks of 16 and
     1-6 of 8 (ragged last chunks), and on a cylinder of 1,100+ assembly
     workgroups (so that dealing in turns of 64 remaps all but the last
@@ -236,7 +236,7 @@ def test_autotune_candidates_bitwise(cfa, oracle_lib, precision):
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
     ty, tac, tx = T(y), T(ac), T(m.x)
     tres4 = torch.zeros((m.numnp, 4), dtype=torch.float64, device="cuda")
-    names = [n for i, n in enumerate(cfa.ElmGMRInterior.AUTOTUNE_CANDIDATES) if precision == 64 or not i & 2]
+    names = [n for i, n in enumerate(cfa.ElmGMRInterior.AUTOTUNE_CANDIDATES) if precision == 64 or not (i // 3) % 2]
     outs = []
     for name in names:
         chosen, times = form.autotune(ty, tac, tx, tres4, lhs, reps=1, only=[name])
