@@ -42,6 +42,7 @@ EXPORTS = (
     "crimson_itr_bc_dev", "crimson_elmpvsq_dev", "crimson_form_sync", "crimson_lhs_set_precision",
     "crimson_form_vjp_dev", "crimson_form_set_instances", "crimson_form_graph_create", "crimson_graph_launch",
     "crimson_graph_free", "crimson_form_set_boundary", "crimson_form_autotune", "crimson_form_set_bc",
+    "crimson_elmpvsq_coef_dev",
 )
 
 
@@ -113,6 +114,7 @@ def lib():
         L.crimson_form_sync.argtypes = [vp]
         L.crimson_form_set_boundary.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_int]
         L.crimson_form_set_bc.argtypes = [vp, vp, C.c_int]
+        L.crimson_elmpvsq_coef_dev.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp, vp]
         L.crimson_form_autotune.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         _lib = L
     return _lib
@@ -322,6 +324,12 @@ class ElmGMRInterior:
     def elmpvsq(self, glob_dev, nabi_dev, R, y, res4, q_dev):
         check(lib().crimson_elmpvsq_dev(self.h, int(glob_dev.numel()), _p(glob_dev), _p(nabi_dev), R, _p(y),
                                         _p(res4), _p(q_dev)), "ElmpvsQ")
+
+    def elmpvsq_coef(self, glob_dev, nabi_dev, c1, c2, y, res4, q_dev):
+        """A coupled surface with its 0D model's implicit coefficients
+        (P = c1 Q + c2; crimsonflowsolver_amd.zerod)."""
+        check(lib().crimson_elmpvsq_coef_dev(self.h, int(glob_dev.numel()), _p(glob_dev), _p(nabi_dev), float(c1),
+                                             float(c2), _p(y), _p(res4), _p(q_dev)), "ElmpvsQ")
 
     def sync(self):
         check(lib().crimson_form_sync(self.h), "crimson_form_sync")

@@ -88,12 +88,14 @@ __global__ void __launch_bounds__(256) flowq_kernel(int nf, const int *__restric
   if (threadIdx.x == 0) *q = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
-// res(i,1:3) += p NABI(i,1:3) with p = -Q R (elmgmr.f90:1153-1164)
+// res(i,1:3) += p NABI(i,1:3) with p = -(c1 Q + c2): the residual side of
+// every coupled surface of ElmpvsQ (elmgmr.f90:1153-1330; a resistance has
+// c1 = R, c2 = 0)
 __global__ void pvsq_kernel(int nf, const int *__restrict__ glob, const double *__restrict__ nabi,
-                            const double *__restrict__ q, double R, double *__restrict__ res4) {
+                            const double *__restrict__ q, double c1, double c2, double *__restrict__ res4) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= nf) return;
-  const double p = -1.0 * *q * R;
+  const double p = -(c1 * *q + c2);
   const size_t g = glob[a];
 #pragma unroll
   for (int c = 0; c < 3; ++c) res4[4 * g + c] = res4[4 * g + c] + p * nabi[3 * (size_t)a + c];
@@ -161,7 +163,23 @@ extern "C" int crimson_elmpvsq_dev(crimson_form *f, int nface, const int *d_glob
   const int N = form_nshg(f);
   hipStream_t s = form_stream(f);
   flowq_kernel<<<1, 256, 0, s>>>(nface, d_glob, d_nabi, d_y, N, d_q);
-  if (nface > 0) pvsq_kernel<<<crimson::grid_for(nface, 256), 256, 0, s>>>(nface, d_glob, d_nabi, d_q, R, d_Res4);
+  if (nface > 0)
+    pvsq_kernel<<<crimson::grid_for(nface, 256), 256, 0, s>>>(nface, d_glob, d_nabi, d_q, R, 0.0, d_Res4);
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_elmpvsq_coef_dev(crimson_form *f, int nface, const int *d_glob, const double *d_nabi,
+                                        double c1, double c2, const double *d_y, double *d_Res4, double *d_q) {
+  if (!f || nface < 0 || (nface > 0 && (!d_glob || !d_nabi)) || !d_y || !d_Res4 || !d_q) {
+    crimson::set_error("crimson_elmpvsq_coef_dev: invalid arguments");
+    return CRIMSON_EARG;
+  }
+  const int N = form_nshg(f);
+  hipStream_t s = form_stream(f);
+  flowq_kernel<<<1, 256, 0, s>>>(nface, d_glob, d_nabi, d_y, N, d_q);
+  if (nface > 0)
+    pvsq_kernel<<<crimson::grid_for(nface, 256), 256, 0, s>>>(nface, d_glob, d_nabi, d_q, c1, c2, d_Res4);
   CR_HIP(hipGetLastError());
   return CRIMSON_OK;
 }

@@ -235,6 +235,14 @@ int crimson_itr_bc_dev(crimson_form *f, const int *d_iBC, const double *d_BC,
 int crimson_elmpvsq_dev(crimson_form *f, int nface, const int *d_glob,
                         const double *d_nabi, double R, const double *d_y,
                         double *d_Res4, double *d_q);
+/* Any coupled surface of ElmpvsQ (elmgmr.f90:1152-1330: resistance,
+ * impedance, RCR, netlist, ...) given its 0D model's implicit coefficients
+ * for this step, P = c1 Q + c2: Res4(1:3,i) += -(c1 Q + c2) NABI(i,1:3)
+ * over d_glob; Q in *d_q.  memLS's face coefficient of the surface is
+ * c1 alfi gami Delt (AddElmpvsQFormemLS, :1775-1856). */
+int crimson_elmpvsq_coef_dev(crimson_form *f, int nface, const int *d_glob,
+                             const double *d_nabi, double c1, double c2,
+                             const double *d_y, double *d_Res4, double *d_q);
 int crimson_form_sync(crimson_form *f);
 
 /* ------------------------------------------------------------------ */

@@ -337,6 +337,16 @@ def itr_bc(ibc, bc, y):
     return y
 
 
+def elmpvsq_coef(res, y, glob, nabi, c1, c2):
+    """ElmpvsQ's term of a coupled surface with implicit coefficients
+    (elmgmr.f90:1210-1228 for RCR, the same form for every 0D model):
+    res(glob,1:3) += -(c1 Q + c2) NABI."""
+    res = np.asfortranarray(res.copy())
+    q = float(np.sum(nabi * y[glob, :3]))
+    res[glob, :3] += (-(c1 * q + c2)) * nabi
+    return res, q
+
+
 def elmpvsq_resistance(res, y, glob, nabi, R):
     """ElmpvsQ's resistance term (elmgmr.f90:1152-1176): Q = sum NABI.u over
     the face's nodes, res(glob,1:3) += -R Q NABI."""
