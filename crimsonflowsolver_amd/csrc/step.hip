@@ -54,17 +54,47 @@ __global__ void correct_kernel(int N, double *__restrict__ y, double *__restrict
   y[i + 3 * n] = y[i + 3 * n] + fct2 * res4[4 * (size_t)i + 3];
 }
 
-// itrBC for codes 0/7 and the pressure bit (itrbc.f90:83-98)
+// itrBC (itrbc.f90:20-98) without periodic nodes: the constrained velocity
+// components from BC(:,3:6) for codes 1-7, the pressure from BC(:,1) for
+// bit 2.  BC(N, ndofBC), ndofBC >= 6.
 __global__ void itrbc_kernel(int N, const int *__restrict__ ibc, const double *__restrict__ BC, int ndofBC,
                              double *__restrict__ y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int c = ibc[i];
   const size_t n = N;
-  if (((c >> 3) & 7) == 7) {
-    y[i] = BC[i + 2 * n];
-    y[i + n] = BC[i + 3 * n];
-    y[i + 2 * n] = BC[i + 4 * n];
+  const int vb = (c >> 3) & 7;
+  if (vb != 0) {
+    const double b3 = BC[i + 2 * n], b4 = BC[i + 3 * n], b5 = BC[i + 4 * n], b6 = BC[i + 5 * n];
+    const double u1 = y[i], u2 = y[i + n], u3 = y[i + 2 * n];
+    switch (vb) {
+      case 1:
+        y[i] = b3 - b4 * u2 - b5 * u3;
+        break;
+      case 2:
+        y[i + n] = b3 - b4 * u1 - b5 * u3;
+        break;
+      case 3:
+        y[i] = b3 - b4 * u3;
+        y[i + n] = b5 - b6 * u3;
+        break;
+      case 4:
+        y[i + 2 * n] = b3 - b4 * u1 - b5 * u2;
+        break;
+      case 5:
+        y[i] = b3 - b4 * u2;
+        y[i + 2 * n] = b5 - b6 * u2;
+        break;
+      case 6:
+        y[i + n] = b3 - b4 * u1;
+        y[i + 2 * n] = b5 - b6 * u1;
+        break;
+      default:  // 7
+        y[i] = b3;
+        y[i + n] = b4;
+        y[i + 2 * n] = b5;
+        break;
+    }
   }
   if ((c >> 2) & 1) y[i + 3 * n] = BC[i];
   (void)ndofBC;
@@ -144,8 +174,8 @@ extern "C" int crimson_itr_correct_dev(crimson_form *f, double *d_y, double *d_a
 }
 
 extern "C" int crimson_itr_bc_dev(crimson_form *f, const int *d_iBC, const double *d_BC, int ndofBC, double *d_y) {
-  if (!f || !d_iBC || !d_BC || !d_y || ndofBC < 5) {
-    crimson::set_error("crimson_itr_bc_dev: invalid arguments (BC(nshg,ndofBC) needs ndofBC >= 5)");
+  if (!f || !d_iBC || !d_BC || !d_y || ndofBC < 6) {
+    crimson::set_error("crimson_itr_bc_dev: invalid arguments (BC(nshg,ndofBC) needs ndofBC >= 6)");
     return CRIMSON_EARG;
   }
   const int N = form_nshg(f);

@@ -225,8 +225,8 @@ int crimson_itr_yalpha_dev(crimson_form *f, const double *d_yold,
 int crimson_itr_correct_dev(crimson_form *f, double *d_y, double *d_ac,
                             const double *d_Res4, double gami, double alfi,
                             double Delt);
-/* itrBC (itrbc.f90:1-100) for velocity codes 0/7 and the pressure bit;
- * d_iBC(nshg), d_BC(nshg,ndofBC) column-major, ndofBC >= 5. */
+/* itrBC (itrbc.f90:1-100, no periodic nodes): velocity codes 1-7 and the
+ * pressure bit; d_iBC(nshg), d_BC(nshg,ndofBC) column-major, ndofBC >= 6. */
 int crimson_itr_bc_dev(crimson_form *f, const int *d_iBC, const double *d_BC,
                        int ndofBC, double *d_y);
 /* Resistance-outlet term of ElmpvsQ (elmgmr.f90:1152-1176): Q = sum NABI.u

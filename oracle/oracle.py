@@ -327,11 +327,23 @@ def itr_correct(y, ac, sol4, gami, alfi, delt):
 
 
 def itr_bc(ibc, bc, y):
-    """itrBC for velocity codes 0/7 and the pressure bit (itrbc.f90:83-98):
-    BC(:,3:5) = the prescribed velocity, BC(:,1) = the pressure."""
+    """itrBC (itrbc.f90:20-98) without periodic nodes: code 7 sets the
+    velocity to BC(:,3:5); codes 1-6 set the constrained components from
+    BC(:,3:6) and the free ones; bit 2 sets the pressure to BC(:,1)."""
     y = np.asfortranarray(y.copy())
-    v = ((ibc >> 3) & 7) == 7
-    y[v, :3] = bc[v, 2:5]
+    code = (ibc >> 3) & 7
+    u1, u2, u3 = y[:, 0].copy(), y[:, 1].copy(), y[:, 2].copy()
+    b = [bc[:, k] for k in range(min(bc.shape[1], 6))] + [np.zeros(y.shape[0])] * max(0, 6 - bc.shape[1])
+    for c, comps in ((1, {0: b[2] - b[3] * u2 - b[4] * u3}),
+                     (2, {1: b[2] - b[3] * u1 - b[4] * u3}),
+                     (3, {0: b[2] - b[3] * u3, 1: b[4] - b[5] * u3}),
+                     (4, {2: b[2] - b[3] * u1 - b[4] * u2}),
+                     (5, {0: b[2] - b[3] * u2, 2: b[4] - b[5] * u2}),
+                     (6, {1: b[2] - b[3] * u1, 2: b[4] - b[5] * u1}),
+                     (7, {0: b[2], 1: b[3], 2: b[4]})):
+        sel = code == c
+        for k, v in comps.items():
+            y[sel, k] = v[sel]
     p = ((ibc >> 2) & 1) == 1
     y[p, 3] = bc[p, 0]
     return y

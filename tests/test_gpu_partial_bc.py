@@ -129,3 +129,19 @@ def test_bc3res_partial_codes_device(cfa, oracle_lib):
     t4 = torch.from_numpy(np.ascontiguousarray(res)).cuda()  # Res4(4,nshg)
     form.bc3res(t4, res4_layout=True)
     assert _rel(t4.cpu().numpy(), want) <= 1e-15
+
+
+def test_itr_bc_partial_codes_device(cfa, oracle_lib):
+    """itrBC on the device for codes 1-7 and the pressure bit (itrbc.f90)."""
+    import torch
+    m, ibc, bc, y, ac = _case(14)
+    codes = np.random.default_rng(14).integers(0, 8, m.numnp)
+    ibc = ((codes << 3) | (ibc & 4)).astype(np.int32)  # every code, no element restriction needed here
+    colm, rowp = cfa.genadj(m.ien, m.numnp)
+    form = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).cuda()
+    ty = T(y)
+    form.itr_bc(torch.from_numpy(ibc).cuda(), T(bc), ty)
+    form.sync()
+    want = oracle_lib.itr_bc(ibc, bc, y)
+    assert _rel(ty.cpu().numpy().T, want) <= 1e-15

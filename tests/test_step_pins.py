@@ -44,12 +44,36 @@ def test_yalpha_endpoints():
 def test_itr_bc_codes():
     y, _ = _fields(6, 5)
     bc = np.asfortranarray(np.arange(42, dtype=float).reshape(6, 7))
-    ibc = np.array([0, 56, 4, 60, 8, 24], np.int32)  # none, velocity, pressure, both, partial x2
+    ibc = np.array([0, 56, 4, 60, 0, 0], np.int32)  # none, velocity, pressure, both
     out = O.itr_bc(ibc, bc, y)
     assert np.array_equal(out[0], y[0]) and np.array_equal(out[4], y[4]) and np.array_equal(out[5], y[5])
     assert np.array_equal(out[1, :3], bc[1, 2:5]) and out[1, 3] == y[1, 3]
     assert out[2, 3] == bc[2, 0] and np.array_equal(out[2, :3], y[2, :3])
     assert np.array_equal(out[3, :3], bc[3, 2:5]) and out[3, 3] == bc[3, 0]
+
+
+def test_itr_bc_partial_codes_lie_on_the_bc3lhs_constraint():
+    """Codes 1-6: the free components are kept and the constrained ones are
+    set on the affine constraint whose linear part is the substitution that
+    bc3lhs / bc3Res project with (u_c = offset_c + (P u_free)_c), the
+    offsets being BC(:,3) and, for the second component, BC(:,5)."""
+    from test_bc3lhs_pins import CONSTRAINED, subst
+    rng = np.random.default_rng(4)
+    for code in range(1, 7):
+        y = np.asfortranarray(rng.standard_normal((3, 4)))
+        bc = np.asfortranarray(rng.uniform(-2, 2, (3, 7)))
+        ibc = np.array([0, code << 3, 0], np.int32)
+        out = O.itr_bc(ibc, bc, y)
+        cons = CONSTRAINED[code]
+        free = [k for k in range(3) if k not in cons]
+        assert np.array_equal(out[1, free], y[1, free]) and out[1, 3] == y[1, 3]
+        u = np.zeros(3)
+        u[free] = y[1, free]
+        lin = subst(code, bc[1]) @ u
+        offs = [bc[1, 2], bc[1, 4]]
+        for i, c in enumerate(cons):
+            assert abs(out[1, c] - (offs[i] + lin[c])) <= 1e-14, (code, c)
+        assert np.array_equal(out[[0, 2]], y[[0, 2]])
 
 
 def test_resistance_flow_of_uniform_velocity():
