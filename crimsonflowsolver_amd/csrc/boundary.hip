@@ -45,10 +45,11 @@ __device__ inline double tri_pt(int nqb, int q, int a) {
 __global__ void __launch_bounds__(256) bnd_elem(int nelb, const int *__restrict__ ienb,
                                                 const int *__restrict__ bcode, const double *__restrict__ bcb,
                                                 const int *__restrict__ permit, int nsrf, int nqb, double influx,
-                                                double rho, double mu, const double *__restrict__ node,
+                                                double rho, double mu0, Ens en, const double *__restrict__ node,
                                                 double *__restrict__ brec) {
   const int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= nelb) return;
+  const double mu = ens_mu(en, e, mu0);
   double xl[4][3], yl[4][4];  // yl(a, (p, u1, u2, u3)) as localy orders it
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -216,11 +217,12 @@ __global__ void __launch_bounds__(256) bnd_node(int nbnode, int nshg, const int 
 // (constants carry no derivative) and the influx term's branch.
 __global__ void __launch_bounds__(256) bnd_vjp_elem(int nelb, int nshg, const int *__restrict__ ienb,
                                                     const int *__restrict__ bcode, const int *__restrict__ permit,
-                                                    int nsrf, int nqb, double influx, double rho, double mu,
+                                                    int nsrf, int nqb, double influx, double rho, double mu0, Ens en,
                                                     const double *__restrict__ node, const double *__restrict__ w,
                                                     double *__restrict__ vrec) {
   const int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= nelb) return;
+  const double mu = ens_mu(en, e, mu0);
   double xl[4][3], yl[4][4], wl[3][4];
   int g[4];
 #pragma unroll
@@ -408,13 +410,34 @@ __global__ void __launch_bounds__(256) bnd_vjp_mu(int nelb, const double *__rest
 
 }  // namespace
 
+// The ensemble instance of each boundary element: that of the first element
+// holding its first node (instances have disjoint nodes).
+__global__ void bnd_inst_kernel(int nelb, const int *__restrict__ ienb, const int *__restrict__ inc_ptr,
+                                const int *__restrict__ inc, const unsigned short *__restrict__ inst,
+                                unsigned short *__restrict__ binst) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelb) return;
+  const int n = ienb[e], p = inc_ptr[n];
+  binst[e] = p < inc_ptr[n + 1] ? inst[inc[p] >> 2] : 0;  // a node of no element: instance 0
+}
+
 namespace crimson {
+int form_boundary_instances(crimson_form *f) {
+  if (f->nelb == 0 || f->ninst == 0) return CRIMSON_OK;
+  CR_TRY(f->d_binst.alloc(f->nelb));
+  bnd_inst_kernel<<<grid_for(f->nelb, 256), 256, 0, f->stream>>>(f->nelb, f->d_ienb.p, f->d_inc_ptr.p, f->d_inc.p,
+                                                                 f->d_inst.p, f->d_binst.p);
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipStreamSynchronize(f->stream));
+  return CRIMSON_OK;
+}
 int form_boundary_pass(crimson_form *f, double *d_res, int mode) {
   if (f->nelb == 0) return CRIMSON_OK;
   hipStream_t s = f->stream;
   bnd_elem<<<grid_for(f->nelb, 256), 256, 0, s>>>(f->nelb, f->d_ienb.p, f->d_bcode.p, f->d_bcb.p,
                                                   f->nsrf >= 0 ? f->d_permit.p : nullptr, f->nsrf, f->nqb,
-                                                  f->influx, f->prm.rho, f->prm.mu, f->d_node.p, f->d_brec.p);
+                                                  f->influx, f->prm.rho, f->prm.mu, f->bens(), f->d_node.p,
+                                                  f->d_brec.p);
   if (mode == 0)
     bnd_node<0><<<grid_for(f->nbnode, 256), 256, 0, s>>>(f->nbnode, f->nshg, f->d_bnode.p, f->d_binc_ptr.p,
                                                          f->d_binc.p, f->d_brec.p, d_res);
@@ -431,7 +454,7 @@ int form_boundary_vjp(crimson_form *f, const double *d_w, double *d_ybar, double
   CR_TRY(f->d_bvrec.alloc((size_t)kBV * f->nelb + 1));
   bnd_vjp_elem<<<grid_for(f->nelb, 256), 256, 0, s>>>(f->nelb, f->nshg, f->d_ienb.p, f->d_bcode.p,
                                                       f->nsrf >= 0 ? f->d_permit.p : nullptr, f->nsrf, f->nqb,
-                                                      f->influx, f->prm.rho, f->prm.mu, f->d_node.p, d_w,
+                                                      f->influx, f->prm.rho, f->prm.mu, f->bens(), f->d_node.p, d_w,
                                                       f->d_bvrec.p);
   bnd_vjp_node<<<grid_for(f->nvnode, 256), 256, 0, s>>>(f->nvnode, f->nshg, f->d_vnode.p, f->d_vptr.p,
                                                         f->d_vinc.p, f->d_bvrec.p, d_ybar);
@@ -458,10 +481,6 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
   }
   if (nelb < 0 || !ienb || !iBCB || !BCB || ndBCB < 2 || (nqb != 1 && nqb != 3) || (permit && nsrf < 0))
     return crimson::set_error("crimson_form_set_boundary: bad arguments"), CRIMSON_EARG;
-  if (f->ninst > 0)
-    return crimson::set_error("crimson_form_set_boundary: boundary elements with ensemble instances are not "
-                              "supported"),
-           CRIMSON_EUNSUPPORTED;
   std::vector<int> ib((size_t)4 * nelb), code((size_t)2 * nelb);
   std::vector<double> bcb((size_t)nelb * 15, 0.0);
   for (int e = 0; e < nelb; ++e) {
@@ -534,6 +553,7 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
   f->nqb = nqb;
   f->influx = influx;
   f->nsrf = permit ? nsrf : -1;
+  CR_TRY(crimson::form_boundary_instances(f));
   if (f->npartial > 0) {
     // the boundary loop's bc3lhs sets the constrained diagonal entries of
     // each boundary element's partial-BC nodes to one (elmgmr.f90:364-372)

@@ -1730,10 +1730,6 @@ extern "C" int crimson_form_autotune(crimson_form *f, const double *d_y, const d
 }
 
 extern "C" int crimson_form_set_instances(crimson_form *f, const unsigned short *inst, int ninst, const double *mu) {
-  if (f && f->nelb > 0 && ninst > 0)
-    return crimson::set_error("crimson_form_set_instances: ensemble instances with boundary elements are not "
-                              "supported"),
-           CRIMSON_EUNSUPPORTED;
   if (!f || ninst < 0 || (ninst > 0 && (!inst || !mu))) {
     crimson::set_error("crimson_form_set_instances: invalid arguments");
     return CRIMSON_EARG;
@@ -1756,7 +1752,7 @@ extern "C" int crimson_form_set_instances(crimson_form *f, const unsigned short 
   CR_TRY(f->d_mu_inst.upload(mu, (size_t)ninst, f->stream));
   CR_HIP(hipStreamSynchronize(f->stream));
   f->ninst = ninst;
-  return CRIMSON_OK;
+  return crimson::form_boundary_instances(f);
 }
 
 struct crimson_graph {

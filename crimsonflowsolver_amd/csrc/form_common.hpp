@@ -184,6 +184,8 @@ struct crimson_form {
   crimson::DBuf<double> d_mu_inst;
   int ninst = 0;
   Ens ens() const { return Ens{ninst ? d_inst.p : nullptr, ninst ? d_mu_inst.p : nullptr}; }
+  crimson::DBuf<unsigned short> d_binst;  // instance of each boundary element (form_boundary_instances)
+  Ens bens() const { return Ens{ninst && nelb ? d_binst.p : nullptr, ninst ? d_mu_inst.p : nullptr}; }
   // boundary elements (crimson_form_set_boundary)
   int nelb = 0, nbnode = 0, nqb = 3, nsrf = -1;
   double influx = 0.5;
@@ -208,6 +210,9 @@ int form_flux_pass(crimson_form *f, const double *d_y, const double *d_ac, const
 // 1: Res4(4,nshg)) from the node records of the current formation; a no-op
 // without boundary elements.  Enqueued on the handle's stream.
 int form_boundary_pass(crimson_form *f, double *d_res, int mode);
+// The ensemble instance of each boundary element (after either of
+// crimson_form_set_boundary / crimson_form_set_instances changes).
+int form_boundary_instances(crimson_form *f);
 // Adjoint of the boundary rows: ybar(nshg,4) += w^T d rl_bnd/d y from the
 // node records of the last flux pass; *mubar = w^T d rl_bnd/d mu.
 int form_boundary_vjp(crimson_form *f, const double *d_w, double *d_ybar, double *mubar);

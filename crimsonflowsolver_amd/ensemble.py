@@ -23,7 +23,7 @@ from . import mesh as M
 
 
 class Ensemble:
-    def __init__(self, meshes, mu, R, precision=64, params=None, outlet_natural=True):
+    def __init__(self, meshes, mu, R, precision=64, params=None, outlet_natural=True, boundary=False):
         self.meshes = list(meshes)
         K = len(self.meshes)
         self.K = K
@@ -58,6 +58,16 @@ class Ensemble:
         self.params = p
         self.form = ElmGMRInterior(self.ien, self.colm, self.rowp, N, iBC=self.iBC, params=p)
         self.form.set_instances(inst, self.mu)
+        self.nelb = 0
+        if boundary:
+            # every instance's boundary elements (inlet, wall, outlet), in
+            # instance order, on the union's node numbers
+            parts = [M.cylinder_boundary(m) for m in self.meshes]
+            ienb = np.concatenate([b[0] + int(o) for b, o in zip(parts, self.node_off)]).astype(np.int32)
+            iBCB = np.concatenate([b[1] for b in parts]).astype(np.int32)
+            BCB = np.concatenate([b[2] for b in parts])
+            self.form.set_boundary(ienb, iBCB, BCB)
+            self.nelb = ienb.shape[0]
         self.lhs = memLS_LHS_CREATE(N, N, self.rowp.size, None, self.colm, self.rowp, 1 + K)
         memLS_BC_CREATE(self.lhs, 1, self.fixed.size, 3, BC_TYPE_Dir, self.fixed + 1,
                         np.zeros((3, self.fixed.size), order="F"))

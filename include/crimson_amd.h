@@ -141,7 +141,8 @@ int crimson_form_timing(const crimson_form *f, double *out4);
  * srfID (callCPPDiscoverWhetherFlowPermittedAcrossSurface, e3b.f90; NULL: all
  * permitted).  nelb = 0 removes the boundary.  Deformable walls, the heart
  * model, Lagrange multipliers and the surface flux integrals (flxID) are not
- * formed.  CRIMSON_EUNSUPPORTED with ensemble instances.  The VJP
+ * formed.  With ensemble instances each boundary element takes the
+ * viscosity of the instance that holds its nodes.  The VJP
  * (crimson_form_vjp_dev) of a handle with boundary elements includes their
  * rows. */
 int crimson_form_set_boundary(crimson_form *f, int nelb, const int *ienb,
@@ -170,7 +171,8 @@ int crimson_form_autotune(crimson_form *f, const double *d_y, const double *d_ac
  * the handle's mesh is the union of several instances' meshes (disjoint
  * node ranges), inst(nel) gives each element's 0-based instance and mu(ninst)
  * each instance's viscosity, which replaces crimson_form_params.mu for its
- * elements.  ninst = 0 returns to the single viscosity. */
+ * elements (and its boundary elements, crimson_form_set_boundary).  ninst =
+ * 0 returns to the single viscosity. */
 int crimson_form_set_instances(crimson_form *f, const unsigned short *inst,
                                int ninst, const double *mu);
 

@@ -134,12 +134,24 @@ def test_mixed_precision_residual_with_boundary(cfa, oracle_lib):
     assert _rel(res, rres) <= TOL
 
 
-def test_boundary_rejects_ensembles(cfa):
+def test_boundary_with_one_instance_equals_plain(cfa):
+    """An ensemble of one instance with boundary elements forms bitwise what
+    the plain handle does (either order of the two calls)."""
     from crimsonflowsolver_amd import mesh as M
-    m = M.cylinder_mesh(4, 4, 4)
+    m = M.cylinder_mesh(4, 4, 6)
+    y, ac = _cylinder_state(m, 3)
     ienb, iBCB, BCB = M.cylinder_boundary(m)
     colm, rowp = cfa.genadj(m.ien, m.numnp)
-    form = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp)
-    form.set_boundary(ienb, iBCB, BCB)
-    with pytest.raises(cfa.CrimsonError):
-        form.set_instances(np.zeros(m.nel, np.uint16), np.array([0.04]))
+    plain = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp, params=cfa.form_params(mu=0.05))
+    plain.set_boundary(ienb, iBCB, BCB)
+    want = plain(y, ac, m.x)
+    for first_boundary in (True, False):
+        form = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp)
+        if first_boundary:
+            form.set_boundary(ienb, iBCB, BCB)
+        form.set_instances(np.zeros(m.nel, np.uint16), np.array([0.05]))
+        if not first_boundary:
+            form.set_boundary(ienb, iBCB, BCB)
+        got = form(y, ac, m.x)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b)

@@ -36,12 +36,16 @@ def _T(torch, a):
     return torch.from_numpy(np.ascontiguousarray(np.asarray(a, order="F").T)).cuda()
 
 
-@pytest.mark.parametrize("precision", [64, 32])
-def test_union_formation_equals_instances(env, precision):
+@pytest.mark.parametrize("precision,boundary", [(64, False), (32, False), (64, True), (32, True)])
+def test_union_formation_equals_instances(env, precision, boundary):
+    """Each instance's rows and blocks of the union formation are bitwise
+    those of the instance formed alone (with its boundary elements, in its
+    own viscosity, when boundary)."""
     cfa, torch = env
     from crimsonflowsolver_amd.ensemble import Ensemble
+    from crimsonflowsolver_amd import mesh as M
     meshes, mu, R, states = _instances()
-    ens = Ensemble(meshes, mu, R, precision=precision)
+    ens = Ensemble(meshes, mu, R, precision=precision, boundary=boundary)
     y = np.concatenate([s[0] for s in states])
     ac = np.concatenate([s[1] for s in states])
     ty, tac, tx = _T(torch, y), _T(torch, ac), _T(torch, ens.x)
@@ -52,6 +56,8 @@ def test_union_formation_equals_instances(env, precision):
         colm, rowp = cfa.genadj(m.ien, m.numnp)
         f = cfa.ElmGMRInterior(m.ien, colm, rowp, m.numnp, params=cfa.form_params(mu=float(mu[k]),
                                                                                    precision=precision))
+        if boundary:
+            f.set_boundary(*M.cylinder_boundary(m))
         lhs = cfa.memLS_LHS_CREATE(m.numnp, m.numnp, rowp.size, None, colm, rowp, 0)
         rk = torch.zeros((m.numnp, 4), dtype=torch.float64, device="cuda")
         f.to_lhs(_T(torch, states[k][0]), _T(torch, states[k][1]), _T(torch, m.x), rk, lhs)
