@@ -223,6 +223,9 @@ def main():
                          "(empty: skip; needs --cpu-seconds > 0)")
     ap.add_argument("--boundary", type=int, default=1,
                     help="form the boundary elements too (inlet code 0, wall 16, outlet 2), as ElmGMR does")
+    ap.add_argument("--ensemble", type=int, default=8,
+                    help="instances of the ensemble formation measurement (uneven cylinders with boundary "
+                         "elements, one union graph; 0: skip)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="time each formation launch configuration (row schedule x assembly chunk) first "
                          "and keep the fastest")
@@ -354,6 +357,42 @@ def main():
         c4 = nnz * (64 + 4) + N * (4 + 16 + 16)
         spmv32 = {"bsr3x3": c3 / (m3 * 1e-3) / 1e9, "bsr4x4": c4 / (m4 * 1e-3) / 1e9, "ms3": m3, "ms4": m4}
         log(f"fp32 spmv 3x3: {m3:.4f} ms {spmv32['bsr3x3']:.0f} GB/s ; 4x4: {m4:.4f} ms {spmv32['bsr4x4']:.0f} GB/s")
+
+    # ---- ensemble: K uneven instances formed as one union, replayed from a HIP graph
+    ens_info = None
+    if args.ensemble > 0:
+        from crimsonflowsolver_amd.ensemble import Ensemble
+        K = args.ensemble
+        grids = [(24 + 4 * (k % 3), 24 + 4 * (k % 3), 60 + 10 * k) for k in range(K)]
+        meshes = [M.cylinder_mesh(*g) for g in grids]
+        ens = Ensemble(meshes, 0.03 + 0.005 * np.arange(K), 1000.0 + 100.0 * np.arange(K), boundary=True)
+        st = [M.poiseuille_state(mm, seed=60 + k) for k, mm in enumerate(meshes)]
+        ey = T(np.concatenate([a[0] for a in st]))
+        eac = T(np.concatenate([a[1] for a in st]))
+        ex = T(ens.x)
+        er4 = torch.zeros((ens.N, 4), dtype=torch.float64, device=dev)
+        g = ens.capture(ey, eac, ex, er4)
+        enel = int(ens.elem_off[-1])
+        reps = max(5, args.steps)
+        for _ in range(2):
+            g.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.launch()
+        torch.cuda.synchronize()
+        ms_graph = (time.perf_counter() - t0) / reps * 1e3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ens.form.to_lhs(ey, eac, ex, er4, ens.lhs)
+        torch.cuda.synchronize()
+        ms_eager = (time.perf_counter() - t0) / reps * 1e3
+        ens_info = {"instances": K, "nel": enel, "nelb": ens.nelb, "nshg": ens.N,
+                    "graph_ms": ms_graph, "eager_ms": ms_eager, "graph_melem_s": enel / (ms_graph * 1e-3) / 1e6,
+                    "autotune": ens.tuned[0] if ens.tuned else None}
+        log(f"ensemble of {K} ({enel} elements, {ens.nelb} boundary): graph {ms_graph:.3f} ms "
+            f"({ens_info['graph_melem_s']:.0f} Melem/s), eager {ms_eager:.3f} ms, autotune {ens_info['autotune']}")
+        del g, ens, ey, eac, ex, er4
 
     # ---- timesteps: FP64, then the mixed form from the same initial state
     ts_ms = None
@@ -498,6 +537,7 @@ def main():
         "timestep_ms_fp64": ts_ms, **ts_info,
         "roofline": roof, "cpu_baseline": cpu, "timestep_slab": slab,
         "autotune": None if tuned is None else {k: {"chosen": c, "assembly_ms": t} for k, (c, t) in tuned.items()},
+        "ensemble": ens_info,
     }
     print(json.dumps(line), flush=True)
 
