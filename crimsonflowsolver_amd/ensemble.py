@@ -57,6 +57,7 @@ class Ensemble:
         p.precision = precision
         self.params = p
         self.form = ElmGMRInterior(self.ien, self.colm, self.rowp, N, iBC=self.iBC, params=p)
+        self.inst = inst
         self.form.set_instances(inst, self.mu)
         self.nelb = 0
         if boundary:
@@ -90,6 +91,15 @@ class Ensemble:
         self.tuned = self.form.autotune(y, ac, x, res4, self.lhs) if autotune else None
         self.graph = self.form.graph_to_lhs(y, ac, x, res4, self.lhs)
         return self.graph
+
+    def set_parameters(self, mu=None, R=None):
+        """New instance viscosities and/or outlet resistances (the union's
+        graph and matrix are unchanged)."""
+        if mu is not None:
+            self.mu = np.asarray(mu, dtype=np.float64)
+            self.form.set_instances(self.inst, self.mu)
+        if R is not None:
+            self.R = np.asarray(R, dtype=np.float64)
 
     def rows(self, k):
         return slice(int(self.node_off[k]), int(self.node_off[k + 1]))
