@@ -42,7 +42,7 @@ EXPORTS = (
     "crimson_itr_bc_dev", "crimson_elmpvsq_dev", "crimson_form_sync", "crimson_lhs_set_precision",
     "crimson_form_vjp_dev", "crimson_form_set_instances", "crimson_form_graph_create", "crimson_graph_launch",
     "crimson_graph_free", "crimson_form_set_boundary", "crimson_form_autotune", "crimson_form_set_bc",
-    "crimson_elmpvsq_coef_dev",
+    "crimson_elmpvsq_coef_dev", "crimson_form_boundary_flux",
 )
 
 
@@ -114,6 +114,7 @@ def lib():
         L.crimson_form_sync.argtypes = [vp]
         L.crimson_form_set_boundary.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_int]
         L.crimson_form_set_bc.argtypes = [vp, vp, C.c_int]
+        L.crimson_form_boundary_flux.argtypes = [vp, C.c_int, vp]
         L.crimson_elmpvsq_coef_dev.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp, vp]
         L.crimson_form_autotune.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         _lib = L
@@ -279,6 +280,13 @@ class ElmGMRInterior:
                                           int(mask), C.byref(choice), ms), "crimson_form_autotune")
         times = {n: ms[i] for i, n in enumerate(self.AUTOTUNE_CANDIDATES) if ms[i] == ms[i]}
         return self.AUTOTUNE_CANDIDATES[choice.value], times
+
+    def boundary_flux(self, nflx):
+        """flxID(5, 0..nflx) of the last formation: area, outflow and force
+        on the fluid by srfID (e3b.f90:216-247)."""
+        out = np.zeros((5, nflx + 1), order="F")
+        check(lib().crimson_form_boundary_flux(self.h, int(nflx), _p(out)), "crimson_form_boundary_flux")
+        return out
 
     def set_bc(self, BC):
         """BC(nshg, ndofBC) of the handle's iBC (ndofBC >= 6 for 1- and

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -46,7 +47,7 @@ __global__ void __launch_bounds__(256) bnd_elem(int nelb, const int *__restrict_
                                                 const int *__restrict__ bcode, const double *__restrict__ bcb,
                                                 const int *__restrict__ permit, int nsrf, int nqb, double influx,
                                                 double rho, double mu0, Ens en, const double *__restrict__ node,
-                                                double *__restrict__ brec) {
+                                                double *__restrict__ brec, double *__restrict__ bflx) {
   const int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= nelb) return;
   const double mu = ens_mu(en, e, mu0);
@@ -125,6 +126,7 @@ __global__ void __launch_bounds__(256) bnd_elem(int nelb, const int *__restrict_
 #pragma unroll
     for (int k = 0; k < 4; ++k) rl[a][k] = 0.0;
   const double qw = nqb == 1 ? 2.0 * 1.0 : 2.0 * kTriA;
+  double fl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // surface integrals (e3b.f90:216-247)
   for (int q = 0; q < nqb; ++q) {
     const double N[3] = {tri_pt(nqb, q, 0), tri_pt(nqb, q, 1), tri_pt(nqb, q, 2)};
     const double WdetJb = qw / (4.0 * temp);
@@ -140,6 +142,10 @@ __global__ void __launch_bounds__(256) bnd_elem(int nelb, const int *__restrict_
     double tau[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) tau[i] = tau0[i] - bn[i] * tn;
+    fl[0] = fl[0] + WdetJb;
+    fl[1] = fl[1] - WdetJb * unm;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) fl[2 + i] = fl[2 + i] - (tau[i] - bn[i] * pres) * WdetJb;
     // e3b.f90: influx stabilisation, then the prescribed fluxes
     double unm_copy = 0.0;
     if ((code & 2) && unm < 0.0) unm_copy = unm;
@@ -183,6 +189,8 @@ __global__ void __launch_bounds__(256) bnd_elem(int nelb, const int *__restrict_
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int k = 0; k < 4; ++k) brec[(size_t)e * 12 + 4 * a + k] = rl[a][k];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) bflx[(size_t)e * 5 + k] = fl[k];
 }
 
 template <int MODE>
@@ -410,6 +418,32 @@ __global__ void __launch_bounds__(256) bnd_vjp_mu(int nelb, const double *__rest
 
 }  // namespace
 
+// flxID(k, s) = sum over the boundary elements of srfID +-s (sorted per
+// surface, in element order: selem[sptr[s] .. sptr[s+1])) of their
+// integrals; one workgroup per surface, fixed order.
+__global__ void __launch_bounds__(256) bnd_flux_sum(int nflx, const int *__restrict__ sptr,
+                                                    const int *__restrict__ selem, const double *__restrict__ bflx,
+                                                    double *__restrict__ out) {
+  __shared__ double sh[5][4];
+  const int s = blockIdx.x;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int p = sptr[s] + threadIdx.x; p < sptr[s + 1]; p += 256) {
+    const double *r = bflx + (size_t)selem[p] * 5;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) acc[k] += r[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[k][threadIdx.x / 64] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) out[5 * (size_t)s + threadIdx.x] = (sh[threadIdx.x][0] + sh[threadIdx.x][1]) +
+                                                           (sh[threadIdx.x][2] + sh[threadIdx.x][3]);
+}
+
 // The ensemble instance of each boundary element: that of the first element
 // holding its first node (instances have disjoint nodes).
 __global__ void bnd_inst_kernel(int nelb, const int *__restrict__ ienb, const int *__restrict__ inc_ptr,
@@ -437,7 +471,7 @@ int form_boundary_pass(crimson_form *f, double *d_res, int mode) {
   bnd_elem<<<grid_for(f->nelb, 256), 256, 0, s>>>(f->nelb, f->d_ienb.p, f->d_bcode.p, f->d_bcb.p,
                                                   f->nsrf >= 0 ? f->d_permit.p : nullptr, f->nsrf, f->nqb,
                                                   f->influx, f->prm.rho, f->prm.mu, f->bens(), f->d_node.p,
-                                                  f->d_brec.p);
+                                                  f->d_brec.p, f->d_bflx.p);
   if (mode == 0)
     bnd_node<0><<<grid_for(f->nbnode, 256), 256, 0, s>>>(f->nbnode, f->nshg, f->d_bnode.p, f->d_binc_ptr.p,
                                                          f->d_binc.p, f->d_brec.p, d_res);
@@ -534,6 +568,16 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
       vinc[vfill[b]++] = e << 2 | a;
     }
   if (nelb > (1 << 29)) return crimson::set_error("crimson_form_set_boundary: too many elements"), CRIMSON_EARG;
+  // boundary elements by |srfID| (0 holds those without a surface), in element order
+  int maxs = 0;
+  for (int e = 0; e < nelb; ++e) maxs = std::max(maxs, std::abs(code[(size_t)nelb + e]));
+  std::vector<int> sptr(maxs + 2, 0), selem(nelb);
+  for (int e = 0; e < nelb; ++e) sptr[std::abs(code[(size_t)nelb + e]) + 1]++;
+  for (int k = 0; k <= maxs; ++k) sptr[k + 1] += sptr[k];
+  {
+    std::vector<int> fillp(sptr.begin(), sptr.end() - 1);
+    for (int e = 0; e < nelb; ++e) selem[fillp[std::abs(code[(size_t)nelb + e])]++] = e;
+  }
   hipStream_t s = f->stream;
   int rc;
   if ((rc = f->d_ienb.upload(ib.data(), ib.size(), s)) || (rc = f->d_bcode.upload(code.data(), code.size(), s)) ||
@@ -541,7 +585,8 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
       (rc = f->d_bnode.upload(bnode.data(), bnode.size(), s)) ||
       (rc = f->d_binc_ptr.upload(bptr.data(), bptr.size(), s)) || (rc = f->d_binc.upload(binc.data(), binc.size(), s)) ||
       (rc = f->d_vnode.upload(vnode.data(), vnode.size(), s)) || (rc = f->d_vptr.upload(vptr.data(), vptr.size(), s)) ||
-      (rc = f->d_vinc.upload(vinc.data(), vinc.size(), s)))
+      (rc = f->d_vinc.upload(vinc.data(), vinc.size(), s)) || (rc = f->d_bflx.alloc((size_t)5 * nelb)) ||
+      (rc = f->d_sptr.upload(sptr.data(), sptr.size(), s)) || (rc = f->d_selem.upload(selem.data(), selem.size(), s)))
     return rc;
   if (permit) {
     if ((rc = f->d_permit.upload(permit, (size_t)nsrf + 1, s))) return rc;
@@ -553,6 +598,7 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
   f->nqb = nqb;
   f->influx = influx;
   f->nsrf = permit ? nsrf : -1;
+  f->maxsrf = maxs;
   CR_TRY(crimson::form_boundary_instances(f));
   if (f->npartial > 0) {
     // the boundary loop's bc3lhs sets the constrained diagonal entries of
@@ -560,5 +606,21 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
     f->h_nbnd = vcnt;
     return crimson::form_bc3lhs_table(f);
   }
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_form_boundary_flux(crimson_form *f, int nflx, double *flxid) {
+  if (!f || nflx < 0 || !flxid) return crimson::set_error("crimson_form_boundary_flux: bad arguments"), CRIMSON_EARG;
+  for (size_t k = 0; k < 5 * ((size_t)nflx + 1); ++k) flxid[k] = 0.0;
+  if (f->nelb == 0) return CRIMSON_OK;
+  const int ns = f->maxsrf + 1;
+  CR_TRY(f->d_flxout.alloc((size_t)5 * ns));
+  bnd_flux_sum<<<ns, 256, 0, f->stream>>>(ns, f->d_sptr.p, f->d_selem.p, f->d_bflx.p, f->d_flxout.p);
+  CR_HIP(hipGetLastError());
+  std::vector<double> h((size_t)5 * ns);
+  CR_HIP(hipMemcpyAsync(h.data(), f->d_flxout.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+  CR_HIP(hipStreamSynchronize(f->stream));
+  for (int sid = 1; sid <= std::min(nflx, f->maxsrf); ++sid)
+    for (int k = 0; k < 5; ++k) flxid[5 * (size_t)sid + k] = h[5 * (size_t)sid + k];
   return CRIMSON_OK;
 }

@@ -194,6 +194,10 @@ struct crimson_form {
   crimson::DBuf<double> d_bcb;    // BCB(nelb, 3, 5): mass flux, pressure, traction (zero past ndBCB)
   crimson::DBuf<int> d_permit;    // flow-permitted flag by srfID, 0..nsrf
   crimson::DBuf<double> d_brec;   // (12, nelb): rl of the three face nodes
+  crimson::DBuf<double> d_bflx;   // (5, nelb): surface integrals of the last formation (flxID terms)
+  crimson::DBuf<int> d_sptr, d_selem;  // boundary elements by |srfID|, 0..maxsrf
+  crimson::DBuf<double> d_flxout;
+  int maxsrf = 0;
   crimson::DBuf<int> d_bnode, d_binc_ptr, d_binc;  // boundary nodes; their incidences e << 2 | a
   int nvnode = 0;                                  // the adjoint's: all four nodes of each element
   crimson::DBuf<int> d_vnode, d_vptr, d_vinc;

@@ -167,6 +167,14 @@ int crimson_form_autotune(crimson_form *f, const double *d_y, const double *d_ac
                           const double *d_x, double *d_res, crimson_lhs *lhs,
                           int reps, int mask, int *choice, double *ms_out);
 
+/* The boundary loop's surface integrals of the last formation (flxID of
+ * e3b.f90:216-247, zeroed by ElmGMR at elmgmr.f90:124): flxid(5, 0..nflx)
+ * column-major; column s sums, over the boundary elements whose srfID is
+ * +-s, at each quadrature point: the area, the outflow -u.n and the force
+ * on the fluid -(tau - n p) (tau, p after e3bvar's normal-traction split).
+ * Column 0 and surfaces without elements are zero. */
+int crimson_form_boundary_flux(crimson_form *f, int nflx, double *flxid);
+
 /* Ensembles (no counterpart in the reference, which runs one case per job):
  * the handle's mesh is the union of several instances' meshes (disjoint
  * node ranges), inst(nel) gives each element's 0-based instance and mu(ninst)

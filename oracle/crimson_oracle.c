@@ -1769,10 +1769,10 @@ static const double TRI_A = 0.3333333333333330000000000000000000; /* symtri.c:20
 static const double TRI_B = 0.666666666666667;                    /* symtri.c:21 */
 static const double TRI_C = 0.166666666666667;                    /* symtri.c:22 */
 
-int cfo_boundary_res(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
+static int boundary_core(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
                      const double *BCB, int ndBCB, const double *y, const double *x,
                      double rho, double mu, double influx, int nqb, const int *permit,
-                     int nsrf, int ibksiz, double *res) {
+                     int nsrf, int ibksiz, double *res, double *flx, int nflx) {
   double qp[3][3], qw[3];
   if (nqb == 1) {
     qp[0][0] = qp[0][1] = qp[0][2] = TRI_A;
@@ -1862,6 +1862,15 @@ int cfo_boundary_res(int nshg, int numnp, int nelb, const int *ienb, const int *
         const double tn = bn[0] * tau[0] + bn[1] * tau[1] + bn[2] * tau[2];
         pres = pres - tn;
         for (int i = 0; i < 3; ++i) tau[i] = tau[i] - bn[i] * tn;
+        /* surface integrals by srfID (e3b.f90:216-247): area, outflow,
+         * force on the fluid */
+        const int iface = srf < 0 ? -srf : srf;
+        if (flx && iface != 0 && iface <= nflx) {
+          double *fl = flx + 5 * (size_t)iface;
+          fl[0] = fl[0] + WdetJb;
+          fl[1] = fl[1] - WdetJb * unm;
+          for (int i = 0; i < 3; ++i) fl[2 + i] = fl[2 + i] - (tau[i] - bn[i] * pres) * WdetJb;
+        }
         /* e3b.f90: influx stabilisation, then the prescribed fluxes */
         double unm_copy = 0.0;
         if ((code & 2) && unm < 0.0) unm_copy = unm;
@@ -1906,4 +1915,21 @@ int cfo_boundary_res(int nshg, int numnp, int nelb, const int *ienb, const int *
   }
   free(rl);
   return CFO_OK;
+}
+
+int cfo_boundary_res(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
+                     const double *BCB, int ndBCB, const double *y, const double *x,
+                     double rho, double mu, double influx, int nqb, const int *permit,
+                     int nsrf, int ibksiz, double *res) {
+  return boundary_core(nshg, numnp, nelb, ienb, iBCB, BCB, ndBCB, y, x, rho, mu, influx, nqb, permit, nsrf,
+                       ibksiz, res, NULL, 0);
+}
+
+int cfo_boundary_res_flx(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
+                         const double *BCB, int ndBCB, const double *y, const double *x,
+                         double rho, double mu, double influx, int nqb, const int *permit,
+                         int nsrf, int ibksiz, double *res, double *flxid, int nflx) {
+  for (size_t k = 0; k < 5 * ((size_t)nflx + 1); ++k) flxid[k] = 0.0; /* elmgmr.f90:124 */
+  return boundary_core(nshg, numnp, nelb, ienb, iBCB, BCB, ndBCB, y, x, rho, mu, influx, nqb, permit, nsrf,
+                       ibksiz, res, flxid, nflx);
 }

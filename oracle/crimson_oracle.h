@@ -98,6 +98,16 @@ int cfo_elmgmr_interior_q(int nshg, int numnp, int nel, const int *ien,
                           const int *rowp, int nnz, double *res, double *lhsK,
                           double *lhsP, const cfo_params *p);
 
+/* cfo_boundary_res plus the boundary loop's surface integrals by srfID
+ * (e3b.f90:216-247): flxid(5, 0..nflx) zeroed, then for each boundary
+ * element with |srfID| in 1..nflx, at each quadrature point: area, the
+ * outflow -u.n, and the force on the fluid -(tau - n p) (tau and p after
+ * e3bvar's normal-traction split). */
+int cfo_boundary_res_flx(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
+                         const double *BCB, int ndBCB, const double *y, const double *x,
+                         double rho, double mu, double influx, int nqb, const int *permit,
+                         int nsrf, int ibksiz, double *res, double *flxid, int nflx);
+
 /* The same with BC(nshg,ndofBC) (ndofBC >= 6): 1- and 2-component velocity
  * codes are projected out of each element's tangent by bc3lhs before the
  * assembly (elmgmr.f90:223-226, bc3lhs.f90:41-690). */

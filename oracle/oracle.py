@@ -78,6 +78,9 @@ def lib():
         L.cfo_boundary_res.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, C.c_int, dp, dp,
                                        C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p,
                                        C.c_int, C.c_int, dp]
+        L.cfo_boundary_res_flx.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, C.c_int, dp, dp,
+                                           C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p,
+                                           C.c_int, C.c_int, dp, dp, C.c_int]
         _lib = L
     return _lib
 
@@ -214,6 +217,23 @@ def boundary_res(ienb, iBCB, BCB, y, x, res, rho=1.06, mu=0.04, influx=0.5, nqb=
     _check(lib().cfo_boundary_res(res.shape[0], x.shape[0], ienb.shape[0], ienb, iBCB, BCB, BCB.shape[2],
                                   y, x, rho, mu, influx, nqb, _ptr(perm), nsrf, ibksiz, res), "boundary_res")
     return res
+
+
+def boundary_flux(ienb, iBCB, BCB, y, x, nflx, rho=1.06, mu=0.04, influx=0.5, nqb=3, permit=None,
+                  ibksiz=255):
+    """The boundary loop's surface integrals flxID(5, 0..nflx) by srfID
+    (e3b.f90:216-247): area, outflow, force on the fluid (3)."""
+    res = np.zeros((np.asarray(y).shape[0], 4), order="F")
+    ienb = _f(ienb, np.int32)
+    iBCB = _f(iBCB, np.int32)
+    BCB = _f(BCB)
+    perm = None if permit is None else np.ascontiguousarray(permit, dtype=np.int32)
+    nsrf = -1 if permit is None else perm.size - 1
+    flx = np.zeros((5, nflx + 1), order="F")
+    _check(lib().cfo_boundary_res_flx(res.shape[0], np.asarray(x).shape[0], ienb.shape[0], ienb, iBCB, BCB,
+                                      BCB.shape[2], _f(y), _f(x), rho, mu, influx, nqb, _ptr(perm), nsrf, ibksiz,
+                                      res, flx, nflx), "boundary_flux")
+    return flx
 
 
 def pack_val4(colm, rowp, lhsK, lhsP):

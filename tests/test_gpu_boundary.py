@@ -155,3 +155,26 @@ def test_boundary_with_one_instance_equals_plain(cfa):
         got = form(y, ac, m.x)
         for a, b in zip(got, want):
             assert np.array_equal(a, b)
+
+
+def test_surface_integrals_match_oracle(cfa, oracle_lib):
+    """flxID of the last formation (crimson_form_boundary_flux) against the
+    oracle, on the reference's own cylinder with a random state."""
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "basic_case.npz"))
+    x, ien = np.asfortranarray(z["x"]), np.asfortranarray(z["ien"])
+    ienb, iBCB, BCB = (np.asfortranarray(z[k]) for k in ("ienb", "iBCB", "BCB"))
+    n = x.shape[0]
+    rng = np.random.default_rng(47)
+    y = np.asfortranarray(rng.uniform(-2, 2, size=(n, 4)))
+    ac = np.asfortranarray(rng.uniform(-5, 5, size=(n, 4)))
+    colm, rowp = oracle_lib.genadj(ien, n)
+    form = cfa.ElmGMRInterior(ien, colm, rowp, n)
+    form.set_boundary(ienb, iBCB, BCB)
+    form(y, ac, x)
+    ns = int(np.abs(iBCB[:, 1]).max()) + 1
+    got = form.boundary_flux(ns)
+    want = oracle_lib.boundary_flux(ienb, iBCB, BCB, y, x, ns)
+    print("flxID", got[:, 1:4])
+    scale = np.abs(want).max(axis=1, keepdims=True)
+    assert np.all(np.abs(got - want) <= 1e-12 * np.maximum(scale, 1e-30))
+    assert np.all(got[:, 0] == 0.0) and np.all(got[:, -1] == 0.0)
