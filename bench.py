@@ -223,9 +223,11 @@ def main():
                          "(empty: skip; needs --cpu-seconds > 0)")
     ap.add_argument("--boundary", type=int, default=1,
                     help="form the boundary elements too (inlet code 0, wall 16, outlet 2), as ElmGMR does")
-    ap.add_argument("--ensemble", type=int, default=8,
-                    help="instances of the ensemble formation measurement (uneven cylinders with boundary "
-                         "elements, one union graph; 0: skip)")
+    ap.add_argument("--ensemble", type=int, default=48,
+                    help="instances of the batched-ensemble measurement (on 8 uneven cylinders with boundary "
+                         "elements, one union graph and one union NS solve; 0: skip)")
+    ap.add_argument("--config1", type=int, default=1,
+                    help="also run the 20x20x100 case under both solver readings (GMRESV 4x4 and NS)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="time each formation launch configuration (row schedule x assembly chunk) first "
                          "and keep the fastest")
@@ -358,14 +360,26 @@ def main():
         spmv32 = {"bsr3x3": c3 / (m3 * 1e-3) / 1e9, "bsr4x4": c4 / (m4 * 1e-3) / 1e9, "ms3": m3, "ms4": m4}
         log(f"fp32 spmv 3x3: {m3:.4f} ms {spmv32['bsr3x3']:.0f} GB/s ; 4x4: {m4:.4f} ms {spmv32['bsr4x4']:.0f} GB/s")
 
-    # ---- ensemble: K uneven instances formed as one union, replayed from a HIP graph
+    # ---- ensemble (the batched-ensemble workload): K instances on 8 cylinders
+    # of uneven size (lengths 4-20 cm, 40k-600k tets), viscosity
+    # lognormal(ln 0.04, 0.2), outlet resistance U(800, 2000); formed as one
+    # union (autotuned, replayed from a HIP graph) and solved by one NS solve
     ens_info = None
     if args.ensemble > 0:
         from crimsonflowsolver_amd.ensemble import Ensemble
         K = args.ensemble
-        grids = [(24 + 4 * (k % 3), 24 + 4 * (k % 3), 60 + 10 * k) for k in range(K)]
-        meshes = [M.cylinder_mesh(*g) for g in grids]
-        ens = Ensemble(meshes, 0.03 + 0.005 * np.arange(K), 1000.0 + 100.0 * np.arange(K), boundary=True)
+        rng = np.random.default_rng(2024)
+        shapes = []
+        for c in range(8):
+            Lc, Tc = rng.uniform(4.0, 20.0), rng.uniform(4e4, 6e5)
+            h = (18.85 * Lc / Tc) ** (1.0 / 3.0)
+            shapes.append(M.cylinder_mesh(max(4, round(2.0 / h)), max(4, round(2.0 / h)), max(4, round(Lc / h)),
+                                          length=Lc))
+        pick = [k % 8 for k in range(K)]
+        meshes = [shapes[c] for c in pick]
+        mu_e = np.exp(rng.normal(np.log(0.04), 0.2, K))
+        R_e = rng.uniform(800.0, 2000.0, K)
+        ens = Ensemble(meshes, mu_e, R_e, boundary=True)
         st = [M.poiseuille_state(mm, seed=60 + k) for k, mm in enumerate(meshes)]
         ey = T(np.concatenate([a[0] for a in st]))
         eac = T(np.concatenate([a[1] for a in st]))
@@ -387,11 +401,25 @@ def main():
             ens.form.to_lhs(ey, eac, ex, er4, ens.lhs)
         torch.cuda.synchronize()
         ms_eager = (time.perf_counter() - t0) / reps * 1e3
-        ens_info = {"instances": K, "nel": enel, "nelb": ens.nelb, "nshg": ens.N,
+        # one NS solve of the union (the residual of the formation above)
+        g.launch()
+        ens.form.bc3res(er4, res4_layout=True)
+        ls_e = cfa.memLS_LS_CREATE(cfa.LS_TYPE_NS, relTol=1e-8, dimKry=100, maxItr=10,
+                                   relTolIn=[0.01, 0.2], maxItrIn=[1, 200])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cfa.memLS_SOLVE_dev(ens.lhs, ls_e, 4, er4, res=ens.face_res())
+        torch.cuda.synchronize()
+        ms_solve = (time.perf_counter() - t0) * 1e3
+        ens_info = {"instances": K, "cylinders": [{"length": float(mm.x[:, 2].max()), "nel": int(mm.nel)}
+                                                  for mm in shapes],
+                    "nel": enel, "nelb": ens.nelb, "nshg": ens.N,
                     "graph_ms": ms_graph, "eager_ms": ms_eager, "graph_melem_s": enel / (ms_graph * 1e-3) / 1e6,
+                    "ns_solve_ms": ms_solve, "ns_solver_iterations": [ls_e.RI["itr"], ls_e.GM["itr"], ls_e.CG["itr"]],
                     "autotune": ens.tuned[0] if ens.tuned else None}
-        log(f"ensemble of {K} ({enel} elements, {ens.nelb} boundary): graph {ms_graph:.3f} ms "
-            f"({ens_info['graph_melem_s']:.0f} Melem/s), eager {ms_eager:.3f} ms, autotune {ens_info['autotune']}")
+        log(f"ensemble of {K} on 8 cylinders ({enel} tets, {ens.nelb} boundary): graph {ms_graph:.3f} ms "
+            f"({ens_info['graph_melem_s']:.0f} Melem/s), eager {ms_eager:.3f} ms, union NS solve {ms_solve:.0f} ms "
+            f"(RI/GM/CG {ens_info['ns_solver_iterations']}), autotune {ens_info['autotune']}")
         del g, ens, ey, eac, ex, er4
 
     # ---- timesteps: FP64, then the mixed form from the same initial state
@@ -480,6 +508,47 @@ def main():
             log(f"mixed: {ms32:.1f} ms/timestep vs fp64 {ts_ms:.1f} -> x{ts_ms / ms32:.2f}; rel err {err}")
             log(f"FP64 residual norms per nonlinear iteration (last: after the step): fp64 {rn64} mixed {rn32}")
 
+    # ---- the CPU-runnable case (20x20x100 cylinder, outlet natural): one
+    # formation and one memLS solve under both readings of "GMRES dim 50, rel
+    # tol 1e-6": GMRESV on the full 4x4 system, and the NS solver
+    c1 = None
+    if args.config1:
+        m1 = M.cylinder_mesh(20, 20, 100)
+        n1 = m1.numnp
+        colm1, rowp1 = cfa.genadj(m1.ien, n1)
+        ibc1, _ = M.dirichlet_codes(m1)
+        f1 = cfa.ElmGMRInterior(m1.ien, colm1, rowp1, n1, iBC=ibc1, params=cfa.form_params())
+        f1.set_boundary(*M.cylinder_boundary(m1))
+        y1, a1 = M.poiseuille_state(m1)
+        fixed1 = np.nonzero(ibc1)[0]
+        c1 = {"workload": "20x20x100 cylinder, outlet natural", "nel": int(m1.nel), "nshg": int(n1)}
+        for label, lt, kw in (("GMRESV 4x4", cfa.LS_TYPE_GMRES, dict(relTol=1e-6, dimKry=50, maxItr=10)),
+                              ("NS", cfa.LS_TYPE_NS, dict(relTol=1e-6, dimKry=50, maxItr=10))):
+            lh1 = cfa.memLS_LHS_CREATE(n1, n1, rowp1.size, None, colm1, rowp1, 1)
+            cfa.memLS_BC_CREATE(lh1, 1, fixed1.size, 3, cfa.BC_TYPE_Dir, fixed1 + 1,
+                                np.zeros((3, fixed1.size), order="F"))
+            r1 = torch.zeros((n1, 4), dtype=torch.float64, device=dev)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            f1.to_lhs(T(y1), T(a1), T(m1.x), r1, lh1)
+            f1.bc3res(r1, res4_layout=True)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            ls1 = cfa.memLS_LS_CREATE(lt, **kw)
+            cfa.memLS_SOLVE_dev(lh1, ls1, 4, r1)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            c1[label] = {"formation_ms": (t1 - t0) * 1e3, "solve_ms": (t2 - t1) * 1e3,
+                         "iterations": [ls1.RI["itr"], ls1.GM["itr"], ls1.CG["itr"]], "converged": ls1.RI["suc"]}
+            log(f"20x20x100, {label}: formation {(t1 - t0) * 1e3:.1f} ms, solve {(t2 - t1) * 1e3:.1f} ms, "
+                f"iterations {c1[label]['iterations']}, converged {ls1.RI['suc']}")
+        del f1, lh1, r1
+
+    # ---- measured ceilings of this device (a plain copy, FMA chains)
+    peaks = cfa.measure_peaks(1 << 31)
+    log(f"measured ceilings: copy {peaks['copy_gbs']:.0f} GB/s, FP64 FMA {peaks['fp64_tflops']:.1f} TFLOP/s, "
+        f"FP32 FMA {peaks['fp32_tflops']:.1f} TFLOP/s (packed FP32 not selected)")
+
     # ---- roofline of the dominant kernel (from the live HIP-event timings)
     fpe = FLOPS_PER_ELEMENT
     phases = {"flux": tm[0], "e3_elem": tm[1], "assembly": tm[2]}
@@ -488,7 +557,8 @@ def main():
         ach = fpe * nel / (tm[1] * 1e-3) / 1e12
         roof = {"kernel": "e3_elem<true>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
-                "traffic": pmc_traffic(grid, "e3_elem<true>")}
+                "traffic": pmc_traffic(grid, "e3_elem<true>"), "measured_peak": peaks["fp64_tflops"],
+                "frac_of_measured": ach / peaks["fp64_tflops"]}
     else:
         # assembly: the element record read once (48 doubles/element), every
         # block written once (16 doubles/nnz), Res4 (4 doubles/node), and the
@@ -499,7 +569,8 @@ def main():
         kname = "asm_row<1,8>" if tuned is not None and tuned["fp64"][0].split("/")[1] == "8" else "asm_row<1,16>"
         roof = {"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": byt,
-                "traffic": pmc_traffic(grid, kname)}
+                "traffic": pmc_traffic(grid, kname), "measured_peak": peaks["copy_gbs"],
+                "frac_of_measured": ach / peaks["copy_gbs"]}
 
     cpu = None
     slab = None
@@ -538,6 +609,9 @@ def main():
         "roofline": roof, "cpu_baseline": cpu, "timestep_slab": slab,
         "autotune": None if tuned is None else {k: {"chosen": c, "assembly_ms": t} for k, (c, t) in tuned.items()},
         "ensemble": ens_info,
+        "case_20x20x100": c1,
+        "ceilings": {"measured": peaks, "published": {"hbm_gbs": HBM_PEAK_GBS, "fp64_tflops": FP64_PEAK_TFLOPS,
+                                                      "fp32_tflops": 157.3}},
     }
     print(json.dumps(line), flush=True)
 

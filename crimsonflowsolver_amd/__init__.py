@@ -43,6 +43,7 @@ EXPORTS = (
     "crimson_form_vjp_dev", "crimson_form_set_instances", "crimson_form_graph_create", "crimson_graph_launch",
     "crimson_graph_free", "crimson_form_set_boundary", "crimson_form_autotune", "crimson_form_set_bc",
     "crimson_elmpvsq_coef_dev", "crimson_form_boundary_flux",
+    "crimson_measure_peaks",
 )
 
 
@@ -115,10 +116,18 @@ def lib():
         L.crimson_form_set_boundary.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_int]
         L.crimson_form_set_bc.argtypes = [vp, vp, C.c_int]
         L.crimson_form_boundary_flux.argtypes = [vp, C.c_int, vp]
+        L.crimson_measure_peaks.argtypes = [C.c_size_t, dp, dp, dp]
         L.crimson_elmpvsq_coef_dev.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp, vp]
         L.crimson_form_autotune.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         _lib = L
     return _lib
+
+
+def measure_peaks(copy_bytes=1 << 31):
+    """Measured device ceilings: {copy_gbs, fp64_tflops, fp32_tflops}."""
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    check(lib().crimson_measure_peaks(int(copy_bytes), C.byref(a), C.byref(b), C.byref(c)), "crimson_measure_peaks")
+    return {"copy_gbs": a.value, "fp64_tflops": b.value, "fp32_tflops": c.value}
 
 
 def last_error() -> str:

@@ -63,7 +63,11 @@ class Ensemble:
         if boundary:
             # every instance's boundary elements (inlet, wall, outlet), in
             # instance order, on the union's node numbers
-            parts = [M.cylinder_boundary(m) for m in self.meshes]
+            cache = {}
+            for m in self.meshes:  # instances may share a mesh object
+                if id(m) not in cache:
+                    cache[id(m)] = M.cylinder_boundary(m)
+            parts = [cache[id(m)] for m in self.meshes]
             ienb = np.concatenate([b[0] + int(o) for b, o in zip(parts, self.node_off)]).astype(np.int32)
             iBCB = np.concatenate([b[1] for b in parts]).astype(np.int32)
             BCB = np.concatenate([b[2] for b in parts])

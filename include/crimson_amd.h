@@ -16,6 +16,8 @@
 #ifndef CRIMSON_AMD_H
 #define CRIMSON_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -174,6 +176,11 @@ int crimson_form_autotune(crimson_form *f, const double *d_y, const double *d_ac
  * on the fluid -(tau - n p) (tau, p after e3bvar's normal-traction split).
  * Column 0 and surfaces without elements are zero. */
 int crimson_form_boundary_flux(crimson_form *f, int nflx, double *flxid);
+
+/* Measured ceilings (no counterpart in the reference; measurement only): a
+ * plain device copy of copy_bytes (GB/s of read + write) and FP64 / FP32
+ * fused multiply-add throughput (TFLOP/s), timed with HIP events. */
+int crimson_measure_peaks(size_t copy_bytes, double *copy_gbs, double *fp64_tflops, double *fp32_tflops);
 
 /* Ensembles (no counterpart in the reference, which runs one case per job):
  * the handle's mesh is the union of several instances' meshes (disjoint
