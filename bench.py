@@ -226,6 +226,9 @@ def main():
     ap.add_argument("--ensemble", type=int, default=48,
                     help="instances of the batched-ensemble measurement (on 8 uneven cylinders with boundary "
                          "elements, one union graph and one union NS solve; 0: skip)")
+    ap.add_argument("--stretched", type=int, default=1,
+                    help="also time the FP64 and mixed timesteps on the cylinder with 5% of its elements stretched "
+                         "up to 50x (FP64 tau island: aspect ratio > 20)")
     ap.add_argument("--config1", type=int, default=1,
                     help="also run the 20x20x100 case under both solver readings (GMRESV 4x4 and NS)")
     ap.add_argument("--autotune", type=int, default=1,
@@ -435,10 +438,11 @@ def main():
         dnabi = torch.from_numpy(np.ascontiguousarray(nabi[out])).to(dev)   # (nf,3) C == (3,nf) F
         y0, ac0 = dy.clone(), dac.clone()
 
-        def run_steps(fm, lh, label):
+        def run_steps(fm, lh, label, dx=dx, fref=form, dnabi=dnabi):
             """Timesteps of 3 nonlinear iterations; also returns the FP64
-            residual norm (after bc3Res and ElmpvsQ) at each formation and once
-            more after the last update (outside the timed region)."""
+            residual norm (after bc3Res and ElmpvsQ, formed by fref) at each
+            formation and once more after the last update (outside the timed
+            region)."""
             ls = cfa.memLS_LS_CREATE(cfa.LS_TYPE_NS, relTol=1e-8, dimKry=100, maxItr=10,
                                      relTolIn=[0.01, 0.2], maxItrIn=[1, 200])
             dBC = T(np.zeros((N, 7), order="F"))  # (7,N) C == BC(N,7) Fortran
@@ -473,14 +477,14 @@ def main():
                 torch.cuda.synchronize()
                 times.append(time.perf_counter() - t0)
                 # the residual the timestep leaves (evaluated in FP64 by the FP64 formation)
-                form.yalpha(yold, acold, yc, acc, ya, aca)
-                form.sync()
-                form.dev(ya, aca, dx, rfin, lhs=0)
-                form.bc3res(rfin)
+                fref.yalpha(yold, acold, yc, acc, ya, aca)
+                fref.sync()
+                fref.dev(ya, aca, dx, rfin, lhs=0)
+                fref.bc3res(rfin)
                 rq = torch.zeros(1, dtype=torch.float64, device=dev)
                 rres4 = rfin.t().contiguous()
-                form.elmpvsq(dglob, dnabi, R, ya, rres4, rq)
-                form.sync()
+                fref.elmpvsq(dglob, dnabi, R, ya, rres4, rq)
+                fref.sync()
                 rn.append(torch.linalg.vector_norm(rres4))
                 rnorms.append([float(v) for v in rn])
                 yold.copy_(yc)
@@ -507,6 +511,45 @@ def main():
                             "solver_iterations_mixed": [list(map(float, v)) for v in its32]})
             log(f"mixed: {ms32:.1f} ms/timestep vs fp64 {ts_ms:.1f} -> x{ts_ms / ms32:.2f}; rel err {err}")
             log(f"FP64 residual norms per nonlinear iteration (last: after the step): fp64 {rn64} mixed {rn32}")
+
+        if args.mixed and args.stretched:
+            # the same cylinder with 5 % of its elements in seeded regions of
+            # anisotropic stretch up to 50x (mesh.stretch_regions); elements
+            # with aspect ratio > 20 form the FP64 tau island of the mixed form
+            mst = M.stretch_regions(m)
+            # aspect ratios of the elements the map moved (the others keep the
+            # plain mesh's, at most 2)
+            asp = M.aspect_ratios(mst.x, np.asarray(mst.ien)[M.moved_elements(m, mst)])
+            dxs = T(mst.x)
+            nabi_s = mst.nabi()
+            dnabi_s = torch.from_numpy(np.ascontiguousarray(nabi_s[out])).to(dev)
+            bnd_s = M.cylinder_boundary(mst) if args.boundary else None
+            pair = []
+            for prec in (64, 32):
+                fs = cfa.ElmGMRInterior(m.ien, colm, rowp, N, iBC=ibc,
+                                        params=cfa.form_params(precision=prec, aspect_fp64=20.0))
+                if bnd_s is not None:
+                    fs.set_boundary(*bnd_s)
+                ls_ = cfa.memLS_LHS_CREATE(N, N, nnz, None, colm, rowp, 2)
+                cfa.memLS_BC_CREATE(ls_, 1, fixed.size, 3, cfa.BC_TYPE_Dir, fixed + 1,
+                                    np.zeros((3, fixed.size), order="F"))
+                cfa.memLS_BC_CREATE(ls_, 2, out.size, 3, cfa.BC_TYPE_Neu, out + 1, np.asfortranarray(nabi_s[out].T))
+                pair.append((fs, ls_))
+            s64, i64s, y64s, _ = run_steps(pair[0][0], pair[0][1], "stretched fp64", dx=dxs, fref=pair[0][0],
+                                           dnabi=dnabi_s)
+            s32, i32s, y32s, _ = run_steps(pair[1][0], pair[1][1], "stretched mixed", dx=dxs, fref=pair[0][0],
+                                           dnabi=dnabi_s)
+            errs = rel_err(y32s, y64s)
+            ts_info["stretched"] = {
+                "elements_stretched_frac": 0.05, "max_stretch": 50.0,
+                "aspect_max": float(asp.max()), "fp64_island_frac": float((asp > 20.0).sum() / m.nel),
+                "timestep_ms_fp64": float(np.mean(s64) * 1e3), "timestep_ms_mixed": float(np.mean(s32) * 1e3),
+                "mixed_rel_err_vs_fp64": errs,
+                "solver_iterations": [list(map(float, v)) for v in i64s],
+                "solver_iterations_mixed": [list(map(float, v)) for v in i32s]}
+            log(f"stretched mesh (aspect max {asp.max():.0f}, FP64 island {(asp > 20).sum() / m.nel:.4f}): fp64 "
+                f"{np.mean(s64) * 1e3:.1f} ms, mixed {np.mean(s32) * 1e3:.1f} ms, rel err {errs}")
+            del pair
 
     # ---- the CPU-runnable case (20x20x100 cylinder, outlet natural): one
     # formation and one memLS solve under both readings of "GMRES dim 50, rel
