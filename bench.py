@@ -229,6 +229,8 @@ def main():
     ap.add_argument("--stretched", type=int, default=1,
                     help="also time the FP64 and mixed timesteps on the cylinder with 5% of its elements stretched "
                          "up to 50x (FP64 tau island: aspect ratio > 20)")
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="also time the VJP of the residual and check it against central differences")
     ap.add_argument("--config1", type=int, default=1,
                     help="also run the 20x20x100 case under both solver readings (GMRESV 4x4 and NS)")
     ap.add_argument("--autotune", type=int, default=1,
@@ -551,6 +553,64 @@ def main():
                 f"{np.mean(s64) * 1e3:.1f} ms, mixed {np.mean(s32) * 1e3:.1f} ms, rel err {errs}")
             del pair
 
+    # ---- adjoint: the VJP of the residual (interior and boundary elements)
+    # with respect to the nodal state and the viscosity, on this mesh, timed,
+    # and checked against central differences of the residual (h = 1e-6
+    # relative) along a seeded state direction and in mu
+    adj = None
+    if args.adjoint:
+        gen = torch.Generator(device=dev).manual_seed(11)
+        w = torch.randn((4, N), dtype=torch.float64, device=dev, generator=gen)
+        ybar = torch.empty_like(w)
+        mb = form.vjp(dy, dac, dx, w, ybar)
+        torch.cuda.synchronize()
+        reps = max(3, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            mb = form.vjp(dy, dac, dx, w, ybar)
+        torch.cuda.synchronize()
+        ms_vjp = (time.perf_counter() - t0) / reps * 1e3
+        # a smooth seeded direction (a node-by-node random one makes the
+        # second differences of the residual so large that no step size
+        # separates truncation from rounding), zero at the outlet, whose
+        # influx term switches on where u.n < 0
+        rs = np.random.default_rng(12)
+        vh = np.zeros((4, N))
+        for c in range(4):
+            k = rs.uniform(0.5, 2.0, 3)
+            ph = rs.uniform(0, 2 * np.pi, 3)
+            vh[c] = (np.sin(k[0] * m.x[:, 0] + ph[0]) * np.cos(k[1] * m.x[:, 1] + ph[1]) *
+                     np.sin(k[2] * m.x[:, 2] / 10.0 + ph[2]))
+        vh[3] *= 100.0  # pressure scale
+        vh[:, m.outlet] = 0.0
+        v = torch.from_numpy(vh).to(dev)
+        rp, rm = torch.empty_like(w), torch.empty_like(w)
+        ad_y = float((ybar * v).sum())
+        fd_ys = {}
+        for hr in (1e-6, 1e-4):
+            h = hr * float(dy.abs().max())
+            form.dev(dy + h * v, dac, dx, rp, lhs=0)
+            form.dev(dy - h * v, dac, dx, rm, lhs=0)
+            torch.cuda.synchronize()
+            fd_ys[hr] = float((w * (rp - rm)).sum() / (2 * h))
+        fd_y = fd_ys[1e-6]
+        inst = np.zeros(nel, np.uint16)
+        hm = 1e-6 * prm.mu
+        form.set_instances(inst, np.array([prm.mu + hm]))
+        form.dev(dy, dac, dx, rp, lhs=0)
+        form.set_instances(inst, np.array([prm.mu - hm]))
+        form.dev(dy, dac, dx, rm, lhs=0)
+        form.set_instances(inst, np.zeros(0))
+        torch.cuda.synchronize()
+        fd_mu = float((w * (rp - rm)).sum() / (2 * hm))
+        adj = {"vjp_ms": ms_vjp, "vjp_melem_s": nel / (ms_vjp * 1e-3) / 1e6,
+               "fd_check_state": {"vjp": ad_y, "central_difference": fd_y, "rel": abs(ad_y - fd_y) / abs(fd_y),
+                                  "rel_h1e-4": abs(ad_y - fd_ys[1e-4]) / abs(fd_ys[1e-4])},
+               "fd_check_mu": {"vjp": mb, "central_difference": fd_mu, "rel": abs(mb - fd_mu) / abs(fd_mu)}}
+        log(f"adjoint: VJP {ms_vjp:.2f} ms ({adj['vjp_melem_s']:.0f} Melem/s); central differences: state "
+            f"{adj['fd_check_state']['rel']:.2e} (h 1e-6), {adj['fd_check_state']['rel_h1e-4']:.2e} (h 1e-4), "
+            f"mu {adj['fd_check_mu']['rel']:.2e} relative")
+
     # ---- the CPU-runnable case (20x20x100 cylinder, outlet natural): one
     # formation and one memLS solve under both readings of "GMRES dim 50, rel
     # tol 1e-6": GMRESV on the full 4x4 system, and the NS solver
@@ -653,6 +713,7 @@ def main():
         "autotune": None if tuned is None else {k: {"chosen": c, "assembly_ms": t} for k, (c, t) in tuned.items()},
         "ensemble": ens_info,
         "case_20x20x100": c1,
+        "adjoint": adj,
         "ceilings": {"measured": peaks, "published": {"hbm_gbs": HBM_PEAK_GBS, "fp64_tflops": FP64_PEAK_TFLOPS,
                                                       "fp32_tflops": 157.3}},
     }
