@@ -874,6 +874,7 @@ int addbcmul(Ctx<T> &c, bool pre, int xdof, const T *X, T *Y) {
   for (size_t f = 0; f < faces.size(); ++f) {
     const CrFace &fa = faces[f];
     if (!fa.coupled) continue;
+    if (fa.nNo == 0) continue;
     const double coef = pre ? -fa.res / (1.0 + fa.res * fa.nS) : fa.res;
     face_axpy_kernel<<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p,
                                                                         fa.d_valM.p, c.fsum.p + f, coef, Y);
@@ -1213,7 +1214,8 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
     g_product_kernel<T><<<gr + nfc * kFaceDotBlocks, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2.p, t1.p,
                                                                      ws.tmp.p, nfc, c.fdev.p, c.fpart.p, done);
     for (int f = 0; f < nfc; ++f)
-      face_pre_apply_kernel<T><<<crimson::grid_for(c.hfdev[f].nNo, 256), 256, 0, c.s>>>(
+      if (c.hfdev[f].nNo > 0)
+        face_pre_apply_kernel<T><<<crimson::grid_for(c.hfdev[f].nNo, 256), 256, 0, c.s>>>(
           c.hfdev[f], c.fpart.p + f * kFaceDotBlocks, t1.p, ws.tmp.p, c.W.p, 4, done);
     schur_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2.p, ws.tmp.p, SPv, done);
     CR_HIP(hipGetLastError());
@@ -1395,7 +1397,7 @@ int solve_impl(crimson_lhs *l, crimson_ls *ls, int dof, double *d_Ri, const int 
                                                                               fa.d_val.p, c.W.p);
   precond_fix_kernel<T><<<gN, 256, 0, c.s>>>(N, dof, l->vdof, V, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
   for (auto &fa : l->face)
-    if (fa.coupled) {
+    if (fa.coupled && fa.nNo > 0) {
       CR_TRY(fa.d_valM.alloc((size_t)fa.dof * fa.nNo));
       face_valm_kernel<T><<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, dof, fa.dof, fa.d_glob.p,
                                                                             fa.d_val.p, c.W.p, fa.d_valM.p);
