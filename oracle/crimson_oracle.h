@@ -20,14 +20,14 @@
  * differences of the assembled residual against the tangent, node-permutation
  * invariance, rotation equivariance, exact flux reconstruction for linear
  * velocity, brute-force dense assembly, dense Val4 packing, and direct
- * solves for the memLS solvers.
- * Parity unpinned: in cfo_e3_block at states with a nonzero strong residual
- * L, the tauBar term tauBar (L.grad u_i) L_j of the momentum flux
- * (e3res.f90, e3stab.f90:356-367) and ubar = u - tauM L / rho inside the
- * convective term (e3stab.f90:372-374) are covered by the invariance checks
- * only (their d/dp through the continuity row is pinned; a wrong
- * coefficient in the tauBar term would pass every check).  The NSSOLVER
- * iterate sequence and iteration counts are not pinned (its solution is).
+ * solves for the memLS solvers.  The tauBar term tauBar (L.grad u_i) L_j of
+ * the momentum flux (e3res.f90:224-245, e3stab.f90:356-367) and ubar = u -
+ * tauM L / rho (e3stab.f90:372-374) are pinned on shear flows whose strong
+ * residual L is known in closed form (the even and odd parts of the residual
+ * in a uniform acceleration).
+ * Parity unpinned: the NSSOLVER iterate sequence and iteration counts (its
+ * solution is pinned); bc3lhs's code-6 row operations, restated as
+ * bc3lhs.f90:648-680 writes them (no projection identity holds for them).
  */
 #ifndef CRIMSON_ORACLE_H
 #define CRIMSON_ORACLE_H

@@ -124,6 +124,65 @@ def test_constant_state_patch(O):
     assert np.abs(rl[0].sum(axis=0)).max() < 1e-11 * V * pr
 
 
+def test_strong_residual_terms_on_a_shear_flow(O):
+    """The terms of the strong residual L, pinned where L is known exactly:
+    u = (a x2 + b1, b2, b3) (grad u nilpotent, so u.grad u = (a b2, 0, 0)
+    everywhere), p = p0 - rho a b2 x1, and a uniform acceleration c, give
+    L = rho c at every quadrature point.  Then tau_M (which does not depend
+    on L) is known at each point from the metric G = 2^(4/3) sum_a g_a g_a^T
+    (the node-order-free tet metric; the unit right tet's G in
+    test_tau_on_unit_right_tet) and R(c) splits into
+      odd part:  -W [N_a rho c_i - N_a rho tau_M (c.grad u_i)   (time, u-bar)
+                     + tau_M rho c_i (g_a.u)]                    (SUPG)
+                 and -W tau_M (c.g_a) in the continuity row       (u-bar)
+      even part: -W tau_M rho (c.grad u_i)(c.g_a) / sqrt(c.G c)  (tau-bar,
+                 tauBar = tau_M / sqrt(L.G L), e3stab.f90:356-374)
+    summed over the 4 points with W = V/4.  A wrong coefficient, index or
+    sign in the tau-bar or u-bar terms fails this."""
+    rng = np.random.default_rng(21)
+    p = O.make_params()
+    nu = MU / RHO
+    kA, kB = (5 + 3 * np.sqrt(5)) / 20, (5 - np.sqrt(5)) / 20
+    Nq = np.full((4, 4), kB) + (kA - kB) * np.eye(4)  # N_a(q), symmetric
+    for trial in range(3):
+        x = random_tet(rng, stretch=1.0 + trial)
+        g, V = p1_grads(x)
+        G = 2.0 ** (4.0 / 3.0) * g.T @ g
+        a, b = 150.0 * (trial + 1), rng.normal(size=3) * 2.0
+        A = np.zeros((3, 3))
+        A[0, 1] = a
+        y = np.zeros((4, 4))
+        y[:, 1:] = x @ A.T + b
+        y[:, 0] = 7.0 - RHO * a * b[1] * x[:, 0]
+        xq = Nq @ x                       # quadrature points (rows q)
+        uq = xq @ A.T + b
+        tauM = ((2.0 / DT) ** 2 + np.einsum("qi,ij,qj->q", uq, G, uq)
+                + 36.0 * nu ** 2 * np.sum(G * G)) ** -0.5
+        W = V / 4.0
+
+        def R(c):
+            ac = np.zeros((4, 4))
+            ac[:, 1:] = c
+            return O.e3_block(*tet_block(x, y=y, ac=ac), p)[0][0]
+
+        R0 = R(np.zeros(3))
+        for c in (rng.normal(size=3) * 30.0, np.array([0.0, 40.0, 0.0]), np.array([5.0, -20.0, 12.0])):
+            Rp, Rm = R(c), R(-c)
+            odd, even = (Rp - Rm) / 2.0, (Rp + Rm) / 2.0 - R0
+            cg = A @ c                    # c.grad u_i
+            want_odd = np.zeros((4, 4))
+            for q in range(4):
+                want_odd[:, :3] -= W * (Nq[:, q, None] * RHO * (c - tauM[q] * cg)
+                                        + tauM[q] * RHO * np.outer(g @ uq[q], c))
+                want_odd[:, 3] -= W * tauM[q] * (g @ c)
+            want_even = np.zeros((4, 4))
+            want_even[:, :3] = -W * tauM.sum() * RHO * np.outer(g @ c, cg) / np.sqrt(c @ G @ c)
+            scale = np.abs(want_odd).max()
+            assert np.abs(odd - want_odd).max() <= 1e-11 * scale, (trial, c)
+            assert np.abs(want_even).max() > 1e-3 * scale or cg[0] == 0.0
+            assert np.abs(even - want_even).max() <= 1e-11 * scale, (trial, c)
+
+
 def _perm_case(rng):
     x = random_tet(rng)
     y = rng.normal(size=(4, 4)) * np.array([100, 10, 10, 10])
