@@ -1938,7 +1938,8 @@ extern "C" int crimson_genadj(int nshg, int nel, const int *ien, int *colm, int 
     buf.clear();
     for (int p = ptr[i]; p < ptr[i + 1]; ++p)
       for (int b = 0; b < 4; ++b) buf.push_back(ien[(size_t)b * nel + inc[p]]);
-    if (buf.empty()) buf.push_back(i + 1);  // isolated node keeps its diagonal
+    // a node no element references gets an empty row, as Asadj leaves it
+    // (crimson_form_setup then rejects the row: it has no diagonal block)
     std::sort(buf.begin(), buf.end());
     const size_t len = std::unique(buf.begin(), buf.end()) - buf.begin();
     if (rowp && nnz + (long long)len <= maxnnz)

@@ -112,6 +112,8 @@ def _check(rc, what):
 def genadj(ien, nshg):
     """colm(nshg+1), rowp(nnz), 1-based (genadj.f90)."""
     ien = _f(ien, np.int32)
+    if ien.size and (ien.min() < 1 or ien.max() > nshg):
+        raise ValueError("genadj: ien entries must lie in 1..nshg")
     nel = ien.shape[0]
     cap = 32 * nshg
     colm = np.zeros(nshg + 1, np.int32)

@@ -46,6 +46,17 @@ def test_genadj_matches_oracle(cfa, oracle_lib):
         a = cfa.genadj(m.ien, m.numnp)
         b = oracle_lib.genadj(m.ien, m.numnp)
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # nodes 5 and 7 belong to no element: their rows stay empty (Asadj adds
+    # nothing for them, genadj.f90:29-44), and the formation rejects such a
+    # graph (no diagonal block)
+    ien = np.asfortranarray(np.array([[1, 2, 3, 4], [2, 3, 4, 6]], np.int32))
+    a = cfa.genadj(ien, 7)
+    b = oracle_lib.genadj(ien, 7)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert a[0][4] == a[0][5] and a[0][6] == a[0][7]
+    with pytest.raises(cfa.CrimsonError) as e:
+        cfa.ElmGMRInterior(ien, a[0], a[1], 7)
+    assert e.value.code == 4 and "row 5" in str(e.value)
 
 
 def test_genadj_capacity_error(cfa):
