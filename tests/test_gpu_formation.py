@@ -46,6 +46,35 @@ def test_formation_matches_oracle(cfa, oracle_lib, grid, seed):
     assert all(e <= TOL for e in errs.values()), errs
 
 
+@pytest.mark.parametrize("nel", [1, 2])
+def test_formation_smallest_meshes(cfa, oracle_lib, nel):
+    """One tet (4 rows, one incidence each) and two tets sharing a face
+    (rows of one and two incidences): the smallest inputs, every grid one
+    workgroup."""
+    rng = np.random.default_rng(40 + nel)
+    x = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.2, 1.0, 0.1], [0.1, 0.2, 1.0], [0.9, 0.9, 0.8]])
+    x += rng.uniform(-0.05, 0.05, x.shape)
+    ien = np.array([[1, 2, 3, 4], [2, 3, 4, 5]], np.int32)[:nel]
+    # positive orientation: det[x1-x4, x2-x4, x3-x4] > 0 (p1 volume sign)
+    for e in range(nel):
+        v = x[ien[e, :3] - 1] - x[ien[e, 3] - 1]
+        if np.linalg.det(v) < 0:
+            ien[e, [0, 1]] = ien[e, [1, 0]]
+    ien = np.asfortranarray(ien)
+    n = 4 + (nel - 1)
+    x = np.asfortranarray(x[:n])
+    y = np.asfortranarray(rng.normal(size=(n, 4)) * np.array([10.0, 1.0, 1.0, 1.0]))
+    ac = np.asfortranarray(rng.uniform(-20, 20, size=(n, 4)))
+    colm, rowp = oracle_lib.genadj(ien, n)
+    form = cfa.ElmGMRInterior(ien, colm, rowp, n)
+    res, K, P, q = form(y, ac, x, want_q=True)
+    rres, rK, rP, rq = oracle_lib.elmgmr_interior(ien, y, ac, x, colm, rowp, oracle_lib.make_params(),
+                                                  want_q=True)
+    errs = {n: _rel(a, b) for n, a, b in (("res", res, rres), ("lhsK", K, rK), ("lhsP", P, rP), ("q", q, rq))}
+    print("nel", nel, errs)
+    assert all(e <= TOL for e in errs.values()), errs
+
+
 def test_formation_residual_only(cfa, oracle_lib):
     m, y, ac = _case((6, 6, 8), 7)
     colm, rowp = oracle_lib.genadj(m.ien, m.numnp)
