@@ -249,7 +249,7 @@ def test_autotune_candidates_bitwise(cfa, oracle_lib, precision):
     matrix, on a jittered Delaunay mesh whose rows take 1-3 chunks of 16 and
     1-6 of 8 (ragged last chunks), and on a cylinder of 1,100+ assembly
     workgroups (so that dealing in turns of 64 remaps all but the last
-    partial round); the FP64 ones match the oracle.  A full run returns a
+    partial turn); the FP64 ones match the oracle.  A full run returns a
     candidate and a time for each one tried."""
     import torch
     from crimsonflowsolver_amd import mesh as M
