@@ -250,9 +250,10 @@ class ElmGMRInterior:
         return mb.value
 
     def set_instances(self, inst, mu):
-        """Ensemble: inst(nel) 0-based instance of each element, mu(ninst)."""
-        inst = np.ascontiguousarray(inst, dtype=np.uint16)
+        """Ensemble: inst(nel) 0-based instance of each element, mu(ninst);
+        an empty mu (inst None) returns to the single viscosity."""
         mu = np.ascontiguousarray(mu, dtype=np.float64)
+        inst = None if mu.size == 0 else np.ascontiguousarray(inst, dtype=np.uint16)
         check(lib().crimson_form_set_instances(self.h, _p(inst), int(mu.size), _p(mu)), "crimson_form_set_instances")
 
     def set_boundary(self, ienb, iBCB, BCB, nqb=3, influx=0.5, permit=None):

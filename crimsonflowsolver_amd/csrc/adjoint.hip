@@ -406,42 +406,62 @@ __global__ void __launch_bounds__(256) mu_final(int n, double *__restrict__ part
 
 }  // namespace
 
+namespace crimson {
+void vjp_work_sizes(const crimson_form *f, size_t n[6]) {
+  const size_t ne = (size_t)std::max(f->nel, 1), nn = (size_t)f->nshg;
+  n[0] = ne * kVRec;
+  n[1] = ne * kVRec2;
+  n[2] = nn * 6;
+  n[3] = nn;
+  n[4] = (ne + kMuChunk - 1) / kMuChunk + 1;
+  n[5] = form_boundary_vjp_doubles(f);
+}
+
+int vjp_run(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, const double *d_w,
+            double *d_ybar, double *mubar, const VjpWork &w) {
+  const size_t nn = (size_t)f->nshg;
+  const int nmu = (int)(((size_t)std::max(f->nel, 1) + kMuChunk - 1) / kMuChunk);
+  hipStream_t s = f->stream;
+  const unsigned T = 256;
+  const FormConst c = make_const(f->prm);
+  CR_TRY(crimson::form_flux_pass(f, d_y, d_ac, d_x, nullptr, w.invm));
+  const unsigned ge = crimson::grid_for(f->nel, T), gn = crimson::grid_for(nn * kNodeLanes, T);
+  if (f->nel > 0)
+    vjp_e3_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, d_w, f->nshg, c, f->ens(), w.vrec);
+  vjp_q_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, w.vrec, w.invm, w.qbar);
+  if (f->nel > 0)
+    vjp_flux_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, f->d_frec.p, w.qbar, c.mu, f->ens(), w.vrec2);
+  vjp_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, w.vrec, w.vrec2, d_ybar);
+  double mb = 0.0;
+  if (f->nel > 0) {
+    mu_part<<<nmu, T, 0, s>>>(f->nel, w.vrec, w.vrec2, w.mupart);
+    mu_final<<<1, T, 0, s>>>(nmu, w.mupart);
+    CR_HIP(hipGetLastError());
+    CR_HIP(hipMemcpyAsync(&mb, w.mupart + nmu, sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipStreamSynchronize(s));
+  double mbb = 0.0;
+  CR_TRY(crimson::form_boundary_vjp(f, d_w, d_ybar, &mbb, w.bvrec));  // the boundary rows (crimson_form_set_boundary)
+  if (mubar) *mubar = mb + mbb;
+  return CRIMSON_OK;
+}
+
+}  // namespace crimson
+
 extern "C" int crimson_form_vjp_dev(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
                                     const double *d_w, double *d_ybar, double *mubar) {
   if (!f || !d_y || !d_ac || !d_x || !d_w || !d_ybar) {
     crimson::set_error("crimson_form_vjp_dev: null argument");
     return CRIMSON_EARG;
   }
-  const size_t ne = (size_t)std::max(f->nel, 1), nn = (size_t)f->nshg;
-  const int nmu = (int)((ne + kMuChunk - 1) / kMuChunk);
-  CR_TRY(f->d_vrec.alloc(ne * kVRec));
-  CR_TRY(f->d_vrec2.alloc(ne * kVRec2));
-  CR_TRY(f->d_qbar.alloc(nn * 6));
-  CR_TRY(f->d_invm.alloc(nn));
-  CR_TRY(f->d_mupart.alloc((size_t)nmu + 1));
-  hipStream_t s = f->stream;
-  const unsigned T = 256;
-  const FormConst c = make_const(f->prm);
-  CR_TRY(crimson::form_flux_pass(f, d_y, d_ac, d_x, nullptr, f->d_invm.p));
-  const unsigned ge = crimson::grid_for(f->nel, T), gn = crimson::grid_for(nn * kNodeLanes, T);
-  if (f->nel > 0)
-    vjp_e3_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, d_w, f->nshg, c, f->ens(), f->d_vrec.p);
-  vjp_q_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_vrec.p, f->d_invm.p, f->d_qbar.p);
-  if (f->nel > 0)
-    vjp_flux_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, f->d_frec.p, f->d_qbar.p, c.mu, f->ens(),
-                                   f->d_vrec2.p);
-  vjp_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_vrec.p, f->d_vrec2.p, d_ybar);
-  double mb = 0.0;
-  if (f->nel > 0) {
-    mu_part<<<nmu, T, 0, s>>>(f->nel, f->d_vrec.p, f->d_vrec2.p, f->d_mupart.p);
-    mu_final<<<1, T, 0, s>>>(nmu, f->d_mupart.p);
-    CR_HIP(hipGetLastError());
-    CR_HIP(hipMemcpyAsync(&mb, f->d_mupart.p + nmu, sizeof(double), hipMemcpyDeviceToHost, s));
-  }
-  CR_HIP(hipGetLastError());
-  CR_HIP(hipStreamSynchronize(s));
-  double mbb = 0.0;
-  CR_TRY(crimson::form_boundary_vjp(f, d_w, d_ybar, &mbb));  // the boundary rows (crimson_form_set_boundary)
-  if (mubar) *mubar = mb + mbb;
-  return CRIMSON_OK;
+  size_t n[6];
+  crimson::vjp_work_sizes(f, n);
+  CR_TRY(f->d_vrec.alloc(n[0]));
+  CR_TRY(f->d_vrec2.alloc(n[1]));
+  CR_TRY(f->d_qbar.alloc(n[2]));
+  CR_TRY(f->d_invm.alloc(n[3]));
+  CR_TRY(f->d_mupart.alloc(n[4]));
+  const crimson::VjpWork w{f->d_vrec.p, f->d_vrec2.p, f->d_qbar.p, f->d_invm.p, f->d_mupart.p, nullptr};
+  return crimson::vjp_run(f, d_y, d_ac, d_x, d_w, d_ybar, mubar, w);
 }

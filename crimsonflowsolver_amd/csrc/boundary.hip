@@ -481,19 +481,23 @@ int form_boundary_pass(crimson_form *f, double *d_res, int mode) {
   CR_HIP(hipGetLastError());
   return CRIMSON_OK;
 }
-int form_boundary_vjp(crimson_form *f, const double *d_w, double *d_ybar, double *mubar) {
+size_t form_boundary_vjp_doubles(const crimson_form *f) { return f->nelb ? (size_t)kBV * f->nelb + 1 : 0; }
+int form_boundary_vjp(crimson_form *f, const double *d_w, double *d_ybar, double *mubar, double *d_bvrec) {
   *mubar = 0.0;
   if (f->nelb == 0) return CRIMSON_OK;
   hipStream_t s = f->stream;
-  CR_TRY(f->d_bvrec.alloc((size_t)kBV * f->nelb + 1));
+  if (!d_bvrec) {
+    CR_TRY(f->d_bvrec.alloc(form_boundary_vjp_doubles(f)));
+    d_bvrec = f->d_bvrec.p;
+  }
   bnd_vjp_elem<<<grid_for(f->nelb, 256), 256, 0, s>>>(f->nelb, f->nshg, f->d_ienb.p, f->d_bcode.p,
                                                       f->nsrf >= 0 ? f->d_permit.p : nullptr, f->nsrf, f->nqb,
                                                       f->influx, f->prm.rho, f->prm.mu, f->bens(), f->d_node.p, d_w,
-                                                      f->d_bvrec.p);
+                                                      d_bvrec);
   bnd_vjp_node<<<grid_for(f->nvnode, 256), 256, 0, s>>>(f->nvnode, f->nshg, f->d_vnode.p, f->d_vptr.p,
-                                                        f->d_vinc.p, f->d_bvrec.p, d_ybar);
-  double *d_mu = f->d_bvrec.p + (size_t)kBV * f->nelb;
-  bnd_vjp_mu<<<1, 256, 0, s>>>(f->nelb, f->d_bvrec.p, d_mu);
+                                                        f->d_vinc.p, d_bvrec, d_ybar);
+  double *d_mu = d_bvrec + (size_t)kBV * f->nelb;
+  bnd_vjp_mu<<<1, 256, 0, s>>>(f->nelb, d_bvrec, d_mu);
   CR_HIP(hipGetLastError());
   CR_HIP(hipMemcpyAsync(mubar, d_mu, sizeof(double), hipMemcpyDeviceToHost, s));
   CR_HIP(hipStreamSynchronize(s));

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdint>
@@ -33,25 +34,37 @@ const std::string &last_error();
     if (rc_ != CRIMSON_OK) return rc_; \
   } while (0)
 
+// Serial numbers of device allocations: every successful allocation gets a
+// new one, so a captured HIP graph can tell whether a buffer it reads has
+// been released or replaced since the capture (even at the same address).
+inline unsigned long long next_alloc_serial() {
+  static std::atomic<unsigned long long> counter{0};
+  return ++counter;
+}
+
 // Owning device allocation.
 template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  unsigned long long serial = 0;  // 0 while nothing is allocated
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) {
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), serial(o.serial) {
     o.p = nullptr;
     o.n = 0;
+    o.serial = 0;
   }
   DBuf &operator=(DBuf &&o) noexcept {
     if (this != &o) {
       release();
       p = o.p;
       n = o.n;
+      serial = o.serial;
       o.p = nullptr;
       o.n = 0;
+      o.serial = 0;
     }
     return *this;
   }
@@ -60,6 +73,7 @@ struct DBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    serial = 0;
   }
   int alloc(size_t count) {
     if (count == n && p) return CRIMSON_OK;
@@ -72,6 +86,7 @@ struct DBuf {
       return CRIMSON_ENOMEM;
     }
     n = count;
+    serial = next_alloc_serial();
     return CRIMSON_OK;
   }
   int upload(const T *h, size_t count, hipStream_t s) {

@@ -22,11 +22,12 @@
 //   e3_tangent32, asm_row32   the FP32 tangent of the mixed-precision form
 //
 // The element record factorisation is exact algebra on the reference's
-// lazy tangent (e3lhs.f90:85-402) for P1 elements, whose metric and shape
-// gradients do not vary over the element:
+// tangent (e3lhs.f90:85-402, iconvflow = 2 branch :124-129) for P1
+// elements, whose metric and shape gradients do not vary over the element:
 //   K^{ij}_ab = delta_ij S_ab + Tc g_a,i g_b,j + Mu g_a,j g_b,i
-//   S_ab = tsFct sum_q N_a N_b + rho tlW sum_q N_a (u_q.g_b)
+//   S_ab = tsFct sum_q N_a N_b + rho tlW sum_q N_a (ubar_q.g_b)
 //          + Mu (g_a.g_b) + rho tlW sum_q tauM_q (u_q.g_a)(u_q.g_b)
+//          + tlW sum_q tauBar_q (L_q.g_a)(L_q.g_b)
 //   G_ab,i = tlW g_b,i sum_q N_a ,   C_ab = Cm (g_a.g_b)
 // with Tc = tlW sum_q tauC_q, Mu = 4 tlW mu, Cm = tlW sum_q tauM_q / rho,
 // tlW = alpha_f gamma Delt WdetJ, tsFct = alpha_m WdetJ rho.
@@ -245,11 +246,14 @@ __global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__rest
   const double divu = gu[0][0] + gu[1][1] + gu[2][2];
   const double ff = c.taucfct / c.dtsfct;
 
-  double RNa[4][3], RGN[3][3], SU[3] = {0, 0, 0}, TC = 0.0, TM = 0.0, SUPG[10];
+  double RNa[4][3], RGN[3][3], SU[3] = {0, 0, 0}, TC = 0.0, TM = 0.0, SUPG[10], UB[4][3];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) RNa[a][i] = 0.0;
+    for (int i = 0; i < 3; ++i) {
+      RNa[a][i] = 0.0;
+      UB[a][i] = 0.0;
+    }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -308,16 +312,29 @@ __global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__rest
       SU[i] += ub[i];
     }
     if (LHS) {
+      // e3LHS, iconvflow = 2 (e3lhs.f90:124-129): convection by u-bar, and
+      // the tauBar (L.g_a)(L.g_b) term beside SUPG's tauM rho (u.g_a)(u.g_b)
       TC += tauC;
       TM += tauM;
-      double z[4];
+      const double tb = tauBar * rhoinv;
+      double z[4], zl[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) z[a] = u[0] * g[a][0] + u[1] * g[a][1] + u[2] * g[a][2];
+      for (int a = 0; a < 4; ++a) {
+        z[a] = u[0] * g[a][0] + u[1] * g[a][1] + u[2] * g[a][2];
+        zl[a] = rL[0] * g[a][0] + rL[1] * g[a][1] + rL[2] * g[a][2];
+      }
       int k = 0;
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = a; b < 4; ++b) SUPG[k++] += tauM * z[a] * z[b];
+        for (int b = a; b < 4; ++b) SUPG[k++] += tauM * z[a] * z[b] + tb * zl[a] * zl[b];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        UB[0][i] += N0 * ub[i];
+        UB[1][i] += N1 * ub[i];
+        UB[2][i] += N2 * ub[i];
+        UB[3][i] += N3 * ub[i];
+      }
     }
   }
 
@@ -338,17 +355,12 @@ __global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__rest
   const double tsFct = c.lhmFct * W * rho;
   const double Mu = 4.0 * (tlW * mu);
   const double rt = rho * tlW;
-  // w[c][b] = u_c . g_b : convection sum_q N_a (u_q.g_b) = sum_c mm(a,c) w[c][b]
-  double w[4][4];
-#pragma unroll
-  for (int cc = 0; cc < 4; ++cc)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) w[cc][b] = ul[cc][0] * g[b][0] + ul[cc][1] * g[b][1] + ul[cc][2] * g[b][2];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const double conv = mm(a, 0) * w[0][b] + mm(a, 1) * w[1][b] + mm(a, 2) * w[2][b] + mm(a, 3) * w[3][b];
+      // convection sum_q N_a (ubar_q . g_b)
+      const double conv = UB[a][0] * g[b][0] + UB[a][1] * g[b][1] + UB[a][2] * g[b][2];
       const int lo = a < b ? a : b, hi = a < b ? b : a;
       const int k = lo * 4 - (lo * (lo - 1)) / 2 + (hi - lo);  // index of (lo,hi) in SUPG
       const double gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
@@ -370,13 +382,14 @@ __global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__rest
 
 // ---------------------------------------------------------------- FP32 tangent
 // The mixed-precision formation keeps the residual in FP64 (flux pass, e3
-// residual, res_node) and forms the lazy tangent in FP32 with FP32
-// accumulation.  The tangent needs no strong residual (tauBar does not enter
-// e3LHS and the tangent's tau depends on u and the metric only), so this
-// kernel evaluates the metric, tau at the 4 quadrature points and the element
-// record S/g/scalars alone.  Elements whose aspect ratio exceeds a threshold
-// are evaluated in FP64 and rounded to FP32 at the end (metric and tau on
-// skewed elements are where FP32 loses most).
+// residual, res_node) and forms the tangent in FP32 with FP32 accumulation.
+// The iconvflow = 2 tangent needs the strong residual L (tauBar and u-bar,
+// e3lhs.f90:124-129), so this kernel gathers the same node records as
+// e3_elem (x, u, p, ac and the reconstructed flux q) and evaluates the metric,
+// gradients, L, tau and u-bar at the 4 quadrature points and the element
+// record S/g/scalars.  Elements whose aspect ratio exceeds a threshold are
+// evaluated in FP64 and rounded to FP32 at the end (metric and tau on skewed
+// elements are where FP32 loses most).
 // FP32 record: 32 floats = one 128-byte line: g[a][j] at a*3+j, tlW, Tc, Mu,
 // Cm at 12..15, S[a][0..3] at 16 + 4a.
 constexpr int kRec32 = 32;
@@ -420,22 +433,38 @@ __device__ inline float tet_aspect(const double (&xd)[4][3]) {
   return sqrtf(L2) * sqrtf(A2) / six_v * 0.8164965809277260f;
 }
 
-// T: arithmetic type; TO: record type; S row a at 16 + SSTRIDE a (4 for the
-// FP32 record, 8 for the FP64 record whose rl shares those lines).
-template <class T, class TO = float, int SSTRIDE = 4>
-__device__ inline void tangent_body(const double (&xd)[4][3], const double (&ud)[4][3], const FormConst &c,
-                                    TO *__restrict__ r) {
-  T xl[4][3], ul[4][3];
+// T: arithmetic type of the tangent; nl: the element's four node records.
+template <class T>
+__device__ inline void tangent_body(const double (&nl)[4][kNRec], const FormConst &c, float *__restrict__ r) {
+  T xl[4][3], ul[4][3], al[4][3], pl[4], ql[4][6];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 4; ++a) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      xl[a][i] = (T)xd[a][i];
-      ul[a][i] = (T)ud[a][i];
+      xl[a][i] = (T)nl[a][N_X + i];
+      ul[a][i] = (T)nl[a][N_U + i];
+      al[a][i] = (T)nl[a][N_A + i];
     }
+    pl[a] = (T)nl[a][N_P];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ql[a][k] = (T)nl[a][N_Q + k];
+  }
   T dx[3][3], W, g[4][3], gg[6];
   p1_metric(xl, dx, W, g);
   gijd_tet(dx, gg);
+  T gu[3][3], gp[3], divq[3] = {T(0), T(0), T(0)};  // gu[j][i] = du_i/dx_j
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      gu[j][i] = g[0][j] * ul[0][i] + g[1][j] * ul[1][i] + g[2][j] * ul[2][i] + g[3][j] * ul[3][i];
+    gp[j] = g[0][j] * pl[0] + g[1][j] * pl[1] + g[2][j] * pl[2] + g[3][j] * pl[3];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      divq[i] += g[a][0] * ql[a][qsym(i)] + g[a][1] * ql[a][qsym(i + 3)] + g[a][2] * ql[a][qsym(i + 6)];
   const T rho = (T)c.rho, mu = (T)c.mu, rhoinv = T(1) / rho;
   const T rnu = mu * rhoinv;
   const T dts = (T)(c.Dtgl * c.dtsfct);
@@ -444,84 +473,109 @@ __device__ inline void tangent_body(const double (&xd)[4][3], const double (&ud)
   const T visc = T(36) * (rnu * rnu) * GG;
   const T trg = gg[0] + gg[1] + gg[2];
   const T ff = (T)(c.taucfct / c.dtsfct);
-  T TC = T(0), TM = T(0), SUPG[10];
+  T TC = T(0), TM = T(0), SUPG[10], UB[4][3];
 #pragma unroll
   for (int k = 0; k < 10; ++k) SUPG[k] = T(0);
 #pragma unroll
-  for (int qp = 0; qp < 4; ++qp) {
-    const T N0 = (T)nq(qp, 0), N1 = (T)nq(qp, 1), N2 = (T)nq(qp, 2), N3 = (T)nq(qp, 3);
-    T u[3];
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) u[i] = N0 * ul[0][i] + N1 * ul[1][i] + N2 * ul[2][i] + N3 * ul[3][i];
+    for (int i = 0; i < 3; ++i) UB[a][i] = T(0);
+#pragma unroll
+  for (int qp = 0; qp < 4; ++qp) {
+    const T Nq[4] = {(T)nq(qp, 0), (T)nq(qp, 1), (T)nq(qp, 2), (T)nq(qp, 3)};
+    T u[3], aq[3], rL[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      u[i] = Nq[0] * ul[0][i] + Nq[1] * ul[1][i] + Nq[2] * ul[2][i] + Nq[3] * ul[3][i];
+      aq[i] = Nq[0] * al[0][i] + Nq[1] * al[1][i] + Nq[2] * al[2][i] + Nq[3] * al[3][i];
+    }
+    // strong residual (e3res.f90:775-789)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      rL[i] = (aq[i] + u[0] * gu[0][i] + u[1] * gu[1][i] + u[2] * gu[2][i]) * rho + gp[i] - divq[i];
     const T uGu = u[0] * (gg[0] * u[0] + gg[3] * u[1] + gg[5] * u[2]) +
                   u[1] * (gg[3] * u[0] + gg[1] * u[1] + gg[4] * u[2]) +
                   u[2] * (gg[5] * u[0] + gg[4] * u[1] + gg[2] * u[2]);
     const T fact = sqrt((tau0 + uGu) + visc);
     const T tauC = rho * T(0.125) * fact / trg * ff;
     const T tauM = T(1) / fact;
+    T tauBar = rL[0] * (gg[0] * rL[0] + gg[3] * rL[1] + gg[5] * rL[2]) +
+               rL[1] * (gg[3] * rL[0] + gg[1] * rL[1] + gg[4] * rL[2]) +
+               rL[2] * (gg[5] * rL[0] + gg[4] * rL[1] + gg[2] * rL[2]);
+    tauBar = (tauBar != T(0)) ? tauM / sqrt(tauBar) : T(0);
     TC += tauC;
     TM += tauM;
-    T z[4];
+    const T tb = tauBar * rhoinv;
+    T z[4], zl[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) z[a] = u[0] * g[a][0] + u[1] * g[a][1] + u[2] * g[a][2];
+    for (int a = 0; a < 4; ++a) {
+      z[a] = u[0] * g[a][0] + u[1] * g[a][1] + u[2] * g[a][2];
+      zl[a] = rL[0] * g[a][0] + rL[1] * g[a][1] + rL[2] * g[a][2];
+    }
     int k = 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int b = a; b < 4; ++b) SUPG[k++] += tauM * z[a] * z[b];
+      for (int b = a; b < 4; ++b) SUPG[k++] += tauM * z[a] * z[b] + tb * zl[a] * zl[b];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const T ub = u[i] - tauM * rL[i] * rhoinv;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) UB[a][i] += Nq[a] * ub;
+    }
   }
   const T tlW = (T)c.lhsFct * W;
   const T tsFct = (T)c.lhmFct * W * rho;
   const T Mu = T(4) * (tlW * mu);
   const T rt = rho * tlW;
-  T w[4][4];
-#pragma unroll
-  for (int cc = 0; cc < 4; ++cc)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) w[cc][b] = ul[cc][0] * g[b][0] + ul[cc][1] * g[b][1] + ul[cc][2] * g[b][2];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const T conv = (T)mm(a, 0) * w[0][b] + (T)mm(a, 1) * w[1][b] + (T)mm(a, 2) * w[2][b] + (T)mm(a, 3) * w[3][b];
+      const T conv = UB[a][0] * g[b][0] + UB[a][1] * g[b][1] + UB[a][2] * g[b][2];
       const int lo = a < b ? a : b, hi = a < b ? b : a;
       const int k = lo * 4 - (lo * (lo - 1)) / 2 + (hi - lo);
       const T gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
-      r[16 + SSTRIDE * a + b] = (TO)(tsFct * (T)mm(a, b) + rt * conv + Mu * gab + rt * SUPG[k]);
+      r[R32_S(a) + b] = (float)(tsFct * (T)mm(a, b) + rt * conv + Mu * gab + rt * SUPG[k]);
     }
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) r[R_G + a * 3 + j] = (TO)g[a][j];
-  r[R_TLW] = (TO)tlW;
-  r[R_TC] = (TO)(tlW * TC);
-  r[R_MU] = (TO)Mu;
-  r[R_CM] = (TO)(tlW * TM / rho);
+    for (int j = 0; j < 3; ++j) r[R_G + a * 3 + j] = (float)g[a][j];
+  r[R_TLW] = (float)tlW;
+  r[R_TC] = (float)(tlW * TC);
+  r[R_MU] = (float)Mu;
+  r[R_CM] = (float)(tlW * TM / rho);
 }
 
 __global__ void __launch_bounds__(256) e3_tangent32(int nel, const int *__restrict__ ien,
-                                                    const double *__restrict__ x, int numnp,
-                                                    const double *__restrict__ y, int nshg, FormConst c0,
-                                                    Ens en, float aspect_fp64, float *__restrict__ rec) {
+                                                    const double *__restrict__ node, FormConst c0, Ens en,
+                                                    float aspect_fp64, float *__restrict__ rec) {
   const int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= nel) return;
   FormConst c = c0;
   c.mu = ens_mu(en, e, c0.mu);
-  double xl[4][3], ul[4][3];
+  double nl[4][kNRec];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int n = ien[a * (size_t)nel + e];
+    const double2 *rr = reinterpret_cast<const double2 *>(node + (size_t)ien[a * (size_t)nel + e] * kNRec);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      xl[a][i] = x[n + (size_t)numnp * i];
-      ul[a][i] = y[n + (size_t)nshg * i];
+    for (int k = 0; k < kNRec / 2; ++k) {
+      const double2 v = rr[k];
+      nl[a][2 * k] = v.x;
+      nl[a][2 * k + 1] = v.y;
     }
   }
+  double xl[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) xl[a][i] = nl[a][N_X + i];
   float r[kRec32];
   if (tet_aspect(xl) > aspect_fp64)
-    tangent_body<double>(xl, ul, c, r);
+    tangent_body<double>(nl, c, r);
   else
-    tangent_body<float>(xl, ul, c, r);
+    tangent_body<float>(nl, c, r);
   float4 *o = reinterpret_cast<float4 *>(rec + (size_t)e * kRec32);
 #pragma unroll
   for (int k = 0; k < kRec32 / 4; ++k) o[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
@@ -1302,8 +1356,8 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
       e3_elem<false><<<crimson::grid_for(f->nel, kE3Threads), kE3Threads, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p,
                                                                                    c, f->ens(), f->d_rec.p);
     if (t32)
-      e3_tangent32<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg, c,
-                                                              f->ens(), (float)f->prm.aspect_fp64, f->d_rec32.p);
+      e3_tangent32<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, c, f->ens(),
+                                                              (float)f->prm.aspect_fp64, f->d_rec32.p);
   }
   CR_HIP(hipGetLastError());
   if (timed) CR_HIP(hipEventRecord(f->ev[2], s));
@@ -1550,6 +1604,7 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
     crimson_form_free(f);
     return code;
   };
+  if (hipGetDevice(&f->device) != hipSuccess) f->device = 0;
   if (hipStreamCreate(&f->stream) != hipSuccess) {
     crimson::set_error("hipStreamCreate failed (is a GPU visible?)");
     f->stream = nullptr;
@@ -1759,7 +1814,31 @@ struct crimson_graph {
   hipGraph_t g = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t s = nullptr;
+  const crimson_form *f = nullptr;
+  const crimson_lhs *lhs = nullptr;
+  std::vector<unsigned long long> sig;  // graph_signature at capture
 };
+
+namespace {
+// What a captured formation graph baked in: the allocation serial of every
+// buffer its kernels read or write, and the scalars that chose its kernels
+// and launch shapes.  A replay is refused once any of them has changed.
+std::vector<unsigned long long> graph_signature(const crimson_form *f, const crimson_lhs *lhs) {
+  std::vector<unsigned long long> v = {
+      f->d_ien.serial, f->d_inc_ptr.serial, f->d_inc.serial, f->d_diagk.serial, f->d_rowptr.serial,
+      f->d_ibc.serial, f->d_lst_ptr.serial, f->d_order.serial, f->d_slot.serial, f->d_lst.serial,
+      f->d_frec.serial, f->d_node.serial, f->d_rec.serial, f->d_rec32.serial, f->d_bc.serial,
+      f->d_bcl.serial, f->d_bcn.serial, f->d_inst.serial, f->d_mu_inst.serial, f->d_binst.serial,
+      f->d_ienb.serial, f->d_bcode.serial, f->d_bcb.serial, f->d_permit.serial, f->d_brec.serial,
+      f->d_bflx.serial, f->d_sptr.serial, f->d_selem.serial, f->d_bnode.serial, f->d_binc_ptr.serial,
+      f->d_binc.serial, lhs->d_V.serial, lhs->d_V32.serial,
+      (unsigned long long)f->ninst, (unsigned long long)f->nelb, (unsigned long long)f->npartial,
+      (unsigned long long)f->nbcl, (unsigned long long)f->prm.precision, (unsigned long long)f->asm_chunk,
+      (unsigned long long)f->asm_xcd, (unsigned long long)f->sched, (unsigned long long)lhs->vdof,
+      (unsigned long long)lhs->prec, (unsigned long long)lhs->gt_is_d};
+  return v;
+}
+}  // namespace
 
 extern "C" int crimson_form_graph_create(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
                                          double *d_res, crimson_lhs *lhs, crimson_graph **out) {
@@ -1801,6 +1880,9 @@ extern "C" int crimson_form_graph_create(crimson_form *f, const double *d_y, con
     if (!rc) crimson::set_error("capture of the formation into a HIP graph failed");
     return rc ? rc : CRIMSON_EHIP;
   }
+  gr->f = f;
+  gr->lhs = lhs;
+  gr->sig = graph_signature(f, lhs);
   *out = gr;
   return CRIMSON_OK;
 }
@@ -1809,6 +1891,12 @@ extern "C" int crimson_graph_launch(crimson_graph *g) {
   if (!g) {
     crimson::set_error("crimson_graph_launch: null graph");
     return CRIMSON_EARG;
+  }
+  if (graph_signature(g->f, g->lhs) != g->sig) {
+    crimson::set_error("crimson_graph_launch: a buffer or setting the graph captured has changed since "
+                       "crimson_form_graph_create (instances, boundary, BC, precision, schedule or the matrix); "
+                       "free the graph and capture it again");
+    return CRIMSON_ESTATE;
   }
   CR_HIP(hipGraphLaunch(g->exec, g->s));
   CR_HIP(hipStreamSynchronize(g->s));

@@ -155,6 +155,7 @@ struct crimson_form {
   int nshg = 0, numnp = 0, nel = 0, nnz = 0;
   crimson_form_params prm{};
   hipStream_t stream = nullptr;
+  int device = 0;  // the device current at setup, which holds every buffer
   crimson::DBuf<int> d_ien, d_inc_ptr, d_inc, d_diagk, d_rowptr, d_ibc;
   crimson::DBuf<int> d_lst_ptr, d_order;  // d_order: assembly row schedule
   crimson::DBuf<int4> d_slot;              // per schedule slot: the row's offsets (slot_table_kernel)
@@ -218,6 +219,18 @@ int form_boundary_pass(crimson_form *f, double *d_res, int mode);
 // crimson_form_set_boundary / crimson_form_set_instances changes).
 int form_boundary_instances(crimson_form *f);
 // Adjoint of the boundary rows: ybar(nshg,4) += w^T d rl_bnd/d y from the
-// node records of the last flux pass; *mubar = w^T d rl_bnd/d mu.
-int form_boundary_vjp(crimson_form *f, const double *d_w, double *d_ybar, double *mubar);
+// node records of the last flux pass; *mubar = w^T d rl_bnd/d mu.  d_bvrec:
+// a workspace of form_boundary_vjp_doubles(f) doubles, or null for the
+// handle's own (allocated on first use).
+int form_boundary_vjp(crimson_form *f, const double *d_w, double *d_ybar, double *mubar, double *d_bvrec);
+size_t form_boundary_vjp_doubles(const crimson_form *f);
+// Device workspaces of one VJP (crimson_form_vjp_dev): the element records,
+// qbar, 1/rmass, the mu partial sums and the boundary records (null without
+// boundary elements), sized by vjp_work_sizes in doubles.
+struct VjpWork {
+  double *vrec, *vrec2, *qbar, *invm, *mupart, *bvrec;
+};
+void vjp_work_sizes(const crimson_form *f, size_t n[6]);
+int vjp_run(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, const double *d_w,
+            double *d_ybar, double *mubar, const VjpWork &w);
 }  // namespace crimson

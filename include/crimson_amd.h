@@ -195,7 +195,16 @@ int crimson_form_set_instances(crimson_form *f, const unsigned short *inst,
  * once into a HIP graph over fixed device arrays, then replayed by
  * crimson_graph_launch (synchronous) for each nonlinear iteration whose
  * inputs were written into the same arrays.  No counterpart in the
- * reference.  No per-phase timing is recorded for replays. */
+ * reference.  No per-phase timing is recorded for replays.  The graph holds
+ * the device addresses of the handle's buffers and of lhs's matrix: once a
+ * later call has replaced any of them or changed what the capture chose
+ * (crimson_form_set_instances with another instance count or 0,
+ * crimson_form_set_boundary, crimson_form_set_bc, crimson_form_autotune,
+ * crimson_lhs_set_precision, a formation into lhs at another precision, ...),
+ * crimson_graph_launch replays nothing and returns CRIMSON_ESTATE; free the
+ * graph and capture again.  New viscosities through
+ * crimson_form_set_instances with the same instance count keep the graph
+ * valid (they are read at each replay).  Free the graph before f and lhs. */
 typedef struct crimson_graph crimson_graph;
 int crimson_form_graph_create(crimson_form *f, const double *d_y,
                               const double *d_ac, const double *d_x,

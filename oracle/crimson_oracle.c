@@ -15,6 +15,18 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* The threaded build (-fopenmp -DCFO_THREADS, liboracle_mt.so) timed as the
+ * all-core CPU baseline splits the O(N) loops of memLS, PRECOND and the Val4
+ * packing over threads with static schedules, and the formation over element
+ * blocks and CSR rows (cfo_elmgmr_interior_plan).  The serial build ignores
+ * CFO_PFOR. */
+#ifdef CFO_THREADS
+#include <omp.h>
+#define CFO_PFOR _Pragma("omp parallel for schedule(static)")
+#else
+#define CFO_PFOR
+#endif
+
 /* 4-point rule (symtet.c:22-23,72-74), Qwt = 4/3 * 1/4 (genint.f90:85) */
 static const double A4 = 0.5854101966249685;
 static const double B4 = 0.1381966011250150;
@@ -260,9 +272,17 @@ void cfo_e3_block(int n, const double *yl, const double *acl,
       tauC = tlW * tauC;
       rmu = tlW * rmu;
       const double tsFct = lhmFct * W * rho;
-      /* lazy tangent: tauBar = 0 and uBar replaced by rho*tlW*u */
+#ifndef CFO_PLANT_LAZY_TANGENT
+      /* iconvflow = 2 branch (e3lhs.f90:124-129): tauBar scaled by
+         lhsFct*WdetJ and the convection taken with rho*tlW*uBar */
+      tauBar = lhsFct * W * tauBar;
+      double ub[3] = {t1w * uBar[0], t1w * uBar[1], t1w * uBar[2]};
+#else
+      /* a planted mistake, built only by the tests that show the tangent
+         pins reject it: the other branch (e3lhs.f90:130-134) */
       tauBar = 0.0;
       double ub[3] = {t1w * u1, t1w * u2, t1w * u3};
+#endif
       for (int b = 0; b < 4; ++b) {
         const double t1 = (ub[0] - multfact * um1) * sg[b][0] + (ub[1] - multfact * um2) * sg[b][1] + (ub[2] - multfact * um3) * sg[b][2];
         for (int a = 0; a < 4; ++a) {
@@ -679,6 +699,233 @@ int cfo_elmgmr_interior(int nshg, int numnp, int nel, const int *ien,
                        rowp, nnz, res, lhsK, lhsP, qres, p);
 }
 
+/* ------------------------------------------------------------------ */
+/* The same interior formation in two phases, for the threaded CPU baseline:
+ * (1) every element block's e3q / e3 (+ bc3lhs) into per-element storage,
+ * blocks independent; (2) the scatters gathered per node or CSR row over the
+ * node's element incidences.  Each sum takes its terms in the order of the
+ * block loop above: local(...,'scatter ') adds by block, then local node,
+ * then element (local.f90:68-75), fillsparseI by element (fillsparse.f90:
+ * 29-63), so both paths give the same bits.  In the serial build CFO_PFOR is
+ * empty and the phases run one after the other on one core. */
+struct cfo_plan {
+  int nshg, nel, bs;
+  int *ptr;   /* (nshg+1): incidences of node i at ptr[i] .. ptr[i+1]-1 */
+  int *inc_b; /* e << 2 | a in scatter order: block, local node, element */
+  int *inc_e; /* e << 2 | a in element order */
+  /* per-element storage of the last formation, kept between calls:
+   * 36 + 4 + 16 + 144 + 64 doubles per element, and qres(nshg,9) */
+  double *store, *qres;
+};
+
+int cfo_plan_create(int nshg, int nel, const int *ien, int ibksiz, cfo_plan **out) {
+  if (nshg <= 0 || nel < 0 || !ien || !out) return CFO_EARG;
+  const int bs = ibksiz > 0 ? ibksiz : 255;
+  cfo_plan *pl = calloc(1, sizeof(cfo_plan));
+  pl->nshg = nshg;
+  pl->nel = nel;
+  pl->bs = bs;
+  pl->ptr = calloc((size_t)nshg + 1, sizeof(int));
+  pl->inc_b = malloc(sizeof(int) * ((size_t)4 * nel + 1));
+  pl->inc_e = malloc(sizeof(int) * ((size_t)4 * nel + 1));
+  int *fill = malloc(sizeof(int) * ((size_t)nshg + 1));
+  for (size_t k = 0; k < (size_t)4 * nel; ++k) {
+    const int g = ien[k];
+    if (g < 1 || g > nshg) {
+      free(fill);
+      cfo_plan_free(pl);
+      return CFO_EARG;
+    }
+    pl->ptr[g] += 1;
+  }
+  for (int i = 0; i < nshg; ++i) pl->ptr[i + 1] += pl->ptr[i];
+  memcpy(fill, pl->ptr, sizeof(int) * nshg);
+  for (int e = 0; e < nel; ++e)
+    for (int a = 0; a < 4; ++a) {
+      const int g = ien[(size_t)nel * a + e] - 1;
+      pl->inc_e[fill[g]++] = e << 2 | a;
+    }
+  memcpy(fill, pl->ptr, sizeof(int) * nshg);
+  for (int iel = 0; iel < nel; iel += bs) {
+    const int n = (nel - iel < bs) ? nel - iel : bs;
+    for (int a = 0; a < 4; ++a)
+      for (int e = iel; e < iel + n; ++e) {
+        const int g = ien[(size_t)nel * a + e] - 1;
+        pl->inc_b[fill[g]++] = e << 2 | a;
+      }
+  }
+  free(fill);
+  *out = pl;
+  return CFO_OK;
+}
+
+void cfo_plan_free(cfo_plan *pl) {
+  if (!pl) return;
+  free(pl->ptr);
+  free(pl->inc_b);
+  free(pl->inc_e);
+  free(pl->store);
+  free(pl->qres);
+  free(pl);
+}
+
+int cfo_threads(int n) {
+#ifdef CFO_THREADS
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+int cfo_elmgmr_interior_plan(cfo_plan *pl, int nshg, int numnp, int nel, const int *ien,
+                             const double *y, const double *ac, const double *x, const int *iBC,
+                             const double *BC, int ndofBC, const int *colm, const int *rowp, int nnz,
+                             double *res, double *lhsK, double *lhsP, const cfo_params *p) {
+  if (!pl || pl->nshg != nshg || pl->nel != nel || nel <= 0 || !ien || !y || !ac || !x || !res) return CFO_EARG;
+  if (pl->bs != (p->ibksiz > 0 ? p->ibksiz : 255)) return CFO_EARG;
+  if (p->lhs == 1 && (!colm || !rowp || !lhsK || !lhsP)) return CFO_EARG;
+  int rc = check_bc_supported(nshg, iBC, BC, ndofBC);
+  if (rc) return rc;
+  const int bs = pl->bs, nblk = (nel + bs - 1) / bs;
+  const int lhs = p->lhs == 1;
+  mesh_view m = {nshg, numnp, nel, ien, y, ac, x};
+  /* per-element storage, element-major so that a node's gather reads each
+   * incidence's values contiguously: qlE[e][a][9], rmE[e][a], rlE[e][a][4],
+   * KE[e][a][b][9], GE[e][a][b][4] */
+  cfo_plan *pw = pl; /* the plan's storage is reused by every call */
+  if (!pw->store) pw->store = malloc(sizeof(double) * (size_t)nel * (36 + 4 + 16 + 144 + 64));
+  if (!pw->qres) pw->qres = malloc(sizeof(double) * (size_t)nshg * 9);
+  if (!pw->store || !pw->qres) return CFO_EARG;
+  double *qlE = pw->store, *rmE = qlE + (size_t)nel * 36, *rlE = rmE + (size_t)nel * 4;
+  double *KE = rlE + (size_t)nel * 16, *GE = KE + (size_t)nel * 144;
+  double *qres = pw->qres;
+  /* flux pass, per block (AsIq, asiq.f90:46-75) */
+#ifdef CFO_THREADS
+#pragma omp parallel
+#endif
+  {
+    double *yl = malloc(sizeof(double) * bs * 16), *xl = malloc(sizeof(double) * bs * 12),
+           *ql = malloc(sizeof(double) * bs * 36), *rm = malloc(sizeof(double) * bs * 4);
+#ifdef CFO_THREADS
+#pragma omp for schedule(static)
+#endif
+    for (int b = 0; b < nblk; ++b) {
+      const int iel = b * bs, n = (nel - iel < bs) ? nel - iel : bs;
+      const int *ienb[4];
+      for (int a = 0; a < 4; ++a) ienb[a] = ien + (size_t)nel * a + iel;
+      gather_block(&m, iel, n, ienb, yl, NULL, xl, NULL, NULL);
+      memset(ql, 0, sizeof(double) * n * 36);
+      memset(rm, 0, sizeof(double) * n * 4);
+      cfo_e3q_block(n, yl, xl, ql, rm, p);
+      for (int e = 0; e < n; ++e)
+        for (int a = 0; a < 4; ++a) {
+          rmE[4 * (size_t)(iel + e) + a] = rm[e + n * a];
+          for (int k = 0; k < 9; ++k) qlE[36 * (size_t)(iel + e) + 9 * a + k] = ql[e + n * (a + 4 * k)];
+        }
+    }
+    free(yl);
+    free(xl);
+    free(ql);
+    free(rm);
+  }
+  /* the flux scatter and qpbc (qpbc.f90:37-71), per node */
+  CFO_PFOR
+  for (int i = 0; i < nshg; ++i) {
+    double rmass = 0.0, sq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = pl->ptr[i]; q < pl->ptr[i + 1]; ++q) {
+      const int e = pl->inc_b[q] >> 2, a = pl->inc_b[q] & 3;
+      rmass = rmass + rmE[4 * (size_t)e + a];
+      const double *v = qlE + 36 * (size_t)e + 9 * a;
+      for (int k = 0; k < 9; ++k) sq[k] = sq[k] + v[k];
+    }
+    rmass = 1.0 / rmass;
+    for (int k = 0; k < 9; ++k) qres[i + (size_t)nshg * k] = rmass * sq[k];
+  }
+  /* interior elements, per block (AsIGMR, asigmr.f90:66-116; bc3lhs) */
+#ifdef CFO_THREADS
+#pragma omp parallel
+#endif
+  {
+    double *yl = malloc(sizeof(double) * bs * 16), *acl = malloc(sizeof(double) * bs * 16),
+           *xl = malloc(sizeof(double) * bs * 12), *ql = malloc(sizeof(double) * bs * 36),
+           *rl = malloc(sizeof(double) * bs * 16), *xK = malloc(sizeof(double) * bs * 144),
+           *xG = malloc(sizeof(double) * bs * 64);
+#ifdef CFO_THREADS
+#pragma omp for schedule(static)
+#endif
+    for (int b = 0; b < nblk; ++b) {
+      const int iel = b * bs, n = (nel - iel < bs) ? nel - iel : bs;
+      const int *ienb[4];
+      for (int a = 0; a < 4; ++a) ienb[a] = ien + (size_t)nel * a + iel;
+      gather_block(&m, iel, n, ienb, yl, acl, xl, qres, ql);
+      memset(rl, 0, sizeof(double) * n * 16);
+      if (lhs) {
+        memset(xK, 0, sizeof(double) * n * 144);
+        memset(xG, 0, sizeof(double) * n * 64);
+      }
+      cfo_e3_block(n, yl, acl, xl, ql, rl, lhs ? xK : NULL, lhs ? xG : NULL, p);
+      if (lhs && iBC && BC)
+        for (int e = 0; e < n; ++e) {
+          const int nodes[4] = {ienb[0][e], ienb[1][e], ienb[2][e], ienb[3][e]};
+          bc3lhs_elem(nshg, iBC, BC, nodes, xK, e, n);
+        }
+      for (int e = 0; e < n; ++e) {
+        const size_t g = (size_t)(iel + e);
+        for (int a = 0; a < 4; ++a)
+          for (int k = 0; k < 4; ++k) rlE[16 * g + 4 * a + k] = rl[e + n * (a + 4 * k)];
+        if (!lhs) continue;
+        for (int a = 0; a < 4; ++a)
+          for (int bb = 0; bb < 4; ++bb) {
+            for (int cc = 0; cc < 9; ++cc) KE[144 * g + 36 * a + 9 * bb + cc] = xK[e + n * (cc + 9 * (a + 4 * bb))];
+            for (int cc = 0; cc < 4; ++cc) GE[64 * g + 16 * a + 4 * bb + cc] = xG[e + n * (cc + 4 * (a + 4 * bb))];
+          }
+      }
+    }
+    free(yl);
+    free(acl);
+    free(xl);
+    free(ql);
+    free(rl);
+    free(xK);
+    free(xG);
+  }
+  /* residual rows (asigmr.f90:116) and fillsparseI rows */
+  int miss = 0;
+  CFO_PFOR
+  for (int i = 0; i < nshg; ++i) {
+    double sr[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = pl->ptr[i]; q < pl->ptr[i + 1]; ++q) {
+      const int e = pl->inc_b[q] >> 2, a = pl->inc_b[q] & 3;
+      const double *v = rlE + 16 * (size_t)e + 4 * a;
+      for (int k = 0; k < 4; ++k) sr[k] = sr[k] + v[k];
+    }
+    for (int k = 0; k < 4; ++k) res[i + (size_t)nshg * k] = sr[k];
+    if (!lhs) continue;
+    const int c = colm[i], len = colm[i + 1] - c;
+    memset(lhsK + 9 * (size_t)(c - 1), 0, sizeof(double) * 9 * len);
+    memset(lhsP + 4 * (size_t)(c - 1), 0, sizeof(double) * 4 * len);
+    for (int q = pl->ptr[i]; q < pl->ptr[i + 1]; ++q) {
+      const int e = pl->inc_e[q] >> 2, a = pl->inc_e[q] & 3;
+      const double *xKa = KE + 144 * (size_t)e + 36 * a, *xGa = GE + 64 * (size_t)e + 16 * a;
+      for (int b = 0; b < 4; ++b) {
+        const int target = ien[(size_t)nel * b + e];
+        const int k = sparseloc(rowp + (c - 1), len, target) + c - 1;
+        if (rowp[k - 1] != target) {
+          miss = 1;
+          continue;
+        }
+        for (int cc = 0; cc < 9; ++cc)
+          lhsK[cc + 9 * (size_t)(k - 1)] = lhsK[cc + 9 * (size_t)(k - 1)] + xKa[9 * b + cc];
+        for (int cc = 0; cc < 4; ++cc)
+          lhsP[cc + 4 * (size_t)(k - 1)] = lhsP[cc + 4 * (size_t)(k - 1)] + xGa[4 * b + cc];
+      }
+    }
+  }
+  return miss ? CFO_ESEARCH : CFO_OK;
+}
+
 int cfo_elmgmr_interior_bc(int nshg, int numnp, int nel, const int *ien,
                            const double *y, const double *ac, const double *x,
                            const int *iBC, const double *BC, int ndofBC,
@@ -848,6 +1095,7 @@ void cfo_pack_val4(int nshg, const int *colm, const int *rowp,
                    const double *lhsK, const double *lhsP, double *Val4) {
   /* solfar.f90:212-235 */
   const int nnz = colm[nshg] - 1;
+  CFO_PFOR
   for (int i = 0; i < nnz; ++i) {
     double *v = Val4 + 16 * (size_t)i;
     const double *k = lhsK + 9 * (size_t)i, *pp = lhsP + 4 * (size_t)i;
@@ -871,6 +1119,7 @@ void cfo_pack_val4(int nshg, const int *colm, const int *rowp,
     v[7] = 0.0;
     v[11] = 0.0;
   }
+  CFO_PFOR
   for (int i = 1; i <= nshg; ++i)
     for (int j = colm[i - 1]; j <= colm[i] - 1; ++j) {
       const int k = rowp[j - 1];
@@ -1052,15 +1301,7 @@ void cfo_ls_stats(const cfo_ls *ls, int sub, double *o) {
   o[7] = ls->Resc;
 }
 
-static double normv(int dof, int nNo, const double *U) {
-  double s = 0.0;
-  for (int i = 0; i < nNo; ++i) {
-    double t = s;
-    for (int k = 0; k < dof; ++k) t = t + U[k + (size_t)dof * i] * U[k + (size_t)dof * i];
-    s = t;
-  }
-  return sqrt(s);
-}
+#ifndef CFO_THREADS
 static double dotv(int dof, int nNo, const double *U, const double *V) {
   double s = 0.0;
   for (int i = 0; i < nNo; ++i) {
@@ -1070,11 +1311,38 @@ static double dotv(int dof, int nNo, const double *U, const double *V) {
   }
   return s;
 }
+#else
+/* threaded build: each thread sums a contiguous range of rows; the partial
+ * sums are added in thread order (deterministic for a fixed thread count) */
+static double dotv(int dof, int nNo, const double *U, const double *V) {
+  enum { kMaxThreads = 1024 };
+  double part[kMaxThreads];
+  int nt = 1;
+#pragma omp parallel
+  {
+    const int T = omp_get_num_threads() < kMaxThreads ? omp_get_num_threads() : kMaxThreads;
+    const int t = omp_get_thread_num();
+    if (t == 0) nt = T;
+    if (t < T) {
+      const int i0 = (int)((long long)nNo * t / T), i1 = (int)((long long)nNo * (t + 1) / T);
+      double s = 0.0;
+      for (int i = i0; i < i1; ++i)
+        for (int k = 0; k < dof; ++k) s = s + U[k + (size_t)dof * i] * V[k + (size_t)dof * i];
+      part[t] = s;
+    }
+  }
+  double s = 0.0;
+  for (int t = 0; t < nt; ++t) s = s + part[t];
+  return s;
+}
+#endif
+static double normv(int dof, int nNo, const double *U) { return sqrt(dotv(dof, nNo, U, U)); }
 
 /* SPARMULVV (memLS.f90:2024-2092): KU(l,i) = sum_j sum_m K(l*dof+m, j) U(m) */
 static void sparmulvv(int dof, const cfo_lhs *l, const double *K,
                       const double *U, double *KU) {
   memset(KU, 0, sizeof(double) * (size_t)dof * l->nNo);
+  CFO_PFOR
   for (int i = 0; i < l->nNo; ++i)
     for (int j = l->rowS[i]; j <= l->rowE[i]; ++j) {
       const int col = l->colPtr[j];
@@ -1089,6 +1357,7 @@ static void sparmulvv(int dof, const cfo_lhs *l, const double *K,
 static void sparmulvs(int dof, const cfo_lhs *l, const double *K,
                       const double *U, double *KU) {
   memset(KU, 0, sizeof(double) * l->nNo);
+  CFO_PFOR
   for (int i = 0; i < l->nNo; ++i)
     for (int j = l->rowS[i]; j <= l->rowE[i]; ++j) {
       const int col = l->colPtr[j];
@@ -1101,6 +1370,7 @@ static void sparmulvs(int dof, const cfo_lhs *l, const double *K,
 static void sparmulsv(int dof, const cfo_lhs *l, const double *K,
                       const double *U, double *KU) {
   memset(KU, 0, sizeof(double) * (size_t)dof * l->nNo);
+  CFO_PFOR
   for (int i = 0; i < l->nNo; ++i)
     for (int j = l->rowS[i]; j <= l->rowE[i]; ++j) {
       const int col = l->colPtr[j];
@@ -1111,6 +1381,7 @@ static void sparmulsv(int dof, const cfo_lhs *l, const double *K,
 static void sparmulss(const cfo_lhs *l, const double *K, const double *U,
                       double *KU) {
   memset(KU, 0, sizeof(double) * l->nNo);
+  CFO_PFOR
   for (int i = 0; i < l->nNo; ++i)
     for (int j = l->rowS[i]; j <= l->rowE[i]; ++j) KU[i] = KU[i] + K[j] * U[l->colPtr[j]];
 }
@@ -1215,6 +1486,7 @@ static void gmres(const cfo_lhs *l, int dof, subls_t *ls, const double *Val,
       ls->itr = ls->itr + 1;
       sparmulvv(dof, l, Val, X, U(0));
       addbcmul(BCOP_ADD, l, dof, X, U(0));
+      CFO_PFOR
       for (size_t k = 0; k < vn; ++k) U(0)[k] = R[k] - U(0)[k];
     }
     if (coupled) {
@@ -1234,6 +1506,7 @@ static void gmres(const cfo_lhs *l, int dof, subls_t *ls, const double *Val,
       eps = (ls->absTol > ls->relTol * eps) ? ls->absTol : ls->relTol * eps;
     }
     ls->dB = ls->fNorm;
+    CFO_PFOR
     for (size_t k = 0; k < vn; ++k) U(0)[k] = U(0)[k] / err[0];
     int i;
     for (i = 1; i <= sD; ++i) {
@@ -1246,9 +1519,11 @@ static void gmres(const cfo_lhs *l, int dof, subls_t *ls, const double *Val,
       }
       for (int j = 1; j <= i; ++j) { /* modified Gram-Schmidt */
         H(j - 1, i - 1) = dotv(dof, nNo, U(i), U(j - 1));
+        CFO_PFOR
         for (size_t k = 0; k < vn; ++k) U(i)[k] = U(i)[k] - H(j - 1, i - 1) * U(j - 1)[k];
       }
       H(i, i - 1) = normv(dof, nNo, U(i));
+      CFO_PFOR
       for (size_t k = 0; k < vn; ++k) U(i)[k] = U(i)[k] / H(i, i - 1);
       for (int j = 1; j <= i - 1; ++j) {
         const double tmp = c[j - 1] * H(j - 1, i - 1) + s[j - 1] * H(j, i - 1);
@@ -1274,6 +1549,7 @@ static void gmres(const cfo_lhs *l, int dof, subls_t *ls, const double *Val,
       y[j - 1] = y[j - 1] / H(j - 1, j - 1);
     }
     for (int j = 1; j <= i; ++j)
+      CFO_PFOR
       for (size_t k = 0; k < vn; ++k) X[k] = X[k] + U(j - 1)[k] * y[j - 1];
     ls->fNorm = fabs(err[i]);
     if (ls->suc) break;
@@ -1319,8 +1595,10 @@ static void gmresv(const cfo_lhs *l, int dof, subls_t *ls, const double *Val,
     ls->itr = ls->itr + 1;
     sparmulvv(dof, l, Val, X, U(0));
     addbcmul(BCOP_ADD, l, dof, X, U(0));
+    CFO_PFOR
     for (size_t k = 0; k < vn; ++k) U(0)[k] = R[k] - U(0)[k];
     err[0] = normv(dof, nNo, U(0));
+    CFO_PFOR
     for (size_t k = 0; k < vn; ++k) U(0)[k] = U(0)[k] / err[0];
     int i;
     for (i = 1; i <= sD; ++i) {
@@ -1329,9 +1607,11 @@ static void gmresv(const cfo_lhs *l, int dof, subls_t *ls, const double *Val,
       addbcmul(BCOP_ADD, l, dof, U(i - 1), U(i));
       for (int j = 1; j <= i; ++j) {
         H(j - 1, i - 1) = dotv(dof, nNo, U(i), U(j - 1));
+        CFO_PFOR
         for (size_t k = 0; k < vn; ++k) U(i)[k] = U(i)[k] - H(j - 1, i - 1) * U(j - 1)[k];
       }
       H(i, i - 1) = normv(dof, nNo, U(i));
+      CFO_PFOR
       for (size_t k = 0; k < vn; ++k) U(i)[k] = U(i)[k] / H(i, i - 1);
       for (int j = 1; j <= i - 1; ++j) {
         const double tmp = c[j - 1] * H(j - 1, i - 1) + s[j - 1] * H(j, i - 1);
@@ -1357,6 +1637,7 @@ static void gmresv(const cfo_lhs *l, int dof, subls_t *ls, const double *Val,
       y[j - 1] = y[j - 1] / H(j - 1, j - 1);
     }
     for (int j = 1; j <= i; ++j)
+      CFO_PFOR
       for (size_t k = 0; k < vn; ++k) X[k] = X[k] + U(j - 1)[k] * y[j - 1];
     ls->fNorm = fabs(err[i]);
     if (ls->suc) break;
@@ -1406,12 +1687,16 @@ static void cgrad(const cfo_lhs *l, int dof, subls_t *ls, const double *D,
     }
     sparmulvs(dof, l, D, GP, DGP);
     sparmulss(l, L, P, SP);
+    CFO_PFOR
     for (int k = 0; k < nNo; ++k) SP[k] = SP[k] - DGP[k];
     const double alpha = errO / dotv(1, nNo, P, SP);
+    CFO_PFOR
     for (int k = 0; k < nNo; ++k) X[k] = X[k] + alpha * P[k];
+    CFO_PFOR
     for (int k = 0; k < nNo; ++k) R[k] = R[k] - alpha * SP[k];
     err = sqrt(dotv(1, nNo, R, R));
     err = err * err;
+    CFO_PFOR
     for (int k = 0; k < nNo; ++k) P[k] = R[k] + err / errO * P[k];
   }
   memcpy(R, X, sizeof(double) * nNo);
@@ -1452,10 +1737,13 @@ static void cgradv(const cfo_lhs *l, int dof, subls_t *ls, const double *K,
     else
       sparmulvv(dof, l, K, P, KP);
     const double alpha = errO / dotv(dof, nNo, P, KP);
+    CFO_PFOR
     for (size_t k = 0; k < vn; ++k) X[k] = X[k] + alpha * P[k];
+    CFO_PFOR
     for (size_t k = 0; k < vn; ++k) R[k] = R[k] - alpha * KP[k];
     err = normv(dof, nNo, R);
     err = err * err;
+    CFO_PFOR
     for (size_t k = 0; k < vn; ++k) P[k] = R[k] + err / errO * P[k];
   }
   memcpy(R, X, sizeof(double) * vn);
@@ -1482,6 +1770,7 @@ static int nssolver(cfo_lhs *l, cfo_ls *ls, int dof, const double *Val,
          *MU = malloc(sizeof(double) * mn * nB), *MP = malloc(sizeof(double) * (size_t)nNo * nB),
          *A = calloc((size_t)nB * nB, sizeof(double)), *B = calloc(nB, sizeof(double)),
          *xB = calloc(nB, sizeof(double));
+  CFO_PFOR
   for (int a = 0; a < nNo; ++a) {
     for (int k = 0; k < nsd; ++k) Rmi[k + (size_t)nsd * a] = Ri[k + (size_t)dof * a];
     Rci[a] = Ri[nsd + (size_t)dof * a];
@@ -1502,6 +1791,7 @@ static int nssolver(cfo_lhs *l, cfo_ls *ls, int dof, const double *Val,
   double *mK = malloc(sizeof(double) * 9 * (size_t)nnz), *mG = malloc(sizeof(double) * 3 * (size_t)nnz),
          *mD = malloc(sizeof(double) * 3 * (size_t)nnz), *mL = malloc(sizeof(double) * (size_t)nnz),
          *Gt = calloc(3 * (size_t)nnz, sizeof(double));
+  CFO_PFOR
   for (int i = 0; i < nnz; ++i) {
     const double *t = Val + 16 * (size_t)i;
     double *k = mK + 9 * (size_t)i;
@@ -1522,6 +1812,7 @@ static int nssolver(cfo_lhs *l, cfo_ls *ls, int dof, const double *Val,
     mD[3 * (size_t)i + 2] = t[14];
     mL[i] = t[15];
   }
+  CFO_PFOR
   for (int i = 0; i < nNo; ++i)
     for (int j = l->rowS[i]; j <= l->rowE[i]; ++j) {
       const int k = l->colPtr[j];
@@ -1549,9 +1840,11 @@ static int nssolver(cfo_lhs *l, cfo_ls *ls, int dof, const double *Val,
     double *MPb = MP + (size_t)nNo * (iB - 1), *MPbb = MP + (size_t)nNo * (iBB - 1);
     gmres(l, nsd, &ls->GM, mK, Rm, Ui);
     sparmulvs(nsd, l, mD, Ui, Pi);
+    CFO_PFOR
     for (int k = 0; k < nNo; ++k) Pi[k] = Rc[k] - Pi[k];
     cgrad(l, nsd, &ls->CG, Gt, mG, mL, Pi);
     sparmulsv(nsd, l, mG, Pi, MUb);
+    CFO_PFOR
     for (size_t k = 0; k < mn; ++k) MUbb[k] = Rm[k] - MUb[k];
     gmres(l, nsd, &ls->GM, mK, MUbb, Ui);
     sparmulvv(nsd, l, mK, Ui, MUbb);
@@ -1583,11 +1876,15 @@ static int nssolver(cfo_lhs *l, cfo_ls *ls, int dof, const double *Val,
       ls->RI.suc = 1;
       break;
     }
+    CFO_PFOR
     for (size_t k = 0; k < mn; ++k) Rm[k] = Rmi[k] - xB[0] * MU[k];
+    CFO_PFOR
     for (int k = 0; k < nNo; ++k) Rc[k] = Rci[k] - xB[0] * MP[k];
     for (int j = 2; j <= iBB; ++j) {
       const double *MUj = MU + mn * (j - 1), *MPj = MP + (size_t)nNo * (j - 1);
+      CFO_PFOR
       for (size_t k = 0; k < mn; ++k) Rm[k] = Rm[k] - xB[j - 1] * MUj[k];
+      CFO_PFOR
       for (int k = 0; k < nNo; ++k) Rc[k] = Rc[k] - xB[j - 1] * MPj[k];
     }
   }
@@ -1596,24 +1893,31 @@ static int nssolver(cfo_lhs *l, cfo_ls *ls, int dof, const double *Val,
       ls->RI.itr = ls->RI.mItr;
     } else {
       ls->RI.itr = i;
+      CFO_PFOR
       for (int k = 0; k < nNo; ++k) Rc[k] = Rci[k] - xB[0] * MP[k];
       for (int j = 2; j <= iBB; ++j) {
         const double *MPj = MP + (size_t)nNo * (j - 1);
+        CFO_PFOR
         for (int k = 0; k < nNo; ++k) Rc[k] = Rc[k] - xB[j - 1] * MPj[k];
       }
     }
     const double rcn = normv(1, nNo, Rc) / ls->RI.fNorm;
     ls->Resc = (int)lround(100.0 * rcn * rcn);
     ls->Resm = 100 - ls->Resc;
+    CFO_PFOR
     for (size_t k = 0; k < mn; ++k) Rmi[k] = xB[1] * U[k];
+    CFO_PFOR
     for (int k = 0; k < nNo; ++k) Rci[k] = xB[0] * P[k];
     for (int ii = 2; ii <= ls->RI.itr; ++ii) {
       const int b1 = 2 * ii - 1, b2 = 2 * ii;
       const double *Uii = U + mn * (ii - 1), *Pii = P + (size_t)nNo * (ii - 1);
+      CFO_PFOR
       for (size_t k = 0; k < mn; ++k) Rmi[k] = Rmi[k] + xB[b2 - 1] * Uii[k];
+      CFO_PFOR
       for (int k = 0; k < nNo; ++k) Rci[k] = Rci[k] + xB[b1 - 1] * Pii[k];
     }
     ls->RI.dB = 10.0 * log(ls->RI.fNorm / ls->RI.dB);
+    CFO_PFOR
     for (int a = 0; a < nNo; ++a) {
       for (int k = 0; k < nsd; ++k) Ri[k + (size_t)dof * a] = Rmi[k + (size_t)nsd * a];
       Ri[nsd + (size_t)dof * a] = Rci[a];
@@ -1641,10 +1945,12 @@ static int nssolver(cfo_lhs *l, cfo_ls *ls, int dof, const double *Val,
 /* PRECOND (memLS.f90:1763-1943) */
 static void precond(cfo_lhs *l, int dof, double *Val, double *R, double *W) {
   const int nNo = l->nNo;
+  CFO_PFOR
   for (int a = 0; a < nNo; ++a) {
     const int d = l->diagPtr[a];
     for (int i = 0; i < dof; ++i) W[i + (size_t)dof * a] = Val[(size_t)dof * dof * d + i * dof + i];
   }
+  CFO_PFOR
   for (int a = 0; a < nNo; ++a) {
     const int d = l->diagPtr[a];
     for (int i = 0; i < dof; ++i)
@@ -1653,6 +1959,7 @@ static void precond(cfo_lhs *l, int dof, double *Val, double *R, double *W) {
         Val[(size_t)dof * dof * d + i * dof + i] = 1.0;
       }
   }
+  CFO_PFOR
   for (size_t k = 0; k < (size_t)dof * nNo; ++k) W[k] = 1.0 / sqrt(fabs(W[k]));
   for (int f = 0; f < l->nFaces; ++f) {
     const face_t *fa = &l->face[f];
@@ -1665,18 +1972,22 @@ static void precond(cfo_lhs *l, int dof, double *Val, double *R, double *W) {
       }
   }
   /* row scaling */
+  CFO_PFOR
   for (int a = 0; a < nNo; ++a)
     for (int j = l->rowS[a]; j <= l->rowE[a]; ++j)
       for (int r = 0; r < dof; ++r)
         for (int c = 0; c < dof; ++c) Val[(size_t)dof * dof * j + r * dof + c] = Val[(size_t)dof * dof * j + r * dof + c] * W[r + (size_t)dof * a];
+  CFO_PFOR
   for (size_t k = 0; k < (size_t)dof * nNo; ++k) R[k] = W[k] * R[k];
   /* column scaling */
+  CFO_PFOR
   for (int a = 0; a < nNo; ++a)
     for (int j = l->rowS[a]; j <= l->rowE[a]; ++j) {
       const int col = l->colPtr[j];
       for (int r = 0; r < dof; ++r)
         for (int c = 0; c < dof; ++c) Val[(size_t)dof * dof * j + r * dof + c] = Val[(size_t)dof * dof * j + r * dof + c] * W[c + (size_t)dof * col];
     }
+  CFO_PFOR
   for (int a = 0; a < nNo; ++a) {
     const int d = l->diagPtr[a];
     for (int i = 0; i < dof; ++i)
@@ -1727,6 +2038,7 @@ int cfo_ls_solve(cfo_lhs *l, cfo_ls *ls, int dof, double *Ri, double *Val,
       rc = CFO_EARG;
   }
   if (rc == CFO_OK) {
+    CFO_PFOR
     for (size_t k = 0; k < vn; ++k) R[k] = R[k] * W[k];
     memcpy(Ri, R, sizeof(double) * vn);
   }

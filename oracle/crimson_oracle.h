@@ -187,6 +187,24 @@ int cfo_boundary_res(int nshg, int numnp, int nelb, const int *ienb, const int *
  * the restatement above (+,-,*,/,sqrt each 1). */
 double cfo_flops_per_element(void);
 
+/* The interior formation of cfo_elmgmr_interior_bc (without qres output)
+ * in two phases over a plan of the mesh's node incidences: element blocks
+ * first (e3q, then e3 + bc3lhs, into per-element storage), the scatters
+ * then gathered per node and CSR row in the block loop's order, so the
+ * result is bit-identical.  Built with -fopenmp -DCFO_THREADS
+ * (liboracle_mt.so) both phases run on cfo_threads() threads: the all-core
+ * CPU baseline.  ibksiz of the plan must equal p->ibksiz.  The plan keeps the
+ * per-element storage between calls (one call at a time per plan). */
+typedef struct cfo_plan cfo_plan;
+int cfo_plan_create(int nshg, int nel, const int *ien, int ibksiz, cfo_plan **out);
+void cfo_plan_free(cfo_plan *pl);
+int cfo_elmgmr_interior_plan(cfo_plan *pl, int nshg, int numnp, int nel, const int *ien,
+                             const double *y, const double *ac, const double *x, const int *iBC,
+                             const double *BC, int ndofBC, const int *colm, const int *rowp, int nnz,
+                             double *res, double *lhsK, double *lhsP, const cfo_params *p);
+/* Threads of the threaded build (n > 0 sets them); 1 in the serial build. */
+int cfo_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,12 +14,13 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
+_LIB_MT = os.path.join(_HERE, "liboracle_mt.so")
 
 ORACLE_HEADER_NOTE = "CPU restatement of the reference path; test infrastructure only"
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(_LIB):
+    if force or not os.path.exists(_LIB) or not os.path.exists(_LIB_MT):
         subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _LIB
 
@@ -33,13 +34,30 @@ class Params(C.Structure):
 
 
 _lib = None
+_lib_mt = None
 
 
 def lib():
     global _lib
     if _lib is None:
         build()
-        L = C.CDLL(_LIB)
+        _lib = _bind(C.CDLL(_LIB))
+    return _lib
+
+
+def lib_mt(threads: int = 0):
+    """The threaded build (OpenMP), with `threads` threads when > 0."""
+    global _lib_mt
+    if _lib_mt is None:
+        build()
+        _lib_mt = _bind(C.CDLL(_LIB_MT))
+    if threads > 0:
+        _lib_mt.cfo_threads(int(threads))
+    return _lib_mt
+
+
+def _bind(L):
+    if True:
         dp = np.ctypeslib.ndpointer(np.float64, flags="F_CONTIGUOUS")
         ip = np.ctypeslib.ndpointer(np.int32, flags="F_CONTIGUOUS")
         L.cfo_elmgmr_interior.argtypes = [C.c_int, C.c_int, C.c_int, ip, dp, dp, dp,
@@ -81,8 +99,14 @@ def lib():
         L.cfo_boundary_res_flx.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, C.c_int, dp, dp,
                                            C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p,
                                            C.c_int, C.c_int, dp, dp, C.c_int]
-        _lib = L
-    return _lib
+        L.cfo_plan_create.argtypes = [C.c_int, C.c_int, ip, C.c_int, C.POINTER(C.c_void_p)]
+        L.cfo_plan_free.argtypes = [C.c_void_p]
+        L.cfo_plan_free.restype = None
+        L.cfo_elmgmr_interior_plan.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, ip, dp, dp, dp,
+                                               C.c_void_p, C.c_void_p, C.c_int, ip, ip, C.c_int, dp, C.c_void_p,
+                                               C.c_void_p, C.POINTER(Params)]
+        L.cfo_threads.argtypes = [C.c_int]
+    return L
 
 
 def time_params(rhoinf: float):
@@ -156,6 +180,41 @@ def elmgmr_interior(ien, y, ac, x, colm, rowp, params, ibc=None, want_q=False, q
                                    _ptr(lhsK), _ptr(lhsP), _ptr(qres), C.byref(params))
     _check(rc, "elmgmr_interior")
     return (res, lhsK, lhsP, qres) if want_q else (res, lhsK, lhsP)
+
+
+class Plan:
+    """cfo_plan over a mesh: the two-phase interior formation, serial
+    (mt False) or on the threaded build's threads (mt True)."""
+
+    def __init__(self, ien, nshg, ibksiz=255, mt=False):
+        self.L = lib_mt() if mt else lib()
+        self.ien = _f(ien, np.int32)
+        self.nshg = int(nshg)
+        h = C.c_void_p()
+        _check(self.L.cfo_plan_create(self.nshg, self.ien.shape[0], self.ien, ibksiz, C.byref(h)), "plan_create")
+        self.h = h
+
+    def interior(self, y, ac, x, colm, rowp, params, ibc=None, bc=None):
+        """res(nshg,4), lhsK(9,nnz), lhsP(4,nnz) as elmgmr_interior."""
+        y, ac, x = _f(y), _f(ac), _f(x)
+        nnz = rowp.shape[0]
+        res = np.empty((self.nshg, 4), order="F")
+        lhsK = np.empty((9, nnz), order="F") if params.lhs == 1 else None
+        lhsP = np.empty((4, nnz), order="F") if params.lhs == 1 else None
+        ibc_a = None if ibc is None else _f(ibc, np.int32)
+        bc_a = None if bc is None else _f(bc)
+        rc = self.L.cfo_elmgmr_interior_plan(self.h, self.nshg, x.shape[0], self.ien.shape[0], self.ien, y, ac,
+                                             x, _ptr(ibc_a), _ptr(bc_a), 0 if bc is None else bc_a.shape[1],
+                                             _f(colm, np.int32), _f(rowp, np.int32), nnz, res, _ptr(lhsK),
+                                             _ptr(lhsP), C.byref(params))
+        _check(rc, "elmgmr_interior_plan")
+        return res, lhsK, lhsP
+
+    def __del__(self):
+        try:
+            self.L.cfo_plan_free(self.h)
+        except Exception:
+            pass
 
 
 def bc3lhs_elem(ibc, bc, nodes, xK):
@@ -238,10 +297,10 @@ def boundary_flux(ienb, iBCB, BCB, y, x, nflx, rho=1.06, mu=0.04, influx=0.5, nq
     return flx
 
 
-def pack_val4(colm, rowp, lhsK, lhsP):
+def pack_val4(colm, rowp, lhsK, lhsP, mt=False):
     nshg = colm.shape[0] - 1
     val = np.zeros((16, rowp.shape[0]), order="F")
-    lib().cfo_pack_val4(nshg, _f(colm, np.int32), _f(rowp, np.int32), _f(lhsK), _f(lhsP), val)
+    (lib_mt() if mt else lib()).cfo_pack_val4(nshg, _f(colm, np.int32), _f(rowp, np.int32), _f(lhsK), _f(lhsP), val)
     return val
 
 
@@ -253,8 +312,8 @@ class MemLS:
     """memLS_LHS_CREATE / BC_CREATE / LS_CREATE / SOLVE on the CPU."""
 
     def __init__(self, colm, rowp, nfaces, ls_type=LS_TYPE_NS, relTol=None, absTol=None,
-                 maxItr=None, dimKry=None, relTolIn=None, absTolIn=None, maxItrIn=None):
-        L = lib()
+                 maxItr=None, dimKry=None, relTolIn=None, absTolIn=None, maxItrIn=None, mt=False):
+        self.L = L = lib_mt() if mt else lib()
         self.nNo = colm.shape[0] - 1
         self.nnz = rowp.shape[0]
         self._keep = []
@@ -288,7 +347,7 @@ class MemLS:
         g = _f(gNodes, np.int32)
         v = None if val is None else _f(val)
         self._keep.append(v)
-        _check(lib().cfo_bc_create(self.lhs, faIn, g.shape[0], dof, bc_type, g, _ptr(v)), "bc_create")
+        _check(self.L.cfo_bc_create(self.lhs, faIn, g.shape[0], dof, bc_type, g, _ptr(v)), "bc_create")
 
     def solve(self, Ri, Val, incL=None, res=None):
         """Returns the solution (dof,nNo); Ri and Val are copied first."""
@@ -296,19 +355,19 @@ class MemLS:
         Val = _f(Val).copy(order="F")
         il = None if incL is None else _f(incL, np.int32)
         rs = None if res is None else _f(res)
-        _check(lib().cfo_ls_solve(self.lhs, self.ls, Ri.shape[0], Ri, Val, _ptr(il), _ptr(rs)), "ls_solve")
+        _check(self.L.cfo_ls_solve(self.lhs, self.ls, Ri.shape[0], Ri, Val, _ptr(il), _ptr(rs)), "ls_solve")
         return Ri
 
     def stats(self, sub=0):
         out = (C.c_double * 8)()
-        lib().cfo_ls_stats(self.ls, sub, out)
+        self.L.cfo_ls_stats(self.ls, sub, out)
         keys = ("iNorm", "fNorm", "itr", "suc", "dB", "callD", "Resm", "Resc")
         return dict(zip(keys, list(out)))
 
     def __del__(self):
         try:
-            lib().cfo_lhs_free(self.lhs)
-            lib().cfo_ls_free(self.ls)
+            self.L.cfo_lhs_free(self.lhs)
+            self.L.cfo_ls_free(self.ls)
         except Exception:
             pass
 

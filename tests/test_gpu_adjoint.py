@@ -5,7 +5,7 @@ Checks:
   * central differences of the oracle's residual (the CPU restatement) in a
     random direction of y and in mu, h = 1e-6 relative: the VJP matches to
     1e-6 relative (the FD truncation error at this h is ~1e-10);
-  * the assembled tangent transposed where the lazy tangent is the exact
+  * the assembled tangent transposed where the tangent is the exact
     Jacobian: the momentum rows' velocity block at u = 0 and the continuity
     rows' pressure block (the PSPG matrix C) at a generic state, to 1e-10
     relative;
@@ -80,7 +80,7 @@ def test_vjp_against_tangent_transpose(env, oracle_lib):
     p = oracle_lib.make_params()
     lf = p.alfi * p.gami * p.Delt
     # momentum rows, velocity columns at u = 0, constant p, a = 0: L = 0 and
-    # every path through q or tau is multiplied by u or L, so the lazy
+    # every path through q or tau is multiplied by u or L, so the
     # tangent is the exact Jacobian there
     m, y, ac, rng = _state((4, 4, 5), 42, generic=False)
     y[:, :3] = 0.0

@@ -71,6 +71,45 @@ def test_union_formation_equals_instances(env, precision, boundary):
     assert np.array_equal(ens.lhs.get_val(4), val)
 
 
+def test_graph_refuses_replay_after_its_buffers_change(env):
+    """New viscosities with the same instance count keep the captured graph
+    valid (a replay equals the direct formation with them); changing the
+    instance count, or dropping the instances, makes crimson_graph_launch
+    return CRIMSON_ESTATE on the host without replaying; a new capture
+    works again."""
+    cfa, torch = env
+    from crimsonflowsolver_amd.ensemble import Ensemble
+    meshes, mu, R, states = _instances()
+    ens = Ensemble(meshes, mu, R)
+    y = np.concatenate([s[0] for s in states])
+    ac = np.concatenate([s[1] for s in states])
+    ty, tac, tx = _T(torch, y), _T(torch, ac), _T(torch, ens.x)
+    r4 = torch.zeros((ens.N, 4), dtype=torch.float64, device="cuda")
+    g = ens.capture(ty, tac, tx, r4, autotune=False)
+    ens.set_parameters(mu=mu * 1.5)
+    g.launch()
+    got = r4.cpu().numpy().copy()
+    ens.form.to_lhs(ty, tac, tx, r4, ens.lhs)
+    assert np.array_equal(got, r4.cpu().numpy())
+    # another instance count: the per-instance viscosity buffer is replaced
+    ens.form.set_instances(np.zeros(ens.inst.size, np.uint16), np.array([0.05]))
+    r4.fill_(7.0)
+    with pytest.raises(cfa.CrimsonError) as ei:
+        g.launch()
+    assert ei.value.code == 8  # CRIMSON_ESTATE
+    assert bool((r4 == 7.0).all())  # nothing was replayed
+    ens.form.set_instances(None, np.zeros(0))
+    with pytest.raises(cfa.CrimsonError):
+        g.launch()
+    g2 = ens.form.graph_to_lhs(ty, tac, tx, r4, ens.lhs)
+    g2.launch()
+    ens.form.to_lhs(ty, tac, tx, r4, ens.lhs)
+    want = r4.cpu().numpy().copy()
+    r4.zero_()
+    g2.launch()
+    assert np.array_equal(r4.cpu().numpy(), want)
+
+
 def test_union_solve_matches_instances(env):
     cfa, torch = env
     from crimsonflowsolver_amd.ensemble import Ensemble

@@ -1,5 +1,6 @@
 """Mixed-precision form on the GPU, through the C ABI: the FP64 residual with
-the lazy tangent formed and assembled in FP32 (precision = 32), and memLS
+the tangent (iconvflow = 2: tauBar and u-bar terms, so the strong residual
+is formed too) formed and assembled in FP32 (precision = 32), and memLS
 solves on an FP32 matrix with FP32 Krylov arithmetic.
 
 Tolerances, normwise relative (max|a - b| / max|b|):

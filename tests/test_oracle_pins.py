@@ -9,11 +9,13 @@ the checks that pin each oracle function; each is chosen so that a plausible
 mistake (a dropped term, a wrong sign or index, a transposed operand) fails.
 """
 import itertools
+import os
 
 import numpy as np
 import pytest
 
 RHO, MU, DT = 1.06, 0.04, 1e-3
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def tet_block(x4, y=None, ac=None, q=None):
@@ -183,6 +185,97 @@ def test_strong_residual_terms_on_a_shear_flow(O):
             assert np.abs(even - want_even).max() <= 1e-11 * scale, (trial, c)
 
 
+def _shear_tangent_mismatch(e3_block, make_params):
+    """Largest relative miss of the L-dependent tangent terms on the shear
+    flow of test_strong_residual_terms_on_a_shear_flow (L = rho c at every
+    quadrature point, tau_M known in closed form).  With iconvflow = 2
+    (e3lhs.f90:124-129) the tangent's c-dependence is, for every velocity
+    component pair (i, j), delta_ij times
+      odd part:  -tlW rho sum_q N_a(q) tau_M,q (c.g_b)      (convection by
+                 u-bar = u - tau_M L / rho, e3lhs.f90:188-190)
+      even part:  tlW rho sum_q tau_M,q (c.g_a)(c.g_b) / sqrt(c.G c)
+                 (tauBar (L.g_a)(L.g_b), tauBar = tau_M / sqrt(L.G L),
+                 e3lhs.f90:237-249)
+    with tlW = alpha_f gamma Delt V/4, and nothing in the cross components.
+    The lazy branch (tauBar = 0, convection by u) has neither part.  Both
+    parts exceed 1e-3 of the tangent's largest entry on these states."""
+    rng = np.random.default_rng(23)
+    p = make_params()
+    lf = p.alfi * p.gami * p.Delt
+    nu = MU / RHO
+    kA, kB = (5 + 3 * np.sqrt(5)) / 20, (5 - np.sqrt(5)) / 20
+    Nq = np.full((4, 4), kB) + (kA - kB) * np.eye(4)
+    worst = 0.0
+    for trial in range(3):
+        x = random_tet(rng, stretch=1.0 + trial)
+        g, V = p1_grads(x)
+        G = 2.0 ** (4.0 / 3.0) * g.T @ g
+        a, b = 150.0 * (trial + 1), rng.normal(size=3) * 2.0
+        A = np.zeros((3, 3))
+        A[0, 1] = a
+        y = np.zeros((4, 4))
+        y[:, 1:] = x @ A.T + b
+        y[:, 0] = 7.0 - RHO * a * b[1] * x[:, 0]
+        uq = (Nq @ x) @ A.T + b
+        tauM = ((2.0 / DT) ** 2 + np.einsum("qi,ij,qj->q", uq, G, uq)
+                + 36.0 * nu ** 2 * np.sum(G * G)) ** -0.5
+        tlW = lf * V / 4.0
+
+        def K(c):
+            ac = np.zeros((4, 4))
+            ac[:, 1:] = c
+            return e3_block(*tet_block(x, y=y, ac=ac), p)[1][0]
+
+        K0 = K(np.zeros(3))
+        # accelerations large enough that the L terms are >= 1e-3 of the
+        # tangent (they scale with |L| against the mass and viscous parts)
+        for c in (rng.normal(size=3) * 3e4, np.array([0.0, 4e4, 0.0]), np.array([5e3, -2e4, 1.2e4])):
+            Kp, Km = K(c), K(-c)
+            odd, even = (Kp - Km) / 2.0, (Kp + Km) / 2.0 - K0
+            want_odd = -tlW * RHO * np.outer(Nq @ tauM, g @ c)
+            want_even = tlW * RHO * tauM.sum() * np.outer(g @ c, g @ c) / np.sqrt(c @ G @ c)
+            scale = max(np.abs(want_odd).max(), np.abs(want_even).max())
+            assert min(np.abs(want_odd).max(), np.abs(want_even).max()) > 1e-3 * np.abs(Kp).max()
+            for m in range(9):
+                wo = want_odd if m in (0, 4, 8) else 0.0
+                we = want_even if m in (0, 4, 8) else 0.0
+                worst = max(worst, np.abs(odd[m] - wo).max() / scale, np.abs(even[m] - we).max() / scale)
+    return worst
+
+
+def test_tangent_iconvflow2_terms_on_a_shear_flow(O):
+    """The reference cases run iconvflow = 2, whose tangent keeps tauBar and
+    convects with u-bar; both vanish where L = 0, so they are pinned here at
+    L = rho c != 0 against closed forms."""
+    assert _shear_tangent_mismatch(O.e3_block, O.make_params) <= 1e-11
+
+
+def test_tangent_pin_rejects_the_lazy_branch(O, tmp_path):
+    """Plant the other e3lhs branch (tauBar = 0, convection by u;
+    e3lhs.f90:130-134) in a private build of the restatement: the shear-flow
+    tangent pin must fail on it."""
+    import ctypes as C
+    import subprocess
+    src = os.path.join(ROOT, "oracle", "crimson_oracle.c")
+    so = str(tmp_path / "liboracle_planted.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-std=c99", "-ffp-contract=off", "-DCFO_PLANT_LAZY_TANGENT",
+                    "-shared", "-o", so, src, "-lm"], check=True)
+    L = C.CDLL(so)
+    dp = np.ctypeslib.ndpointer(np.float64, flags="F_CONTIGUOUS")
+    L.cfo_e3_block.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp, C.POINTER(O.Params)]
+    L.cfo_e3_block.restype = None
+
+    def planted(yl, acl, xl, ql, params):
+        n = yl.shape[0]
+        rl = np.zeros((n, 4, 4), order="F")
+        K = np.zeros((n, 9, 4, 4), order="F")
+        G = np.zeros((n, 4, 4, 4), order="F")
+        L.cfo_e3_block(n, yl, acl, xl, ql, rl, K, G, C.byref(params))
+        return rl, K, G
+
+    assert _shear_tangent_mismatch(planted, O.make_params) > 1e-3
+
+
 def _perm_case(rng):
     x = random_tet(rng)
     y = rng.normal(size=(4, 4)) * np.array([100, 10, 10, 10])
@@ -291,14 +384,15 @@ def _p1_mass(m):
 
 @pytest.mark.parametrize("state", ["zero", "constu"])
 def test_tangent_momentum_block_is_exact_jacobian(O, state):
-    """At u = 0 or at constant u, p with a = 0 (so L = 0), the lazy tangent's
+    """At u = 0 or at constant u, p with a = 0 (so L = 0, u-bar = u and
+    tauBar = 0), the tangent's
     momentum-velocity block is exact: K = alpha_m rho M - d(res)/d(y_alpha)
     * alpha_f gamma Delt (mass, convection, viscous, grad-div and SUPG terms),
     with K^{ij} at lhsK(i + 3(j-1)) for equation i, variable j.  Central
     differences of the assembled residual with q frozen pin index placement,
     signs and factors; the mass is the closed-form P1 mass.  (Perturbing the
     acceleration as well would add the SUPG acceleration term tauM rho N_b
-    (u.grad N_a), which the lazy tangent omits; at u = 0 it vanishes and the
+    (u.grad N_a), which the tangent omits; at u = 0 it vanishes and the
     zero-state case also checks d(res)/d(ac) = -mass.)"""
     m, colm, rowp, y, ac, q = _fd_setup(O, state)
     p = O.make_params()
