@@ -28,6 +28,7 @@ LIB_PATH = os.path.join(_HERE, "libcrimson_amd.so")
 
 LS_TYPE_CG, LS_TYPE_GMRES, LS_TYPE_NS = 1, 2, 3
 BC_TYPE_Dir, BC_TYPE_Neu = 0, 1
+FLUX_NOALLOC_MAX_SURFACES = 4096  # CRIMSON_FLUX_NOALLOC_MAX_SURFACES
 
 ERRORS = {1: "EARG", 2: "EUNSUPPORTED", 3: "EGRAPH", 4: "ENODIAG", 5: "EHIP", 6: "ENOMEM",
           7: "ESINGULAR", 8: "ESTATE"}
@@ -42,7 +43,7 @@ EXPORTS = (
     "crimson_itr_bc_dev", "crimson_elmpvsq_dev", "crimson_form_sync", "crimson_lhs_set_precision",
     "crimson_form_vjp_dev", "crimson_form_set_instances", "crimson_form_graph_create", "crimson_graph_launch",
     "crimson_graph_free", "crimson_form_set_boundary", "crimson_form_autotune", "crimson_form_set_bc",
-    "crimson_elmpvsq_coef_dev", "crimson_form_boundary_flux",
+    "crimson_elmpvsq_coef_dev", "crimson_form_boundary_flux", "crimson_form_boundary_flux_noalloc",
     "crimson_measure_peaks",
 )
 
@@ -116,6 +117,7 @@ def lib():
         L.crimson_form_set_boundary.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_int]
         L.crimson_form_set_bc.argtypes = [vp, vp, C.c_int]
         L.crimson_form_boundary_flux.argtypes = [vp, C.c_int, vp]
+        L.crimson_form_boundary_flux_noalloc.argtypes = [vp, C.c_int, vp]
         L.crimson_measure_peaks.argtypes = [C.c_size_t, dp, dp, dp]
         L.crimson_elmpvsq_coef_dev.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp, vp]
         L.crimson_form_autotune.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
@@ -296,6 +298,14 @@ class ElmGMRInterior:
         on the fluid by srfID (e3b.f90:216-247)."""
         out = np.zeros((5, nflx + 1), order="F")
         check(lib().crimson_form_boundary_flux(self.h, int(nflx), _p(out)), "crimson_form_boundary_flux")
+        return out
+
+    def boundary_flux_noalloc(self, nflx):
+        """boundary_flux through the entry point that allocates no device
+        memory (|srfID| below FLUX_NOALLOC_MAX_SURFACES)."""
+        out = np.zeros((5, nflx + 1), order="F")
+        check(lib().crimson_form_boundary_flux_noalloc(self.h, int(nflx), _p(out)),
+              "crimson_form_boundary_flux_noalloc")
         return out
 
     def set_bc(self, BC):

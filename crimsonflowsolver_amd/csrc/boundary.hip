@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -626,5 +627,62 @@ extern "C" int crimson_form_boundary_flux(crimson_form *f, int nflx, double *flx
   CR_HIP(hipStreamSynchronize(f->stream));
   for (int sid = 1; sid <= std::min(nflx, f->maxsrf); ++sid)
     for (int k = 0; k < 5; ++k) flxid[5 * (size_t)sid + k] = h[5 * (size_t)sid + k];
+  return CRIMSON_OK;
+}
+
+// Staging of crimson_form_boundary_flux_noalloc: bnd_flux_sum's per-surface
+// sums, 5 doubles for each |srfID| 0..CRIMSON_FLUX_NOALLOC_MAX_SURFACES-1
+// (160 KiB).  Each device holds its own copy for the life of the process.
+// bnd_flux_sum writes every column it covers in full, so nothing in it has
+// to start at zero; every call on a device shares that device's copy.
+static __device__ __align__(256) double scratch[5 * CRIMSON_FLUX_NOALLOC_MAX_SURFACES];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kScratchDevices] = {};
+
+// The device address of `scratch` on `device`, looked up once per device
+// with that device current (the host-side name is not a device address).
+int flux_scratch(int device, double **out) {
+  if (device < 0 || device >= kScratchDevices)
+    return crimson::set_error("crimson_form_boundary_flux_noalloc: device %d out of range", device), CRIMSON_EARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    CR_HIP(hipGetDevice(&cur));
+    if (cur != device) CR_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) CR_HIP(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return crimson::set_error("hipGetSymbolAddress(scratch): %s", hipGetErrorString(e)), CRIMSON_EHIP;
+    g_scratch_ptr[device] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return CRIMSON_OK;
+}
+}  // namespace
+
+extern "C" int crimson_form_boundary_flux_noalloc(crimson_form *f, int nflx, double *flxid) {
+  if (!f || nflx < 0 || !flxid)
+    return crimson::set_error("crimson_form_boundary_flux_noalloc: bad arguments"), CRIMSON_EARG;
+  const int ns = f->maxsrf + 1;
+  if (f->nelb > 0 && ns > CRIMSON_FLUX_NOALLOC_MAX_SURFACES)
+    return crimson::set_error("crimson_form_boundary_flux_noalloc: srfID up to %d; this entry point holds "
+                              "|srfID| < %d (use crimson_form_boundary_flux)",
+                              f->maxsrf, CRIMSON_FLUX_NOALLOC_MAX_SURFACES),
+           CRIMSON_EARG;
+  for (size_t k = 0; k < 5 * ((size_t)nflx + 1); ++k) flxid[k] = 0.0;
+  if (f->nelb == 0) return CRIMSON_OK;
+  double *d_scratch = nullptr;
+  CR_TRY(flux_scratch(f->device, &d_scratch));
+  bnd_flux_sum<<<ns, 256, 0, f->stream>>>(ns, f->d_sptr.p, f->d_selem.p, f->d_bflx.p, d_scratch);
+  CR_HIP(hipGetLastError());
+  const int ncol = std::min(nflx, f->maxsrf);  // columns 1..ncol; column 0 stays zero
+  if (ncol > 0)
+    CR_HIP(hipMemcpyAsync(flxid + 5, d_scratch + 5, (size_t)5 * ncol * sizeof(double), hipMemcpyDeviceToHost,
+                          f->stream));
+  CR_HIP(hipStreamSynchronize(f->stream));
   return CRIMSON_OK;
 }

@@ -177,6 +177,20 @@ int crimson_form_autotune(crimson_form *f, const double *d_y, const double *d_ac
  * Column 0 and surfaces without elements are zero. */
 int crimson_form_boundary_flux(crimson_form *f, int nflx, double *flxid);
 
+/* crimson_form_boundary_flux without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code near HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-surface sums are staged in a fixed device array of the library
+ * (5 doubles per |srfID| below CRIMSON_FLUX_NOALLOC_MAX_SURFACES, 160 KiB,
+ * held by each device the library runs on until the process ends) instead
+ * of a buffer of the handle.  Boundary elements with |srfID| >=
+ * CRIMSON_FLUX_NOALLOC_MAX_SURFACES give CRIMSON_EARG.  Results equal
+ * crimson_form_boundary_flux's bit for bit.  Every call on one device shares
+ * that array: calls that overlap on one device (other threads, other
+ * handles' streams) race and must be serialised by the caller. */
+#define CRIMSON_FLUX_NOALLOC_MAX_SURFACES 4096
+int crimson_form_boundary_flux_noalloc(crimson_form *f, int nflx, double *flxid);
+
 /* Measured ceilings (no counterpart in the reference; measurement only): a
  * plain device copy of copy_bytes (GB/s of read + write) and FP64 / FP32
  * fused multiply-add throughput (TFLOP/s), timed with HIP events. */
