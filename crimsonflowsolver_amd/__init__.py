@@ -44,7 +44,8 @@ EXPORTS = (
     "crimson_form_vjp_dev", "crimson_form_set_instances", "crimson_form_graph_create", "crimson_graph_launch",
     "crimson_graph_free", "crimson_form_set_boundary", "crimson_form_autotune", "crimson_form_set_bc",
     "crimson_elmpvsq_coef_dev", "crimson_form_boundary_flux", "crimson_form_boundary_flux_noalloc",
-    "crimson_measure_peaks",
+    "crimson_measure_peaks", "crimson_form_vjp_ac_dev", "crimson_elmpvsq_vjp_dev", "crimson_lhs_transpose_dev",
+    "crimson_ls_graph_create", "crimson_step_graph_create",
 )
 
 
@@ -60,7 +61,7 @@ class FormParams(C.Structure):
                 ("dtsfct", C.c_double), ("taucfct", C.c_double), ("flmpl", C.c_double),
                 ("flmpr", C.c_double), ("itau", C.c_int), ("iconvflow", C.c_int),
                 ("idiff", C.c_int), ("ipord", C.c_int), ("precision", C.c_int),
-                ("aspect_fp64", C.c_double)]
+                ("aspect_fp64", C.c_double), ("ideformwall", C.c_int)]
 
 
 _lib = None
@@ -120,6 +121,14 @@ def lib():
         L.crimson_form_boundary_flux_noalloc.argtypes = [vp, C.c_int, vp]
         L.crimson_measure_peaks.argtypes = [C.c_size_t, dp, dp, dp]
         L.crimson_elmpvsq_coef_dev.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp, vp]
+        L.crimson_form_vjp_ac_dev.argtypes = [vp, vp, vp, vp, vp, vp, vp, dp]
+        L.crimson_elmpvsq_vjp_dev.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp, vp, vp, vp]
+        L.crimson_lhs_transpose_dev.argtypes = [vp, vp]
+        L.crimson_ls_graph_create.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.POINTER(C.c_void_p)]
+        L.crimson_step_graph_create.argtypes = ([vp] * 13 + [C.c_int, C.c_int] + [vp] * 7 +
+                                                [C.c_int, C.POINTER(C.c_void_p)])
+        L.crimson_graph_launch.argtypes = [vp]
+        L.crimson_graph_free.argtypes = [vp]
         L.crimson_form_autotune.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         _lib = L
     return _lib
@@ -159,8 +168,9 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def form_params(rho=1.06, mu=0.04, delt=1e-3, rhoinf=0.5, precision=64, aspect_fp64=20.0):
-    """Material and generalized-alpha parameters (itrPC.f90:16-37)."""
+def form_params(rho=1.06, mu=0.04, delt=1e-3, rhoinf=0.5, precision=64, aspect_fp64=20.0, ideformwall=0):
+    """Material and generalized-alpha parameters (itrPC.f90:16-37);
+    ideformwall 1 (deformable walls) is refused by the formation setup."""
     if rhoinf < 0 or rhoinf > 1:
         almi = alfi = gami = 1.0
     else:
@@ -168,7 +178,7 @@ def form_params(rho=1.06, mu=0.04, delt=1e-3, rhoinf=0.5, precision=64, aspect_f
         alfi = 1.0 / (1.0 + rhoinf)
         gami = 0.5 + almi - alfi
     return FormParams(rho, mu, delt, alfi, almi, gami, 1.0, 1.0, 0.0, 0.0, 0, 2, 1, 1,
-                      precision, aspect_fp64)
+                      precision, aspect_fp64, ideformwall)
 
 
 def genadj(ien, nshg):
@@ -184,7 +194,8 @@ def genadj(ien, nshg):
 
 
 class FormGraph:
-    """A captured formation (crimson_form_graph_create); keeps its arrays alive."""
+    """A captured HIP graph (crimson_form_graph_create, crimson_ls_graph_create,
+    crimson_step_graph_create); keeps its arrays alive."""
 
     def __init__(self, h, keep):
         self.h = h
@@ -250,6 +261,21 @@ class ElmGMRInterior:
         check(lib().crimson_form_vjp_dev(self.h, _p(y), _p(ac), _p(x), _p(w), _p(ybar), C.byref(mb)),
               "crimson_form_vjp_dev")
         return mb.value
+
+    def vjp_ac(self, y, ac, x, w, ybar, acbar):
+        """vjp with the acceleration's cotangent: ybar <- w^T d res/d y,
+        acbar <- w^T d res/d ac (fourth column zero); returns w^T d res/d mu."""
+        mb = C.c_double()
+        check(lib().crimson_form_vjp_ac_dev(self.h, _p(y), _p(ac), _p(x), _p(w), _p(ybar), _p(acbar),
+                                            C.byref(mb)), "crimson_form_vjp_ac_dev")
+        return mb.value
+
+    def elmpvsq_vjp(self, glob_dev, nabi_dev, c1, y, w4, ybar, cbar):
+        """Adjoint of elmpvsq_coef's term for w4 laid out like Res4: ybar
+        (like y) gets the face term added; cbar (2 doubles, device) <- the
+        cotangents of c1 and c2."""
+        check(lib().crimson_elmpvsq_vjp_dev(self.h, int(glob_dev.numel()), _p(glob_dev), _p(nabi_dev), float(c1),
+                                            _p(y), _p(w4), _p(ybar), _p(cbar)), "crimson_elmpvsq_vjp_dev")
 
     def set_instances(self, inst, mu):
         """Ensemble: inst(nel) 0-based instance of each element, mu(ninst);
@@ -327,6 +353,39 @@ class ElmGMRInterior:
         check(lib().crimson_form_graph_create(self.h, _p(y), _p(ac), _p(x), _p(res4), lhs_handle.h, C.byref(g)),
               "crimson_form_graph_create")
         return FormGraph(g, keep=(y, ac, x, res4, lhs_handle))
+
+    def step_graph(self, lhs_handle, ls, yold, acold, y, ac, ya, aca, x, res4, ibc_dev, bc_dev, surfaces=(),
+                   faceres=None, nonlinear=3):
+        """Capture one generalized-alpha timestep (crimson_step_graph_create):
+        itrPredict, itrBC, then `nonlinear` times itrYAlpha, the formation
+        into lhs, bc3Res, ElmpvsQ of each coupled surface, the device-driven
+        NS solve, itrCorrect and itrBC, and yold, acold <- y, ac, all on
+        these device arrays.  surfaces: [(glob_dev int32 0-based, nabi_dev
+        (nf,3), c1, c2)] of the coupled surfaces; faceres: memLS's face
+        coefficients (res of memLS_SOLVE).  Each launch() advances one
+        timestep; the BC array may be rewritten between launches."""
+        import torch
+        dev = y.device
+        sizes = [int(s[0].numel()) for s in surfaces]
+        surf_ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+        if surfaces:
+            glob = torch.cat([s[0] for s in surfaces]).to(torch.int32).contiguous()
+            nabi = torch.cat([s[1] for s in surfaces]).to(torch.float64).contiguous()
+        else:
+            glob = torch.zeros(1, dtype=torch.int32, device=dev)
+            nabi = torch.zeros((1, 3), dtype=torch.float64, device=dev)
+        c1 = np.array([float(s[2]) for s in surfaces] or [0.0])
+        c2 = np.array([float(s[3]) for s in surfaces] or [0.0])
+        q = torch.zeros(max(1, len(surfaces)), dtype=torch.float64, device=dev)
+        fr = None if faceres is None else _f(faceres)
+        g = C.c_void_p()
+        check(lib().crimson_step_graph_create(self.h, lhs_handle.h, ls.h, _p(yold), _p(acold), _p(y), _p(ac), _p(ya),
+                                              _p(aca), _p(x), _p(res4), _p(ibc_dev), _p(bc_dev),
+                                              int(bc_dev.shape[0]), len(surfaces), _p(surf_ptr), _p(glob),
+                                              _p(nabi), _p(c1), _p(c2), _p(q), _p(fr), int(nonlinear), C.byref(g)),
+              "crimson_step_graph_create")
+        return FormGraph(g, keep=(yold, acold, y, ac, ya, aca, x, res4, ibc_dev, bc_dev, glob, nabi, c1, c2, q, fr,
+                                  lhs_handle, ls, surf_ptr))
 
     def bc3res(self, res, res4_layout=False):
         check(lib().crimson_form_bc3res_dev(self.h, _p(res), 1 if res4_layout else 0),
@@ -432,6 +491,10 @@ class memLS_lhsType:
         """32: hold the matrix in FP32 and solve in FP32 (mixed form); 64: back to FP64."""
         check(lib().crimson_lhs_set_precision(self.h, int(bits)), "crimson_lhs_set_precision")
 
+    def transpose_from(self, src):
+        """Hold the block transpose of src's matrix (same CSR graph)."""
+        check(lib().crimson_lhs_transpose_dev(self.h, src.h), "crimson_lhs_transpose_dev")
+
     def get_val(self, dof=4):
         out = np.zeros((dof * dof, self.nnz), order="F")
         check(lib().crimson_lhs_get_val(self.h, dof, _p(out)), "crimson_lhs_get_val")
@@ -483,6 +546,18 @@ def memLS_SOLVE(lhs, ls, dof, Ri, Val, incL=None, res=None):
     il = None if incL is None else _f(incL, np.int32)
     rs = None if res is None else _f(res)
     check(lib().crimson_ls_solve(lhs.h, ls.h, dof, _p(Ri), _p(Val), _p(il), _p(rs)), "memLS_SOLVE")
+
+
+def memLS_SOLVE_graph(lhs, ls, dof, Ri_dev, incL=None, res=None):
+    """The NS solve of ``lhs``'s matrix on the device array Ri_dev captured
+    device-driven into a HIP graph (crimson_ls_graph_create); launch() solves
+    in place and leaves the statistics in ``ls``."""
+    il = None if incL is None else _f(incL, np.int32)
+    rs = None if res is None else _f(res)
+    g = C.c_void_p()
+    check(lib().crimson_ls_graph_create(lhs.h, ls.h, dof, _p(Ri_dev), _p(il), _p(rs), C.byref(g)),
+          "crimson_ls_graph_create")
+    return FormGraph(g, keep=(lhs, ls, Ri_dev, il, rs))
 
 
 def memLS_SOLVE_dev(lhs, ls, dof, Ri_dev, incL=None, res=None):

@@ -2,6 +2,7 @@
 //
 // Given a cotangent w(nshg,4) (same layout as res), computes
 //   ybar  = w^T d res / d y      (nshg,4), y = (u1,u2,u3,p)
+//   acbar = w^T d res / d ac     (nshg,4), ac = (a1,a2,a3, unused), optional
 //   mubar = w^T d res / d mu     (scalar)
 // for res the interior ElmGMR residual that crimson_form_interior returns
 // (flux pass, e3 with itau = 0 and iconvflow = 2, scatter; before bc3Res).
@@ -9,7 +10,9 @@
 // tangent): tauM and tauC through u (e3stab.f90:108-133), tauBar and ubar
 // through the strong residual L (e3stab.f90:356-374), and the reconstructed
 // flux q through qpbc's lumped projection of mu (grad u + grad u^T)
-// (e3q.f90:100-145, qpbc.f90:67-71).  The acceleration is held fixed.
+// (e3q.f90:100-145, qpbc.f90:67-71).  The acceleration enters the momentum
+// residual's rNa and the strong residual L (e3res.f90:775-789), both with
+// factor rho.
 //
 // Reverse mode written by hand and deterministic (no atomics):
 //   vjp_e3_elem    element: recompute each quadrature point forward, then
@@ -32,8 +35,8 @@
 
 namespace {
 
-constexpr int kVRec = 48;  // ybar (4 nodes x 4) | qbar (4 nodes x 6) | mubar
-constexpr int V_Y = 0, V_Q = 16, V_MU = 40;
+constexpr int kVRec = 56;  // ybar (4 nodes x 4) | qbar (4 nodes x 6) | mubar | acbar (4 nodes x 3)
+constexpr int V_Y = 0, V_Q = 16, V_MU = 40, V_A = 41;
 constexpr int kVRec2 = 16;  // ubar (4 nodes x 3) | mubar
 constexpr int V2_MU = 12;
 
@@ -112,7 +115,7 @@ __global__ void __launch_bounds__(256) vjp_e3_elem(int nel, const int *__restric
   }
 
   double gub[3][3], gpb[3] = {0, 0, 0}, divqb[3] = {0, 0, 0}, divub = 0.0, mub = 0.0;
-  double ulb[4][3], plb[4] = {0, 0, 0, 0};
+  double ulb[4][3], alb[4][3], plb[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -120,7 +123,7 @@ __global__ void __launch_bounds__(256) vjp_e3_elem(int nel, const int *__restric
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) ulb[a][i] = 0.0;
+    for (int i = 0; i < 3; ++i) ulb[a][i] = alb[a][i] = 0.0;
 
 #pragma unroll
   for (int qp = 0; qp < 4; ++qp) {
@@ -234,11 +237,18 @@ __global__ void __launch_bounds__(256) vjp_e3_elem(int nel, const int *__restric
       gpb[i] += rLb[i];
       divqb[i] -= rLb[i];
     }
-    // u, p at the quadrature point
+    // the acceleration enters rna_i and L_i, each times rho
+    double aqb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) aqb[i] = rho * (rnab[i] + rLb[i]);
+    // u, p and a at the quadrature point
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
 #pragma unroll
-      for (int i = 0; i < 3; ++i) ulb[a][i] += Nq[a] * uqb[i];
+      for (int i = 0; i < 3; ++i) {
+        ulb[a][i] += Nq[a] * uqb[i];
+        alb[a][i] += Nq[a] * aqb[i];
+      }
       plb[a] += Nq[a] * pqb;
     }
   }
@@ -258,6 +268,8 @@ __global__ void __launch_bounds__(256) vjp_e3_elem(int nel, const int *__restric
       for (int j = 0; j < 3; ++j) qb[qsym(i + 3 * j)] += g[a][j] * divqb[i];
 #pragma unroll
     for (int s = 0; s < 6; ++s) o[V_Q + 6 * a + s] = qb[s];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o[V_A + 3 * a + i] = alb[a][i];
   }
   o[V_MU] = mub;
 }
@@ -346,13 +358,15 @@ __global__ void __launch_bounds__(256) vjp_flux_elem(int nel, const int *__restr
   o[V2_MU] = mub;
 }
 
-// ybar at the nodes, (nshg,4) column-major like res.
+// ybar (and, when acbar is given, acbar) at the nodes, (nshg,4)
+// column-major like res; acbar's fourth column is zero.
 __global__ void __launch_bounds__(256) vjp_node(int nshg, const int *__restrict__ inc_ptr,
                                                 const int *__restrict__ inc, const double *__restrict__ vrec,
-                                                const double *__restrict__ vrec2, double *__restrict__ ybar) {
+                                                const double *__restrict__ vrec2, double *__restrict__ ybar,
+                                                double *__restrict__ acbar) {
   const int gid = xcd_block() * blockDim.x + threadIdx.x;
   const int i = gid / kNodeLanes, lane = gid % kNodeLanes;
-  double s[4] = {0, 0, 0, 0};
+  double s[7] = {0, 0, 0, 0, 0, 0, 0};
   if (i < nshg)
     for (int p = inc_ptr[i] + lane; p < inc_ptr[i + 1]; p += kNodeLanes) {
       const int ea = inc[p];
@@ -363,15 +377,26 @@ __global__ void __launch_bounds__(256) vjp_node(int nshg, const int *__restrict_
       s[1] += r[1] + r2[1];
       s[2] += r[2] + r2[2];
       s[3] += r[3];
+      if (acbar) {
+        const double *ra = vrec + (size_t)e * kVRec + V_A + 3 * a;
+        s[4] += ra[0];
+        s[5] += ra[1];
+        s[6] += ra[2];
+      }
     }
 #pragma unroll
   for (int off = 1; off < kNodeLanes; off <<= 1)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) s[k] += __shfl_xor(s[k], off, kNodeLanes);
+    for (int k = 0; k < 7; ++k) s[k] += __shfl_xor(s[k], off, kNodeLanes);
   if (i >= nshg || lane != 0) return;
   const size_t N = nshg;
 #pragma unroll
   for (int k = 0; k < 4; ++k) ybar[i + N * k] = s[k];
+  if (acbar) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acbar[i + N * k] = s[4 + k];
+    acbar[i + N * 3] = 0.0;
+  }
 }
 
 // mubar: per-workgroup sums over fixed element ranges, then one workgroup.
@@ -418,7 +443,7 @@ void vjp_work_sizes(const crimson_form *f, size_t n[6]) {
 }
 
 int vjp_run(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, const double *d_w,
-            double *d_ybar, double *mubar, const VjpWork &w) {
+            double *d_ybar, double *d_acbar, double *mubar, const VjpWork &w) {
   const size_t nn = (size_t)f->nshg;
   const int nmu = (int)(((size_t)std::max(f->nel, 1) + kMuChunk - 1) / kMuChunk);
   hipStream_t s = f->stream;
@@ -431,7 +456,7 @@ int vjp_run(crimson_form *f, const double *d_y, const double *d_ac, const double
   vjp_q_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, w.vrec, w.invm, w.qbar);
   if (f->nel > 0)
     vjp_flux_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, f->d_frec.p, w.qbar, c.mu, f->ens(), w.vrec2);
-  vjp_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, w.vrec, w.vrec2, d_ybar);
+  vjp_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, w.vrec, w.vrec2, d_ybar, d_acbar);
   double mb = 0.0;
   if (f->nel > 0) {
     mu_part<<<nmu, T, 0, s>>>(f->nel, w.vrec, w.vrec2, w.mupart);
@@ -449,8 +474,9 @@ int vjp_run(crimson_form *f, const double *d_y, const double *d_ac, const double
 
 }  // namespace crimson
 
-extern "C" int crimson_form_vjp_dev(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
-                                    const double *d_w, double *d_ybar, double *mubar) {
+namespace {
+int vjp_entry(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, const double *d_w,
+              double *d_ybar, double *d_acbar, double *mubar) {
   if (!f || !d_y || !d_ac || !d_x || !d_w || !d_ybar) {
     crimson::set_error("crimson_form_vjp_dev: null argument");
     return CRIMSON_EARG;
@@ -463,5 +489,20 @@ extern "C" int crimson_form_vjp_dev(crimson_form *f, const double *d_y, const do
   CR_TRY(f->d_invm.alloc(n[3]));
   CR_TRY(f->d_mupart.alloc(n[4]));
   const crimson::VjpWork w{f->d_vrec.p, f->d_vrec2.p, f->d_qbar.p, f->d_invm.p, f->d_mupart.p, nullptr};
-  return crimson::vjp_run(f, d_y, d_ac, d_x, d_w, d_ybar, mubar, w);
+  return crimson::vjp_run(f, d_y, d_ac, d_x, d_w, d_ybar, d_acbar, mubar, w);
+}
+}  // namespace
+
+extern "C" int crimson_form_vjp_dev(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
+                                    const double *d_w, double *d_ybar, double *mubar) {
+  return vjp_entry(f, d_y, d_ac, d_x, d_w, d_ybar, nullptr, mubar);
+}
+
+extern "C" int crimson_form_vjp_ac_dev(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
+                                       const double *d_w, double *d_ybar, double *d_acbar, double *mubar) {
+  if (!d_acbar) {
+    crimson::set_error("crimson_form_vjp_ac_dev: null acbar");
+    return CRIMSON_EARG;
+  }
+  return vjp_entry(f, d_y, d_ac, d_x, d_w, d_ybar, d_acbar, mubar);
 }

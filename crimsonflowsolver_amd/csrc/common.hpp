@@ -126,10 +126,50 @@ inline unsigned grid_for(size_t n, unsigned threads) {
 }  // namespace crimson
 
 struct crimson_form;
+struct crimson_lhs;
 namespace crimson {
 hipStream_t form_stream(const crimson_form *f);
 int form_nshg(const crimson_form *f);
+crimson_form_params form_params_of(const crimson_form *f);
+
+// What a graph of the solver side adds to a replay: the state it holds on
+// the device, read back after each launch, and the buffers and settings it
+// baked in (a replay is refused once they change).
+struct GraphExtra {
+  virtual ~GraphExtra() = default;
+  virtual int after_launch() { return CRIMSON_OK; }
+  virtual std::vector<unsigned long long> signature() const { return {}; }
+};
+
+// The formation's graph signature (form.hip): the allocation serial of every
+// buffer a captured formation reads or writes and the settings that chose
+// its kernels.
+std::vector<unsigned long long> graph_signature(const crimson_form *f, const crimson_lhs *lhs);
+// Pre-capture set-up of a formation into lhs's matrix (its allocation and
+// the assembly schedule); then form_enqueue_to_lhs enqueues that formation
+// on the handle's stream without any synchronisation (capturable), and
+// form_enqueue_bc3res bc3Res on Res4.
+int form_prepare_to_lhs(crimson_form *f, const double *d_x, crimson_lhs *lhs);
+int form_enqueue_to_lhs(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, double *d_Res4,
+                        crimson_lhs *lhs);
+int form_enqueue_bc3res(crimson_form *f, double *d_res, int res4_layout);
 }  // namespace crimson
+
+// A captured HIP graph (crimson_form_graph_create, crimson_ls_graph_create,
+// crimson_step_graph_create).
+struct crimson_graph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t s = nullptr;
+  const crimson_form *f = nullptr;  // with lhs: the formation's signature applies
+  const crimson_lhs *lhs = nullptr;
+  std::vector<unsigned long long> sig;  // the signature at capture
+  crimson::GraphExtra *extra = nullptr;  // owned
+  // a capture cut into segments: graphs launched back to back on s, with no
+  // host decision between them (g and exec unused then)
+  std::vector<hipGraph_t> seg;
+  std::vector<hipGraphExec_t> segx;
+};
 
 // memLS face (memLS_faceType, memLS.f90:83-100), single task.
 struct CrFace {

@@ -1232,6 +1232,12 @@ int validate_params(const crimson_form_params *p) {
         p->itau, p->iconvflow, p->idiff, p->ipord, p->flmpl, p->flmpr);
     return CRIMSON_EUNSUPPORTED;
   }
+  if (p->ideformwall != 0) {
+    crimson::set_error("ideformwall = %d: deformable walls (the membrane terms of e3b.f90:488-562 and "
+                       "e3bvar.f90:1010-1017) are not formed; only rigid walls (ideformwall = 0)",
+                       p->ideformwall);
+    return CRIMSON_EUNSUPPORTED;
+  }
   if (p->precision != 64 && p->precision != 32) {
     crimson::set_error("formation precision %d not available (64: FP64; 32: FP64 residual, FP32 tangent)",
                        p->precision);
@@ -1435,6 +1441,7 @@ int form_flux_pass(crimson_form *f, const double *d_y, const double *d_ac, const
   return CRIMSON_OK;
 }
 hipStream_t form_stream(const crimson_form *f) { return f->stream; }
+crimson_form_params form_params_of(const crimson_form *f) { return f->prm; }
 int form_nshg(const crimson_form *f) { return f->nshg; }
 }  // namespace crimson
 
@@ -1810,16 +1817,7 @@ extern "C" int crimson_form_set_instances(crimson_form *f, const unsigned short 
   return crimson::form_boundary_instances(f);
 }
 
-struct crimson_graph {
-  hipGraph_t g = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipStream_t s = nullptr;
-  const crimson_form *f = nullptr;
-  const crimson_lhs *lhs = nullptr;
-  std::vector<unsigned long long> sig;  // graph_signature at capture
-};
-
-namespace {
+namespace crimson {
 // What a captured formation graph baked in: the allocation serial of every
 // buffer its kernels read or write, and the scalars that chose its kernels
 // and launch shapes.  A replay is refused once any of them has changed.
@@ -1838,7 +1836,39 @@ std::vector<unsigned long long> graph_signature(const crimson_form *f, const cri
       (unsigned long long)lhs->prec, (unsigned long long)lhs->gt_is_d};
   return v;
 }
-}  // namespace
+
+int form_prepare_to_lhs(crimson_form *f, const double *d_x, crimson_lhs *lhs) {
+  if (lhs->nNo != f->nshg || lhs->nnz != f->nnz) {
+    crimson::set_error("lhs (nNo=%d nnz=%d) does not match the formation graph (nshg=%d nnz=%d)", lhs->nNo,
+                       lhs->nnz, f->nshg, f->nnz);
+    return CRIMSON_EARG;
+  }
+  if (f->npartial > 0 && !f->d_bc.p) {
+    crimson::set_error("iBC holds 1- or 2-component velocity codes: BC is needed first (crimson_form_set_bc)");
+    return CRIMSON_ESTATE;
+  }
+  lhs->vdof = 4;
+  lhs->gt_is_d = true;
+  if (f->prm.precision == 32) {
+    CR_TRY(lhs->d_V32.alloc((size_t)16 * f->nnz));
+    lhs->prec = 32;
+  } else {
+    CR_TRY(lhs->d_V.alloc((size_t)16 * f->nnz));
+    lhs->prec = 64;
+  }
+  return ensure_order(f, d_x);
+}
+
+int form_enqueue_to_lhs(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, double *d_Res4,
+                        crimson_lhs *lhs) {
+  AsmOut o{nullptr, nullptr, nullptr, nullptr, (size_t)f->nnz};
+  if (f->prm.precision == 32)
+    o.V32 = lhs->d_V32.p;
+  else
+    o.V = lhs->d_V.p;
+  return run_formation(f, d_y, d_ac, d_x, d_Res4, 1, 1, o, nullptr, false);
+}
+}  // namespace crimson
 
 extern "C" int crimson_form_graph_create(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x,
                                          double *d_res, crimson_lhs *lhs, crimson_graph **out) {
@@ -1882,7 +1912,7 @@ extern "C" int crimson_form_graph_create(crimson_form *f, const double *d_y, con
   }
   gr->f = f;
   gr->lhs = lhs;
-  gr->sig = graph_signature(f, lhs);
+  gr->sig = crimson::graph_signature(f, lhs);
   *out = gr;
   return CRIMSON_OK;
 }
@@ -1892,14 +1922,25 @@ extern "C" int crimson_graph_launch(crimson_graph *g) {
     crimson::set_error("crimson_graph_launch: null graph");
     return CRIMSON_EARG;
   }
-  if (graph_signature(g->f, g->lhs) != g->sig) {
+  std::vector<unsigned long long> sig;
+  if (g->f) sig = crimson::graph_signature(g->f, g->lhs);
+  if (g->extra) {
+    const std::vector<unsigned long long> e = g->extra->signature();
+    sig.insert(sig.end(), e.begin(), e.end());
+  }
+  if (sig != g->sig) {
     crimson::set_error("crimson_graph_launch: a buffer or setting the graph captured has changed since "
                        "crimson_form_graph_create (instances, boundary, BC, precision, schedule or the matrix); "
                        "free the graph and capture it again");
     return CRIMSON_ESTATE;
   }
-  CR_HIP(hipGraphLaunch(g->exec, g->s));
+  if (g->segx.empty()) {
+    CR_HIP(hipGraphLaunch(g->exec, g->s));
+  } else {
+    for (hipGraphExec_t x : g->segx) CR_HIP(hipGraphLaunch(x, g->s));
+  }
   CR_HIP(hipStreamSynchronize(g->s));
+  if (g->extra) return g->extra->after_launch();
   return CRIMSON_OK;
 }
 
@@ -1907,6 +1948,9 @@ extern "C" int crimson_graph_free(crimson_graph *g) {
   if (!g) return CRIMSON_OK;
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->g) (void)hipGraphDestroy(g->g);
+  for (hipGraphExec_t x : g->segx) (void)hipGraphExecDestroy(x);
+  for (hipGraph_t h : g->seg) (void)hipGraphDestroy(h);
+  delete g->extra;
   delete g;
   return CRIMSON_OK;
 }
@@ -1979,12 +2023,24 @@ extern "C" int crimson_form_bc3res_dev(crimson_form *f, double *d_res, int res4_
     crimson::set_error("iBC holds 1- or 2-component velocity codes: BC is needed first (crimson_form_set_bc)");
     return CRIMSON_ESTATE;
   }
-  bc3res_kernel<<<crimson::grid_for(f->nshg, 256), 256, 0, f->stream>>>(f->nshg, f->d_ibc.p, f->d_bc.p, d_res,
-                                                                         res4_layout ? 1 : 0);
-  CR_HIP(hipGetLastError());
+  CR_TRY(crimson::form_enqueue_bc3res(f, d_res, res4_layout));
   CR_HIP(hipStreamSynchronize(f->stream));
   return CRIMSON_OK;
 }
+
+namespace crimson {
+int form_enqueue_bc3res(crimson_form *f, double *d_res, int res4_layout) {
+  if (!f->has_ibc) return CRIMSON_OK;
+  if (f->npartial > 0 && !f->d_bc.p) {
+    crimson::set_error("iBC holds 1- or 2-component velocity codes: BC is needed first (crimson_form_set_bc)");
+    return CRIMSON_ESTATE;
+  }
+  bc3res_kernel<<<crimson::grid_for(f->nshg, 256), 256, 0, f->stream>>>(f->nshg, f->d_ibc.p, f->d_bc.p, d_res,
+                                                                         res4_layout ? 1 : 0);
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+}  // namespace crimson
 
 extern "C" int crimson_form_timing(const crimson_form *f, double *out4) {
   if (!f || !out4) {

@@ -232,5 +232,5 @@ struct VjpWork {
 };
 void vjp_work_sizes(const crimson_form *f, size_t n[6]);
 int vjp_run(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, const double *d_w,
-            double *d_ybar, double *mubar, const VjpWork &w);
+            double *d_ybar, double *d_acbar, double *mubar, const VjpWork &w);
 }  // namespace crimson

@@ -29,6 +29,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "common.hpp"
@@ -70,7 +71,9 @@ template <int NR, int NC, class T, int LANES>
 __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restrict__ rowptr,
                                                    const int *__restrict__ col, MatView<T> A,
                                                    const T *__restrict__ x, T *__restrict__ y, T ya,
-                                                   const T *__restrict__ z, T za) {
+                                                   const T *__restrict__ z, T za,
+                                                   const double *__restrict__ skip) {
+  if (skip && *skip != 0.0) return;
   const int gid = crimson::xcd_block(kRowXcdChunk) * blockDim.x + threadIdx.x;
   const int row = gid / LANES;
   const int lane = threadIdx.x % LANES;
@@ -123,6 +126,8 @@ struct FaceDev {
   const int *glob;
   const double *valM;
   double coef;
+  const double *coefp;  // the coefficient on the device (device-driven solves), or null: coef
+  double res;           // the face's coupling coefficient (ADDBCMUL without PRE)
   int nNo, dof;
   const int *sup;
   const double *q;
@@ -336,8 +341,9 @@ __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restri
 template <class T, class Acc>
 __global__ void __launch_bounds__(256) mdot_kernel(size_t L, const T *__restrict__ v, const T *__restrict__ U0,
                                                    size_t ustride, int m, int self, Acc *__restrict__ partial,
-                                                   int pstride, const int *__restrict__ gate) {
-  if (gate && *gate == 0) return;
+                                                   int pstride, const int *__restrict__ gate,
+                                                   const double *__restrict__ skip) {
+  if ((gate && *gate == 0) || (skip && *skip != 0.0)) return;
   __shared__ Acc sh[4];
   const size_t base = (size_t)blockIdx.x * kChunk;
   Acc vv[kChunk / 256];
@@ -371,9 +377,11 @@ __global__ void __launch_bounds__(256) mdot_kernel(size_t L, const T *__restrict
 // (summed in T, stored as double for the host-side FP64 scalars).
 template <class T>
 __global__ void __launch_bounds__(256) reduce_kernel(int nblk, const T *__restrict__ partial, int pstride,
-                                                     double *__restrict__ out, const int *__restrict__ gate) {
+                                                     double *__restrict__ out, const int *__restrict__ gate,
+                                                     const double *__restrict__ skip) {
   __shared__ T sh[4];
   const int j = blockIdx.x;
+  if (skip && *skip != 0.0) return;
   if (gate && *gate == 0) {
     if (threadIdx.x == 0) out[j] = 0.0;
     return;
@@ -390,8 +398,9 @@ template <class T>
 __global__ void __launch_bounds__(256) msub_norm_kernel(size_t L, T *__restrict__ y, const T *__restrict__ U0,
                                                         size_t ustride, int m, const double *__restrict__ coef,
                                                         T *__restrict__ partial, int pstride,
-                                                        const int *__restrict__ gate) {
-  if (gate && *gate == 0) return;
+                                                        const int *__restrict__ gate,
+                                                        const double *__restrict__ skip) {
+  if ((gate && *gate == 0) || (skip && *skip != 0.0)) return;
   __shared__ T sh[4];
   __shared__ T cs[256];
   for (int j = threadIdx.x; j < m && j < 256; j += 256) cs[j] = (T)coef[j];
@@ -506,7 +515,8 @@ __global__ void face_valm_kernel(int n, int dof, int fdof, const int *__restrict
 template <class T>
 __global__ void __launch_bounds__(1024) face_dot_kernel(int n, int xdof, int fdof, const int *__restrict__ glob,
                                                         const double *__restrict__ valM, const T *__restrict__ X,
-                                                        double *__restrict__ out) {
+                                                        double *__restrict__ out, const double *__restrict__ skip) {
+  if (skip && *skip != 0.0) return;
   __shared__ double sh[16];
   const int nc = fdof < xdof ? fdof : xdof;
   double s = 0.0;
@@ -522,15 +532,17 @@ __global__ void __launch_bounds__(1024) face_dot_kernel(int n, int xdof, int fdo
     *out = t;
   }
 }
-// Y(i, glob(a)) += valM(i,a) * coef * S
+// Y(i, glob(a)) += valM(i,a) * coef * S (coef from *coefp when given)
 template <class T>
 __global__ void face_axpy_kernel(int n, int xdof, int fdof, const int *__restrict__ glob,
                                  const double *__restrict__ valM, const double *__restrict__ S, double coef,
-                                 T *__restrict__ Y) {
+                                 T *__restrict__ Y, const double *__restrict__ coefp,
+                                 const double *__restrict__ skip) {
+  if (skip && *skip != 0.0) return;
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
   const int nc = fdof < xdof ? fdof : xdof;
-  const double s = coef * *S;
+  const double s = (coefp ? *coefp : coef) * *S;
   for (int i = 0; i < nc; ++i) Y[i + (size_t)xdof * glob[a]] += (T)(valM[i + (size_t)fdof * a] * s);
 }
 // A coupled face on the device, with its PRE coefficient -res/(1 + res nS)
@@ -552,7 +564,65 @@ __global__ void __launch_bounds__(256) face_pre_apply_kernel(FaceDev F, const do
   double S = 0.0;
 #pragma unroll
   for (int b = 0; b < kFaceDotBlocks; ++b) S += part[b];
-  const double sc = F.coef * S;
+  const double sc = (F.coefp ? *F.coefp : F.coef) * S;
+  const size_t g = F.glob[a];
+  for (int i = 0; i < nc; ++i) t1[i + 3 * g] += (T)(F.valM[i + (size_t)F.dof * a] * sc);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) t1s[c + 3 * g] = W[c + (size_t)wdof * g] * t1[c + 3 * g];
+}
+// The same three face operations for every coupled face in one launch, when
+// the faces share no node (then the faces' updates commute): one
+// 1024-thread workgroup per face for the dots (face_dot_kernel's order), and
+// a grid row per face for the updates.  Per face the arithmetic is the
+// single-face kernels'.
+template <class T>
+__global__ void __launch_bounds__(1024) face_dot_multi_kernel(const FaceDev *__restrict__ fd, int xdof,
+                                                              const T *__restrict__ X, double *__restrict__ out,
+                                                              const double *__restrict__ skip) {
+  if (skip && *skip != 0.0) return;
+  __shared__ double sh[16];
+  const FaceDev F = fd[blockIdx.x];
+  const int nc = F.dof < xdof ? F.dof : xdof;
+  double s = 0.0;
+  for (int a = threadIdx.x; a < F.nNo; a += 1024)
+    for (int i = 0; i < nc; ++i) s += F.valM[i + (size_t)F.dof * a] * (double)X[i + (size_t)xdof * F.glob[a]];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x / 64] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += sh[w];
+    out[blockIdx.x] = t;
+  }
+}
+template <class T>
+__global__ void face_axpy_multi_kernel(const FaceDev *__restrict__ fd, int xdof, const double *__restrict__ S,
+                                       int pre, T *__restrict__ Y, const double *__restrict__ skip) {
+  if (skip && *skip != 0.0) return;
+  const FaceDev F = fd[blockIdx.y];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= F.nNo) return;
+  const int nc = F.dof < xdof ? F.dof : xdof;
+  const double s = (pre ? (F.coefp ? *F.coefp : F.coef) : F.res) * S[blockIdx.y];
+  for (int i = 0; i < nc; ++i) Y[i + (size_t)xdof * F.glob[a]] += (T)(F.valM[i + (size_t)F.dof * a] * s);
+}
+template <class T>
+__global__ void __launch_bounds__(256) face_pre_apply_multi_kernel(const FaceDev *__restrict__ fd,
+                                                                   const double *__restrict__ part,
+                                                                   T *__restrict__ t1, T *__restrict__ t1s,
+                                                                   const T *__restrict__ W, int wdof,
+                                                                   const double *__restrict__ done) {
+  if (done && *done != 0.0) return;
+  const FaceDev F = fd[blockIdx.y];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= F.nNo) return;
+  const int nc = F.dof < 3 ? F.dof : 3;
+  const double *pf = part + (size_t)kFaceDotBlocks * blockIdx.y;
+  double Sv = 0.0;
+#pragma unroll
+  for (int b = 0; b < kFaceDotBlocks; ++b) Sv += pf[b];
+  const double sc = (F.coefp ? *F.coefp : F.coef) * Sv;
   const size_t g = F.glob[a];
   for (int i = 0; i < nc; ++i) t1[i + 3 * g] += (T)(F.valM[i + (size_t)F.dof * a] * sc);
 #pragma unroll
@@ -578,7 +648,9 @@ __global__ void face_q_kernel(int nsup, const int *__restrict__ qptr, const int 
   q[s] = v;
 }
 // DGKS decision: gate = ||w'||^2 < 0.5 ||w||^2
-__global__ void dgks_kernel(const double *__restrict__ n0, const double *__restrict__ n1, int *__restrict__ gate) {
+__global__ void dgks_kernel(const double *__restrict__ n0, const double *__restrict__ n1, int *__restrict__ gate,
+                            const double *__restrict__ skip) {
+  if (skip && *skip != 0.0) return;
   *gate = (*n1 < 0.5 * *n0) ? 1 : 0;
 }
 // AoS (vdof*vdof, nnz) -> planes, and Gt = -mG^T via the transpose map
@@ -782,7 +854,7 @@ MatView<T> view(const Ctx<T> &c, const T *base, int vdof, int r0, int c0, int nr
 // y = ya * (A x) + za * z
 template <class T>
 int spmv(Ctx<T> &c, const MatView<T> &A, int nr, int nc, const T *x, T *y, double ya = 1.0, const T *z = nullptr,
-         double za = 0.0) {
+         double za = 0.0, const double *skip = nullptr) {
   const unsigned TB = 256;
   const unsigned G = crimson::grid_for((size_t)c.N * kRowLanes, TB);
   const unsigned Gn = crimson::grid_for((size_t)c.N * kNarrowLanes, TB);
@@ -790,9 +862,9 @@ int spmv(Ctx<T> &c, const MatView<T> &A, int nr, int nc, const T *x, T *y, doubl
 #define CR_SPMV(R_, C_)                                                                              \
   do {                                                                                                \
     if (R_ * C_ >= 9)                                                                                 \
-      spmv_kernel<R_, C_, T, kRowLanes><<<G, TB, 0, c.s>>>(c.N, rp, cl, A, x, y, (T)ya, z, (T)za);    \
+      spmv_kernel<R_, C_, T, kRowLanes><<<G, TB, 0, c.s>>>(c.N, rp, cl, A, x, y, (T)ya, z, (T)za, skip); \
     else                                                                                              \
-      spmv_kernel<R_, C_, T, kNarrowLanes><<<Gn, TB, 0, c.s>>>(c.N, rp, cl, A, x, y, (T)ya, z, (T)za); \
+      spmv_kernel<R_, C_, T, kNarrowLanes><<<Gn, TB, 0, c.s>>>(c.N, rp, cl, A, x, y, (T)ya, z, (T)za, skip); \
   } while (0)
   if (nr == 3 && nc == 3) CR_SPMV(3, 3);
   else if (nr == 4 && nc == 4) CR_SPMV(4, 4);
@@ -816,7 +888,7 @@ int spmv(Ctx<T> &c, const MatView<T> &A, int nr, int nc, const T *x, T *y, doubl
 // the norms that set the stopping tests of an FP32 solve).
 template <class T>
 int mdot(Ctx<T> &c, size_t L, const T *v, const T *U0, size_t ustride, int m, bool self, double *dst,
-         const int *gate = nullptr, bool acc64 = false) {
+         const int *gate = nullptr, bool acc64 = false, const double *skip = nullptr) {
   const int nblk = (int)((L + kChunk - 1) / kChunk);
   const int pst = m + (self ? 1 : 0);
   if (pst > 256) {
@@ -824,12 +896,13 @@ int mdot(Ctx<T> &c, size_t L, const T *v, const T *U0, size_t ustride, int m, bo
     return CRIMSON_EUNSUPPORTED;
   }
   if (acc64 || sizeof(T) == sizeof(double)) {
-    mdot_kernel<T, double><<<nblk, 256, 0, c.s>>>(L, v, U0, ustride, m, self ? 1 : 0, c.partial.p, pst, gate);
-    reduce_kernel<double><<<pst, 256, 0, c.s>>>(nblk, c.partial.p, pst, dst, gate);
+    mdot_kernel<T, double><<<nblk, 256, 0, c.s>>>(L, v, U0, ustride, m, self ? 1 : 0, c.partial.p, pst, gate,
+                                                   skip);
+    reduce_kernel<double><<<pst, 256, 0, c.s>>>(nblk, c.partial.p, pst, dst, gate, skip);
   } else {
     T *pt = reinterpret_cast<T *>(c.partial.p);
-    mdot_kernel<T, T><<<nblk, 256, 0, c.s>>>(L, v, U0, ustride, m, self ? 1 : 0, pt, pst, gate);
-    reduce_kernel<T><<<pst, 256, 0, c.s>>>(nblk, pt, pst, dst, gate);
+    mdot_kernel<T, T><<<nblk, 256, 0, c.s>>>(L, v, U0, ustride, m, self ? 1 : 0, pt, pst, gate, skip);
+    reduce_kernel<T><<<pst, 256, 0, c.s>>>(nblk, pt, pst, dst, gate, skip);
   }
   CR_HIP(hipGetLastError());
   return CRIMSON_OK;
@@ -861,23 +934,29 @@ double host_norm(Ctx<T> &c, size_t L, const T *v, int *rc) {
 // ADDBCMUL (memLS.f90:206-268) over all coupled faces, X (xdof, N) -> Y.
 // X may be Y: every face's dot product is taken (stream order) before any
 // face's update is applied, which is what memLS's copy unCondU provides.
+// skip (device, may be null): the device-driven solve's flag that makes
+// these launches return at once; fcoef (device, may be null): every face's
+// PRE coefficient computed on the device (indexed like lhs->face).
 template <class T>
-int addbcmul(Ctx<T> &c, bool pre, int xdof, const T *X, T *Y) {
+int addbcmul(Ctx<T> &c, bool pre, int xdof, const T *X, T *Y, const double *skip = nullptr,
+             const double *fcoef = nullptr) {
   if (!c.coupled) return CRIMSON_OK;
   auto &faces = c.lhs->face;
   // all face dots first (PRE reads the unmodified vector, as unCondU does)
   for (size_t f = 0; f < faces.size(); ++f) {
     const CrFace &fa = faces[f];
     if (!fa.coupled) continue;
-    face_dot_kernel<<<1, 1024, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p, fa.d_valM.p, X, c.fsum.p + f);
+    face_dot_kernel<<<1, 1024, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p, fa.d_valM.p, X, c.fsum.p + f,
+                                         skip);
   }
   for (size_t f = 0; f < faces.size(); ++f) {
     const CrFace &fa = faces[f];
     if (!fa.coupled) continue;
     if (fa.nNo == 0) continue;
     const double coef = pre ? -fa.res / (1.0 + fa.res * fa.nS) : fa.res;
-    face_axpy_kernel<<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, xdof, fa.dof, fa.d_glob.p,
-                                                                        fa.d_valM.p, c.fsum.p + f, coef, Y);
+    face_axpy_kernel<<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(
+        fa.nNo, xdof, fa.dof, fa.d_glob.p, fa.d_valM.p, c.fsum.p + f, coef, Y, (pre && fcoef) ? fcoef + f : nullptr,
+        skip);
   }
   CR_HIP(hipGetLastError());
   return CRIMSON_OK;
@@ -946,13 +1025,13 @@ int gmres_dev(Ctx<T> &c, const MatView<T> &A, int nd, SubLs &ls, const T *R, T *
       const int nblk = (int)((L + kChunk - 1) / kChunk);
       CR_TRY(mdot(c, L, ui, U, L, i, true, h1));  // h1[i] = n0
       T *pt = reinterpret_cast<T *>(c.partial.p);
-      msub_norm_kernel<T><<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h1, pt, 1, nullptr);
-      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, nrm + 1, nullptr);
-      dgks_kernel<<<1, 1, 0, c.s>>>(h1 + i, nrm + 1, c.gate.p);
+      msub_norm_kernel<T><<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h1, pt, 1, nullptr, nullptr);
+      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, nrm + 1, nullptr, nullptr);
+      dgks_kernel<<<1, 1, 0, c.s>>>(h1 + i, nrm + 1, c.gate.p, nullptr);
       // DGKS pass 2 (gated)
       CR_TRY(mdot(c, L, ui, U, L, i, false, h2, c.gate.p));
-      msub_norm_kernel<T><<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h2, pt, 1, c.gate.p);
-      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, nrm + 2, c.gate.p);
+      msub_norm_kernel<T><<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h2, pt, 1, c.gate.p, nullptr);
+      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, nrm + 2, c.gate.p, nullptr);
       CR_HIP(hipGetLastError());
       // read h1[0..i], h2[0..i-1], n1, n2, gate
       CR_HIP(hipMemcpyAsync(hc.data(), h1, (i + 1) * sizeof(double), hipMemcpyDeviceToHost, c.s));
@@ -1039,7 +1118,7 @@ int cg_dev(Ctx<T> &c, size_t L, SubLs &ls, T *R, Op op, T *X, T *P, T *SP, bool 
       CR_TRY(op(P, SP, scal));
       // den = P . SP
       T *pt = reinterpret_cast<T *>(c.partial.p);
-      mdot_kernel<T, T><<<nblk, 256, 0, c.s>>>(L, SP, P, 0, 1, 0, pt, 1, nullptr);
+      mdot_kernel<T, T><<<nblk, 256, 0, c.s>>>(L, SP, P, 0, 1, 0, pt, 1, nullptr, nullptr);
       cg_update_kernel<T><<<nblk, 256, 0, c.s>>>(L, scal, pt, nblk, X, R, P, SP, pt + nblk);
       cg_err_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt + nblk, scal);
       cg_p_kernel<T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, scal, R, P, ps.W, ps.wdof, ps.off, ps.Ps);
@@ -1197,7 +1276,7 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
           fa.nsup, fa.d_qptr.p, fa.d_qa.p, fa.d_qk.p, fa.d_glob.p, fa.d_valM.p, fa.dof, mG, fa.d_q.p);
       CR_HIP(hipGetLastError());
     }
-    c.hfdev.push_back(FaceDev{fa.d_glob.p, fa.d_valM.p, -fa.res / (1.0 + fa.res * fa.nS), fa.nNo, fa.dof,
+    c.hfdev.push_back(FaceDev{fa.d_glob.p, fa.d_valM.p, -fa.res / (1.0 + fa.res * fa.nS), nullptr, fa.res, fa.nNo, fa.dof,
                               fa.d_sup.p, fa.d_q.p, fa.nsup});
   }
   const int nfc = (int)c.hfdev.size();
@@ -1470,6 +1549,806 @@ int solve_dev(crimson_lhs *l, crimson_ls *ls, int dof, double *d_Ri, const int *
   if (l->prec == 32) return solve_impl<float>(l, ls, dof, d_Ri, incL, res);
   return solve_impl<double>(l, ls, dof, d_Ri, incL, res);
 }
+
+
+// ---------------------------------------------------------------------------
+// Device-driven NSSOLVER: the same kernels and the same arithmetic as
+// nssolver_dev, with every host decision of the solve (GMRES's Givens
+// rotations, back substitution and convergence test, CGRAD's iteration
+// count, NSSOLVER's least-squares Gram system, its pivoted GE and stopping
+// test, BCPRE's nS and the PRE coefficients) moved into one-thread kernels,
+// so that the whole solve is one fixed launch sequence that a HIP graph can
+// hold.  Every possible iteration is enqueued (GMRES: mItr cycles of sD
+// Arnoldi steps; CGRAD: mItr iterations; NSSOLVER: mItr outer iterations);
+// the launches past convergence see their flag and return at once.  The
+// scalar kernels keep the host's operation order with FMA contraction off,
+// so the solve gives the host-driven solve's bits.
+//
+// gst (double, device): the state of the solve and of its last GMRES and
+// CG calls, then the GMRES Hessenberg system, the Gram system and the
+// solution coefficients.
+enum GstIdx : int {
+  G_RI_SKIP = 0,  // the outer loop has stopped (converged or failed)
+  G_RI_ITR,
+  G_RI_SUC,
+  G_RI_INORM,
+  G_RI_FNORM,
+  G_RI_EPS,
+  G_RI_DBPRE,
+  G_ERR,  // 1: singular GE
+  G_GM_CYC = 8,  // this GMRES call does nothing more (done, or not running)
+  G_GM_ARN,      // no further Arnoldi step in this cycle
+  G_GM_EPS,
+  G_GM_I,
+  G_GM_ITR,
+  G_GM_SUC,
+  G_GM_INORM,
+  G_GM_FNORM,
+  G_GM_DBPRE,
+  G_GM_SCALE,
+  G_GM_APPLY,
+  G_GM_RAN,
+  G_GM_EARLY,
+  G_GM_LASTF,  // fNorm and dB of the last GMRES call that ran to the end
+  G_GM_LASTD,
+  G_CG_ITR = 24,
+  G_CG_SUC,
+  G_CG_INORM,
+  G_CG_ERR,
+  G_CG_ERRO,
+  G_RC_N2 = 30,
+  G_NS_CNT,
+  G_ARR = 64
+};
+
+template <class T>
+__global__ void gfill_kernel(size_t L, T *__restrict__ x, const double *__restrict__ skip) {
+  if (skip && *skip != 0.0) return;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < L) x[e] = T(0);
+}
+// dst = src when *skip is 0 (skip may be null) and *runif is not 0 (runif may be null)
+template <class T>
+__global__ void gcopy_kernel(size_t L, const T *__restrict__ src, T *__restrict__ dst,
+                             const double *__restrict__ skip, const double *__restrict__ runif) {
+  if ((skip && *skip != 0.0) || (runif && *runif == 0.0)) return;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < L) dst[e] = src[e];
+}
+// x = (T)(*s) x, as axpby(s, x, 0, x)
+template <class T>
+__global__ void gscale_kernel(size_t L, T *__restrict__ x, const double *__restrict__ s,
+                              const double *__restrict__ skip) {
+  if (skip && *skip != 0.0) return;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < L) x[e] = (T)(*s) * x[e];
+}
+template <class T>
+__global__ void gaxpby_kernel(size_t L, T a, const T *__restrict__ x, T b, T *__restrict__ y,
+                              const double *__restrict__ skip) {
+  if (skip && *skip != 0.0) return;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L) return;
+  y[e] = a * x[e] + (b == T(0) ? T(0) : b * y[e]);
+}
+// maxpy_kernel with the count m_host, or *m_dev when given
+template <class T>
+__global__ void gmaxpy_kernel(size_t L, T *__restrict__ y, const T *__restrict__ U0, size_t ustride, int m_host,
+                              const double *__restrict__ m_dev, const double *__restrict__ coef, double sign,
+                              const double *__restrict__ skip, const double *__restrict__ runif) {
+  if ((skip && *skip != 0.0) || (runif && *runif == 0.0)) return;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L) return;
+  const int m = m_dev ? (int)*m_dev : m_host;
+  T w = y[e];
+  for (int j = 0; j < m; ++j) w += (T)(sign * coef[j]) * U0[ustride * j + e];
+  y[e] = w;
+}
+
+__device__ inline double dmax(double a, double b) { return a < b ? b : a; }  // std::max's choice
+
+__global__ void ns_begin_kernel(double *__restrict__ g, const double *__restrict__ n2a,
+                                const double *__restrict__ n2b, double absTol, double relTol) {
+#pragma clang fp contract(off)
+  const double a1 = sqrt(*n2a), a2 = sqrt(*n2b);
+  const double eps = sqrt(a1 * a1 + a2 * a2);
+  g[G_RI_INORM] = eps;
+  g[G_RI_FNORM] = eps;
+  g[G_RI_EPS] = dmax(absTol, relTol * eps);
+  g[G_RI_SKIP] = 0.0;
+  g[G_RI_SUC] = 0.0;
+  g[G_RI_ITR] = 0.0;
+  g[G_ERR] = 0.0;
+  g[G_GM_ITR] = 0.0;
+  g[G_CG_ITR] = 0.0;
+  g[G_GM_LASTF] = g[G_GM_LASTD] = 0.0;
+}
+// BCPRE's nS of a coupled face (memLS.f90:1683-1716, summed in the host's
+// order) and its PRE coefficient -res / (1 + res nS)
+__global__ void face_pre_coef_kernel(int nNo, int fdof, int nsd, const double *__restrict__ valM, double res,
+                                     double *__restrict__ coef) {
+#pragma clang fp contract(off)
+  double nS = 0.0;
+  const int nc = fdof < nsd ? fdof : nsd;
+  for (int a = 0; a < nNo; ++a)
+    for (int i = 0; i < nc; ++i) nS = nS + valM[i + (size_t)fdof * a] * valM[i + (size_t)fdof * a];
+  *coef = -res / (1.0 + res * nS);
+}
+__global__ void gm_begin_kernel(double *__restrict__ g) {
+  if (g[G_RI_SKIP] != 0.0) {
+    g[G_GM_CYC] = 1.0;
+    g[G_GM_ARN] = 1.0;
+    g[G_GM_RAN] = 0.0;
+    return;
+  }
+  g[G_GM_CYC] = 0.0;
+  g[G_GM_ARN] = 1.0;
+  g[G_GM_SUC] = 0.0;
+  g[G_GM_RAN] = 0.0;
+  g[G_GM_EARLY] = 0.0;
+}
+__global__ void gm_count_kernel(double *__restrict__ g) {
+  if (g[G_GM_CYC] != 0.0) return;
+  g[G_GM_ITR] = g[G_GM_ITR] + 1.0;
+}
+__global__ void gm_cycle_kernel(double *__restrict__ g, int l, double absTol, double relTol,
+                                const double *__restrict__ n2, double *__restrict__ err) {
+#pragma clang fp contract(off)
+  if (g[G_GM_CYC] != 0.0) return;
+  const double e0 = sqrt(*n2);
+  err[0] = e0;
+  if (l == 1) {
+    const double eps = e0;
+    if (eps <= absTol) {
+      g[G_GM_EARLY] = 1.0;
+      g[G_GM_CYC] = 1.0;
+      g[G_GM_ARN] = 1.0;
+      return;
+    }
+    g[G_GM_INORM] = eps;
+    g[G_GM_FNORM] = eps;
+    g[G_GM_EPS] = dmax(absTol, relTol * eps);
+  }
+  g[G_GM_DBPRE] = g[G_GM_FNORM];
+  g[G_GM_SCALE] = 1.0 / e0;
+  g[G_GM_ARN] = 0.0;
+  g[G_GM_RAN] = 1.0;
+  g[G_GM_I] = 0.0;
+}
+// Arnoldi step i's Hessenberg column, rotations and test (gmres_dev's host
+// part); H (sD+1, sD) column-major
+__global__ void gm_givens_kernel(double *__restrict__ g, int i, int sD, const double *__restrict__ h1,
+                                 const double *__restrict__ h2, const double *__restrict__ nrm,
+                                 const int *__restrict__ gate, double *__restrict__ H, double *__restrict__ cs,
+                                 double *__restrict__ sn, double *__restrict__ err) {
+#pragma clang fp contract(off)
+  if (g[G_GM_ARN] != 0.0) return;
+  g[G_GM_ITR] = g[G_GM_ITR] + 1.0;
+  const int ld = sD + 1;
+  const bool gt = *gate != 0;
+  for (int j = 1; j <= i; ++j) H[(j - 1) + (size_t)ld * (i - 1)] = h1[j - 1] + (gt ? h2[j - 1] : 0.0);
+  H[i + (size_t)ld * (i - 1)] = sqrt(gt ? nrm[2] : nrm[1]);
+  g[G_GM_SCALE] = 1.0 / H[i + (size_t)ld * (i - 1)];
+  for (int j = 1; j <= i - 1; ++j) {
+    const double a = H[(j - 1) + (size_t)ld * (i - 1)], b = H[j + (size_t)ld * (i - 1)];
+    const double tmp = cs[j - 1] * a + sn[j - 1] * b;
+    H[j + (size_t)ld * (i - 1)] = -sn[j - 1] * a + cs[j - 1] * b;
+    H[(j - 1) + (size_t)ld * (i - 1)] = tmp;
+  }
+  const double a = H[(i - 1) + (size_t)ld * (i - 1)], b = H[i + (size_t)ld * (i - 1)];
+  const double tmp = sqrt(a * a + b * b);
+  cs[i - 1] = a / tmp;
+  sn[i - 1] = b / tmp;
+  H[(i - 1) + (size_t)ld * (i - 1)] = tmp;
+  H[i + (size_t)ld * (i - 1)] = 0.0;
+  err[i] = -sn[i - 1] * err[i - 1];
+  err[i - 1] = cs[i - 1] * err[i - 1];
+  g[G_GM_I] = (double)i;
+  if (fabs(err[i]) < g[G_GM_EPS]) {
+    g[G_GM_SUC] = 1.0;
+    g[G_GM_ARN] = 1.0;
+  }
+}
+// the cycle's back substitution (gmres_dev's host part)
+__global__ void gm_solve_kernel(double *__restrict__ g, int sD, const double *__restrict__ H,
+                                const double *__restrict__ err, double *__restrict__ y) {
+#pragma clang fp contract(off)
+  if (g[G_GM_CYC] != 0.0) return;
+  const int ld = sD + 1;
+  const int i = (int)g[G_GM_I];
+  for (int j = 0; j < i; ++j) y[j] = err[j];
+  for (int j = i; j >= 1; --j) {
+    for (int k = j + 1; k <= i; ++k) y[j - 1] = y[j - 1] - H[(j - 1) + (size_t)ld * (k - 1)] * y[k - 1];
+    y[j - 1] = y[j - 1] / H[(j - 1) + (size_t)ld * (j - 1)];
+  }
+  g[G_GM_APPLY] = (double)i;
+  g[G_GM_FNORM] = fabs(err[i]);
+}
+__global__ void gm_endcycle_kernel(double *__restrict__ g) {
+  if (g[G_GM_CYC] != 0.0) return;
+  if (g[G_GM_SUC] != 0.0) g[G_GM_CYC] = 1.0;
+  g[G_GM_ARN] = 1.0;
+}
+__global__ void gm_finish_kernel(double *__restrict__ g) {
+  if (g[G_RI_SKIP] != 0.0 || g[G_GM_RAN] == 0.0) return;
+  g[G_GM_LASTF] = g[G_GM_FNORM];
+  g[G_GM_LASTD] = g[G_GM_DBPRE];
+}
+__global__ void cg_begin_kernel(double *__restrict__ scal, double *__restrict__ g, const double *__restrict__ n2,
+                                double absTol, double relTol, int mItr) {
+#pragma clang fp contract(off)
+  if (g[G_RI_SKIP] != 0.0) {
+    scal[5] = 1.0;
+    return;
+  }
+  const double e = sqrt(*n2);
+  g[G_CG_INORM] = e;
+  double eps = dmax(absTol, relTol * e);
+  eps = eps * eps;
+  scal[0] = e * e;
+  scal[1] = e * e;
+  scal[2] = eps;
+  scal[3] = scal[4] = scal[5] = scal[6] = 0.0;
+  scal[7] = (double)mItr;
+  scal[8] = 0.0;
+}
+__global__ void cg_end_kernel(const double *__restrict__ scal, double *__restrict__ g) {
+  if (g[G_RI_SKIP] != 0.0) return;
+  g[G_CG_ITR] = g[G_CG_ITR] + (double)(int)scal[6];
+  g[G_CG_SUC] = scal[8];
+  g[G_CG_ERR] = scal[0];
+  g[G_CG_ERRO] = scal[1];
+}
+// NSSOLVER iteration i's Gram rows iB = 2i-1, iBB = 2i from their dots
+// (dk: for each of the two, h1 at 0, r1 at 256, h2 at 512, r2 at 768),
+// then GE (memLS.f90:607-677) and the stopping test; A (nB, nB), C the GE
+// scratch (Nn, Nn+1)
+__global__ void ns_gram_kernel(double *__restrict__ g, int i, int nB, const double *__restrict__ dk,
+                               double *__restrict__ A, double *__restrict__ Bv, double *__restrict__ xB,
+                               double *__restrict__ C) {
+#pragma clang fp contract(off)
+  if (g[G_RI_SKIP] != 0.0) return;
+  const int iB = 2 * i - 1, iBB = 2 * i;
+  g[G_RI_DBPRE] = g[G_RI_FNORM];
+  for (int k = iB; k <= iBB; ++k) {
+    const double *d = dk + (size_t)1024 * (k - iB);
+    const double *h1 = d, *r1 = d + 256, *h2 = d + 512, *r2 = d + 768;
+    for (int j = 1; j <= k - 1; ++j) {
+      A[(j - 1) + (size_t)nB * (k - 1)] = h1[j - 1] + h2[j - 1];
+      A[(k - 1) + (size_t)nB * (j - 1)] = A[(j - 1) + (size_t)nB * (k - 1)];
+    }
+    const double n1 = sqrt(h1[k - 1]), n2 = sqrt(h2[k - 1]);
+    A[(k - 1) + (size_t)nB * (k - 1)] = n1 * n1 + n2 * n2;
+    Bv[k - 1] = r1[0] + r2[0];
+  }
+  for (int k = 0; k < iBB; ++k) xB[k] = Bv[k];
+  const int Nn = iBB;
+  if (Nn == 2) {
+    const double a11 = A[0], a21 = A[1], a12 = A[nB], a22 = A[nB + 1];
+    const double pivot = a11 * a22 - a21 * a12;
+    const double saveEl = (xB[0] * a22 - xB[1] * a12) / pivot;
+    xB[1] = (xB[1] * a11 - xB[0] * a21) / pivot;
+    xB[0] = saveEl;
+  } else {
+#define CC(r, cc) C[(r) + (size_t)Nn * (cc)]
+    for (int cc = 0; cc < Nn; ++cc)
+      for (int r = 0; r < Nn; ++r) CC(r, cc) = A[r + (size_t)nB * cc];
+    for (int r = 0; r < Nn; ++r) CC(r, Nn) = xB[r];
+    for (int m = 0; m < Nn - 1; ++m) {
+      int ipv = m;
+      double pivot = fabs(CC(m, m));
+      for (int r = m + 1; r < Nn; ++r)
+        if (fabs(CC(r, m)) > pivot) {
+          ipv = r;
+          pivot = fabs(CC(r, m));
+        }
+      if (pivot < 2.0 * 2.220446049250313e-16) {
+        g[G_ERR] = 1.0;
+        g[G_RI_SKIP] = 1.0;
+        return;
+      }
+      if (ipv != m)
+        for (int cc = m; cc <= Nn; ++cc) {
+          const double t = CC(m, cc);
+          CC(m, cc) = CC(ipv, cc);
+          CC(ipv, cc) = t;
+        }
+      for (int r = m + 1; r < Nn; ++r) {
+        const double sv = CC(r, m) / CC(m, m);
+        CC(r, m) = 0.0;
+        for (int cc = m + 1; cc <= Nn; ++cc) CC(r, cc) = CC(r, cc) - sv * CC(m, cc);
+      }
+    }
+    for (int j = Nn - 1; j >= 0; --j) {
+      for (int r = j + 1; r < Nn; ++r) CC(j, Nn) = CC(j, Nn) - CC(j, r) * CC(r, Nn);
+      CC(j, Nn) = CC(j, Nn) / CC(j, j);
+    }
+    for (int r = 0; r < Nn; ++r) xB[r] = CC(r, Nn);
+#undef CC
+  }
+  double sxb = 0.0;
+  for (int k = 0; k < iBB; ++k) sxb = sxb + xB[k] * Bv[k];
+  const double iN = g[G_RI_INORM];
+  g[G_RI_FNORM] = sqrt(iN * iN - sxb);
+  if (g[G_RI_FNORM] < g[G_RI_EPS]) {
+    g[G_RI_SUC] = 1.0;
+    g[G_RI_ITR] = (double)i;
+    g[G_RI_SKIP] = 1.0;
+  }
+}
+// after the outer loop: the iteration count, and the solution's
+// coefficients cu(k) = xB(2k), cp(k) = xB(2k-1) (1-based)
+__global__ void ns_final_kernel(double *__restrict__ g, int mItr, const double *__restrict__ xB,
+                                double *__restrict__ cu, double *__restrict__ cp) {
+  if (g[G_ERR] != 0.0) return;
+  if (g[G_RI_SUC] == 0.0) g[G_RI_ITR] = (double)mItr;
+  const int nit = (int)g[G_RI_ITR];
+  g[G_NS_CNT] = 2.0 * nit;
+  for (int k = 1; k <= nit; ++k) {
+    cu[k - 1] = xB[2 * k - 1];
+    cp[k - 1] = xB[2 * k - 2];
+  }
+}
+
+// Device workspaces of one device-driven solve (allocated before capture).
+template <class T>
+struct NsGraphWork {
+  Ctx<T> c;
+  crimson::DBuf<T> Rm, Rc, Rmi, Rci, U, P, MU, MP, t1, t2, t3, t4, Rw, gmU, gmTmp;
+  crimson::DBuf<double> gst, fcoef, dots;
+  int sD = 0, nB = 0, mRI = 0, mGM = 0, mCG = 0;
+  double riAbs = 0, riRel = 0, gmAbs = 0, gmRel = 0, cgAbs = 0, cgRel = 0;
+  bool disjoint = false;  // the coupled faces share no node: one launch for all of them
+  int maxfn = 0;          // the largest coupled face
+  crimson::DBuf<double> msum;  // the faces' dots of the batched ADDBCMUL
+  struct SegCapture *cap = nullptr;  // the capture in progress, cut between the solve's phases
+  int cut();
+  size_t oH = 0, ocs = 0, osn = 0, oerr = 0, oy = 0, oA = 0, oBv = 0, oxB = 0, oC = 0, ocu = 0, ocp = 0;
+  double *G(size_t off) { return gst.p + off; }
+};
+
+// ADDBCMUL of the device-driven solve: the batched face kernels when the
+// coupled faces are disjoint, addbcmul's per-face launches otherwise.
+template <class T>
+int addbcmul_graph(NsGraphWork<T> &w, bool pre, int xdof, const T *X, T *Y, const double *skip) {
+  Ctx<T> &c = w.c;
+  if (!c.coupled) return CRIMSON_OK;
+  const int nfc = (int)c.hfdev.size();
+  if (!w.disjoint || nfc < 2) return addbcmul(c, pre, xdof, X, Y, skip, pre ? w.fcoef.p : nullptr);
+  face_dot_multi_kernel<T><<<nfc, 1024, 0, c.s>>>(c.fdev.p, xdof, X, w.msum.p, skip);
+  face_axpy_multi_kernel<T><<<dim3(crimson::grid_for(std::max(1, w.maxfn), 256), nfc), 256, 0, c.s>>>(
+      c.fdev.p, xdof, w.msum.p, pre ? 1 : 0, Y, skip);
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+
+// GMRES (gmres_dev) device-driven, on the operator x -> A x + ADD(x) with
+// PRE (the NS solver's momentum solves: pre_ok, GMRESV's first-product
+// count not kept)
+template <class T>
+int gmres_graph(NsGraphWork<T> &w, const MatView<T> &A, int nd, const T *R, T *X) {
+  Ctx<T> &c = w.c;
+  const size_t L = (size_t)nd * c.N;
+  const int sD = w.sD;
+  double *g = w.gst.p;
+  const double *cyc = g + G_GM_CYC, *arn = g + G_GM_ARN, *ri = g + G_RI_SKIP;
+  T *U = w.gmU.p;
+  const unsigned GL = crimson::grid_for(L, 256);
+  gm_begin_kernel<<<1, 1, 0, c.s>>>(g);
+  gfill_kernel<T><<<GL, 256, 0, c.s>>>(L, X, ri);
+  const int nblk = (int)((L + kChunk - 1) / kChunk);
+  double *h1 = c.hcol.p, *h2 = c.hcol.p + 256, *nrm = c.hcol.p + 512;
+  T *pt = reinterpret_cast<T *>(c.partial.p);
+  for (int l = 1; l <= w.mGM; ++l) {
+    if (l == 1) {
+      gcopy_kernel<T><<<GL, 256, 0, c.s>>>(L, R, U, cyc, nullptr);
+    } else {
+      gm_count_kernel<<<1, 1, 0, c.s>>>(g);
+      CR_TRY(spmv(c, A, nd, nd, X, U, 1.0, static_cast<const T *>(nullptr), 0.0, cyc));
+      CR_TRY(addbcmul_graph(w, false, nd, X, U, cyc));
+      gaxpby_kernel<T><<<GL, 256, 0, c.s>>>(L, T(1), R, T(-1), U, cyc);
+    }
+    CR_TRY(addbcmul_graph(w, true, nd, U, U, cyc));
+    CR_TRY(mdot<T>(c, L, U, nullptr, 0, 0, true, c.dsc.p + 40, nullptr, true, cyc));
+    gm_cycle_kernel<<<1, 1, 0, c.s>>>(g, l, w.gmAbs, w.gmRel, c.dsc.p + 40, w.G(w.oerr));
+    gscale_kernel<T><<<GL, 256, 0, c.s>>>(L, U, g + G_GM_SCALE, cyc);
+    for (int i = 1; i <= sD; ++i) {
+      T *ui = U + L * (size_t)i;
+      const T *uprev = U + L * (size_t)(i - 1);
+      CR_TRY(spmv(c, A, nd, nd, uprev, ui, 1.0, static_cast<const T *>(nullptr), 0.0, arn));
+      CR_TRY(addbcmul_graph(w, false, nd, uprev, ui, arn));
+      CR_TRY(addbcmul_graph(w, true, nd, ui, ui, arn));
+      CR_TRY(mdot(c, L, ui, U, L, i, true, h1, nullptr, false, arn));
+      msub_norm_kernel<T><<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h1, pt, 1, nullptr, arn);
+      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, nrm + 1, nullptr, arn);
+      dgks_kernel<<<1, 1, 0, c.s>>>(h1 + i, nrm + 1, c.gate.p, arn);
+      CR_TRY(mdot(c, L, ui, U, L, i, false, h2, c.gate.p, false, arn));
+      msub_norm_kernel<T><<<nblk, 256, 0, c.s>>>(L, ui, U, L, i, h2, pt, 1, c.gate.p, arn);
+      reduce_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt, 1, nrm + 2, c.gate.p, arn);
+      gm_givens_kernel<<<1, 1, 0, c.s>>>(g, i, sD, h1, h2, nrm, c.gate.p, w.G(w.oH), w.G(w.ocs), w.G(w.osn),
+                                         w.G(w.oerr));
+      gscale_kernel<T><<<GL, 256, 0, c.s>>>(L, ui, g + G_GM_SCALE, arn);
+    }
+    gm_solve_kernel<<<1, 1, 0, c.s>>>(g, sD, w.G(w.oH), w.G(w.oerr), w.G(w.oy));
+    gmaxpy_kernel<T><<<GL, 256, 0, c.s>>>(L, X, U, L, 0, g + G_GM_APPLY, w.G(w.oy), 1.0, cyc, nullptr);
+    gm_endcycle_kernel<<<1, 1, 0, c.s>>>(g);
+  }
+  gm_finish_kernel<<<1, 1, 0, c.s>>>(g);
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+
+// CGRAD on the Schur operator (cg_dev with schur = true) device-driven:
+// mItr iteration bodies, each launch returning at once after the flag.
+template <class T, class Op>
+int cg_graph(NsGraphWork<T> &w, size_t L, T *R, Op op, T *X, T *P, T *SP, Prescale<T> ps) {
+  Ctx<T> &c = w.c;
+  double *g = w.gst.p, *scal = c.dsc.p;
+  const unsigned GL = crimson::grid_for(L, 256);
+  CR_TRY(mdot<T>(c, L, R, nullptr, 0, 0, true, c.dsc.p + 41, nullptr, true, g + G_RI_SKIP));
+  cg_begin_kernel<<<1, 1, 0, c.s>>>(scal, g, c.dsc.p + 41, w.cgAbs, w.cgRel, w.mCG);
+  gfill_kernel<T><<<GL, 256, 0, c.s>>>(L, X, scal + 5);
+  gcopy_kernel<T><<<GL, 256, 0, c.s>>>(L, R, P, scal + 5, nullptr);
+  if (ps.Ps) scale_p_kernel<T><<<GL, 256, 0, c.s>>>((int)L, ps.W, ps.wdof, ps.off, P, ps.Ps, scal + 5);
+  const int nblk = (int)((L + kChunk - 1) / kChunk);
+  cg_check_kernel<<<1, 1, 0, c.s>>>(scal);
+  for (int it = 0; it < w.mCG; ++it) {
+    CR_TRY(op(P, SP, scal));
+    T *pt = reinterpret_cast<T *>(c.partial.p);
+    mdot_kernel<T, T><<<nblk, 256, 0, c.s>>>(L, SP, P, 0, 1, 0, pt, 1, nullptr, scal + 5);
+    cg_update_kernel<T><<<nblk, 256, 0, c.s>>>(L, scal, pt, nblk, X, R, P, SP, pt + nblk);
+    cg_err_kernel<T><<<1, 256, 0, c.s>>>(nblk, pt + nblk, scal);
+    cg_p_kernel<T><<<GL, 256, 0, c.s>>>(L, scal, R, P, ps.W, ps.wdof, ps.off, ps.Ps);
+  }
+  gcopy_kernel<T><<<GL, 256, 0, c.s>>>(L, X, R, g + G_RI_SKIP, nullptr);
+  cg_end_kernel<<<1, 1, 0, c.s>>>(scal, g);
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+
+// NSSOLVER (nssolver_dev) device-driven on Ri (4, N): the whole solve is
+// enqueued on w.c.s with no host synchronisation.
+template <class T>
+int nssolver_graph(NsGraphWork<T> &w, T *Ri) {
+  Ctx<T> &c = w.c;
+  crimson_lhs *l = c.lhs;
+  const int N = c.N, nsd = 3;
+  const size_t mn = (size_t)nsd * N;
+  double *g = w.gst.p;
+  const double *ri = g + G_RI_SKIP;
+  const unsigned gN = crimson::grid_for(N, 256), Gmn = crimson::grid_for(mn, 256);
+  extract_kernel<<<gN, 256, 0, c.s>>>(N, 4, 0, 3, Ri, w.Rmi.p);
+  extract_kernel<<<gN, 256, 0, c.s>>>(N, 4, 3, 1, Ri, w.Rci.p);
+  gcopy_kernel<T><<<Gmn, 256, 0, c.s>>>(mn, w.Rmi.p, w.Rm.p, nullptr, nullptr);
+  gcopy_kernel<T><<<gN, 256, 0, c.s>>>(N, w.Rci.p, w.Rc.p, nullptr, nullptr);
+  CR_TRY(mdot<T>(c, mn, w.Rm.p, nullptr, 0, 0, true, c.dsc.p + 42, nullptr, true));
+  CR_TRY(mdot<T>(c, N, w.Rc.p, nullptr, 0, 0, true, c.dsc.p + 43, nullptr, true));
+  ns_begin_kernel<<<1, 1, 0, c.s>>>(g, c.dsc.p + 42, c.dsc.p + 43, w.riAbs, w.riRel);
+  const T *V = planes(l, T(0));
+  const MatView<T> mK = view<T>(c, V, 4, 0, 0, 3, 3, c.W.p, c.fix.p, 4);
+  const MatView<T> mG = view<T>(c, V, 4, 0, 3, 3, 1, c.W.p, nullptr, 4);
+  const MatView<T> mD = view<T>(c, V, 4, 3, 0, 1, 3, c.W.p, nullptr, 4);
+  const MatView<T> mL = view<T>(c, V, 4, 3, 3, 1, 1, c.W.p, c.fix.p, 4);
+  MatView<T> Gt = mD;
+  if (!l->gt_is_d) {
+    crimson::DBuf<T> &gtb = gt_buf(l, T(0));
+    gt_kernel<T><<<crimson::grid_for(l->nnz, 256), 256, 0, c.s>>>(l->nnz, V, l->d_tr.p, gtb.p);
+    Gt = view<T>(c, gtb.p, 1, 0, 0, 1, 3, c.W.p, nullptr, 4);
+    Gt.plane[0] = 0;
+    Gt.plane[1] = 1;
+    Gt.plane[2] = 2;
+    Gt.r0 = 3;
+    Gt.c0 = 0;
+  }
+  // BCPRE and the faces' PRE coefficients on the device; the face
+  // functionals (c.hfdev / c.fdev: set up once by ns_graph_setup)
+  for (size_t f = 0; f < l->face.size(); ++f) {
+    CrFace &fa = l->face[f];
+    if (!fa.coupled) continue;
+    face_pre_coef_kernel<<<1, 1, 0, c.s>>>(fa.nNo, fa.dof, nsd, fa.d_valM.p, fa.res, w.fcoef.p + f);
+    if (fa.nsup > 0)
+      face_q_kernel<T><<<crimson::grid_for(fa.nsup, 256), 256, 0, c.s>>>(
+          fa.nsup, fa.d_qptr.p, fa.d_qa.p, fa.d_qk.p, fa.d_glob.p, fa.d_valM.p, fa.dof, mG, fa.d_q.p);
+  }
+  CR_HIP(hipGetLastError());
+  const int nfc = (int)c.hfdev.size();
+  T *t1 = w.t1.p, *t2 = w.t2.p, *tmp = w.gmTmp.p;
+  auto schur_op = [&](const T *Pv, T *SPv, const double *scal) -> int {
+    const double *done = scal + 5;
+    const unsigned gr = crimson::grid_for((size_t)N * kNarrowLanes, 256);
+    g_product_kernel<T><<<gr + nfc * kFaceDotBlocks, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2, t1, tmp,
+                                                                     nfc, c.fdev.p, c.fpart.p, done);
+    if (w.disjoint && nfc > 1)
+      face_pre_apply_multi_kernel<T><<<dim3(crimson::grid_for(std::max(1, w.maxfn), 256), nfc), 256, 0, c.s>>>(
+          c.fdev.p, c.fpart.p, t1, tmp, c.W.p, 4, done);
+    else
+      for (int f = 0; f < nfc; ++f)
+        if (c.hfdev[f].nNo > 0)
+          face_pre_apply_kernel<T><<<crimson::grid_for(c.hfdev[f].nNo, 256), 256, 0, c.s>>>(
+              c.hfdev[f], c.fpart.p + f * kFaceDotBlocks, t1, tmp, c.W.p, 4, done);
+    schur_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2, tmp, SPv, done);
+    CR_HIP(hipGetLastError());
+    return CRIMSON_OK;
+  };
+  Prescale<T> pre_p;
+  pre_p.W = c.W.p;
+  pre_p.wdof = 4;
+  pre_p.off = mG.c0;
+  pre_p.Ps = t2;
+  const int nB = w.nB;
+  for (int i = 1; i <= w.mRI; ++i) {
+    const int iB = 2 * i - 1, iBB = 2 * i;
+    T *Ui = w.U.p + mn * (i - 1), *Pi = w.P.p + (size_t)N * (i - 1);
+    T *MUb = w.MU.p + mn * (iB - 1), *MUbb = w.MU.p + mn * (iBB - 1);
+    T *MPb = w.MP.p + (size_t)N * (iB - 1), *MPbb = w.MP.p + (size_t)N * (iBB - 1);
+    CR_TRY(w.cut());
+    CR_TRY(gmres_graph(w, mK, 3, w.Rm.p, Ui));
+    CR_TRY(w.cut());
+    CR_TRY(spmv(c, mD, 1, 3, Ui, Pi, -1.0, w.Rc.p, 1.0, ri));  // P = Rc - D U
+    CR_TRY(cg_graph(w, (size_t)N, Pi, schur_op, w.t3.p, w.t4.p, MPb, pre_p));
+    CR_TRY(w.cut());
+    CR_TRY(spmv(c, mG, 3, 1, Pi, MUb, 1.0, static_cast<const T *>(nullptr), 0.0, ri));
+    gaxpby_kernel<T><<<Gmn, 256, 0, c.s>>>(mn, T(1), w.Rm.p, T(0), MUbb, ri);
+    gaxpby_kernel<T><<<Gmn, 256, 0, c.s>>>(mn, T(-1), MUb, T(1), MUbb, ri);  // MU(iBB) = Rm - MU(iB)
+    CR_TRY(gmres_graph(w, mK, 3, MUbb, Ui));
+    CR_TRY(w.cut());
+    CR_TRY(spmv(c, mK, 3, 3, Ui, MUbb, 1.0, static_cast<const T *>(nullptr), 0.0, ri));
+    CR_TRY(addbcmul_graph(w, false, 3, Ui, MUbb, ri));
+    CR_TRY(spmv(c, mL, 1, 1, Pi, MPb, 1.0, static_cast<const T *>(nullptr), 0.0, ri));
+    CR_TRY(spmv(c, mD, 1, 3, Ui, MPbb, 1.0, static_cast<const T *>(nullptr), 0.0, ri));
+    for (int k = iB; k <= iBB; ++k) {
+      const T *MUk = w.MU.p + mn * (k - 1), *MPk = w.MP.p + (size_t)N * (k - 1);
+      double *d = w.dots.p + (size_t)1024 * (k - iB);
+      CR_TRY(mdot<T>(c, mn, MUk, w.MU.p, mn, k - 1, true, d, nullptr, true, ri));
+      CR_TRY(mdot<T>(c, mn, MUk, w.Rmi.p, 0, 1, false, d + 256, nullptr, true, ri));
+      CR_TRY(mdot<T>(c, N, MPk, w.MP.p, N, k - 1, true, d + 512, nullptr, true, ri));
+      CR_TRY(mdot<T>(c, N, MPk, w.Rci.p, 0, 1, false, d + 768, nullptr, true, ri));
+    }
+    ns_gram_kernel<<<1, 1, 0, c.s>>>(g, i, nB, w.dots.p, w.G(w.oA), w.G(w.oBv), w.G(w.oxB), w.G(w.oC));
+    gcopy_kernel<T><<<Gmn, 256, 0, c.s>>>(mn, w.Rmi.p, w.Rm.p, ri, nullptr);
+    gcopy_kernel<T><<<gN, 256, 0, c.s>>>(N, w.Rci.p, w.Rc.p, ri, nullptr);
+    gmaxpy_kernel<T><<<Gmn, 256, 0, c.s>>>(mn, w.Rm.p, w.MU.p, mn, iBB, nullptr, w.G(w.oxB), -1.0, ri, nullptr);
+    gmaxpy_kernel<T><<<gN, 256, 0, c.s>>>(N, w.Rc.p, w.MP.p, N, iBB, nullptr, w.G(w.oxB), -1.0, ri, nullptr);
+  }
+  ns_final_kernel<<<1, 1, 0, c.s>>>(g, w.mRI, w.G(w.oxB), w.G(w.ocu), w.G(w.ocp));
+  const double *err = g + G_ERR, *suc = g + G_RI_SUC;
+  gcopy_kernel<T><<<gN, 256, 0, c.s>>>(N, w.Rci.p, w.Rc.p, err, suc);
+  gmaxpy_kernel<T><<<gN, 256, 0, c.s>>>(N, w.Rc.p, w.MP.p, N, 0, g + G_NS_CNT, w.G(w.oxB), -1.0, err, suc);
+  CR_TRY(mdot<T>(c, N, w.Rc.p, nullptr, 0, 0, true, g + G_RC_N2, nullptr, true, err));
+  // solution: Rmi = sum cu(k) U_k, Rci = sum cp(k) P_k
+  gfill_kernel<T><<<Gmn, 256, 0, c.s>>>(mn, w.Rmi.p, err);
+  gfill_kernel<T><<<gN, 256, 0, c.s>>>(N, w.Rci.p, err);
+  gmaxpy_kernel<T><<<Gmn, 256, 0, c.s>>>(mn, w.Rmi.p, w.U.p, mn, 0, g + G_RI_ITR, w.G(w.ocu), 1.0, err, nullptr);
+  gmaxpy_kernel<T><<<gN, 256, 0, c.s>>>(N, w.Rci.p, w.P.p, N, 0, g + G_RI_ITR, w.G(w.ocp), 1.0, err, nullptr);
+  insert_kernel<<<gN, 256, 0, c.s>>>(N, 4, 0, 3, w.Rmi.p, Ri);
+  insert_kernel<<<gN, 256, 0, c.s>>>(N, 4, 3, 1, w.Rci.p, Ri);
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+
+// The device-driven solve's workspaces and face set-up for lhs and ls
+// (allocations, face supports and their uploads: all before any capture).
+// res: memLS's face coefficients (res[f] != 0 couples Neumann face f), incL
+// as in memLS_SOLVE.
+template <class T>
+int ns_graph_setup(NsGraphWork<T> &w, crimson_lhs *l, crimson_ls *ls, const int *incL, const double *res,
+                   hipStream_t s) {
+  if (ls->LS_type != CRIMSON_LS_NS) {
+    crimson::set_error("the device-driven solve is the NS solver's (LS_TYPE_NS)");
+    return CRIMSON_EUNSUPPORTED;
+  }
+  Ctx<T> &c = w.c;
+  CR_TRY(c.init(l, 4));
+  c.s = s;
+  c.coupled = false;
+  for (int f = 0; f < l->nFaces; ++f) {
+    CrFace &fa = l->face[f];
+    fa.inc = !(incL && incL[f] == 0);
+    fa.coupled = false;
+    if (!fa.inc || !fa.created) continue;
+    if (fa.bGrp == CRIMSON_BC_NEU && res && res[f] != 0.0) {
+      fa.res = res[f];
+      fa.coupled = true;
+      c.coupled = true;
+    }
+  }
+  const int N = l->nNo;
+  const size_t mn = (size_t)3 * N;
+  w.mRI = ls->RI.mItr;
+  w.mGM = ls->GM.mItr;
+  w.mCG = ls->CG.mItr;
+  w.sD = ls->GM.sD;
+  w.nB = 2 * w.mRI;
+  w.riAbs = ls->RI.absTol;
+  w.riRel = ls->RI.relTol;
+  w.gmAbs = ls->GM.absTol;
+  w.gmRel = ls->GM.relTol;
+  w.cgAbs = ls->CG.absTol;
+  w.cgRel = ls->CG.relTol;
+  if (w.mRI < 1 || w.mGM < 1 || w.mCG < 1 || w.sD < 1 || w.nB > 255 || w.sD > 255) {
+    crimson::set_error("device-driven NS solve: iteration limits out of range (RI %d, GM %d x %d, CG %d)", w.mRI,
+                       w.mGM, w.sD, w.mCG);
+    return CRIMSON_EARG;
+  }
+  CR_TRY(w.Rm.alloc(mn));
+  CR_TRY(w.Rc.alloc(N));
+  CR_TRY(w.Rmi.alloc(mn));
+  CR_TRY(w.Rci.alloc(N));
+  CR_TRY(w.U.alloc(mn * w.mRI));
+  CR_TRY(w.P.alloc((size_t)N * w.mRI));
+  CR_TRY(w.MU.alloc(mn * w.nB));
+  CR_TRY(w.MP.alloc((size_t)N * w.nB));
+  CR_TRY(w.t1.alloc(mn));
+  CR_TRY(w.t2.alloc(N));
+  CR_TRY(w.t3.alloc(N));
+  CR_TRY(w.t4.alloc(N));
+  CR_TRY(w.gmTmp.alloc(mn));
+  CR_TRY(w.gmU.alloc(mn * (w.sD + 1)));
+  if (sizeof(T) != sizeof(double)) CR_TRY(w.Rw.alloc((size_t)4 * N));
+  size_t o = G_ARR;
+  w.oH = o;
+  o += (size_t)(w.sD + 1) * w.sD;
+  w.ocs = o;
+  o += w.sD;
+  w.osn = o;
+  o += w.sD;
+  w.oerr = o;
+  o += w.sD + 1;
+  w.oy = o;
+  o += w.sD;
+  w.oA = o;
+  o += (size_t)w.nB * w.nB;
+  w.oBv = o;
+  o += w.nB;
+  w.oxB = o;
+  o += w.nB;
+  w.oC = o;
+  o += (size_t)w.nB * (w.nB + 1);
+  w.ocu = o;
+  o += w.mRI;
+  w.ocp = o;
+  o += w.mRI;
+  CR_TRY(w.gst.alloc(o));
+  CR_TRY(w.fcoef.alloc(std::max<size_t>(1, l->face.size())));
+  CR_TRY(w.dots.alloc(2048));
+  if (!l->gt_is_d) CR_TRY(gt_buf(l, T(0)).alloc((size_t)3 * l->nnz));
+  c.hfdev.clear();
+  for (size_t f = 0; f < l->face.size(); ++f) {
+    CrFace &fa = l->face[f];
+    if (!fa.coupled) continue;
+    CR_TRY(fa.d_valM.alloc((size_t)fa.dof * fa.nNo));
+    CR_TRY(face_support(l, fa));
+    c.hfdev.push_back(FaceDev{fa.d_glob.p, fa.d_valM.p, 0.0, w.fcoef.p + f, fa.res, fa.nNo, fa.dof, fa.d_sup.p,
+                              fa.d_q.p, fa.nsup});
+  }
+  if (!c.hfdev.empty()) {
+    CR_TRY(c.fdev.alloc(c.hfdev.size()));
+    CR_HIP(hipMemcpyAsync(c.fdev.p, c.hfdev.data(), c.hfdev.size() * sizeof(FaceDev), hipMemcpyHostToDevice, s));
+  }
+  CR_TRY(w.msum.alloc(std::max<size_t>(1, c.hfdev.size())));
+  {
+    std::vector<char> seen(N, 0);
+    w.disjoint = true;
+    w.maxfn = 0;
+    for (auto &fa : l->face) {
+      if (!fa.coupled) continue;
+      w.maxfn = std::max(w.maxfn, fa.nNo);
+      for (int a = 0; a < fa.nNo; ++a) {
+        if (seen[fa.glob[a]]) w.disjoint = false;
+        seen[fa.glob[a]] = 1;
+      }
+    }
+  }
+  CR_HIP(hipStreamSynchronize(s));
+  return CRIMSON_OK;
+}
+
+// One memLS NS solve (solve_impl) enqueued device-driven on w.c.s: PRECOND,
+// R = W R, the solve, R W; d_Ri the FP64 (4, N) right-hand side in and
+// solution out.  No synchronisation, no allocation.
+template <class T>
+int ns_graph_enqueue(NsGraphWork<T> &w, double *d_Ri) {
+  Ctx<T> &c = w.c;
+  crimson_lhs *l = c.lhs;
+  const int N = l->nNo, dof = 4;
+  const size_t L = (size_t)dof * N;
+  const unsigned gN = crimson::grid_for(N, 256);
+  const T *V = planes(l, T(0));
+  precond_diag_kernel<T><<<gN, 256, 0, c.s>>>(N, dof, l->vdof, V, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
+  for (auto &fa : l->face)
+    if (fa.created && fa.inc && fa.bGrp == CRIMSON_BC_DIR && fa.nNo > 0)
+      precond_face_kernel<T><<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, dof, fa.dof, fa.d_glob.p,
+                                                                              fa.d_val.p, c.W.p);
+  precond_fix_kernel<T><<<gN, 256, 0, c.s>>>(N, dof, l->vdof, V, l->nnz, l->d_diag.p, c.W.p, c.fix.p);
+  for (auto &fa : l->face)
+    if (fa.coupled && fa.nNo > 0)
+      face_valm_kernel<T><<<crimson::grid_for(fa.nNo, 256), 256, 0, c.s>>>(fa.nNo, dof, fa.dof, fa.d_glob.p,
+                                                                            fa.d_val.p, c.W.p, fa.d_valM.p);
+  T *R = sizeof(T) == sizeof(double) ? reinterpret_cast<T *>(d_Ri) : w.Rw.p;
+  mul_kernel<double, T, T><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, d_Ri, c.W.p, R);
+  CR_HIP(hipGetLastError());
+  CR_TRY(nssolver_graph<T>(w, R));
+  mul_kernel<T, T, double><<<crimson::grid_for(L, 256), 256, 0, c.s>>>(L, R, c.W.p, d_Ri);  // R = R W
+  CR_HIP(hipGetLastError());
+  return CRIMSON_OK;
+}
+
+// The solve's statistics from a copy of the first G_ARR entries of gst
+// into ls's fields (as nssolver_dev leaves them); CRIMSON_ESINGULAR when GE
+// met a singular pivot.
+int ns_graph_stats(const double *h, crimson_ls *ls) {
+  if (h[G_ERR] != 0.0) {
+    crimson::set_error("memLS GE: singular matrix");
+    return CRIMSON_ESINGULAR;
+  }
+  ls->RI.iNorm = h[G_RI_INORM];
+  ls->RI.fNorm = h[G_RI_FNORM];
+  ls->RI.itr = (int)h[G_RI_ITR];
+  ls->RI.suc = h[G_RI_SUC] != 0.0;
+  ls->RI.dB = 10.0 * std::log(h[G_RI_FNORM] / h[G_RI_DBPRE]);
+  ls->RI.callD = 0.0;
+  ls->GM.iNorm = h[G_GM_INORM];
+  ls->GM.fNorm = h[G_GM_FNORM];
+  ls->GM.itr = (int)h[G_GM_ITR];
+  ls->GM.suc = h[G_GM_SUC] != 0.0;
+  ls->GM.dB = h[G_GM_LASTD] != 0.0 ? 10.0 * std::log(h[G_GM_LASTF] / h[G_GM_LASTD]) : 0.0;
+  ls->GM.callD = 0.0;
+  ls->CG.iNorm = h[G_CG_INORM];
+  ls->CG.fNorm = std::sqrt(h[G_CG_ERR]);
+  ls->CG.itr = (int)h[G_CG_ITR];
+  ls->CG.suc = h[G_CG_SUC] != 0.0;
+  ls->CG.dB = 5.0 * std::log(h[G_CG_ERR] / h[G_CG_ERRO]);
+  ls->CG.callD = 0.0;
+  const double rcn = std::sqrt(h[G_RC_N2]) / ls->RI.fNorm;
+  ls->Resc = (int)std::lround(100.0 * rcn * rcn);
+  ls->Resm = 100 - ls->Resc;
+  return CRIMSON_OK;
+}
+
+// The signature of what a device-driven solve baked in: the matrix and
+// face buffers it reads and the solver's settings.
+std::vector<unsigned long long> ns_signature(const crimson_lhs *l, const crimson_ls *ls) {
+  std::vector<unsigned long long> v = {l->d_V.serial, l->d_V32.serial, l->d_Gt.serial, l->d_Gt32.serial,
+                                       (unsigned long long)l->vdof, (unsigned long long)l->prec,
+                                       (unsigned long long)l->gt_is_d, (unsigned long long)ls->RI.mItr,
+                                       (unsigned long long)ls->GM.mItr, (unsigned long long)ls->GM.sD,
+                                       (unsigned long long)ls->CG.mItr};
+  for (const auto &fa : l->face) {
+    v.push_back(fa.d_glob.serial);
+    v.push_back(fa.d_val.serial);
+    v.push_back(fa.d_valM.serial);
+    v.push_back(fa.d_sup.serial);
+    v.push_back(fa.d_q.serial);
+    v.push_back((unsigned long long)fa.coupled);
+  }
+  return v;
+}
+
+// A graph holding device-driven solves: the workspaces, and after a replay
+// each captured solve's statistics (the last one's into ls).
+template <class T>
+struct NsGraphExtra : crimson::GraphExtra {
+  NsGraphWork<T> w;
+  crimson_lhs *lhs = nullptr;
+  crimson_ls *ls = nullptr;
+  int nsolve = 0;
+  crimson::DBuf<double> stat;  // G_ARR doubles per captured solve
+  std::vector<double> hstat;
+  std::vector<unsigned long long> form_sig_unused;
+  int after_launch() override {
+    hstat.resize((size_t)G_ARR * nsolve);
+    CR_HIP(hipMemcpy(hstat.data(), stat.p, hstat.size() * sizeof(double), hipMemcpyDeviceToHost));
+    int rc = CRIMSON_OK;
+    for (int k = 0; k < nsolve && rc == CRIMSON_OK; ++k) rc = ns_graph_stats(hstat.data() + (size_t)G_ARR * k, ls);
+    return rc;
+  }
+  std::vector<unsigned long long> signature() const override { return ns_signature(lhs, ls); }
+  int record_stats(int k) {
+    CR_HIP(hipMemcpyAsync(stat.p + (size_t)G_ARR * k, w.gst.p, (size_t)G_ARR * sizeof(double),
+                          hipMemcpyDeviceToDevice, w.c.s));
+    return CRIMSON_OK;
+  }
+};
 
 }  // namespace
 
@@ -1744,6 +2623,53 @@ extern "C" int crimson_lhs_get_val(crimson_lhs *l, int dof, double *Val) {
   return CRIMSON_OK;
 }
 
+// Block transpose: block k = (i, j) of dst is block (j, i) of src transposed,
+// B'_k(r, c) = B_tr(k)(c, r), every plane of every block.
+template <class T>
+__global__ void transpose_blocks_kernel(size_t nnz, int m, const int *__restrict__ tr, const T *__restrict__ V,
+                                        T *__restrict__ Vt) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const size_t t = (size_t)tr[k];
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < m; ++c) Vt[(size_t)(r * m + c) * nnz + k] = V[(size_t)(c * m + r) * nnz + t];
+}
+
+extern "C" int crimson_lhs_transpose_dev(crimson_lhs *dst, const crimson_lhs *src) {
+  if (!dst || !src || dst == src) {
+    crimson::set_error("crimson_lhs_transpose_dev: two distinct handles needed");
+    return CRIMSON_EARG;
+  }
+  if (dst->nNo != src->nNo || dst->nnz != src->nnz || dst->h_rowptr != src->h_rowptr || dst->h_col != src->h_col) {
+    crimson::set_error("crimson_lhs_transpose_dev: the handles' CSR graphs differ");
+    return CRIMSON_EARG;
+  }
+  const bool f32 = src->prec == 32;
+  if (src->vdof < 1 || !(f32 ? (const void *)src->d_V32.p : (const void *)src->d_V.p)) {
+    crimson::set_error("crimson_lhs_transpose_dev: src holds no matrix");
+    return CRIMSON_ESTATE;
+  }
+  const int m = src->vdof;
+  const size_t nnz = src->nnz, n = (size_t)m * m * nnz;
+  CR_HIP(hipStreamSynchronize(src->stream));
+  const unsigned G = crimson::grid_for(nnz, 256);
+  if (f32) {
+    CR_TRY(dst->d_V32.alloc(n));
+    transpose_blocks_kernel<float><<<G, 256, 0, dst->stream>>>(nnz, m, dst->d_tr.p, src->d_V32.p, dst->d_V32.p);
+  } else {
+    CR_TRY(dst->d_V.alloc(n));
+    transpose_blocks_kernel<double><<<G, 256, 0, dst->stream>>>(nnz, m, dst->d_tr.p, src->d_V.p, dst->d_V.p);
+  }
+  CR_HIP(hipGetLastError());
+  CR_HIP(hipStreamSynchronize(dst->stream));
+  dst->vdof = m;
+  dst->prec = src->prec;
+  // a formation-produced matrix has D = -G^T blockwise; its transpose keeps
+  // that relation (D'_ij = G_ji^T = -D_ij = -G'_ji^T)
+  dst->gt_is_d = src->gt_is_d;
+  return CRIMSON_OK;
+}
+
 extern "C" int crimson_ls_solve_dev(crimson_lhs *l, crimson_ls *ls, int dof, double *d_Ri, const int *incL,
                                     const double *res) {
   if (!l || !ls || !d_Ri) {
@@ -1826,4 +2752,230 @@ extern "C" int crimson_lhs_spmv_dev(crimson_lhs *l, int dof, const double *d_x, 
   CR_HIP(hipGetLastError());
   CR_HIP(hipStreamSynchronize(l->stream));
   return CRIMSON_OK;
+}
+
+namespace {
+// Launches per captured segment at most: a device-driven solve enqueues
+// every possible iteration, so its launch count grows with its iteration
+// limits; the capture is cut into segments (each GMRES call, each CG call,
+// the glue between them) and a configuration whose segment would exceed
+// this is refused before it is captured.
+constexpr size_t kMaxSegmentLaunches = 40000;
+
+// A stream capture cut into segments (crimson_graph::seg), each
+// instantiated on its own and launched back to back.
+struct SegCapture {
+  crimson_graph *g;
+  hipStream_t s;
+  bool open = false;
+  int begin() {
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      crimson::set_error("hipStreamBeginCapture failed");
+      return CRIMSON_EHIP;
+    }
+    open = true;
+    return CRIMSON_OK;
+  }
+  int close() {
+    hipGraph_t gr = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &gr);
+    open = false;
+    if (ec != hipSuccess || !gr) {
+      crimson::set_error("capture into a HIP graph failed");
+      return CRIMSON_EHIP;
+    }
+    size_t n = 0;
+    if (hipGraphGetNodes(gr, nullptr, &n) != hipSuccess) {
+      (void)hipGraphDestroy(gr);
+      crimson::set_error("hipGraphGetNodes failed");
+      return CRIMSON_EHIP;
+    }
+    if (n == 0) {
+      (void)hipGraphDestroy(gr);
+      return CRIMSON_OK;
+    }
+    g->seg.push_back(gr);
+    hipGraphExec_t x = nullptr;
+    if (hipGraphInstantiate(&x, gr, nullptr, nullptr, 0) != hipSuccess) {
+      crimson::set_error("hipGraphInstantiate failed");
+      return CRIMSON_EHIP;
+    }
+    g->segx.push_back(x);
+    return CRIMSON_OK;
+  }
+  int cut() {
+    CR_TRY(close());
+    return begin();
+  }
+  void abort() {
+    if (!open) return;
+    hipGraph_t gr = nullptr;
+    (void)hipStreamEndCapture(s, &gr);
+    if (gr) (void)hipGraphDestroy(gr);
+    open = false;
+  }
+};
+
+// The launches one segment of a device-driven solve holds at most, from the
+// solver's limits (GMRES: per Arnoldi step about 16 launches, every coupled
+// face batched or not; CG: about 8 per iteration), checked before capturing.
+int check_segment_budget(int mGM, int sD, int mCG, int nfc_launch) {
+  const size_t gm = (size_t)mGM * ((size_t)sD * (16 + 4 * nfc_launch) + 16);
+  const size_t cg = (size_t)mCG * (8 + nfc_launch) + 8;
+  if (gm > kMaxSegmentLaunches || cg > kMaxSegmentLaunches) {
+    crimson::set_error("device-driven NS solve: a GMRES call of %d cycles x %d steps (%zu launches) or a CG call of "
+                       "%d iterations (%zu launches) exceeds the %zu launches a captured segment may hold",
+                       mGM, sD, gm, mCG, cg, kMaxSegmentLaunches);
+    return CRIMSON_EUNSUPPORTED;
+  }
+  return CRIMSON_OK;
+}
+
+template <class T>
+int NsGraphWork<T>::cut() {
+  return cap ? cap->cut() : CRIMSON_OK;
+}
+
+template <class T>
+int capture_into(crimson_graph *g, hipStream_t s, NsGraphWork<T> &w, const std::function<int()> &body) {
+  SegCapture cap{g, s};
+  const int nfc = (int)w.c.hfdev.size();
+  CR_TRY(check_segment_budget(w.mGM, w.sD, w.mCG, (w.disjoint && nfc > 1) ? 1 : nfc));
+  CR_TRY(cap.begin());
+  w.cap = &cap;
+  int rc = body();
+  if (!rc) rc = cap.close();
+  w.cap = nullptr;
+  if (rc) cap.abort();
+  g->s = s;
+  return rc;
+}
+
+void graph_discard(crimson_graph *g) {
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->g) (void)hipGraphDestroy(g->g);
+  for (hipGraphExec_t x : g->segx) (void)hipGraphExecDestroy(x);
+  for (hipGraph_t h : g->seg) (void)hipGraphDestroy(h);
+  delete g->extra;
+  delete g;
+}
+
+template <class T>
+int ls_graph_create_t(crimson_lhs *l, crimson_ls *ls, double *d_Ri, const int *incL, const double *res,
+                      crimson_graph **out) {
+  crimson_graph *g = new crimson_graph();
+  auto *ex = new NsGraphExtra<T>();
+  g->extra = ex;
+  ex->lhs = l;
+  ex->ls = ls;
+  ex->nsolve = 1;
+  int rc = ns_graph_setup<T>(ex->w, l, ls, incL, res, l->stream);
+  if (!rc) rc = ex->stat.alloc(G_ARR);
+  if (!rc)
+    rc = capture_into<T>(g, l->stream, ex->w, [&]() -> int {
+      CR_TRY(ns_graph_enqueue<T>(ex->w, d_Ri));
+      return ex->record_stats(0);
+    });
+  if (rc) {
+    graph_discard(g);
+    return rc;
+  }
+  g->lhs = l;
+  g->sig = ex->signature();
+  *out = g;
+  return CRIMSON_OK;
+}
+
+template <class T>
+int step_graph_create_t(crimson_form *f, crimson_lhs *l, crimson_ls *ls, double *d_yold, double *d_acold,
+                        double *d_y, double *d_ac, double *d_ya, double *d_aca, const double *d_x, double *d_Res4,
+                        const int *d_iBC, const double *d_BC, int ndofBC, int nsurf, const int *surf_ptr,
+                        const int *d_glob, const double *d_nabi, const double *c1, const double *c2, double *d_q,
+                        const double *faceRes, int nonlinear, crimson_graph **out) {
+  crimson_graph *g = new crimson_graph();
+  auto *ex = new NsGraphExtra<T>();
+  g->extra = ex;
+  ex->lhs = l;
+  ex->ls = ls;
+  ex->nsolve = nonlinear;
+  hipStream_t s = crimson::form_stream(f);
+  const int N = crimson::form_nshg(f);
+  int rc = crimson::form_prepare_to_lhs(f, d_x, l);
+  if (!rc) rc = ns_graph_setup<T>(ex->w, l, ls, nullptr, faceRes, s);
+  if (!rc) rc = ex->stat.alloc((size_t)G_ARR * nonlinear);
+  const crimson_form_params prm = crimson::form_params_of(f);
+  if (!rc)
+    rc = capture_into<T>(g, s, ex->w, [&]() -> int {
+      CR_TRY(crimson_itr_predict_dev(f, d_yold, d_acold, d_y, d_ac, prm.gami));
+      CR_TRY(crimson_itr_bc_dev(f, d_iBC, d_BC, ndofBC, d_y));
+      for (int k = 0; k < nonlinear; ++k) {
+        CR_TRY(crimson_itr_yalpha_dev(f, d_yold, d_acold, d_y, d_ac, d_ya, d_aca, prm.alfi, prm.almi));
+        CR_TRY(crimson::form_enqueue_to_lhs(f, d_ya, d_aca, d_x, d_Res4, l));
+        CR_TRY(crimson::form_enqueue_bc3res(f, d_Res4, 1));
+        for (int q = 0; q < nsurf; ++q)
+          CR_TRY(crimson_elmpvsq_coef_dev(f, surf_ptr[q + 1] - surf_ptr[q], d_glob + surf_ptr[q],
+                                          d_nabi + (size_t)3 * surf_ptr[q], c1[q], c2[q], d_ya, d_Res4, d_q + q));
+        CR_TRY(ns_graph_enqueue<T>(ex->w, d_Res4));
+        CR_TRY(ex->record_stats(k));
+        CR_TRY(crimson_itr_correct_dev(f, d_y, d_ac, d_Res4, prm.gami, prm.alfi, prm.Delt));
+        CR_TRY(crimson_itr_bc_dev(f, d_iBC, d_BC, ndofBC, d_y));
+      }
+      CR_HIP(hipMemcpyAsync(d_yold, d_y, (size_t)4 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      CR_HIP(hipMemcpyAsync(d_acold, d_ac, (size_t)4 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      return CRIMSON_OK;
+    });
+  if (rc) {
+    graph_discard(g);
+    return rc;
+  }
+  g->f = f;
+  g->lhs = l;
+  g->sig = crimson::graph_signature(f, l);
+  const std::vector<unsigned long long> e = ex->signature();
+  g->sig.insert(g->sig.end(), e.begin(), e.end());
+  *out = g;
+  return CRIMSON_OK;
+}
+}  // namespace
+
+extern "C" int crimson_ls_graph_create(crimson_lhs *lhs, crimson_ls *ls, int dof, double *d_Ri, const int *incL,
+                                       const double *res, crimson_graph **out) {
+  if (!lhs || !ls || !d_Ri || !out || dof != 4) {
+    crimson::set_error("crimson_ls_graph_create: invalid arguments (the NS solver, dof 4)");
+    return CRIMSON_EARG;
+  }
+  if (lhs->vdof != 4 || !(lhs->prec == 32 ? (const void *)lhs->d_V32.p : (const void *)lhs->d_V.p)) {
+    crimson::set_error("crimson_ls_graph_create: lhs holds no 4x4 matrix");
+    return CRIMSON_ESTATE;
+  }
+  if (lhs->prec == 32) return ls_graph_create_t<float>(lhs, ls, d_Ri, incL, res, out);
+  return ls_graph_create_t<double>(lhs, ls, d_Ri, incL, res, out);
+}
+
+extern "C" int crimson_step_graph_create(crimson_form *f, crimson_lhs *lhs, crimson_ls *ls, double *d_yold,
+                                         double *d_acold, double *d_y, double *d_ac, double *d_yAlpha,
+                                         double *d_acAlpha, const double *d_x, double *d_Res4, const int *d_iBC,
+                                         const double *d_BC, int ndofBC, int nsurf, const int *surf_ptr,
+                                         const int *d_glob, const double *d_nabi, const double *c1,
+                                         const double *c2, double *d_q, const double *faceRes, int nonlinear,
+                                         crimson_graph **out) {
+  if (!f || !lhs || !ls || !d_yold || !d_acold || !d_y || !d_ac || !d_yAlpha || !d_acAlpha || !d_x || !d_Res4 ||
+      !d_iBC || !d_BC || ndofBC < 6 || nsurf < 0 || (nsurf > 0 && (!surf_ptr || !d_glob || !d_nabi || !c1 || !c2 ||
+                                                                    !d_q)) ||
+      nonlinear < 1 || !out) {
+    crimson::set_error("crimson_step_graph_create: invalid arguments");
+    return CRIMSON_EARG;
+  }
+  for (int q = 0; q < nsurf; ++q)
+    if (surf_ptr[q + 1] < surf_ptr[q]) {
+      crimson::set_error("crimson_step_graph_create: surf_ptr not monotone");
+      return CRIMSON_EARG;
+    }
+  if (crimson::form_params_of(f).precision == 32)
+    return step_graph_create_t<float>(f, lhs, ls, d_yold, d_acold, d_y, d_ac, d_yAlpha, d_acAlpha, d_x, d_Res4,
+                                      d_iBC, d_BC, ndofBC, nsurf, surf_ptr, d_glob, d_nabi, c1, c2, d_q, faceRes,
+                                      nonlinear, out);
+  return step_graph_create_t<double>(f, lhs, ls, d_yold, d_acold, d_y, d_ac, d_yAlpha, d_acAlpha, d_x, d_Res4,
+                                     d_iBC, d_BC, ndofBC, nsurf, surf_ptr, d_glob, d_nabi, c1, c2, d_q, faceRes,
+                                     nonlinear, out);
 }

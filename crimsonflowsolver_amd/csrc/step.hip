@@ -131,6 +131,52 @@ __global__ void pvsq_kernel(int nf, const int *__restrict__ glob, const double *
   for (int c = 0; c < 3; ++c) res4[4 * g + c] = res4[4 * g + c] + p * nabi[3 * (size_t)a + c];
 }
 
+// Adjoint of the coupled-surface term for a cotangent w4 laid out like
+// Res4(4,nshg): s = sum_a w4(1:3, glob a) . NABI(a,:) and Q = sum NABI . u in
+// one workgroup (fixed order), then cbar = (-Q s, -s), the cotangents of c1
+// and c2.
+__global__ void __launch_bounds__(256) pvsq_vjp_dot_kernel(int nf, const int *__restrict__ glob,
+                                                           const double *__restrict__ nabi,
+                                                           const double *__restrict__ y, int N,
+                                                           const double *__restrict__ w4,
+                                                           double *__restrict__ cbar) {
+  __shared__ double sh[2][4];
+  double s = 0.0, q = 0.0;
+  for (int a = threadIdx.x; a < nf; a += 256) {
+    const size_t g = glob[a];
+    const double *nb = nabi + 3 * (size_t)a;
+    s += nb[0] * w4[4 * g] + nb[1] * w4[4 * g + 1] + nb[2] * w4[4 * g + 2];
+    q += nb[0] * y[g] + nb[1] * y[g + (size_t)N] + nb[2] * y[g + 2 * (size_t)N];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off, 64);
+    q += __shfl_xor(q, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x / 64] = s;
+    sh[1][threadIdx.x / 64] = q;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double ss = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+    const double qq = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+    cbar[0] = -qq * ss;
+    cbar[1] = -ss;
+  }
+}
+
+// ybar(glob a, 1:3) += c1 cbar[1] NABI(a,:), ybar (nshg,4) column-major
+__global__ void pvsq_vjp_axpy_kernel(int nf, const int *__restrict__ glob, const double *__restrict__ nabi, int N,
+                                     double c1, const double *__restrict__ cbar, double *__restrict__ ybar) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nf) return;
+  const double f = c1 * cbar[1];
+  const size_t g = glob[a];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) ybar[g + c * (size_t)N] = ybar[g + c * (size_t)N] + f * nabi[3 * (size_t)a + c];
+}
+
 }  // namespace
 
 extern "C" int crimson_itr_predict_dev(crimson_form *f, const double *d_yold, const double *d_acold, double *d_y,
@@ -220,5 +266,22 @@ extern "C" int crimson_form_sync(crimson_form *f) {
     return CRIMSON_EARG;
   }
   CR_HIP(hipStreamSynchronize(form_stream(f)));
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_elmpvsq_vjp_dev(crimson_form *f, int nface, const int *d_glob, const double *d_nabi,
+                                       double c1, const double *d_y, const double *d_w4, double *d_ybar,
+                                       double *d_cbar) {
+  if (!f || nface < 0 || (nface > 0 && (!d_glob || !d_nabi)) || !d_y || !d_w4 || !d_ybar || !d_cbar) {
+    crimson::set_error("crimson_elmpvsq_vjp_dev: invalid arguments");
+    return CRIMSON_EARG;
+  }
+  const int N = form_nshg(f);
+  hipStream_t s = form_stream(f);
+  pvsq_vjp_dot_kernel<<<1, 256, 0, s>>>(nface, d_glob, d_nabi, d_y, N, d_w4, d_cbar);
+  if (nface > 0)
+    pvsq_vjp_axpy_kernel<<<crimson::grid_for(nface, 256), 256, 0, s>>>(nface, d_glob, d_nabi, N, c1, d_cbar,
+                                                                         d_ybar);
+  CR_HIP(hipGetLastError());
   return CRIMSON_OK;
 }

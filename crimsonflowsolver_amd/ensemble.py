@@ -22,6 +22,26 @@ from . import ElmGMRInterior, form_params, genadj, memLS_BC_CREATE, memLS_LHS_CR
 from . import mesh as M
 
 
+def uneven_cylinders(K=48, seed=2024):
+    """The batched-ensemble workload: 8 cylinders (radius 1) of seeded
+    lengths uniform in 4-20 cm and element counts uniform in 40k-600k tets
+    (the hex size h follows from the count: 6 tets per hex, pi/4 of the box
+    kept by the clip), and K instances dealt to them in turn with viscosity
+    lognormal(ln 0.04, 0.2) and outlet resistance uniform 800-2000.
+    Returns (meshes of the instances, the 8 distinct meshes, mu(K), R(K))."""
+    rng = np.random.default_rng(seed)
+    shapes = []
+    for _ in range(8):
+        Lc, Tc = rng.uniform(4.0, 20.0), rng.uniform(4e4, 6e5)
+        h = (18.85 * Lc / Tc) ** (1.0 / 3.0)
+        n = max(4, round(2.0 / h))
+        shapes.append(M.cylinder_mesh(n, n, max(4, round(Lc / h)), length=Lc))
+    meshes = [shapes[k % 8] for k in range(K)]
+    mu = np.exp(rng.normal(np.log(0.04), 0.2, K))
+    R = rng.uniform(800.0, 2000.0, K)
+    return meshes, shapes, mu, R
+
+
 class Ensemble:
     def __init__(self, meshes, mu, R, precision=64, params=None, outlet_natural=True, boundary=False):
         self.meshes = list(meshes)

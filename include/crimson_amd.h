@@ -64,6 +64,10 @@ typedef struct {
   double aspect_fp64; /* aspect-ratio threshold of the FP64 tau island:
                          longest edge over shortest altitude, 1 for the
                          regular tet; used when precision == 32 (>= 1) */
+  int ideformwall;    /* "Deformable Wall": only 0 (rigid walls; vessel-wall
+                         faces, iBCB bit 4, then carry no membrane terms, as
+                         in the reference); 1 (the e3b.f90:488-562 membrane
+                         terms) gives CRIMSON_EUNSUPPORTED */
 } crimson_form_params;
 
 typedef struct crimson_form crimson_form;
@@ -99,7 +103,8 @@ int crimson_form_interior_dev(crimson_form *f, const double *d_y,
  * -G blocks taken from the same element contributions instead of the
  * transpose search) and the residual as d_Res4 (4,nshg), the transposed
  * copy SolFlow passes to memLS_SOLVE (solfar.f90:206-210). */
-struct crimson_lhs;
+typedef struct crimson_lhs crimson_lhs;
+typedef struct crimson_ls crimson_ls;
 int crimson_form_interior_to_lhs(crimson_form *f, const double *d_y,
                                  const double *d_ac, const double *d_x,
                                  double *d_Res4, struct crimson_lhs *lhs);
@@ -227,6 +232,43 @@ int crimson_form_graph_create(crimson_form *f, const double *d_y,
 int crimson_graph_launch(crimson_graph *g);
 int crimson_graph_free(crimson_graph *g);
 
+/* The NS solve of crimson_ls_solve_dev (LS_TYPE_NS, dof 4) captured into a
+ * HIP graph on lhs's matrix and the device array d_Ri (solved in place at
+ * each crimson_graph_launch, which then leaves the statistics in ls).  The
+ * solve is device-driven: every decision the host takes in memLS's NSSOLVER
+ * (memLS.f90:1460-1759: GMRES's Givens rotations and convergence test,
+ * CGRAD's iteration count, the least-squares Gram system and its GE, BCPRE)
+ * is taken by the GPU, every possible iteration is enqueued and those past
+ * convergence return at once; the results equal crimson_ls_solve_dev's.
+ * res and incL as in crimson_ls_solve_dev, fixed at capture.  No
+ * counterpart in the reference.  The matrix must be held at capture (a later
+ * formation into lhs refills the same buffer). */
+int crimson_ls_graph_create(struct crimson_lhs *lhs, struct crimson_ls *ls, int dof,
+                            double *d_Ri, const int *incL, const double *res,
+                            crimson_graph **out);
+
+/* One generalized-alpha timestep (itrdrv.f90:1284-1528, without its host
+ * side) captured into a HIP graph: itrPredict (ipred 1), itrBC, then
+ * `nonlinear` times itrYAlpha, the formation into lhs's matrix (interior and
+ * boundary elements), bc3Res, the ElmpvsQ term of each coupled surface
+ * (surface q: nodes d_glob[surf_ptr[q]..surf_ptr[q+1]) 0-based with NABI
+ * rows d_nabi(3, .), implicit coefficients c1[q], c2[q], flow into
+ * d_q[q]), the device-driven NS solve of crimson_ls_graph_create (faceRes:
+ * memLS's face coefficients), itrCorrect and itrBC; then yold, acold <- y,
+ * ac.  Each crimson_graph_launch advances one timestep; d_BC may be rewritten
+ * between launches (time-dependent Dirichlet values).  For ensembles (the
+ * union mesh of crimson_form_set_instances with one coupled surface per
+ * instance) the whole timestep of every instance is one launch.  No
+ * counterpart in the reference. */
+int crimson_step_graph_create(crimson_form *f, struct crimson_lhs *lhs, struct crimson_ls *ls,
+                              double *d_yold, double *d_acold, double *d_y, double *d_ac,
+                              double *d_yAlpha, double *d_acAlpha, const double *d_x,
+                              double *d_Res4, const int *d_iBC, const double *d_BC,
+                              int ndofBC, int nsurf, const int *surf_ptr, const int *d_glob,
+                              const double *d_nabi, const double *c1, const double *c2,
+                              double *d_q, const double *faceRes, int nonlinear,
+                              crimson_graph **out);
+
 /* Adjoint of the interior residual (the vector-Jacobian product the
  * estimation subsystem needs for parameter fitting; no counterpart in the
  * reference, which has no derivative code): for a cotangent d_w(nshg,4)
@@ -238,6 +280,16 @@ int crimson_graph_free(crimson_graph *g);
 int crimson_form_vjp_dev(crimson_form *f, const double *d_y, const double *d_ac,
                          const double *d_x, const double *d_w, double *d_ybar,
                          double *mubar);
+/* crimson_form_vjp_dev with the acceleration's cotangent as well:
+ * d_acbar(nshg,4) = w^T d res/d ac (its fourth column zero; the acceleration
+ * enters the momentum residual and the strong residual L, e3res.f90:775-789).
+ * The implicit step's adjoint needs it: the tangent of one generalized-alpha
+ * step with respect to its acceleration increment is almi d res/d ac +
+ * alfi gami Delt d res/d y (itrPC.f90:176-181, 326-334).  No counterpart in
+ * the reference. */
+int crimson_form_vjp_ac_dev(crimson_form *f, const double *d_y, const double *d_ac,
+                            const double *d_x, const double *d_w, double *d_ybar,
+                            double *d_acbar, double *mubar);
 int crimson_form_free(crimson_form *f);
 
 /* Native genadj (genadj.f90:1-84): node-adjacency CSR including self, rows
@@ -283,6 +335,15 @@ int crimson_elmpvsq_dev(crimson_form *f, int nface, const int *d_glob,
 int crimson_elmpvsq_coef_dev(crimson_form *f, int nface, const int *d_glob,
                              const double *d_nabi, double c1, double c2,
                              const double *d_y, double *d_Res4, double *d_q);
+/* Adjoint of crimson_elmpvsq_coef_dev's residual term for a cotangent
+ * d_w4 laid out like Res4(4,nshg): with s = sum over d_glob of w(1:3).NABI
+ * and Q = sum NABI.u(d_y), d_ybar(nshg,4) (column-major) gets -c1 s NABI
+ * added on the face nodes' velocity components, and d_cbar(2) receives the
+ * cotangents (-Q s, -s) of c1 and c2 (of R and of a constant pressure for a
+ * resistance).  No counterpart in the reference. */
+int crimson_elmpvsq_vjp_dev(crimson_form *f, int nface, const int *d_glob,
+                            const double *d_nabi, double c1, const double *d_y,
+                            const double *d_w4, double *d_ybar, double *d_cbar);
 int crimson_form_sync(crimson_form *f);
 
 /* ------------------------------------------------------------------ */
@@ -291,8 +352,6 @@ int crimson_form_sync(crimson_form *f);
 enum { CRIMSON_LS_CG = 1, CRIMSON_LS_GMRES = 2, CRIMSON_LS_NS = 3 };
 enum { CRIMSON_BC_DIR = 0, CRIMSON_BC_NEU = 1 };
 
-typedef struct crimson_lhs crimson_lhs;
-typedef struct crimson_ls crimson_ls;
 
 /* memLS_LS_CREATE (memLS.f90:1379-1442).  `present` marks the optional
  * arguments given: 1 relTol, 2 absTol, 4 maxItr, 8 dimKry, 16 relTolIn,
@@ -344,6 +403,14 @@ int crimson_lhs_get_val(crimson_lhs *lhs, int dof, double *Val);
  * and out.  crimson_form_interior_to_lhs with a precision-32 formation
  * handle leaves an FP32 matrix directly. */
 int crimson_lhs_set_precision(crimson_lhs *lhs, int bits);
+/* The block transpose of src's matrix into dst (both created on the same
+ * CSR graph): block (i,j) of dst becomes block (j,i) of src transposed, at
+ * src's precision, so that crimson_ls_solve_dev on dst solves with the
+ * transposed memLS matrix (the adjoint's transpose solve; memLS has no
+ * transpose mode).  dst's faces are its own; the PRECOND scaling, the
+ * Dirichlet masks and the rank-one face terms are symmetric.  Waits for
+ * src's stream; synchronous. */
+int crimson_lhs_transpose_dev(crimson_lhs *dst, const crimson_lhs *src);
 
 /* Stats of the last solve: sub 0 = RI, 1 = GM, 2 = CG; out[0] iNorm,
  * out[1] fNorm, out[2] itr, out[3] suc, out[4] dB, out[5] callD (s),

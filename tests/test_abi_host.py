@@ -108,7 +108,8 @@ def _small_graph():
 
 
 @pytest.mark.parametrize("field,value,code", [("itau", 1, 2), ("iconvflow", 1, 2), ("idiff", 2, 2),
-                                              ("ipord", 2, 2), ("precision", 16, 2), ("rho", -1.0, 1)])
+                                              ("ipord", 2, 2), ("precision", 16, 2), ("rho", -1.0, 1),
+                                              ("ideformwall", 1, 2)])
 def test_form_setup_rejects_unsupported_flags(cfa, field, value, code):
     m, colm, rowp = _small_graph()
     prm = cfa.form_params()
@@ -201,3 +202,18 @@ def test_unstructured_mesh_exercises_long_rows(cfa):
     x = m.x[m.ien - 1]
     V = np.einsum("ei,ei->e", x[:, 0] - x[:, 3], np.cross(x[:, 1] - x[:, 3], x[:, 2] - x[:, 3]))
     assert (V > 0).all()
+
+
+def test_header_compiles_as_c99_and_cxx():
+    """include/crimson_amd.h is usable from C (the Fortran host's ISO_C_BINDING
+    interfaces mirror it) and from C++."""
+    import shutil
+    import subprocess
+    hdr = os.path.join(ROOT, "include", "crimson_amd.h")
+    for comp, std, lang in (("gcc", "-std=c99", "c"), ("g++", "-std=c++17", "c++")):
+        exe = shutil.which(comp)
+        if not exe:
+            pytest.skip(f"{comp} not available")
+        r = subprocess.run([exe, "-fsyntax-only", std, "-Wall", "-Werror", "-x", lang, hdr], capture_output=True,
+                           text=True)
+        assert r.returncode == 0, r.stderr
