@@ -2048,20 +2048,16 @@ int cfo_ls_solve(cfo_lhs *l, cfo_ls *ls, int dof, double *Ri, double *Val,
 }
 
 double cfo_flops_per_element(void) {
-  /* Hand count of the restatement above, per element (+,-,*,/,sqrt = 1):
-   *   e3q pass, per qp: metric 98, velocity gradients 72, qdi 21,
-   *     ql/rmassl 4*(9*3+2) = 116  -> 307, x4 qp               = 1228
-   *   flux scatter 40 adds, qpbc amortized ~1 node/5.8 elems    ~   46
-   *   e3, per qp: metric 98, interpolation p,u,a 56, gradients 96,
-   *     divq 72, strong residual 30, gijd 54, tauM/tauC 42,
-   *     tauBar+uBar 26, rNa/rGNa 82, rl 4*(4+3*8)+... = 128,
-   *     LHS scaling 10, mass/convection 4*(5+4*4) = 84,
-   *     K symmetric terms 4*(18+... ) = 474, G 4*(3+12) = 60,
-   *     C 1 + 4*3 + 10*6 = 73                                 -> 1385, x4
-   *                                                          = 5540
-   *   residual scatter 16 adds, fillsparseI 16*13 = 208 adds  =  224
-   */
-  return 1228.0 + 46.0 + 5540.0 + 224.0;
+  /* Counted by the counting build (flopcount.hpp; +, -, *, /, sqrt = 1 each,
+   * as written, no FMA) and pinned by the oracle tests: per element
+   *   e3 (4 qp: metric, e3ivar, e3stab, e3Res, e3LHS)      7208
+   *   e3q (4 qp: metric, grad u, qdi, ql/rmassl)            1524
+   *   flux scatter (9 q + 1 mass) x 4 nodes                   40
+   *   residual scatter 4 x 4                                  16
+   *   fillsparseI 16 blocks x (9 + 4)                        208
+   * = 8996; besides, 10 per node (qpbc's reciprocal and 9 products) and 25
+   * per block of up to 255 elements (the shape tables of e3 and e3q). */
+  return 8996.0;
 }
 
 /* ------------------------------------------------------------ boundary

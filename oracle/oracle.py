@@ -56,6 +56,42 @@ def lib_mt(threads: int = 0):
     return _lib_mt
 
 
+_LIB_COUNT = os.path.join(_HERE, "liboracle_count.so")
+_lib_count = None
+
+
+def lib_count():
+    """The counting build (flopcount.hpp: every +, -, *, / and sqrt of the
+    restatement counted, the values unchanged)."""
+    global _lib_count
+    if _lib_count is None:
+        if not os.path.exists(_LIB_COUNT):
+            subprocess.run(["make", "-s", "-C", _HERE, _LIB_COUNT], check=True)
+        L = _bind(C.CDLL(_LIB_COUNT))
+        L.cfo_flop_count.restype = C.c_ulonglong
+        L.cfo_flop_count.argtypes = [C.c_int]
+        _lib_count = L
+    return _lib_count
+
+
+def count_interior_flops(ien, y, ac, x, colm, rowp, params):
+    """Floating-point operations of one interior formation (flux pass, e3,
+    scatter, fillsparseI) of the restatement on this mesh, counted by the
+    counting build; returns (count, res, lhsK, lhsP)."""
+    L = lib_count()
+    ien = _f(ien, np.int32)
+    y, ac, x = _f(y), _f(ac), _f(x)
+    nshg, nnz = y.shape[0], rowp.shape[0]
+    res = np.zeros((nshg, 4), order="F")
+    lhsK = np.zeros((9, nnz), order="F")
+    lhsP = np.zeros((4, nnz), order="F")
+    L.cfo_flop_count(1)
+    rc = L.cfo_elmgmr_interior(nshg, x.shape[0], ien.shape[0], ien, y, ac, x, None, _f(colm, np.int32),
+                               _f(rowp, np.int32), nnz, res, _ptr(lhsK), _ptr(lhsP), None, C.byref(params))
+    _check(rc, "elmgmr_interior (counting build)")
+    return int(L.cfo_flop_count(1)), res, lhsK, lhsP
+
+
 def _bind(L):
     if True:
         dp = np.ctypeslib.ndpointer(np.float64, flags="F_CONTIGUOUS")
