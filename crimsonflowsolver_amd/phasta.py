@@ -22,10 +22,11 @@ _HDR = re.compile(r"(.*?):\s*<\s*(\d+)\s*>(.*)")
 
 
 def read_blocks(path):
-    """{keyword: (header ints, raw bytes without the trailing newline)} in
-    file order, and the byte order ('<' or '>')."""
+    """[(keyword, header ints, raw bytes without the trailing newline)] in
+    file order (a keyword repeats once per element block), and the byte
+    order ('<' or '>')."""
     data = open(path, "rb").read()
-    pos, blocks = 0, {}
+    pos, blocks = 0, []
     while pos < len(data):
         nl = data.find(b"\n", pos)
         if nl < 0:
@@ -42,11 +43,12 @@ def read_blocks(path):
         raw = data[pos:pos + nb]
         if nb and raw[-1:] != b"\n":
             raise ValueError(f"{path}: block {key!r} does not end with a newline")
-        blocks[key] = (ints, raw[:-1] if nb else b"")
+        blocks.append((key, ints, raw[:-1] if nb else b""))
         pos += nb
     order = "<"
-    if "byteorder magic number" in blocks:
-        raw = blocks["byteorder magic number"][1][:4]
+    magic = _find(blocks, "byteorder magic number")
+    if magic is not None:
+        raw = magic[2][:4]
         if np.frombuffer(raw, "<i4")[0] == MAGIC:
             order = "<"
         elif np.frombuffer(raw, ">i4")[0] == MAGIC:
@@ -56,8 +58,16 @@ def read_blocks(path):
     return blocks, order
 
 
-def _arr(blocks, order, key, dtype, shape):
-    ints, raw = blocks[key]
+def _find(blocks, key):
+    """The first block named `key`, or None."""
+    for b in blocks:
+        if b[0] == key:
+            return b
+    return None
+
+
+def _arr(block, order, dtype, shape):
+    key, ints, raw = block
     a = np.frombuffer(raw, order + dtype)
     n = int(np.prod(shape))
     if a.size < n:
@@ -66,41 +76,60 @@ def _arr(blocks, order, key, dtype, shape):
 
 
 def read_geombc(path):
-    """Mesh and boundary data of one part: x(numnp,3), ien(nel,4) 1-based
-    (interior blocks concatenated), ienb(nelb,4), iBCB(nelb,2), BCB(nelb,
-    nshlb, ndBCB), iBC(nshg) and BC(nshg, ndofBC) expanded through the bc
-    mapping array (zero rows where no code is set)."""
+    """Mesh and boundary data of one part: x(numnp,3), ien(nel,4) 1-based,
+    ienb(nelb,4), iBCB(nelb,2), BCB(nelb, nshlb, ndBCB), each with its element
+    blocks concatenated in file order (readnblk.f90:98-408 reads one block
+    per topology; the boundary blocks' connectivity, codes and values come
+    in the same order and must agree in length), and iBC(nshg) and BC(nshg,
+    ndofBC) expanded through the bc mapping array (zero rows where no code
+    is set)."""
     b, o = read_blocks(path)
-    numnp = b["number of nodes"][0][0]
-    x = _arr(b, o, "co-ordinates", "f8", (numnp, 3))
-    ien = [_arr(b, o, k, "i4", (v[0][0], v[0][1])) for k, v in b.items() if k.startswith("connectivity interior")]
-    ienb = [_arr(b, o, k, "i4", (v[0][0], v[0][1])) for k, v in b.items() if k.startswith("connectivity boundary")]
+    numnp = _find(b, "number of nodes")[1][0]
+    x = _arr(_find(b, "co-ordinates"), o, "f8", (numnp, 3))
+
+    def blocks_of(prefix):
+        return [blk for blk in b if blk[0].startswith(prefix)]
+
+    ien = [_arr(blk, o, "i4", (blk[1][0], blk[1][1])) for blk in blocks_of("connectivity interior")]
+    bcon = blocks_of("connectivity boundary")
+    ienb = [_arr(blk, o, "i4", (blk[1][0], blk[1][1])) for blk in bcon]
     out = {"x": x, "numnp": numnp,
            "ien": np.asfortranarray(np.concatenate(ien)) if ien else np.zeros((0, 4), np.int32, order="F"),
            "ienb": np.asfortranarray(np.concatenate(ienb)) if ienb else np.zeros((0, 4), np.int32, order="F")}
-    codes = [(k, v) for k, v in b.items() if k.startswith("nbc codes")]
-    vals = [(k, v) for k, v in b.items() if k.startswith("nbc values")]
-    if codes:
-        k, v = codes[0]
-        out["iBCB"] = _arr(b, o, k, "i4", (v[0][0], 2))
-    if vals:
-        k, v = vals[0]
-        npro = v[0][0]
-        nbytes = len(v[1])
-        ndbcb = nbytes // 8 // npro // 3  # nshlb = 3 for a tet face
-        out["BCB"] = _arr(b, o, k, "f8", (npro, 3, ndbcb))
-    if "bc mapping array" in b:
-        nshg = b["bc mapping array"][0][0]
-        mp = _arr(b, o, "bc mapping array", "i4", (nshg,))
-        nb = b["bc codes array"][0][0]
-        codes = _arr(b, o, "bc codes array", "i4", (nb,))
-        nvals = b["boundary condition array"][0][0]
-        ndof = nvals // max(nb, 1)
-        bcv = _arr(b, o, "boundary condition array", "f8", (nb, ndof)) if nb else np.zeros((0, ndof))
+    codes = blocks_of("nbc codes")
+    vals = blocks_of("nbc values")
+    if codes or vals:
+        if len(codes) != len(bcon) or len(vals) != len(bcon):
+            raise ValueError(f"{path}: {len(bcon)} boundary blocks but {len(codes)} code and {len(vals)} value "
+                             f"blocks")
+        ib, bv, ndbcb = [], [], None
+        for con, cb, vb in zip(bcon, codes, vals):
+            npro = con[1][0]
+            if cb[1][0] != npro or vb[1][0] != npro:
+                raise ValueError(f"{path}: boundary block of {npro} elements with {cb[1][0]} codes and "
+                                 f"{vb[1][0]} value rows")
+            ib.append(_arr(cb, o, "i4", (npro, 2)))
+            nd = len(vb[2]) // 8 // max(npro, 1) // 3  # nshlb = 3 for a tet face
+            if ndbcb is not None and nd != ndbcb:
+                raise ValueError(f"{path}: boundary value blocks of {ndbcb} and {nd} values per face node")
+            ndbcb = nd
+            bv.append(_arr(vb, o, "f8", (npro, 3, nd)))
+        out["iBCB"] = np.asfortranarray(np.concatenate(ib))
+        out["BCB"] = np.asfortranarray(np.concatenate(bv))
+    mapping = _find(b, "bc mapping array")
+    if mapping is not None:
+        nshg = mapping[1][0]
+        mp = _arr(mapping, o, "i4", (nshg,))
+        cblk = _find(b, "bc codes array")
+        nb = cblk[1][0]
+        bcodes = _arr(cblk, o, "i4", (nb,))
+        vblk = _find(b, "boundary condition array")
+        ndof = vblk[1][0] // max(nb, 1)
+        bcv = _arr(vblk, o, "f8", (nb, ndof)) if nb else np.zeros((0, ndof))
         iBC = np.zeros(nshg, np.int32)
         BC = np.zeros((nshg, ndof), order="F")
         has = mp > 0
-        iBC[has] = codes[mp[has] - 1]
+        iBC[has] = bcodes[mp[has] - 1]
         BC[has] = bcv[mp[has] - 1]
         out["iBC"], out["BC"] = iBC, BC
     return out
@@ -112,11 +141,12 @@ def read_restart(path):
     b, o = read_blocks(path)
     out = {}
     for key, name in (("solution", "y"), ("time derivative of solution", "ac")):
-        if key in b:
-            n, ndof = b[key][0][0], b[key][0][1]
-            out[name] = _arr(b, o, key, "f8", (n, ndof))
-            if len(b[key][0]) > 2:
-                out["step"] = b[key][0][2]
+        blk = _find(b, key)
+        if blk is not None:
+            n, ndof = blk[1][0], blk[1][1]
+            out[name] = _arr(blk, o, "f8", (n, ndof))
+            if len(blk[1]) > 2:
+                out["step"] = blk[1][2]
     return out
 
 

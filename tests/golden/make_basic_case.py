@@ -1,8 +1,11 @@
 """Writes tests/golden/basic_case.npz from the reference's mainTests/basic
 input files (testbin/mainTests/basic/geombc.dat.1 and restart.0.1, the
 1072-node / 4157-tet cylinder its own end-to-end tests run), read with the
-package's PHASTA reader.  Data only: the mesh, its boundary codes and the
-restart state.  Run in a container that holds the reference:
+package's PHASTA reader, and tests/golden/basic_bct.npz from the same case's
+bct.dat (26 inlet nodes x 1101 samples) together with the inlet velocities
+of testbin/mainTests/restart/restart.5.1, the state the reference wrote
+after 5 steps of 0.001 on the same mesh and bct.dat.  Data only.  Run in a
+container that holds the reference:
 
     python tests/golden/make_basic_case.py /root/reference
 """
@@ -12,7 +15,7 @@ import sys
 import numpy as np
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", ".."))
-from crimsonflowsolver_amd import phasta  # noqa: E402
+from crimsonflowsolver_amd import bct, phasta  # noqa: E402
 
 
 def main(ref):
@@ -23,6 +26,17 @@ def main(ref):
     np.savez_compressed(out, x=g["x"], ien=g["ien"], ienb=g["ienb"], iBCB=g["iBCB"], BCB=g["BCB"],
                         iBC=g["iBC"], BC=g["BC"], y=phasta.to_solver_order(r["y"]),
                         ac=phasta.to_solver_order(r["ac"]), step=np.int32(r.get("step", 0)))
+    print("wrote", out)
+    recs = bct.read_bct(os.path.join(d, "bct.dat"))
+    tab = np.stack([t for _, t in recs])
+    if np.any(tab[:, :, 3] != tab[:1, :, 3]):
+        raise SystemExit("bct.dat: the nodes' sample times differ")
+    r5 = phasta.read_restart(os.path.join(ref, "testbin", "mainTests", "restart", "restart.5.1"))
+    y5 = phasta.to_solver_order(r5["y"])
+    attached = bct.BCt(recs, g["x"], g["iBC"]).nodes
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "basic_bct.npz")
+    np.savez_compressed(out, x=np.stack([x for x, _ in recs]), u=tab[:, :, :3], t=tab[0, :, 3],
+                        step5=np.int32(r5["step"]), u5_nodes=attached, u5=y5[attached, :3])
     print("wrote", out)
 
 

@@ -54,3 +54,52 @@ def test_oracle_patch_test_on_real_geometry(oracle_lib):
     interior = np.setdiff1d(np.arange(n), bnd)
     assert interior.size > 100
     assert np.abs(res[interior]).max() < 1e-10 * np.abs(res).max()
+
+
+def _write_geombc(path, x, ien_blocks, bnd_blocks):
+    """A geombc file with several interior and boundary element blocks (the
+    layout read_geombc reads; each boundary block: connectivity, then its
+    nbc codes and nbc values, as PHASTA's writer orders them)."""
+    from crimsonflowsolver_amd import phasta as P
+    parts = [b"# PHASTA Input File Version 2.0\n",
+             P._block("byteorder magic number", [1], np.array([P.MAGIC], "<i4").tobytes()),
+             P._block("number of nodes", [x.shape[0]]),
+             P._block("co-ordinates", [x.shape[0], 3], np.asfortranarray(x, "<f8").tobytes(order="F"))]
+    for ien in ien_blocks:
+        parts.append(P._block("connectivity interior linear tetrahedron", [ien.shape[0], 4, 0, 1, 4],
+                              np.asfortranarray(ien, "<i4").tobytes(order="F")))
+    for ienb, ibcb, bcb in bnd_blocks:
+        n = ienb.shape[0]
+        parts.append(P._block("connectivity boundary linear tetrahedron", [n, 4, 0, 1, 4, 3],
+                              np.asfortranarray(ienb, "<i4").tobytes(order="F")))
+        parts.append(P._block("nbc codes linear tetrahedron", [n, 2],
+                              np.asfortranarray(ibcb, "<i4").tobytes(order="F")))
+        parts.append(P._block("nbc values linear tetrahedron", [n, bcb.shape[2]],
+                              np.asfortranarray(bcb, "<f8").tobytes(order="F")))
+    with open(path, "wb") as f:
+        f.write(b"".join(parts))
+
+
+def test_geombc_with_several_element_blocks(tmp_path):
+    """Every interior and boundary block is kept, concatenated in file order,
+    with each boundary block's codes and values beside its elements."""
+    from crimsonflowsolver_amd import phasta
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(20, 3))
+    ien = [rng.integers(1, 21, size=(n, 4)).astype(np.int32) for n in (5, 3)]
+    bnd = [(rng.integers(1, 21, size=(n, 4)).astype(np.int32), rng.integers(0, 9, size=(n, 2)).astype(np.int32),
+            rng.normal(size=(n, 3, 2))) for n in (4, 2, 3)]
+    p = tmp_path / "geombc.dat.1"
+    _write_geombc(str(p), x, ien, bnd)
+    g = phasta.read_geombc(str(p))
+    assert np.array_equal(g["x"], x)
+    assert np.array_equal(g["ien"], np.concatenate(ien))
+    assert np.array_equal(g["ienb"], np.concatenate([b[0] for b in bnd]))
+    assert np.array_equal(g["iBCB"], np.concatenate([b[1] for b in bnd]))
+    assert np.array_equal(g["BCB"], np.concatenate([b[2] for b in bnd]))
+    # a boundary block whose codes disagree with its elements is refused
+    bad = [(bnd[0][0], bnd[1][1], bnd[0][2])]
+    _write_geombc(str(p), x, ien, bad)
+    import pytest
+    with pytest.raises(ValueError):
+        phasta.read_geombc(str(p))
