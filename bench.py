@@ -401,7 +401,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the formation/solve path here has no multi-GPU form "
-              "(one GPU holds the whole mesh; see DESIGN.md for how it would shard).", file=sys.stderr)
+              "(one GPU holds the whole mesh); run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -693,9 +693,9 @@ def main():
                 # right-hand side at unit norm: memLS's GE refuses pivots below
                 # 2 eps in absolute terms, which the Gram matrix of a residual
                 # driven this far down reaches
-                dres4 /= rn[-1]
+                dres4.div_(rn[-1])
                 cfa.memLS_SOLVE_dev(lh, ls, 4, dres4, res=faceres)
-                dres4 *= rn[-1]
+                dres4.mul_(rn[-1])
                 its.append((ls.RI["itr"], ls.GM["itr"], ls.CG["itr"], ls.RI["fNorm"] / ls.RI["iNorm"]))
                 fm.correct(yc, acc, dres4)
                 fm.itr_bc(dibc, dBC, yc)
