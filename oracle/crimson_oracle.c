@@ -2077,10 +2077,225 @@ static const double TRI_A = 0.3333333333333330000000000000000000; /* symtri.c:20
 static const double TRI_B = 0.666666666666667;                    /* symtri.c:21 */
 static const double TRI_C = 0.166666666666667;                    /* symtri.c:22 */
 
+/* The deformable-wall inputs of the boundary loop (NULL: rigid walls). */
+typedef struct {
+  const double *ac, *u, *uref;
+  const int *tag;
+  const cfo_wall *w;
+  const int *iBC;
+  const double *BC;
+  int ndofBC;
+  const int *colm, *rowp;
+  double *lhsK;
+} wall_ctx;
+
+/* e3bvar's deformable-wall section at one boundary quadrature point of a
+ * vessel-wall face (e3bvar.f90:462-1017), P1 tet, lnode = 1,2,3, isotropic
+ * membrane (nProps = 10, no SWB/BET files), as written: the rotation to the
+ * face's flat frame, the "jacobian" of the rotated face nodes (node 4's
+ * rotated coordinates left zero) and its cofactor inverse, the strain-
+ * displacement matrix B(5,3 nodes,3), the membrane residual
+ * rlKwall = B^T D sum_m B_m (u_m + uref_m) WdetJb h and its tangent
+ * LHSwall, the wall mass vdot, the tissue support usup1 and damping velsup,
+ * and at the first point the ring tethering.  Adds this point's xKebe
+ * (mass, support and damping on the diagonal components, then LHSwall)
+ * into xK when xK is not NULL. */
+static void wall_qp(const double xl[4][3], const double bn[3], const double N[3], double WdetJb,
+                    const double yl[4][4], const double acl[3][3], const double ul[3][3],
+                    const double urefl[3][3], const int tagl[3], int first, const cfo_wall *w,
+                    double vdot[3], double velsup[3], double usup1[3], double rlKwall[3][3], double *xK) {
+  const double shglb[3][4] = {{0.5, 0.0, 0.0, -0.5}, {0.0, 0.5, 0.0, -0.5}, {0.0, 0.0, 0.5, -0.5}};
+  double v1[3], v2[3], v3[3], rot[3][3];
+  for (int i = 0; i < 3; ++i) {
+    v3[i] = bn[i];
+    v1[i] = xl[1][i] - xl[0][i];
+  }
+  double temp = 1.0 / sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+  for (int i = 0; i < 3; ++i) v1[i] = v1[i] * temp;
+  const double c1 = v3[1] * v1[2] - v1[1] * v3[2];
+  const double c2 = v1[0] * v3[2] - v3[0] * v1[2];
+  const double c3 = v3[0] * v1[1] - v1[0] * v3[1];
+  temp = 1.0 / sqrt(c1 * c1 + c2 * c2 + c3 * c3);
+  v2[0] = c1 * temp;
+  v2[1] = c2 * temp;
+  v2[2] = c3 * temp;
+  for (int j = 0; j < 3; ++j) {
+    rot[0][j] = v1[j];
+    rot[1][j] = v2[j];
+    rot[2][j] = v3[j];
+  }
+  double xr[4][3] = {{0}};
+  for (int n = 0; n < 3; ++n)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) xr[n][i] = xr[n][i] + rot[i][j] * xl[n][j];
+  double J[3][3] = {{0}}, D[3][3];
+  for (int n = 0; n < 4; ++n)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) J[i][j] = J[i][j] + xr[n][i] * shglb[j][n];
+  D[0][0] = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+  D[0][1] = J[2][1] * J[0][2] - J[0][1] * J[2][2];
+  D[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  temp = 1.0 / (D[0][0] * J[0][0] + D[0][1] * J[1][0] + D[0][2] * J[2][0]);
+  D[0][0] = D[0][0] * temp;
+  D[0][1] = D[0][1] * temp;
+  D[0][2] = D[0][2] * temp;
+  D[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * temp;
+  D[1][1] = (J[0][0] * J[2][2] - J[2][0] * J[0][2]) * temp;
+  D[1][2] = (J[1][0] * J[0][2] - J[0][0] * J[1][2]) * temp;
+  D[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * temp;
+  D[2][1] = (J[2][0] * J[0][1] - J[0][0] * J[2][1]) * temp;
+  D[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * temp;
+  /* shgb_el(i,n) = sum_j dxidxb_el(j,i) shglb(j,n) */
+  double sg[3][4] = {{0}};
+  for (int n = 0; n < 4; ++n)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) sg[i][n] = sg[i][n] + D[j][i] * shglb[j][n];
+  /* the material matrix (e3bvar.f90:576-624, isotropic) */
+  const double h = w->thicknessvw;
+  const double tc = 1.0 / (1.0 - w->rnuvw * w->rnuvw);
+  const double swb7 = w->evw * tc * 1.0;
+  const double swb8 = w->evw * tc * w->rnuvw;
+  const double swb9 = w->evw * tc * 0.5 * (1.0 - w->rnuvw);
+  const double swb10 = w->evw * tc * 0.5 * (1.0 - w->rnuvw) * w->rshearconstantvw;
+  double Dm[5][5] = {{0}};
+  Dm[0][0] = swb7;
+  Dm[1][1] = swb7;
+  Dm[0][1] = swb8;
+  Dm[1][0] = swb8;
+  Dm[2][2] = swb9;
+  Dm[3][3] = swb10;
+  Dm[4][4] = swb10;
+  /* B(j, n, i), the strain, D strain, D B */
+  double B[5][3][3], strain[5][3] = {{0}}, Ds[5][3] = {{0}}, DB[5][3][3] = {{{0}}};
+  for (int n = 0; n < 3; ++n) {
+    for (int i = 0; i < 3; ++i) {
+      B[0][n][i] = sg[0][n] * rot[0][i];
+      B[1][n][i] = sg[1][n] * rot[1][i];
+      B[2][n][i] = sg[1][n] * rot[0][i] + sg[0][n] * rot[1][i];
+      B[3][n][i] = sg[0][n] * rot[2][i];
+      B[4][n][i] = sg[1][n] * rot[2][i];
+    }
+    for (int j = 0; j < 5; ++j)
+      for (int i = 0; i < 3; ++i) strain[j][n] = strain[j][n] + B[j][n][i] * (ul[n][i] + urefl[n][i]);
+    for (int i = 0; i < 5; ++i)
+      for (int j = 0; j < 5; ++j) Ds[i][n] = Ds[i][n] + Dm[i][j] * strain[j][n];
+    for (int k = 0; k < 3; ++k)
+      for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) DB[i][n][k] = DB[i][n][k] + Dm[i][j] * B[j][n][k];
+  }
+  double LW[9][3][3] = {{{0}}}; /* LHSwall(3(i-1)+k, n, m) */
+  for (int n = 0; n < 3; ++n) {
+    for (int i = 0; i < 3; ++i) rlKwall[n][i] = 0.0;
+    for (int m = 0; m < 3; ++m) {
+      for (int j = 0; j < 5; ++j)
+        for (int i = 0; i < 3; ++i) rlKwall[n][i] = rlKwall[n][i] + B[j][n][i] * Ds[j][m];
+      for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 5; ++j)
+          for (int i = 0; i < 3; ++i)
+            LW[3 * i + k][n][m] = LW[3 * i + k][n][m] + B[j][n][i] * DB[j][m][k] * WdetJb * h *
+                                                            w->iwallstiffactor * w->betai * w->Delt * w->Delt *
+                                                            w->alfi;
+    }
+    for (int i = 0; i < 3; ++i) rlKwall[n][i] = rlKwall[n][i] * WdetJb * h;
+  }
+  /* mass, support, damping (e3bvar.f90:783-870) */
+  for (int i = 0; i < 3; ++i) vdot[i] = 0.0;
+  for (int n = 0; n < 3; ++n)
+    for (int i = 0; i < 3; ++i) vdot[i] = vdot[i] + N[n] * acl[n][i];
+  for (int i = 0; i < 3; ++i) vdot[i] = vdot[i] * h * w->rhovw;
+  double wlf = w->almi * (1.0 - w->flmpl) * w->rhovw * h;
+  for (int i = 0; i < 3; ++i) usup1[i] = velsup[i] = 0.0;
+  if (w->iwallsupp > 0) {
+    for (int n = 0; n < 3; ++n)
+      for (int i = 0; i < 3; ++i) usup1[i] = usup1[i] + N[n] * ul[n][i];
+    for (int i = 0; i < 3; ++i) usup1[i] = usup1[i] * w->tissSuppStiffCoeff;
+    wlf = wlf + w->Delt * w->alfi * w->betai * w->Delt * w->tissSuppStiffCoeff;
+  }
+  if (w->iwalldamp > 0) {
+    for (int n = 0; n < 3; ++n)
+      for (int i = 0; i < 3; ++i) velsup[i] = velsup[i] + N[n] * yl[n][i + 1];
+    for (int i = 0; i < 3; ++i) velsup[i] = velsup[i] * w->tissSuppDampCoeff;
+    wlf = wlf + w->Delt * w->alfi * w->gami * w->tissSuppDampCoeff;
+  }
+  /* the ring tethering, once per element (e3bvar.f90:876-975) */
+  if (first && (w->iringsupp > 0 || w->iringdamp > 0) && tagl) {
+    int r[3], nr = 0;
+    double xm[3] = {0.0, 0.0, 0.0}, rm[3] = {0.0, 0.0, 0.0};
+    for (int n = 0; n < 3; ++n) {
+      if (tagl[n] & 1) {
+        r[nr++] = n;
+        for (int i = 0; i < 3; ++i) rm[i] = rm[i] + xl[n][i];
+      }
+      for (int i = 0; i < 3; ++i) xm[i] = xm[i] + xl[n][i];
+    }
+    for (int i = 0; i < 3; ++i) xm[i] = xm[i] / 3;
+    for (int i = 0; i < 3; ++i) rm[i] = rm[i] / nr;
+    if (nr > 1) {
+      double d[3], t2v[3], t1v[3];
+      for (int i = 0; i < 3; ++i) {
+        d[i] = rm[i] - xm[i];
+        t2v[i] = xl[r[1]][i] - rm[i];
+      }
+      double nn = sqrt(t2v[0] * t2v[0] + t2v[1] * t2v[1] + t2v[2] * t2v[2]);
+      for (int i = 0; i < 3; ++i) t2v[i] = t2v[i] / nn;
+      const double dp = d[0] * t2v[0] + d[1] * t2v[1] + d[2] * t2v[2];
+      for (int i = 0; i < 3; ++i) d[i] = d[i] - t2v[i] * dp;
+      nn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      for (int i = 0; i < 3; ++i) d[i] = d[i] / nn;
+      for (int i = 0; i < 3; ++i) t2v[i] = xl[r[1]][i] - xl[r[0]][i];
+      const double len = sqrt(t2v[0] * t2v[0] + t2v[1] * t2v[1] + t2v[2] * t2v[2]) / 6.0;
+      for (int i = 0; i < 3; ++i) t1v[i] = t2v[i] = 0.0;
+      if (w->iringsupp > 0)
+        for (int i = 0; i < 3; ++i) {
+          t1v[i] = t1v[i] + w->tissSuppRingStiffCoeff * ul[r[0]][i];
+          t2v[i] = t2v[i] + w->tissSuppRingStiffCoeff * ul[r[1]][i];
+        }
+      if (w->iringdamp > 0) /* both ends take the first ring node's velocity, as written */
+        for (int i = 0; i < 3; ++i) {
+          t1v[i] = t1v[i] + w->tissSuppRingDampCoeff * yl[r[0]][i + 1];
+          t2v[i] = t2v[i] + w->tissSuppRingDampCoeff * yl[r[0]][i + 1];
+        }
+      const double p1 = d[0] * t1v[0] + d[1] * t1v[1] + d[2] * t1v[2];
+      const double p2 = d[0] * t2v[0] + d[1] * t2v[1] + d[2] * t2v[2];
+      for (int i = 0; i < 3; ++i) {
+        t1v[i] = d[i] * p1;
+        t2v[i] = d[i] * p2;
+      }
+      for (int i = 0; i < 3; ++i) {
+        rlKwall[r[0]][i] = rlKwall[r[0]][i] + len * (2.0 * t1v[i] + t2v[i]);
+        rlKwall[r[1]][i] = rlKwall[r[1]][i] + len * (t1v[i] + 2.0 * t2v[i]);
+      }
+      double hf = 0.0;
+      if (w->iringsupp > 0) hf = hf + len * w->tissSuppRingStiffCoeff * w->betai * w->Delt * w->Delt * w->alfi;
+      if (w->iringdamp > 0) hf = hf + len * w->tissSuppRingDampCoeff * w->Delt * w->alfi * w->gami;
+      for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+          LW[3 * i + k][r[0]][r[0]] = LW[3 * i + k][r[0]][r[0]] + d[i] * d[k] * hf * 2.0;
+          LW[3 * i + k][r[0]][r[1]] = LW[3 * i + k][r[0]][r[1]] + d[i] * d[k] * hf;
+          LW[3 * i + k][r[1]][r[0]] = LW[3 * i + k][r[1]][r[0]] + d[i] * d[k] * hf;
+          LW[3 * i + k][r[1]][r[1]] = LW[3 * i + k][r[1]][r[1]] + d[i] * d[k] * hf * 2.0;
+        }
+    }
+  }
+  if (!xK) return;
+  wlf = w->iwallmassfactor * wlf * WdetJb;
+  for (int b = 0; b < 3; ++b)
+    for (int a = 0; a < 3; ++a) {
+      const double t = wlf * N[a] * N[b];
+      double *blk = xK + 9 * (a + 4 * b);
+      blk[0] = blk[0] + t;
+      blk[4] = blk[4] + t;
+      blk[8] = blk[8] + t;
+    }
+  for (int b = 0; b < 3; ++b)
+    for (int a = 0; a < 3; ++a)
+      for (int m = 0; m < 9; ++m) xK[m + 9 * (a + 4 * b)] = xK[m + 9 * (a + 4 * b)] + LW[m][a][b];
+}
+
 static int boundary_core(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
                      const double *BCB, int ndBCB, const double *y, const double *x,
                      double rho, double mu, double influx, int nqb, const int *permit,
-                     int nsrf, int ibksiz, double *res, double *flx, int nflx) {
+                     int nsrf, int ibksiz, double *res, double *flx, int nflx, const wall_ctx *wc) {
   double qp[3][3], qw[3];
   if (nqb == 1) {
     qp[0][0] = qp[0][1] = qp[0][2] = TRI_A;
@@ -2113,6 +2328,22 @@ static int boundary_core(int nshg, int numnp, int nelb, const int *ienb, const i
         yl[a][0] = y[g[a] + (size_t)nshg * 3];
         for (int i = 0; i < 3; ++i) yl[a][i + 1] = y[g[a] + (size_t)nshg * i];
         for (int i = 0; i < 3; ++i) xl[a][i] = x[g[a] + (size_t)numnp * i];
+      }
+      /* the wall's gathers (AsBMFG: localy of ac, localx of u) and its
+       * element matrix */
+      double acl[3][3] = {{0}}, ul[3][3] = {{0}}, urefl[3][3] = {{0}}, xK[144];
+      int tagl[3] = {0, 0, 0};
+      const int wall = wc && (iBCB[e] & 16);
+      if (wc) {
+        for (int a = 0; a < 3; ++a)
+          for (int i = 0; i < 3; ++i) {
+            acl[a][i] = wc->ac[g[a] + (size_t)nshg * i];
+            ul[a][i] = wc->u[g[a] + (size_t)nshg * i];
+            if (wc->uref) urefl[a][i] = wc->uref[g[a] + (size_t)nshg * i];
+          }
+        if (wc->tag)
+          for (int a = 0; a < 3; ++a) tagl[a] = wc->tag[g[a]];
+        for (int k = 0; k < 144; ++k) xK[k] = 0.0;
       }
       /* e3bvar.f90: dxdxib, its inverse by cofactors, global gradients */
       double J[3][3] = {{0}}, D[3][3], gl[3][4] = {{0}}, gy[3][4];
@@ -2206,11 +2437,41 @@ static int boundary_core(int nshg, int numnp, int nelb, const int *ienb, const i
         double rNa[4];
         for (int i = 0; i < 3; ++i) rNa[i] = -tau[i] + bn[i] * pres;
         rNa[3] = unm;
+        double rlK[3][3];
+        if (wall) { /* e3b.f90:488-511 */
+          double vdot[3], velsup[3], usup1[3];
+          wall_qp(xl, bn, N, WdetJb, yl, acl, ul, urefl, wc->tag ? tagl : NULL, q == 0, wc->w, vdot, velsup, usup1,
+                  rlK, wc->lhsK ? xK : NULL);
+          for (int i = 0; i < 3; ++i) rNa[i] = rNa[i] + vdot[i];
+          if (wc->w->iwalldamp == 1)
+            for (int i = 0; i < 3; ++i) rNa[i] = rNa[i] + velsup[i];
+          if (wc->w->iwallsupp == 1)
+            for (int i = 0; i < 3; ++i) rNa[i] = rNa[i] + usup1[i];
+        }
         const double rou_copy = rho * unm_copy;
         for (int i = 0; i < 3; ++i) rNa[i] = rNa[i] - influx * rou_copy * u[i];
         for (int k = 0; k < 4; ++k) rNa[k] = rNa[k] * WdetJb;
         for (int a = 0; a < 3; ++a)
           for (int k = 0; k < 4; ++k) rl[el + n * (a + 4 * k)] = rl[el + n * (a + 4 * k)] - N[a] * rNa[k];
+        if (wall) /* e3b.f90:562-575 */
+          for (int a = 0; a < 3; ++a)
+            for (int i = 0; i < 3; ++i) rl[el + n * (a + 4 * i)] = rl[el + n * (a + 4 * i)] - rlK[a][i];
+      }
+      if (wc && wc->lhsK) { /* bc3lhs and fillsparseI of this element's xKebe (elmgmr.f90:425-436) */
+        const int nodes[4] = {g[0] + 1, g[1] + 1, g[2] + 1, g[3] + 1};
+        bc3lhs_elem(nshg, wc->iBC, wc->BC, nodes, xK, 0, 1);
+        for (int a = 0; a < 4; ++a) {
+          const int c = wc->colm[g[a]];
+          const int len = wc->colm[g[a] + 1] - c;
+          for (int b = 0; b < 4; ++b) {
+            const int k = sparseloc(wc->rowp + (c - 1), len, nodes[b]) + c - 1;
+            if (wc->rowp[k - 1] != nodes[b]) {
+              free(rl);
+              return CFO_ESEARCH;
+            }
+            for (int m = 0; m < 9; ++m) wc->lhsK[m + 9 * (size_t)(k - 1)] += xK[m + 9 * (a + 4 * b)];
+          }
+        }
       }
     }
     /* local(res, rl, ienb, nflow, 'scatter ') (local.f90:68-75) */
@@ -2230,7 +2491,7 @@ int cfo_boundary_res(int nshg, int numnp, int nelb, const int *ienb, const int *
                      double rho, double mu, double influx, int nqb, const int *permit,
                      int nsrf, int ibksiz, double *res) {
   return boundary_core(nshg, numnp, nelb, ienb, iBCB, BCB, ndBCB, y, x, rho, mu, influx, nqb, permit, nsrf,
-                       ibksiz, res, NULL, 0);
+                       ibksiz, res, NULL, 0, NULL);
 }
 
 int cfo_boundary_res_flx(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
@@ -2239,5 +2500,19 @@ int cfo_boundary_res_flx(int nshg, int numnp, int nelb, const int *ienb, const i
                          int nsrf, int ibksiz, double *res, double *flxid, int nflx) {
   for (size_t k = 0; k < 5 * ((size_t)nflx + 1); ++k) flxid[k] = 0.0; /* elmgmr.f90:124 */
   return boundary_core(nshg, numnp, nelb, ienb, iBCB, BCB, ndBCB, y, x, rho, mu, influx, nqb, permit, nsrf,
-                       ibksiz, res, flxid, nflx);
+                       ibksiz, res, flxid, nflx, NULL);
+}
+
+int cfo_boundary_wall(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
+                      const double *BCB, int ndBCB, const double *y, const double *ac,
+                      const double *u, const double *uref, const double *x, const int *nodetag,
+                      double rho, double mu, double influx, int nqb, const int *permit, int nsrf,
+                      int ibksiz, const cfo_wall *w, double *res, const int *iBC, const double *BC,
+                      int ndofBC, const int *colm, const int *rowp, double *lhsK) {
+  if (!w || !ac || !u) return CFO_EARG;
+  if (lhsK && (!iBC || !BC || ndofBC < 6 || !colm || !rowp)) return CFO_EARG;
+  if ((w->iringsupp > 0 || w->iringdamp > 0) && !nodetag) return CFO_EARG;
+  const wall_ctx wc = {ac, u, uref, nodetag, w, iBC, BC, ndofBC, colm, rowp, lhsK};
+  return boundary_core(nshg, numnp, nelb, ienb, iBCB, BCB, ndBCB, y, x, rho, mu, influx, nqb, permit, nsrf,
+                       ibksiz, res, NULL, 0, &wc);
 }

@@ -127,6 +127,38 @@ void cfo_bc3lhs_elem(int nshg, const int *iBC, const double *BC, const int *node
 int cfo_boundary_lhs(int nshg, int nelb, const int *ienb, const int *iBC, const double *BC,
                      int ndofBC, const int *colm, const int *rowp, double *lhsK);
 
+/* Deformable-wall parameters (solver.inp "Deformable Wall" block,
+ * input_fform.cxx:1237-1262; isotropic membrane, no SWB/BET files). */
+typedef struct {
+  double rhovw;            /* Density of Vessel Wall */
+  double thicknessvw;      /* Thickness of Vessel Wall = SWB(:,1) */
+  double evw;              /* Young Mod of Vessel Wall */
+  double rnuvw;            /* Poisson Ratio of Vessel Wall */
+  double rshearconstantvw; /* Shear Constant of Vessel Wall */
+  double tissSuppStiffCoeff, tissSuppDampCoeff;         /* iwallsupp, iwalldamp */
+  double tissSuppRingStiffCoeff, tissSuppRingDampCoeff; /* iringsupp, iringdamp */
+  int iwallmassfactor, iwallstiffactor;                 /* Wall Mass/Stiffness Matrix for LHS */
+  int iwallsupp, iwalldamp, iringsupp, iringdamp;
+  double Delt, alfi, almi, gami, betai, flmpl;          /* time integration (betai = 1) */
+} cfo_wall;
+
+/* The boundary loop with deformable walls (ideformwall = 1): cfo_boundary_res
+ * plus, on the vessel-wall faces (iBCB bit 4), e3bvar's membrane, mass,
+ * tissue-support, damping and ring-tethering terms (e3bvar.f90:462-1017)
+ * and e3b's use of them (e3b.f90:374-395,488-575).  ac(nshg,4) as y;
+ * u(nshg,3) the displacement (uAlpha), uref(nshg,3) the reference
+ * displacement (NULL: zero), nodetag(numnp) the node tags (bit 0: ring;
+ * needed for the ring terms).  When lhsK is not NULL, each boundary
+ * element's xKebe (the wall tangent, zero elsewhere) goes through bc3lhs
+ * (iBC, BC(nshg,ndofBC)) and fillsparseI into lhsK(9,nnz)
+ * (elmgmr.f90:425-436). */
+int cfo_boundary_wall(int nshg, int numnp, int nelb, const int *ienb, const int *iBCB,
+                      const double *BCB, int ndBCB, const double *y, const double *ac,
+                      const double *u, const double *uref, const double *x, const int *nodetag,
+                      double rho, double mu, double influx, int nqb, const int *permit, int nsrf,
+                      int ibksiz, const cfo_wall *w, double *res, const int *iBC, const double *BC,
+                      int ndofBC, const int *colm, const int *rowp, double *lhsK);
+
 /* bc3Res without periodicity (bc3res.f90:1-80). BC(nshg,ndofBC). */
 int cfo_bc3res(int nshg, const int *iBC, const double *BC, int ndofBC,
                double *res);

@@ -135,6 +135,10 @@ def _bind(L):
         L.cfo_boundary_res_flx.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, C.c_int, dp, dp,
                                            C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p,
                                            C.c_int, C.c_int, dp, dp, C.c_int]
+        L.cfo_boundary_wall.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip, dp, C.c_int, dp, dp, dp, C.c_void_p,
+                                        dp, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p,
+                                        C.c_int, C.c_int, C.POINTER(Wall), dp, C.c_void_p, C.c_void_p, C.c_int,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]
         L.cfo_plan_create.argtypes = [C.c_int, C.c_int, ip, C.c_int, C.POINTER(C.c_void_p)]
         L.cfo_plan_free.argtypes = [C.c_void_p]
         L.cfo_plan_free.restype = None
@@ -331,6 +335,61 @@ def boundary_flux(ienb, iBCB, BCB, y, x, nflx, rho=1.06, mu=0.04, influx=0.5, nq
                                       BCB.shape[2], _f(y), _f(x), rho, mu, influx, nqb, _ptr(perm), nsrf, ibksiz,
                                       res, flx, nflx), "boundary_flux")
     return flx
+
+
+class Wall(C.Structure):
+    """cfo_wall: the deformable-wall parameters and the time integration."""
+    _fields_ = [("rhovw", C.c_double), ("thicknessvw", C.c_double), ("evw", C.c_double), ("rnuvw", C.c_double),
+                ("rshearconstantvw", C.c_double), ("tissSuppStiffCoeff", C.c_double),
+                ("tissSuppDampCoeff", C.c_double), ("tissSuppRingStiffCoeff", C.c_double),
+                ("tissSuppRingDampCoeff", C.c_double), ("iwallmassfactor", C.c_int), ("iwallstiffactor", C.c_int),
+                ("iwallsupp", C.c_int), ("iwalldamp", C.c_int), ("iringsupp", C.c_int), ("iringdamp", C.c_int),
+                ("Delt", C.c_double), ("alfi", C.c_double), ("almi", C.c_double), ("gami", C.c_double),
+                ("betai", C.c_double), ("flmpl", C.c_double)]
+
+
+def make_wall(p=None, **kw):
+    """cfo_wall with the reference basic case's wall values (testbin/mainTests/
+    basic/solver.inp: rhovw 1.06, h 0.3, E 7e5, nu 0.5, kappa 0.833333,
+    support 0.1, damping 0.01, ring 5e5 / 0.01, every term on) in the units
+    of the formation tests (g, cm, s), the time integration of `p`
+    (make_params) and betai = 1; keywords override any field."""
+    p = p if p is not None else make_params()
+    d = dict(rhovw=1.06, thicknessvw=0.3, evw=7.0e5, rnuvw=0.5, rshearconstantvw=0.833333,
+             tissSuppStiffCoeff=0.1, tissSuppDampCoeff=0.01, tissSuppRingStiffCoeff=5e5, tissSuppRingDampCoeff=0.01,
+             iwallmassfactor=1, iwallstiffactor=1, iwallsupp=1, iwalldamp=1, iringsupp=1, iringdamp=1,
+             Delt=p.Delt, alfi=p.alfi, almi=p.almi, gami=p.gami, betai=1.0, flmpl=0.0)
+    d.update(kw)
+    return Wall(**d)
+
+
+def boundary_wall(ienb, iBCB, BCB, y, ac, u, x, res, wall, uref=None, tag=None, rho=1.06, mu=0.04, influx=0.5,
+                  nqb=3, permit=None, ibksiz=255, ibc=None, bc=None, colm=None, rowp=None, lhsK=None):
+    """res(nshg,4) plus the boundary loop with deformable walls (e3b/e3bvar
+    with ideformwall = 1); when lhsK(9,nnz) is given, each boundary element's
+    xKebe (the wall tangent) through bc3lhs and fillsparseI is added to it.
+    u, uref (nshg,3); tag(numnp) node tags (bit 0: ring)."""
+    res = np.asfortranarray(res, dtype=np.float64)
+    ienb = _f(ienb, np.int32)
+    iBCB = _f(iBCB, np.int32)
+    BCB = _f(BCB)
+    perm = None if permit is None else np.ascontiguousarray(permit, dtype=np.int32)
+    nsrf = -1 if permit is None else perm.size - 1
+    nshg = res.shape[0]
+    if lhsK is not None:
+        ibc = np.ascontiguousarray(ibc if ibc is not None else np.zeros(nshg), dtype=np.int32)
+        bc = _f(bc if bc is not None else np.zeros((nshg, 6)))
+        colm, rowp = _f(colm, np.int32), _f(rowp, np.int32)
+    tag_a = None if tag is None else np.ascontiguousarray(tag, dtype=np.int32)
+    uref_a = None if uref is None else _f(uref)
+    _check(lib().cfo_boundary_wall(nshg, np.asarray(x).shape[0], ienb.shape[0], ienb, iBCB, BCB, BCB.shape[2],
+                                   _f(y), _f(ac), _f(u), _ptr(uref_a), _f(x), _ptr(tag_a), rho, mu, influx, nqb,
+                                   _ptr(perm), nsrf, ibksiz, C.byref(wall), res,
+                                   _ptr(ibc) if lhsK is not None else None,
+                                   _ptr(bc) if lhsK is not None else None, bc.shape[1] if lhsK is not None else 0,
+                                   _ptr(colm) if lhsK is not None else None,
+                                   _ptr(rowp) if lhsK is not None else None, _ptr(lhsK)), "boundary_wall")
+    return res
 
 
 def pack_val4(colm, rowp, lhsK, lhsP, mt=False):
