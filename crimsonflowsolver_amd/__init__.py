@@ -45,7 +45,9 @@ EXPORTS = (
     "crimson_graph_free", "crimson_form_set_boundary", "crimson_form_autotune", "crimson_form_set_bc",
     "crimson_elmpvsq_coef_dev", "crimson_form_boundary_flux", "crimson_form_boundary_flux_noalloc",
     "crimson_measure_peaks", "crimson_form_vjp_ac_dev", "crimson_elmpvsq_vjp_dev", "crimson_lhs_transpose_dev",
-    "crimson_ls_graph_create", "crimson_step_graph_create",
+    "crimson_ls_graph_create", "crimson_step_graph_create", "crimson_form_set_wall",
+    "crimson_form_set_displacement_dev", "crimson_wall_predict_dev", "crimson_wall_correct_dev",
+    "crimson_wall_alpha_dev",
 )
 
 
@@ -62,6 +64,26 @@ class FormParams(C.Structure):
                 ("flmpr", C.c_double), ("itau", C.c_int), ("iconvflow", C.c_int),
                 ("idiff", C.c_int), ("ipord", C.c_int), ("precision", C.c_int),
                 ("aspect_fp64", C.c_double), ("ideformwall", C.c_int)]
+
+
+class WallParams(C.Structure):
+    """crimson_wall_params: the vessel wall's values (solver.inp)."""
+    _fields_ = [("rhovw", C.c_double), ("thicknessvw", C.c_double), ("evw", C.c_double), ("rnuvw", C.c_double),
+                ("rshearconstantvw", C.c_double), ("tissSuppStiffCoeff", C.c_double),
+                ("tissSuppDampCoeff", C.c_double), ("tissSuppRingStiffCoeff", C.c_double),
+                ("tissSuppRingDampCoeff", C.c_double), ("iwallmassfactor", C.c_int), ("iwallstiffactor", C.c_int),
+                ("iwallsupp", C.c_int), ("iwalldamp", C.c_int), ("iringsupp", C.c_int), ("iringdamp", C.c_int)]
+
+
+def wall_params(rhovw=1.06, thicknessvw=0.3, evw=7.0e5, rnuvw=0.5, rshearconstantvw=0.833333,
+                tissSuppStiffCoeff=0.1, tissSuppDampCoeff=0.01, tissSuppRingStiffCoeff=5e5,
+                tissSuppRingDampCoeff=0.01, iwallmassfactor=1, iwallstiffactor=1, iwallsupp=1, iwalldamp=1,
+                iringsupp=1, iringdamp=1):
+    """The wall block of solver.inp ("Deformable Wall: True"); the defaults
+    are the reference's basic case's values with every term on."""
+    return WallParams(rhovw, thicknessvw, evw, rnuvw, rshearconstantvw, tissSuppStiffCoeff, tissSuppDampCoeff,
+                      tissSuppRingStiffCoeff, tissSuppRingDampCoeff, iwallmassfactor, iwallstiffactor, iwallsupp,
+                      iwalldamp, iringsupp, iringdamp)
 
 
 _lib = None
@@ -127,6 +149,11 @@ def lib():
         L.crimson_ls_graph_create.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.POINTER(C.c_void_p)]
         L.crimson_step_graph_create.argtypes = ([vp] * 13 + [C.c_int, C.c_int] + [vp] * 7 +
                                                 [C.c_int, C.POINTER(C.c_void_p)])
+        L.crimson_form_set_wall.argtypes = [vp, C.POINTER(WallParams), vp, vp]
+        L.crimson_form_set_displacement_dev.argtypes = [vp, vp]
+        L.crimson_wall_predict_dev.argtypes = [vp, vp, vp, vp, vp]
+        L.crimson_wall_correct_dev.argtypes = [vp, vp, vp]
+        L.crimson_wall_alpha_dev.argtypes = [vp, vp, vp, vp]
         L.crimson_graph_launch.argtypes = [vp]
         L.crimson_graph_free.argtypes = [vp]
         L.crimson_form_autotune.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
@@ -170,7 +197,7 @@ def _p(a):
 
 def form_params(rho=1.06, mu=0.04, delt=1e-3, rhoinf=0.5, precision=64, aspect_fp64=20.0, ideformwall=0):
     """Material and generalized-alpha parameters (itrPC.f90:16-37);
-    ideformwall 1 (deformable walls) is refused by the formation setup."""
+    ideformwall 1: deformable walls (set_wall before the first formation)."""
     if rhoinf < 0 or rhoinf > 1:
         almi = alfi = gami = 1.0
     else:
@@ -300,6 +327,35 @@ class ElmGMRInterior:
                                               int(BCB.shape[2]), int(nqb), float(influx), _p(perm),
                                               -1 if perm is None else int(perm.size - 1)),
               "crimson_form_set_boundary")
+
+    def set_wall(self, params=None, nodetag=None, uref=None):
+        """Deformable walls on the boundary's iBCB-bit-4 faces (the handle
+        needs ideformwall = 1): params a WallParams (wall_params()), nodetag
+        (numnp) node tags (bit 0: ring), uref(nshg,3) the reference
+        displacement.  set_displacement gives the displacement each later
+        formation uses."""
+        params = params if params is not None else wall_params()
+        tag = None if nodetag is None else np.ascontiguousarray(nodetag, dtype=np.int32)
+        ur = None if uref is None else np.asfortranarray(uref, dtype=np.float64)
+        if ur is not None and ur.shape != (self.nshg, 3):
+            raise ValueError("uref must be (nshg, 3)")
+        check(lib().crimson_form_set_wall(self.h, C.byref(params), _p(tag), _p(ur)), "crimson_form_set_wall")
+
+    def set_displacement(self, u_dev):
+        """The device displacement u(nshg,3) (a (3, nshg) C tensor, its
+        Fortran layout) of the following formations; kept alive here."""
+        self._disp = u_dev
+        check(lib().crimson_form_set_displacement_dev(self.h, _p(u_dev)), "crimson_form_set_displacement_dev")
+
+    def wall_predict(self, uold, yold, acold, u):
+        check(lib().crimson_wall_predict_dev(self.h, _p(uold), _p(yold), _p(acold), _p(u)),
+              "crimson_wall_predict_dev")
+
+    def wall_correct(self, u, res4):
+        check(lib().crimson_wall_correct_dev(self.h, _p(u), _p(res4)), "crimson_wall_correct_dev")
+
+    def wall_alpha(self, uold, u, ualpha):
+        check(lib().crimson_wall_alpha_dev(self.h, _p(uold), _p(u), _p(ualpha)), "crimson_wall_alpha_dev")
 
     # row schedule (numbering, Morton, Morton-ordered runs of 64 rows) / FP64 incidence chunk /
     # XCD dealing (one range per XCD, or turns of 64 workgroups)

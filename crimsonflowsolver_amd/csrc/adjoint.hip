@@ -444,6 +444,10 @@ void vjp_work_sizes(const crimson_form *f, size_t n[6]) {
 
 int vjp_run(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, const double *d_w,
             double *d_ybar, double *d_acbar, double *mubar, const VjpWork &w) {
+  if (f->wall) {
+    crimson::set_error("crimson_form_vjp_dev: the deformable wall's rows have no adjoint here");
+    return CRIMSON_EUNSUPPORTED;
+  }
   const size_t nn = (size_t)f->nshg;
   const int nmu = (int)(((size_t)std::max(f->nel, 1) + kMuChunk - 1) / kMuChunk);
   hipStream_t s = f->stream;

@@ -473,6 +473,7 @@ int form_boundary_pass(crimson_form *f, double *d_res, int mode) {
                                                   f->nsrf >= 0 ? f->d_permit.p : nullptr, f->nsrf, f->nqb,
                                                   f->influx, f->prm.rho, f->prm.mu, f->bens(), f->d_node.p,
                                                   f->d_brec.p, f->d_bflx.p);
+  CR_TRY(form_wall_rows(f));  // the wall's rows join each face's rl (e3b.f90:488-575)
   if (mode == 0)
     bnd_node<0><<<grid_for(f->nbnode, 256), 256, 0, s>>>(f->nbnode, f->nshg, f->d_bnode.p, f->d_binc_ptr.p,
                                                          f->d_binc.p, f->d_brec.p, d_res);
@@ -510,6 +511,11 @@ extern "C" int crimson_form_set_boundary(crimson_form *f, int nelb, const int *i
                                          const double *BCB, int ndBCB, int nqb, double influx,
                                          const int *permit, int nsrf) {
   if (!f) return crimson::set_error("crimson_form_set_boundary: null handle"), CRIMSON_EARG;
+  if (f->wall) {  // the wall's faces index the old boundary
+    f->wall = false;
+    f->nwf = f->nwn = f->nwk = 0;
+    f->wall_gen++;
+  }
   if (nelb == 0) {
     f->nelb = f->nbnode = 0;
     if (f->npartial > 0) {

@@ -131,6 +131,7 @@ namespace crimson {
 hipStream_t form_stream(const crimson_form *f);
 int form_nshg(const crimson_form *f);
 crimson_form_params form_params_of(const crimson_form *f);
+bool form_has_wall(const crimson_form *f);  // crimson_form_set_wall holds
 
 // What a graph of the solver side adds to a replay: the state it holds on
 // the device, read back after each launch, and the buffers and settings it

@@ -37,6 +37,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -1232,11 +1233,9 @@ int validate_params(const crimson_form_params *p) {
         p->itau, p->iconvflow, p->idiff, p->ipord, p->flmpl, p->flmpr);
     return CRIMSON_EUNSUPPORTED;
   }
-  if (p->ideformwall != 0) {
-    crimson::set_error("ideformwall = %d: deformable walls (the membrane terms of e3b.f90:488-562 and "
-                       "e3bvar.f90:1010-1017) are not formed; only rigid walls (ideformwall = 0)",
-                       p->ideformwall);
-    return CRIMSON_EUNSUPPORTED;
+  if (p->ideformwall != 0 && p->ideformwall != 1) {
+    crimson::set_error("ideformwall = %d: 0 (rigid walls) or 1 (deformable walls)", p->ideformwall);
+    return CRIMSON_EARG;
   }
   if (p->precision != 64 && p->precision != 32) {
     crimson::set_error("formation precision %d not available (64: FP64; 32: FP64 residual, FP32 tangent)",
@@ -1348,6 +1347,11 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
     crimson::set_error("iBC holds 1- or 2-component velocity codes: BC is needed first (crimson_form_set_bc)");
     return CRIMSON_ESTATE;
   }
+  if (f->prm.ideformwall && (!f->wall || !f->d_u)) {
+    crimson::set_error(f->wall ? "ideformwall = 1: the displacement is needed first (crimson_form_set_displacement_dev)"
+                               : "ideformwall = 1: the wall is needed first (crimson_form_set_wall)");
+    return CRIMSON_ESTATE;
+  }
   if (lhs) CR_TRY(ensure_order(f, d_x));
   if (timed) CR_HIP(hipEventRecord(f->ev[0], s));
   CR_TRY(crimson::form_flux_pass(f, d_y, d_ac, d_x, d_qout, nullptr));
@@ -1399,6 +1403,8 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
       asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
                                   d_res, o, f->asm_xcd);
   }
+  // the wall's rows and tangent (AsBMFG's xKebe goes through bc3lhs too)
+  CR_TRY(crimson::form_wall_pass(f, lhs, mode, o.lhsK, o.V, mode == 1 && t32 ? o.V32 : nullptr, o.nnz));
   if (lhs && f->nbcl > 0) {
     {
       const unsigned gb = crimson::grid_for(f->nbcl, 256);
@@ -1641,6 +1647,8 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
     f->has_ibc = true;
     if ((rc = f->d_ibc.upload(iBC, nshg, s))) return fail(rc);
   }
+  f->h_wrp = rp;
+  f->h_wcols = cols;
   f->npartial = npartial;
   if (npartial > 0) {
     f->h_ibc.assign(iBC, iBC + nshg);
@@ -1829,7 +1837,9 @@ std::vector<unsigned long long> graph_signature(const crimson_form *f, const cri
       f->d_bcl.serial, f->d_bcn.serial, f->d_inst.serial, f->d_mu_inst.serial, f->d_binst.serial,
       f->d_ienb.serial, f->d_bcode.serial, f->d_bcb.serial, f->d_permit.serial, f->d_brec.serial,
       f->d_bflx.serial, f->d_sptr.serial, f->d_selem.serial, f->d_bnode.serial, f->d_binc_ptr.serial,
-      f->d_binc.serial, lhs->d_V.serial, lhs->d_V32.serial,
+      f->d_binc.serial, lhs->d_V.serial, lhs->d_V32.serial, f->wall_gen, (unsigned long long)f->wall,
+      (unsigned long long)(uintptr_t)f->d_u, f->d_wface.serial, f->d_wtag.serial, f->d_uref.serial,
+      f->d_wres.serial, f->d_wrec.serial, f->d_wk.serial, f->d_wptr.serial, f->d_wsrc.serial,
       (unsigned long long)f->ninst, (unsigned long long)f->nelb, (unsigned long long)f->npartial,
       (unsigned long long)f->nbcl, (unsigned long long)f->prm.precision, (unsigned long long)f->asm_chunk,
       (unsigned long long)f->asm_xcd, (unsigned long long)f->sched, (unsigned long long)lhs->vdof,

@@ -138,6 +138,21 @@ inline FormConst make_const(const crimson_form_params &p) {
 
 }  // namespace
 
+// The deformable wall's constants (crimson_wall_params and the time
+// integration folded as e3bvar uses them).
+struct WallConst {
+  double h, rhovw;             // SWB(:,1), the wall density
+  double d7, d8, d9, d10;      // the isotropic material values SWB(:,7:10)
+  double kstiff;               // iwallstiffactor betai Delt^2 alfi (LHSwall's factor besides WdetJb h)
+  double mass;                 // almi (1 - flmpl) rhovw h (Wall_LHSfactor's first term)
+  double supp, supp_lhs;       // tissSuppStiffCoeff, Delt alfi betai Delt tissSuppStiffCoeff
+  double damp, damp_lhs;       // tissSuppDampCoeff, Delt alfi gami tissSuppDampCoeff
+  double ring_k, ring_c;       // tissSuppRingStiffCoeff, tissSuppRingDampCoeff
+  double ring_klhs, ring_clhs; // betai Delt^2 alfi, Delt alfi gami (the ring tangent's factors)
+  int massf;                   // iwallmassfactor
+  int isupp, idamp, iring_k, iring_c;
+};
+
 // A node with a 1- or 2-component velocity code (bc3lhs): its code, the
 // number of elements (interior and boundary) that hold it, and BC(4:6).
 struct BclNode {
@@ -203,6 +218,20 @@ struct crimson_form {
   int nvnode = 0;                                  // the adjoint's: all four nodes of each element
   crimson::DBuf<int> d_vnode, d_vptr, d_vinc;
   crimson::DBuf<double> d_bvrec;                   // boundary VJP records
+  // deformable walls (crimson_form_set_wall)
+  bool wall = false;
+  unsigned long long wall_gen = 0;  // bumped by every crimson_form_set_wall (captured graphs check it)
+  WallConst wc{};
+  int nwf = 0, nwn = 0, nwk = 0;   // wall faces, wall nodes, matrix blocks the wall tangent touches
+  crimson::DBuf<int> d_wface;      // boundary-element index of each wall face
+  crimson::DBuf<int> d_wnode;      // the wall nodes (mWNodes), 0-based
+  crimson::DBuf<int> d_wtag;       // (3, nwf): the face nodes' ring bits
+  crimson::DBuf<double> d_uref;    // uref(nshg,3)
+  const double *d_u = nullptr;     // the displacement of the next formations (the caller's)
+  crimson::DBuf<double> d_wres;    // (9, nwf): the wall rows of the three face nodes
+  crimson::DBuf<double> d_wrec;    // (81, nwf): xKebe(9, a, b) of the face node pairs
+  crimson::DBuf<int> d_wk, d_wptr, d_wsrc;  // the tangent gather: block k, its (face, pair) sources
+  std::vector<int> h_wrp, h_wcols;          // the CSR graph (host), kept for the wall's set-up
 };
 
 namespace crimson {
@@ -215,6 +244,13 @@ int form_flux_pass(crimson_form *f, const double *d_y, const double *d_ac, const
 // 1: Res4(4,nshg)) from the node records of the current formation; a no-op
 // without boundary elements.  Enqueued on the handle's stream.
 int form_boundary_pass(crimson_form *f, double *d_res, int mode);
+// The wall faces' rows and, when lhs, their tangent added into the matrix
+// (mode 0: lhsK(9,nnz); mode 1: the FP64 or FP32 planes of o), from the
+// node records of the current formation; enqueued before bc3lhs.  A no-op
+// without a wall.
+int form_wall_pass(crimson_form *f, int lhs, int mode, double *lhsK, double *V, float *V32, size_t nnz);
+// brec += the wall rows (after bnd_elem wrote brec).
+int form_wall_rows(crimson_form *f);
 // The ensemble instance of each boundary element (after either of
 // crimson_form_set_boundary / crimson_form_set_instances changes).
 int form_boundary_instances(crimson_form *f);

@@ -2966,6 +2966,10 @@ extern "C" int crimson_step_graph_create(crimson_form *f, crimson_lhs *lhs, crim
     crimson::set_error("crimson_step_graph_create: invalid arguments");
     return CRIMSON_EARG;
   }
+  if (crimson::form_has_wall(f)) {
+    crimson::set_error("crimson_step_graph_create: a deformable wall's displacement update is not captured");
+    return CRIMSON_EUNSUPPORTED;
+  }
   for (int q = 0; q < nsurf; ++q)
     if (surf_ptr[q + 1] < surf_ptr[q]) {
       crimson::set_error("crimson_step_graph_create: surf_ptr not monotone");

@@ -116,6 +116,9 @@ def read_geombc(path):
             bv.append(_arr(vb, o, "f8", (npro, 3, nd)))
         out["iBCB"] = np.asfortranarray(np.concatenate(ib))
         out["BCB"] = np.asfortranarray(np.concatenate(bv))
+    tags = _find(b, "node tags")
+    if tags is not None:  # readnblk.f90:216-228 (bit 0: ring node of a deformable wall)
+        out["nodetag"] = _arr(tags, o, "i4", (tags[1][0],)).ravel()
     mapping = _find(b, "bc mapping array")
     if mapping is not None:
         nshg = mapping[1][0]
@@ -137,10 +140,13 @@ def read_geombc(path):
 
 def read_restart(path):
     """solution y(nshg,ndof) and 'time derivative of solution' ac(nshg,ndof),
-    columns (p,u1,u2,u3,...) as PHASTA stores them, and the step number."""
+    columns (p,u1,u2,u3,...) as PHASTA stores them, the step number, and
+    when present the deformable wall's 'displacement' and 'displacement_ref'
+    (nshg,3) (readnblk.f90:589-640)."""
     b, o = read_blocks(path)
     out = {}
-    for key, name in (("solution", "y"), ("time derivative of solution", "ac")):
+    for key, name in (("solution", "y"), ("time derivative of solution", "ac"), ("displacement", "u"),
+                      ("displacement_ref", "uref")):
         blk = _find(b, key)
         if blk is not None:
             n, ndof = blk[1][0], blk[1][1]

@@ -13,7 +13,10 @@ Per timestep (itrPredict with ipred = 1, itrPC.f90:104-108; itrBC):
 
 Inlet and wall nodes carry the three velocity codes (inlet: parabolic
 profile of the mean velocity `inflow(t)`, walls: no slip); the outlet is
-natural with the resistance surface on its nodes that carry no code.
+natural with the resistance surface on its nodes that carry no code.  With a
+deformable wall (`wall`, ideformwall = 1) only the inlet nodes carry codes;
+the wall nodes move with the displacement u, predicted, set at alpha_f and
+corrected with the step (itrPC.f90:127-132, 183-186, 333-336).
 
 Tensors are torch tensors on the GPU in the C ABI's column-major layouts:
 y and ac as (4, nshg) C-ordered (y(nshg,4) Fortran), Res4 as (nshg, 4).
@@ -42,21 +45,31 @@ class ResistanceCylinder:
     it holds (BCint)."""
 
     def __init__(self, mesh, R=1333.0, params=None, inflow=lambda t: 10.0, boundary=True, ls=None,
-                 transpose=False, bct=None):
+                 transpose=False, bct=None, wall=None, nodetag=None, uref=None):
         import torch
         self.torch = torch
         self.dev = torch.device("cuda:0")
         self.mesh = m = mesh
         self.N = N = m.numnp
         self.R = float(R)
-        self.params = p = params if params is not None else form_params()
+        self.params = p = params if params is not None else form_params(ideformwall=1 if wall else 0)
         self.inflow = inflow
         self.bct = bct
+        self.wall = wall is not None
+        if self.wall and (p.ideformwall != 1 or not boundary):
+            raise ValueError("a deformable wall needs ideformwall = 1 and the boundary elements")
         self.colm, self.rowp = genadj(m.ien, N)
-        self.ibc, _ = M.dirichlet_codes(m)
+        if self.wall:
+            self.ibc = np.where(m.inlet, M.IBC_VEL, 0).astype(np.int32)
+        else:
+            self.ibc, _ = M.dirichlet_codes(m)
         self.form = ElmGMRInterior(m.ien, self.colm, self.rowp, N, iBC=self.ibc, params=p)
         if boundary:
             self.form.set_boundary(*M.cylinder_boundary(m))
+        if self.wall:
+            self.form.set_wall(wall, nodetag=nodetag, uref=uref)
+            self.ua = torch.zeros((3, N), dtype=torch.float64, device=self.dev)
+            self.form.set_displacement(self.ua)
         self.fixed = np.nonzero(self.ibc)[0]
         self.out = np.nonzero(m.outlet & (self.ibc == 0))[0]
         self.nabi = m.nabi()
@@ -114,13 +127,18 @@ class ResistanceCylinder:
         y, ac = M.poiseuille_state(self.mesh, seed=seed)
         return self.T(y), self.T(ac)
 
-    def residual(self, yold, acold, y, ac, ya=None, aca=None, lhs=None):
+    def residual(self, yold, acold, y, ac, ya=None, aca=None, lhs=None, uold=None, u=None):
         """Res4 of the current iterate (formation into `lhs`, default the
-        case's matrix; bc3Res; ElmpvsQ) in self.res4; returns its norm."""
+        case's matrix; bc3Res; ElmpvsQ) in self.res4; returns its norm.  With
+        a deformable wall, uold and u (3, nshg) give the displacement."""
         torch = self.torch
         if ya is None:
             ya, aca = torch.empty_like(y), torch.empty_like(ac)
         f = self.form
+        if self.wall:
+            if uold is None or u is None:
+                raise ValueError("a deformable wall's residual needs uold and u")
+            f.wall_alpha(uold, u, self.ua)
         f.yalpha(yold, acold, y, ac, ya, aca)
         f.sync()
         f.to_lhs(ya, aca, self.dx, self.res4, lhs if lhs is not None else self.lhs)
@@ -128,6 +146,13 @@ class ResistanceCylinder:
         f.elmpvsq(self.dglob, self.dnabi, self.R, ya, self.res4, self.dq)
         f.sync()
         return float(torch.linalg.vector_norm(self.res4))
+
+    def step_wall(self, yold, acold, uold, t, **kw):
+        """step() with the deformable wall's displacement: returns (y, ac,
+        u, residual norms)."""
+        if not self.wall:
+            raise ValueError("no deformable wall")
+        return self._step(yold, acold, t, uold=uold, **kw)
 
     def step(self, yold, acold, t, iters=3, tol=None, max_iters=200, unit_rhs=False, atol=0.0):
         """One timestep from (yold, acold) to time t.  With tol None, `iters`
@@ -138,17 +163,28 @@ class ResistanceCylinder:
         back): memLS's GE rejects pivots below 2 eps in absolute terms
         (memLS.f90:641-642), which the Gram matrix of a residual driven far
         down reaches.  Returns (y, ac, residual norms)."""
+        if self.wall:
+            raise ValueError("a deformable wall steps with step_wall")
+        y, ac, _, norms = self._step(yold, acold, t, iters=iters, tol=tol, max_iters=max_iters, unit_rhs=unit_rhs,
+                                     atol=atol)
+        return y, ac, norms
+
+    def _step(self, yold, acold, t, uold=None, iters=3, tol=None, max_iters=200, unit_rhs=False, atol=0.0):
         torch = self.torch
         f = self.form
         y, ac = torch.empty_like(yold), torch.empty_like(acold)
         ya, aca = torch.empty_like(yold), torch.empty_like(acold)
+        u = None
         dBC = self.bc(t)
         f.predict(yold, acold, y, ac)
+        if self.wall:
+            u = uold.clone()
+            f.wall_predict(uold, yold, acold, u)
         f.itr_bc(self.dibc, dBC, y)
         norms = []
         n = iters if tol is None else max_iters
         for k in range(n + (0 if tol is None else 1)):
-            norms.append(self.residual(yold, acold, y, ac, ya, aca))
+            norms.append(self.residual(yold, acold, y, ac, ya, aca, uold=uold, u=u))
             if tol is not None and (norms[-1] <= max(tol * norms[0], atol) or k == n):
                 break
             if unit_rhs:
@@ -157,6 +193,8 @@ class ResistanceCylinder:
             if unit_rhs:
                 self.res4 *= norms[-1]
             f.correct(y, ac, self.res4)
+            if self.wall:
+                f.wall_correct(u, self.res4)
             f.itr_bc(self.dibc, dBC, y)
             f.sync()
-        return y, ac, norms
+        return y, ac, u, norms

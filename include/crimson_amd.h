@@ -64,11 +64,35 @@ typedef struct {
   double aspect_fp64; /* aspect-ratio threshold of the FP64 tau island:
                          longest edge over shortest altitude, 1 for the
                          regular tet; used when precision == 32 (>= 1) */
-  int ideformwall;    /* "Deformable Wall": only 0 (rigid walls; vessel-wall
-                         faces, iBCB bit 4, then carry no membrane terms, as
-                         in the reference); 1 (the e3b.f90:488-562 membrane
-                         terms) gives CRIMSON_EUNSUPPORTED */
+  int ideformwall;    /* "Deformable Wall": 0 rigid walls (vessel-wall faces,
+                         iBCB bit 4, carry no traction and no membrane
+                         terms); 1 the vessel-wall faces carry e3bvar's
+                         membrane, mass, support and damping terms and their
+                         tangent (crimson_form_set_wall, required before the
+                         first formation) */
 } crimson_form_params;
+
+/* Deformable-wall values (solver.inp's wall block, input_fform.cxx:
+ * 1237-1262): an isotropic membrane of uniform thickness ("Number of Wall
+ * Properties per Node" 10, no SWB, TWB or BET files).  The time integration
+ * (Delt, alfi, almi, gami) is the formation's, betai = 1 (itrPC.f90:31-35). */
+typedef struct {
+  double rhovw;            /* Density of Vessel Wall */
+  double thicknessvw;      /* Thickness of Vessel Wall (SWB(:,1)) */
+  double evw;              /* Young Mod of Vessel Wall */
+  double rnuvw;            /* Poisson Ratio of Vessel Wall */
+  double rshearconstantvw; /* Shear Constant of Vessel Wall */
+  double tissSuppStiffCoeff;     /* Stiffness Coefficient for Tissue Support */
+  double tissSuppDampCoeff;      /* Damping Coefficient for Tissue Support */
+  double tissSuppRingStiffCoeff; /* Axial Tethering Stiffness Coefficient */
+  double tissSuppRingDampCoeff;  /* Axial Tethering Damping Coefficient */
+  int iwallmassfactor;     /* Wall Mass Matrix for LHS (1) */
+  int iwallstiffactor;     /* Wall Stiffness Matrix for LHS (1) */
+  int iwallsupp;           /* Wall External Support Term */
+  int iwalldamp;           /* Wall Damping Term */
+  int iringsupp;           /* Axial Tethering Stiffness Term */
+  int iringdamp;           /* Axial Tethering Damping Term */
+} crimson_wall_params;
 
 typedef struct crimson_form crimson_form;
 
@@ -138,24 +162,61 @@ int crimson_form_timing(const crimson_form *f, double *out4);
  * (elmgmr.f90:325-460: AsBMFG asbmfg.f90:1-80 -> e3b e3b.f90:1-560 -> e3bvar
  * e3bvar.f90).  After this call every formation of the handle adds the
  * boundary elements' residual rows to res / Res4 after the interior rows, as
- * the reference's boundary loop does; the matrix is unchanged (rigid walls:
- * the boundary xKebe is zero).  ienb(nelb,4) 1-based with the three face
+ * the reference's boundary loop does; with rigid walls the matrix is
+ * unchanged (the boundary xKebe is zero).  ienb(nelb,4) 1-based with the three face
  * nodes first (the mienb blocks concatenated), iBCB(nelb,2) (code bits: 1
  * mass flux, 2 pressure, 4 traction, 16 vessel wall; srfID), BCB(nelb,3,
  * ndBCB) (mass flux, pressure, traction x3; ndBCB >= 5 when bit 4 is used),
  * nqb = 1 or 3 boundary points ("Quadrature Rule on Boundary" 1 or 2), influx
  * = "Influx Coefficient", permit(0..nsrf) the flow-permitted flag of each
  * srfID (callCPPDiscoverWhetherFlowPermittedAcrossSurface, e3b.f90; NULL: all
- * permitted).  nelb = 0 removes the boundary.  Deformable walls, the heart
- * model, Lagrange multipliers and the surface flux integrals (flxID) are not
- * formed.  With ensemble instances each boundary element takes the
- * viscosity of the instance that holds its nodes.  The VJP
+ * permitted).  nelb = 0 removes the boundary (and the wall).  Deformable
+ * walls come with crimson_form_set_wall; the heart model and Lagrange
+ * multipliers are not formed.  With ensemble instances each boundary element
+ * takes the viscosity of the instance that holds its nodes.  The VJP
  * (crimson_form_vjp_dev) of a handle with boundary elements includes their
- * rows. */
+ * rows (rigid walls only). */
 int crimson_form_set_boundary(crimson_form *f, int nelb, const int *ienb,
                               const int *iBCB, const double *BCB, int ndBCB,
                               int nqb, double influx, const int *permit,
                               int nsrf);
+
+/* Deformable walls (ideformwall = 1; vlmwStTri vlmwsttri.f90:60-210 and
+ * gendat.f90:172-225 at set-up): after crimson_form_set_boundary, the faces
+ * with iBCB bit 4 become the vessel wall and every formation adds, on them,
+ * e3bvar's wall terms (e3bvar.f90:462-1017 through e3b.f90:374-575): the
+ * membrane B^T D B (u + uref) in each face's flat frame, the wall mass
+ * rhovw h ac, the tissue support k u and damping c v, the ring tethering
+ * along the end rings, and their tangent into the velocity blocks of the
+ * matrix (before bc3lhs, as AsBMFG's xKebe).  nodetag(numnp): the node tags
+ * (bit 0 "ring"; NULL when neither ring term is on); uref(nshg,3)
+ * column-major: the reference displacement ("displacement_ref", NULL:
+ * zero).  The wall nodes (mWNodes) are the faces' nodes.  A face with three
+ * ring nodes is refused (CRIMSON_EUNSUPPORTED: e3bvar's tethering direction
+ * is then 0/0); a wall face whose plane passes through the origin gives
+ * non-finite rows, as e3bvar's flat-frame inverse does.  Calling
+ * crimson_form_set_boundary again removes the wall.  The VJP and the
+ * captured timestep (crimson_step_graph_create) refuse a handle with a
+ * wall. */
+int crimson_form_set_wall(crimson_form *f, const crimson_wall_params *w,
+                          const int *nodetag, const double *uref);
+/* The displacement u(nshg,3) (column-major, device) at which the following
+ * formations evaluate the wall terms: the reference passes uAlpha to ElmGMR
+ * (solfar.f90:148).  The pointer is kept; it must stay valid while used. */
+int crimson_form_set_displacement_dev(crimson_form *f, const double *d_u);
+/* The wall nodes' displacement in the generalized-alpha step, on device
+ * arrays u(nshg,3), y(nshg,4), ac(nshg,4), Res4(4,nshg):
+ *   predict  u = uold + Delt yold + (gami - 2 betai)/(2 gami) Delt^2 acold
+ *            (itrPredict, itrPC.f90:127-132)
+ *   correct  u += Delt^2 betai solinc (itrCorrect, itrPC.f90:183-186)
+ *   alpha    uAlpha = uold + alfi (u - uold) (itrYAlpha, itrPC.f90:333-336)
+ * Other nodes are left alone. */
+int crimson_wall_predict_dev(crimson_form *f, const double *d_uold,
+                             const double *d_yold, const double *d_acold,
+                             double *d_u);
+int crimson_wall_correct_dev(crimson_form *f, double *d_u, const double *d_Res4);
+int crimson_wall_alpha_dev(crimson_form *f, const double *d_uold,
+                           const double *d_u, double *d_uAlpha);
 
 /* Autotuner (no counterpart in the reference): times the formation into
  * the memLS handle `lhs` (crimson_form_interior_to_lhs, reps calls after one

@@ -1,7 +1,8 @@
 """Writes tests/golden/basic_case.npz from the reference's mainTests/basic
 input files (testbin/mainTests/basic/geombc.dat.1 and restart.0.1, the
 1072-node / 4157-tet cylinder its own end-to-end tests run), read with the
-package's PHASTA reader, and tests/golden/basic_bct.npz from the same case's
+package's PHASTA reader (with the node tags and the restart's wall displacement
+and reference displacement), and tests/golden/basic_bct.npz from the same case's
 bct.dat (26 inlet nodes x 1101 samples) together with the inlet velocities
 of testbin/mainTests/restart/restart.5.1, the state the reference wrote
 after 5 steps of 0.001 on the same mesh and bct.dat.  Data only.  Run in a
@@ -25,7 +26,8 @@ def main(ref):
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "basic_case.npz")
     np.savez_compressed(out, x=g["x"], ien=g["ien"], ienb=g["ienb"], iBCB=g["iBCB"], BCB=g["BCB"],
                         iBC=g["iBC"], BC=g["BC"], y=phasta.to_solver_order(r["y"]),
-                        ac=phasta.to_solver_order(r["ac"]), step=np.int32(r.get("step", 0)))
+                        ac=phasta.to_solver_order(r["ac"]), step=np.int32(r.get("step", 0)),
+                        nodetag=g["nodetag"], u=r["u"], uref=r["uref"])
     print("wrote", out)
     recs = bct.read_bct(os.path.join(d, "bct.dat"))
     tab = np.stack([t for _, t in recs])

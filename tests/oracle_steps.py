@@ -7,7 +7,12 @@ import numpy as np
 
 
 class OracleCylinder:
-    def __init__(self, O, mesh, R=1333.0, mu=0.04, inflow=lambda t: 10.0, relTol=1e-6):
+    """With `wall` (oracle.make_wall), the vessel wall is deformable: only
+    the inlet nodes carry velocity codes and the wall nodes' displacement
+    follows itrPC.f90's predictor, alpha_f level and corrector."""
+
+    def __init__(self, O, mesh, R=1333.0, mu=0.04, inflow=lambda t: 10.0, relTol=1e-6, wall=None, nodetag=None,
+                 uref=None):
         from crimsonflowsolver_amd import mesh as M
         self.O, self.m = O, mesh
         self.N = N = mesh.numnp
@@ -15,7 +20,11 @@ class OracleCylinder:
         self.p = O.make_params(mu=mu)
         self.mu = mu
         self.colm, self.rowp = O.genadj(mesh.ien, N)
-        self.ibc, _ = M.dirichlet_codes(mesh)
+        self.wall, self.tag, self.uref = wall, nodetag, uref
+        if wall is not None:
+            self.ibc = np.where(mesh.inlet, M.IBC_VEL, 0).astype(np.int32)
+        else:
+            self.ibc, _ = M.dirichlet_codes(mesh)
         self.bnd = M.cylinder_boundary(mesh)
         self.nabi = mesh.nabi()
         self.out = np.nonzero(mesh.outlet & (self.ibc == 0))[0]
@@ -36,25 +45,37 @@ class OracleCylinder:
         bc[self.inl, 4] = self.inflow(t) * self.prof
         return bc
 
-    def residual(self, yold, acold, y, ac):
+    def residual(self, yold, acold, y, ac, uold=None, u=None):
         O, p, m = self.O, self.p, self.m
         ya, aca = O.itr_yalpha(yold, acold, y, ac, p.alfi, p.almi)
         res, K, P = O.elmgmr_interior(m.ien, ya, aca, m.x, self.colm, self.rowp, p)
-        res = O.boundary_res(*self.bnd, ya, m.x, res, mu=self.mu)
+        if self.wall is not None:
+            ua = uold + p.alfi * (u - uold)  # itrYAlpha (itrPC.f90:333-336), on every node here
+            res = O.boundary_wall(*self.bnd, ya, aca, ua, m.x, res, self.wall, uref=self.uref, tag=self.tag,
+                                  mu=self.mu, ibc=self.ibc, bc=np.zeros((self.N, 6)), colm=self.colm,
+                                  rowp=self.rowp, lhsK=K)
+        else:
+            res = O.boundary_res(*self.bnd, ya, m.x, res, mu=self.mu)
         res = O.bc3res(self.ibc, res)
         res, _ = O.elmpvsq_resistance(res, ya, self.out, self.nabi[self.out], self.R)
         return res, K, P
 
-    def step(self, yold, acold, t, tol=1e-12, max_iters=200, atol=0.0):
+    def step(self, yold, acold, t, tol=1e-12, max_iters=200, atol=0.0, uold=None):
         """Iterations until |res| <= max(tol |res_0|, atol); returns (y, ac,
-        norms)."""
+        norms), or (y, ac, u, norms) with a deformable wall."""
         O, p = self.O, self.p
         bc = self.bc(t)
         y, ac = O.itr_predict(yold, acold, p.gami)
+        u = None
+        if self.wall is not None:
+            betai, wn = 1.0, self.wall_nodes
+            u = uold.copy()
+            u[wn] = (uold[wn] + p.Delt * yold[wn, :3] +
+                     0.5 * ((p.gami - 2.0 * betai) / p.gami) * p.Delt * p.Delt * acold[wn, :3])
         y = O.itr_bc(self.ibc, bc, y)
         norms = []
         for k in range(max_iters + 1):
-            res, K, P = self.residual(yold, acold, y, ac)
+            res, K, P = self.residual(yold, acold, y, ac, uold, u)
             norms.append(float(np.linalg.norm(res)))
             if norms[-1] <= max(tol * norms[0], atol) or k == max_iters:
                 break
@@ -64,5 +85,17 @@ class OracleCylinder:
             sol = self.ls.solve(np.asfortranarray(res.T) / norms[-1], O.pack_val4(self.colm, self.rowp, K, P),
                                 res=self.faceres) * norms[-1]
             y, ac = O.itr_correct(y, ac, sol, p.gami, p.alfi, p.Delt)
+            if self.wall is not None:
+                u[self.wall_nodes] += p.Delt * p.Delt * 1.0 * np.asarray(sol).T[self.wall_nodes, :3]
             y = O.itr_bc(self.ibc, bc, y)
+        if self.wall is not None:
+            return y, ac, u, norms
         return y, ac, norms
+
+    @property
+    def wall_nodes(self):
+        """The deformable wall's nodes: those of the boundary faces with
+        iBCB bit 4 (gendat.f90:172-225)."""
+        ienb, iBCB, _ = self.bnd
+        f = np.asarray(ienb)[(np.asarray(iBCB)[:, 0] & 16) != 0, :3] - 1
+        return np.unique(f)

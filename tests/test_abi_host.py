@@ -109,7 +109,7 @@ def _small_graph():
 
 @pytest.mark.parametrize("field,value,code", [("itau", 1, 2), ("iconvflow", 1, 2), ("idiff", 2, 2),
                                               ("ipord", 2, 2), ("precision", 16, 2), ("rho", -1.0, 1),
-                                              ("ideformwall", 1, 2)])
+                                              ("ideformwall", 2, 1)])
 def test_form_setup_rejects_unsupported_flags(cfa, field, value, code):
     m, colm, rowp = _small_graph()
     prm = cfa.form_params()
