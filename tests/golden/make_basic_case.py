@@ -24,10 +24,13 @@ def main(ref):
     g = phasta.read_geombc(os.path.join(d, "geombc.dat.1"))
     r = phasta.read_restart(os.path.join(d, "restart.0.1"))
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "basic_case.npz")
+    blocks, order = phasta.read_blocks(os.path.join(d, "geombc.dat.1"))
+    nds = phasta._find(blocks, "global node surface number")
+    ndsurf = phasta._arr(nds, order, "i4", (nds[1][0],)).ravel()
     np.savez_compressed(out, x=g["x"], ien=g["ien"], ienb=g["ienb"], iBCB=g["iBCB"], BCB=g["BCB"],
                         iBC=g["iBC"], BC=g["BC"], y=phasta.to_solver_order(r["y"]),
                         ac=phasta.to_solver_order(r["ac"]), step=np.int32(r.get("step", 0)),
-                        nodetag=g["nodetag"], u=r["u"], uref=r["uref"])
+                        nodetag=g["nodetag"], u=r["u"], uref=r["uref"], ndsurf=ndsurf)
     print("wrote", out)
     recs = bct.read_bct(os.path.join(d, "bct.dat"))
     tab = np.stack([t for _, t in recs])
@@ -39,6 +42,24 @@ def main(ref):
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "basic_bct.npz")
     np.savez_compressed(out, x=np.stack([x for x, _ in recs]), u=tab[:, :, :3], t=tab[0, :, 3],
                         step5=np.int32(r5["step"]), u5_nodes=attached, u5=y5[attached, :3])
+    print("wrote", out)
+    # the reference's own run of this case (5 steps of 0.001; the restart
+    # test's inputs are its outputs): state at step 5, the RCR outlet's flow
+    # and pressure history, the inlet/outlet flow and mean pressure history,
+    # and the outlet's RCR values (rcrt.dat: Rp, C, Rd, distal pressure table)
+    rdir = os.path.join(ref, "testbin", "mainTests", "restart")
+
+    def table(name, ncol, skip):
+        rows = [ln.split() for ln in open(os.path.join(rdir, name)).read().splitlines()[skip:] if ln.strip()]
+        return np.array([[float(v) for v in r[:ncol]] for r in rows])
+
+    vals = [float(v) for v in open(os.path.join(d, "rcrt.dat")).read().split()]
+    npd = int(vals[1])
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "basic_outputs.npz")
+    np.savez_compressed(out, y5=y5, ac5=phasta.to_solver_order(r5["ac"]), u5=r5["u"], uref5=r5["uref"],
+                        qhist=table("QHistRCR.dat", 2, 0), phist=table("PHistRCR.dat", 2, 0),
+                        flowhist=table("FlowHist.dat", 2, 1), presshist=table("PressHist.dat", 2, 1),
+                        rcr=np.array(vals[2:5]), pdist=np.array(vals[5:5 + 2 * npd]).reshape(npd, 2))
     print("wrote", out)
 
 
