@@ -366,8 +366,8 @@ def main():
     ap.add_argument("--grid", default="64,64,320")
     ap.add_argument("--timesteps", type=int, default=10,
                     help="timesteps of 3 nonlinear iterations to time, FP64 and mixed (0: skip)")
-    ap.add_argument("--max-nonlinear", type=int, default=40,
-                    help="nonlinear iterations allowed to reach the 1e-10 residual in the converged timestep")
+    ap.add_argument("--max-nonlinear", type=int, default=60,
+                    help="nonlinear iterations allowed to reach the 1e-12 residual in the converged timestep")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0: skip)")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision timesteps (0: skip)")
@@ -659,7 +659,7 @@ def main():
                 log(f"{label} timestep {n + 1}: {times[-1] * 1e3:.1f} ms, solver (RI,GM,CG itr, rel) {its[-3:]}")
             return times, its, yc, rnorms
 
-        def converge_step(fm, lh, label, dx=dx, dnabi=dnabi, tol=1e-10, max_it=args.max_nonlinear):
+        def converge_step(fm, lh, label, dx=dx, dnabi=dnabi, tol=1e-12, max_it=args.max_nonlinear):
             """The first timestep driven until its FP64 nonlinear residual
             (Res4 after bc3Res and ElmpvsQ, formed in FP64 by both forms) is
             <= tol times its value at the first nonlinear iteration, each
@@ -711,7 +711,7 @@ def main():
 
         def conv_pair(fa, la, fb, lb, tag, target=1e-8, **kw):
             """Time to equal final error.  The FP64 form is driven to a
-            residual of 1e-10 of the first; its last state is the FP64
+            residual of 1e-12 of the first; its last state is the FP64
             solution y*.  For each form (FP64, mixed), the time to target is
             the time at which it first holds a state whose FP64 residual is
             <= target of the first and whose velocity and pressure are each
@@ -720,7 +720,7 @@ def main():
             ta, ra, ia, ysa = converge_step(fa, la, f"{tag}fp64", **kw)
             tb, rb, ib, ysb = converge_step(fb, lb, f"{tag}mixed", **kw)
             ystar = ysa[-1]
-            ref_ok = ra[-1] <= 1e-10 * ra[0]
+            ref_ok = ra[-1] <= 1e-12 * ra[0]
 
             def first_hit(tk, rn, ys):
                 errs = [rel_err(v, ystar) for v in ys]
@@ -732,7 +732,7 @@ def main():
             ka, ea = first_hit(ta, ra, ysa)
             kb, eb = first_hit(tb, rb, ysb)
             ok = ref_ok and ka is not None and kb is not None
-            out = {"reference": "FP64 driven to 1e-10 of the first residual",
+            out = {"reference": "FP64 driven to 1e-12 of the first residual",
                    "reference_reached": bool(ref_ok),
                    "fp64_ms": ta[ka] * 1e3 if ka is not None else None,
                    "mixed_ms": tb[kb] * 1e3 if kb is not None else None,
