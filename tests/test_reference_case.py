@@ -14,6 +14,12 @@ deformable wall and its ring tethering and reference displacement, the
 bct.dat inflow, the RCR coupling (ElmpvsQ and memLS's coupled face), the NS
 solver at the case's loose tolerances and the wall displacement's
 generalized-alpha update.
+
+The reference's restart test (testMain.cxx:25-42,
+checkRestartWorks_RCRSimpleShortSimulation) continues from its restart.5.1
+for 5 more steps, the RCR pressure taken from PHistRCR row 5 and the flow
+from the restart state, and asks for PHistRCR row 10 = 11625.41088 within
+1e-4: `test_restart_continues_as_the_reference_test_asks`.
 """
 import os
 import sys
@@ -92,3 +98,26 @@ def test_product_surface_integrals_agree(run):
     n = nabi[out].sum(axis=0)
     assert abs(np.linalg.norm(n) - area) <= 1e-12 * area
     assert abs(abs(n[2]) - area) <= 1e-12 * area  # the outlet is the plane z = const
+
+
+def test_restart_continues_as_the_reference_test_asks():
+    sys.path.insert(0, HERE)
+    from oracle import oracle as O
+    from oracle_case import OracleBasicCase, OUTLET
+    z = np.load(os.path.join(HERE, "golden", "basic_case.npz"))
+    b = np.load(os.path.join(HERE, "golden", "basic_bct.npz"))
+    o = np.load(os.path.join(HERE, "golden", "basic_outputs.npz"))
+    recs = [(b["x"][k], np.concatenate([b["u"][k], b["t"][:, None]], axis=1)) for k in range(b["x"].shape[0])]
+    c = OracleBasicCase(O, z, recs, o)
+    # the restart: restart.5.1's state, currentTimestepIndex 5, the RCR's
+    # pressure from PHistRCR row 5 and its flow from the restart state
+    c.y = np.asfortranarray(o["y5"])
+    c.ac = np.asfortranarray(o["ac5"])
+    c.u = np.asfortranarray(o["u5"])
+    c.n = 5
+    c.rcr.P = float(o["phist"][4, 1])
+    c.rcr.Q = c.flow(c.y, OUTLET)
+    log = [c.step() for _ in range(5)]
+    p10 = log[-1]["P"]
+    print("PHistRCR rows 6-10:", [r["P"] for r in log])
+    assert abs(p10 - 0.1162541088e5) <= 10e-5  # testMain.cxx:38
