@@ -231,7 +231,7 @@ class PhastaCase:
             m = self.rcr[s]
             m.P = cu[s][0] * Q1 + cu[s][1]
             m.Q = Q1
-            rec[s] = {"Q": Q1, "P": m.P}
+            rec[s] = {"Q": Q1, "P": m.P, "p_mean": self.mean_pressure(s)}
         return rec
 
     def restart(self, y, ac, u, step, pressures):

@@ -74,6 +74,18 @@ def test_histories_every_step(run):
         assert all(e <= 1e-9 for e in errs.values()), (n + 1, errs)
 
 
+def test_step_4_meets_the_emulation_tests(run):
+    """testMain.cxx:102-159 (checkCoronaryCanEmulateKnownRCRResults,
+    checkNetlistCanEmulateKnownRCRResults) hold this run's outlet mean
+    pressure and flow after step 4 as the values an RCR-emulating outlet
+    must give, within 1e-7."""
+    c, o, log, _ = run
+    r = log[3]
+    print("step 4 outlet mean pressure", repr(r["p_out"]), "flow", repr(r["Q"]))
+    assert abs(r["p_out"] - 10645.5858581080) <= 1e-7
+    assert abs(r["Q"] - 478.116982120136) <= 1e-7
+
+
 def test_state_after_five_steps(run):
     c, o, log, _ = run
     rel = lambda a, b: float(np.abs(a - b).max() / np.abs(b).max())
