@@ -358,6 +358,35 @@ def implicit_adjoint(cfa, M, torch, m, nsteps, every, R=1333.0):
     return out
 
 
+def reference_case(cfa, np):
+    """The reference's basic case on the device (phasta_case.PhastaCase): 5
+    steps from its restart.0.1; returns the outlet flow after each step,
+    the reference's values and the wall time per step."""
+    from crimsonflowsolver_amd.phasta_case import PhastaCase, wall_from_solver_inp
+    g = os.path.join(ROOT, "tests", "golden")
+    z = np.load(os.path.join(g, "basic_case.npz"))
+    b = np.load(os.path.join(g, "basic_bct.npz"))
+    o = np.load(os.path.join(g, "basic_outputs.npz"))
+    recs = [(b["x"][k], np.concatenate([b["u"][k], b["t"][:, None]], axis=1)) for k in range(b["x"].shape[0])]
+    c = PhastaCase(z["x"], z["ien"], z["ienb"], z["iBCB"], z["BCB"], z["iBC"], z["y"], z["ac"], z["u"], z["uref"],
+                   z["nodetag"], z["ndsurf"], recs, {3: (o["rcr"][0], o["rcr"][1], o["rcr"][2], o["pdist"])},
+                   rho=0.00106, mu=0.004, Delt=0.001, rhoinf=0.0, wall=wall_from_solver_inp(), dirichlet_srfs=(2,))
+    qs, ts, its = [], [], []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        r = c.step()
+        ts.append((time.perf_counter() - t0) * 1e3)
+        qs.append(r[3]["Q"])
+        its.append([list(map(float, v[1:])) for v in r["iterations"]])
+    out = {"case": "testbin/mainTests/basic (1072 nodes, 4157 tets, deformable wall, bct.dat, RCR outlet)",
+           "outlet_flow": qs, "reference_outlet_flow": [float(v) for v in o["qhist"][:, 1]],
+           "step5_abs_diff": abs(qs[-1] - 714.921079082528), "reference_test_tolerance": 1e-7,
+           "ms_per_step": ts, "solver_iterations": its}
+    log(f"reference basic case: outlet flow after step 5 {qs[-1]!r} (reference 714.921079082528), "
+        f"{np.mean(ts[1:]):.1f} ms per step")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -391,6 +420,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=10,
                     help="timesteps of the implicit-step adjoint (0: skip; the workload's figure is 50)")
     ap.add_argument("--adjoint-every", type=int, default=5, help="its checkpoint interval (the workload's: 10)")
+    ap.add_argument("--reference-case", type=int, default=1,
+                    help="run the reference's basic case (5 steps) against its own output (0: skip)")
     ap.add_argument("--config1", type=int, default=1,
                     help="also run the 20x20x100 case under both solver readings (GMRESV 4x4 and NS)")
     ap.add_argument("--autotune", type=int, default=1,
@@ -899,6 +930,15 @@ def main():
                 f"iterations {c1[label]['iterations']}, converged {ls1.RI['suc']}")
         del f1, lh1, r1
 
+    # ---- the reference's own basic case (testbin/mainTests/basic, held in
+    # tests/golden: 1072 nodes, deformable wall, bct.dat inflow, RCR outlet)
+    # for its 5 steps on the device, against the reference's output of that
+    # run: the RCR outlet's flow after step 5 (its test: 714.921079082528
+    # within 1e-7) and the per-step wall time
+    refcase = None
+    if args.reference_case:
+        refcase = reference_case(cfa, np)
+
     # ---- measured ceilings of this device (a plain copy, FMA chains)
     peaks = cfa.measure_peaks(1 << 31)
     log(f"measured ceilings: copy {peaks['copy_gbs']:.0f} GB/s, FP64 FMA {peaks['fp64_tflops']:.1f} TFLOP/s, "
@@ -984,6 +1024,7 @@ def main():
         "ensemble": ens_info,
         "case_20x20x100": c1,
         "adjoint": adj,
+        "reference_case": refcase,
         "ceilings": {"measured": peaks, "published": {"hbm_gbs": HBM_PEAK_GBS, "fp64_tflops": FP64_PEAK_TFLOPS,
                                                       "fp32_tflops": 157.3}},
     }
