@@ -47,7 +47,7 @@ EXPORTS = (
     "crimson_measure_peaks", "crimson_form_vjp_ac_dev", "crimson_elmpvsq_vjp_dev", "crimson_lhs_transpose_dev",
     "crimson_ls_graph_create", "crimson_step_graph_create", "crimson_form_set_wall",
     "crimson_form_set_displacement_dev", "crimson_wall_predict_dev", "crimson_wall_correct_dev",
-    "crimson_wall_alpha_dev",
+    "crimson_wall_alpha_dev", "crimson_form_set_wall_state_dev",
 )
 
 
@@ -151,6 +151,7 @@ def lib():
                                                 [C.c_int, C.POINTER(C.c_void_p)])
         L.crimson_form_set_wall.argtypes = [vp, C.POINTER(WallParams), vp, vp]
         L.crimson_form_set_displacement_dev.argtypes = [vp, vp]
+        L.crimson_form_set_wall_state_dev.argtypes = [vp, vp, vp]
         L.crimson_wall_predict_dev.argtypes = [vp, vp, vp, vp, vp]
         L.crimson_wall_correct_dev.argtypes = [vp, vp, vp]
         L.crimson_wall_alpha_dev.argtypes = [vp, vp, vp, vp]
@@ -346,6 +347,13 @@ class ElmGMRInterior:
         Fortran layout) of the following formations; kept alive here."""
         self._disp = u_dev
         check(lib().crimson_form_set_displacement_dev(self.h, _p(u_dev)), "crimson_form_set_displacement_dev")
+
+    def set_wall_state(self, uold_dev, u_dev):
+        """The displacement at t_n and t_{n+1} a captured timestep
+        (step_graph) updates; kept alive here."""
+        self._wstate = (uold_dev, u_dev)
+        check(lib().crimson_form_set_wall_state_dev(self.h, _p(uold_dev), _p(u_dev)),
+              "crimson_form_set_wall_state_dev")
 
     def wall_predict(self, uold, yold, acold, u):
         check(lib().crimson_wall_predict_dev(self.h, _p(uold), _p(yold), _p(acold), _p(u)),

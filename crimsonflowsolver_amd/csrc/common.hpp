@@ -132,6 +132,14 @@ hipStream_t form_stream(const crimson_form *f);
 int form_nshg(const crimson_form *f);
 crimson_form_params form_params_of(const crimson_form *f);
 bool form_has_wall(const crimson_form *f);  // crimson_form_set_wall holds
+// The wall displacement's steps inside a captured timestep (no-ops without
+// a wall): predict u from uold, uAlpha, correct u, uold = u at the end.
+// A wall needs crimson_form_set_wall_state_dev first (CRIMSON_ESTATE).
+int form_wall_step_ready(crimson_form *f);
+int form_wall_enqueue_predict(crimson_form *f, const double *d_yold, const double *d_acold);
+int form_wall_enqueue_alpha(crimson_form *f);
+int form_wall_enqueue_correct(crimson_form *f, const double *d_Res4);
+int form_wall_enqueue_update(crimson_form *f);
 
 // What a graph of the solver side adds to a replay: the state it holds on
 // the device, read back after each launch, and the buffers and settings it

@@ -228,6 +228,7 @@ struct crimson_form {
   crimson::DBuf<int> d_wtag;       // (3, nwf): the face nodes' ring bits
   crimson::DBuf<double> d_uref;    // uref(nshg,3)
   const double *d_u = nullptr;     // the displacement of the next formations (the caller's)
+  double *d_wuold = nullptr, *d_wu = nullptr;  // u at t_n and t_{n+1} a captured timestep updates
   crimson::DBuf<double> d_wres;    // (9, nwf): the wall rows of the three face nodes
   crimson::DBuf<double> d_wrec;    // (81, nwf): xKebe(9, a, b) of the face node pairs
   crimson::DBuf<int> d_wk, d_wptr, d_wsrc;  // the tangent gather: block k, its (face, pair) sources

@@ -2907,21 +2907,26 @@ int step_graph_create_t(crimson_form *f, crimson_lhs *l, crimson_ls *ls, double 
   if (!rc)
     rc = capture_into<T>(g, s, ex->w, [&]() -> int {
       CR_TRY(crimson_itr_predict_dev(f, d_yold, d_acold, d_y, d_ac, prm.gami));
+      CR_TRY(crimson::form_wall_enqueue_predict(f, d_yold, d_acold));
       CR_TRY(crimson_itr_bc_dev(f, d_iBC, d_BC, ndofBC, d_y));
       for (int k = 0; k < nonlinear; ++k) {
+        CR_TRY(crimson::form_wall_enqueue_alpha(f));
         CR_TRY(crimson_itr_yalpha_dev(f, d_yold, d_acold, d_y, d_ac, d_ya, d_aca, prm.alfi, prm.almi));
         CR_TRY(crimson::form_enqueue_to_lhs(f, d_ya, d_aca, d_x, d_Res4, l));
-        CR_TRY(crimson::form_enqueue_bc3res(f, d_Res4, 1));
+        // ElmpvsQ, then bc3Res (elmgmr.f90:528-555)
         for (int q = 0; q < nsurf; ++q)
           CR_TRY(crimson_elmpvsq_coef_dev(f, surf_ptr[q + 1] - surf_ptr[q], d_glob + surf_ptr[q],
                                           d_nabi + (size_t)3 * surf_ptr[q], c1[q], c2[q], d_ya, d_Res4, d_q + q));
+        CR_TRY(crimson::form_enqueue_bc3res(f, d_Res4, 1));
         CR_TRY(ns_graph_enqueue<T>(ex->w, d_Res4));
         CR_TRY(ex->record_stats(k));
         CR_TRY(crimson_itr_correct_dev(f, d_y, d_ac, d_Res4, prm.gami, prm.alfi, prm.Delt));
+        CR_TRY(crimson::form_wall_enqueue_correct(f, d_Res4));
         CR_TRY(crimson_itr_bc_dev(f, d_iBC, d_BC, ndofBC, d_y));
       }
       CR_HIP(hipMemcpyAsync(d_yold, d_y, (size_t)4 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
       CR_HIP(hipMemcpyAsync(d_acold, d_ac, (size_t)4 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
+      CR_TRY(crimson::form_wall_enqueue_update(f));
       return CRIMSON_OK;
     });
   if (rc) {
@@ -2966,10 +2971,7 @@ extern "C" int crimson_step_graph_create(crimson_form *f, crimson_lhs *lhs, crim
     crimson::set_error("crimson_step_graph_create: invalid arguments");
     return CRIMSON_EARG;
   }
-  if (crimson::form_has_wall(f)) {
-    crimson::set_error("crimson_step_graph_create: a deformable wall's displacement update is not captured");
-    return CRIMSON_EUNSUPPORTED;
-  }
+  CR_TRY(crimson::form_wall_step_ready(f));
   for (int q = 0; q < nsurf; ++q)
     if (surf_ptr[q + 1] < surf_ptr[q]) {
       crimson::set_error("crimson_step_graph_create: surf_ptr not monotone");

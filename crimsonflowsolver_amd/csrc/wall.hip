@@ -392,6 +392,33 @@ int form_wall_pass(crimson_form *f, int lhs, int mode, double *lhsK, double *V, 
   return CRIMSON_OK;
 }
 bool form_has_wall(const crimson_form *f) { return f->wall; }
+int form_wall_step_ready(crimson_form *f) {
+  if (!f->wall) return CRIMSON_OK;
+  if (!f->d_wuold || !f->d_wu || !f->d_u) {
+    set_error("a captured timestep with a deformable wall needs crimson_form_set_wall_state_dev and "
+              "crimson_form_set_displacement_dev");
+    return CRIMSON_ESTATE;
+  }
+  return CRIMSON_OK;
+}
+int form_wall_enqueue_predict(crimson_form *f, const double *d_yold, const double *d_acold) {
+  if (!f->wall) return CRIMSON_OK;
+  return crimson_wall_predict_dev(f, f->d_wuold, d_yold, d_acold, f->d_wu);
+}
+int form_wall_enqueue_alpha(crimson_form *f) {
+  if (!f->wall) return CRIMSON_OK;
+  return crimson_wall_alpha_dev(f, f->d_wuold, f->d_wu, const_cast<double *>(f->d_u));
+}
+int form_wall_enqueue_correct(crimson_form *f, const double *d_Res4) {
+  if (!f->wall) return CRIMSON_OK;
+  return crimson_wall_correct_dev(f, f->d_wu, d_Res4);
+}
+int form_wall_enqueue_update(crimson_form *f) {
+  if (!f->wall) return CRIMSON_OK;
+  CR_HIP(hipMemcpyAsync(f->d_wuold, f->d_wu, (size_t)3 * f->nshg * sizeof(double), hipMemcpyDeviceToDevice,
+                        f->stream));
+  return CRIMSON_OK;
+}
 int form_wall_rows(crimson_form *f) {
   if (!f->wall || f->nwf == 0) return CRIMSON_OK;
   wall_rows_add<<<grid_for(f->nwf, 256), 256, 0, f->stream>>>(f->nwf, f->d_wface.p, f->d_wres.p, f->d_brec.p);
@@ -541,6 +568,14 @@ extern "C" int crimson_form_set_wall(crimson_form *f, const crimson_wall_params 
     f->d_uref.release();
   }
   CR_HIP(hipStreamSynchronize(s));
+  return CRIMSON_OK;
+}
+
+extern "C" int crimson_form_set_wall_state_dev(crimson_form *f, double *d_uold, double *d_u) {
+  if (!f || !d_uold || !d_u) return crimson::set_error("crimson_form_set_wall_state_dev: null argument"), CRIMSON_EARG;
+  f->d_wuold = d_uold;
+  f->d_wu = d_u;
+  f->wall_gen++;
   return CRIMSON_OK;
 }
 

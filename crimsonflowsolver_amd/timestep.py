@@ -142,8 +142,8 @@ class ResistanceCylinder:
         f.yalpha(yold, acold, y, ac, ya, aca)
         f.sync()
         f.to_lhs(ya, aca, self.dx, self.res4, lhs if lhs is not None else self.lhs)
+        f.elmpvsq(self.dglob, self.dnabi, self.R, ya, self.res4, self.dq)  # ElmpvsQ, then bc3Res (elmgmr.f90:528-555)
         f.bc3res(self.res4, res4_layout=True)
-        f.elmpvsq(self.dglob, self.dnabi, self.R, ya, self.res4, self.dq)
         f.sync()
         return float(torch.linalg.vector_norm(self.res4))
 

@@ -195,15 +195,21 @@ int crimson_form_set_boundary(crimson_form *f, int nelb, const int *ienb,
  * ring nodes is refused (CRIMSON_EUNSUPPORTED: e3bvar's tethering direction
  * is then 0/0); a wall face whose plane passes through the origin gives
  * non-finite rows, as e3bvar's flat-frame inverse does.  Calling
- * crimson_form_set_boundary again removes the wall.  The VJP and the
- * captured timestep (crimson_step_graph_create) refuse a handle with a
- * wall. */
+ * crimson_form_set_boundary again removes the wall.  The VJP refuses a
+ * handle with a wall; the captured timestep (crimson_step_graph_create)
+ * carries the wall displacement given by crimson_form_set_wall_state_dev. */
 int crimson_form_set_wall(crimson_form *f, const crimson_wall_params *w,
                           const int *nodetag, const double *uref);
 /* The displacement u(nshg,3) (column-major, device) at which the following
  * formations evaluate the wall terms: the reference passes uAlpha to ElmGMR
  * (solfar.f90:148).  The pointer is kept; it must stay valid while used. */
 int crimson_form_set_displacement_dev(crimson_form *f, const double *d_u);
+/* The displacement at t_n and t_{n+1} (device u(nshg,3) arrays) that a
+ * timestep captured by crimson_step_graph_create updates with the wall's
+ * predictor, alpha level and corrector, copying u into uold at its end (as
+ * it copies y and ac); with crimson_form_set_displacement_dev's uAlpha.
+ * Pointers kept; no counterpart in the reference (its arrays are global). */
+int crimson_form_set_wall_state_dev(crimson_form *f, double *d_uold, double *d_u);
 /* The wall nodes' displacement in the generalized-alpha step, on device
  * arrays u(nshg,3), y(nshg,4), ac(nshg,4), Res4(4,nshg):
  *   predict  u = uold + Delt yold + (gami - 2 betai)/(2 gami) Delt^2 acold

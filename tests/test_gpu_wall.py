@@ -12,6 +12,9 @@ test_wall_pins.py):
   displacement's predictor, alpha level and corrector) against the oracle's
   timesteps, each iterated to a residual 1e-10 of its first: y, ac and the
   displacement within 1e-8 relative after 3 steps;
+* a timestep captured with the wall (crimson_step_graph_create with the
+  wall displacement's predictor, alpha level and corrector) equals the same
+  timestep run entry point by entry point, bit for bit, over 3 replays;
 * the entry points that do not carry the wall refuse it.
 """
 import numpy as np
@@ -182,6 +185,45 @@ def test_wall_timesteps_match_oracle(cfa, oracle_lib):
     print("after 3 steps", errs, "wall displacement max", np.abs(U[wn]).max())
     assert np.abs(U[wn]).max() > 0
     assert all(e <= 1e-8 for e in errs.values()), errs
+
+
+def test_wall_graph_timestep_equals_eager(cfa):
+    import torch
+    from crimsonflowsolver_amd import mesh as M
+    from crimsonflowsolver_amd.timestep import ResistanceCylinder, ns_solver
+    m = M.cylinder_mesh(5, 5, 8)
+    rng = np.random.default_rng(12)
+    uref = rng.normal(size=(m.numnp, 3)) * 1e-3
+    tag = np.zeros(m.numnp, np.int32)
+    tag[m.wall & (m.inlet | m.outlet)] = 1
+    mk = lambda: ns_solver(relTol=1e-6, dimKry=40, maxItr=6, relTolIn=(0.01, 0.2), maxItrIn=(1, 200))
+    c = ResistanceCylinder(m, wall=cfa.wall_params(), nodetag=tag, uref=uref, ls=mk())
+    y0, ac0 = c.state()
+    u0 = torch.zeros((3, m.numnp), dtype=torch.float64, device="cuda")
+    p = c.params
+    states = []
+    y, ac, u = y0.clone(), ac0.clone(), u0.clone()
+    for n in range(3):
+        y, ac, u, _ = c.step_wall(y, ac, u, (n + 1) * p.Delt, iters=3)
+        states.append((y.clone(), ac.clone(), u.clone()))
+    ls_g = mk()
+    gy0, gac0, gu0 = y0.clone(), ac0.clone(), u0.clone()
+    gy, gac, gya, gaca = (torch.empty_like(y0) for _ in range(4))
+    gu = u0.clone()
+    res4 = torch.zeros((c.N, 4), dtype=torch.float64, device="cuda")
+    bc = c.bc(p.Delt)
+    c.form.set_wall_state(gu0, gu)
+    g = c.form.step_graph(c.lhs, ls_g, gy0, gac0, gy, gac, gya, gaca, c.dx, res4, c.dibc, bc,
+                          surfaces=[(c.dglob, c.dnabi, c.R, 0.0)], faceres=c.faceres, nonlinear=3)
+    for n in range(3):
+        bc.copy_(c.bc((n + 1) * p.Delt))
+        torch.cuda.synchronize()
+        g.launch()
+        ok = [torch.equal(gy, states[n][0]), torch.equal(gac, states[n][1]), torch.equal(gu, states[n][2])]
+        print(f"timestep {n + 1}: y, ac, u bitwise equal {ok}; displacement max {float(gu.abs().max()):.3e}",
+              flush=True)
+        assert all(ok), n
+        assert torch.equal(gu0, gu)  # uold = u at the end of the captured step
 
 
 def test_entry_points_without_the_wall_refuse_it(cfa):
