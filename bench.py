@@ -51,7 +51,7 @@ def build_problem(grid, cfa, M):
 FLOPS_PER_ELEMENT = 8996.0
 FLOPS_PER_NODE = 10.0
 # the formation's kernels as rocprofv3 names them (profiles/*_pmc_traffic.md)
-FORMATION_KERNELS = ("flux_elem", "flux_node", "e3_elem<true>", "asm_row<1,16>", "bnd_elem", "bnd_node<1>")
+FORMATION_KERNELS = ("node_pack", "flux_node", "e3_elem<true>", "asm_row<1,16>", "bnd_elem", "bnd_node<1>")
 
 
 def pmc_traffic(grid, kernel):
@@ -960,11 +960,12 @@ def main():
             "algorithmic_flops": flops, "duration_ms": form_ms,
             "traffic": sum(traffic) if all(t is not None for t in traffic) else None,
             "measured_peak": peaks["fp64_tflops"], "frac_of_measured": ach / peaks["fp64_tflops"]}
-    # assembly: the element record read once (48 doubles/element), every
-    # block written once (16 doubles/nnz), Res4 (4 doubles/node), and the
-    # incidence (4 B per element-node) and contribution-list (2 B per
-    # off-diagonal contribution, 12 per element) indices
-    byt = nel * 48 * 8 + nnz * 16 * 8 + N * 4 * 8 + nel * 4 * 4 + nel * 12 * 2
+    # assembly: each incidence record read once (12 doubles per element-node)
+    # with its element's nodes (16 B per element-node), the packed nodes
+    # once (6 doubles/node), every block written once (16 doubles/nnz), Res4
+    # (4 doubles/node) and the contribution lists (2 B per off-diagonal
+    # contribution, 12 per element)
+    byt = nel * 4 * (12 * 8 + 16) + N * 6 * 8 + nnz * 16 * 8 + N * 4 * 8 + nel * 12 * 2
     kname = "asm_row<1,8>" if tuned is not None and tuned["fp64"][0].split("/")[1] == "8" else "asm_row<1,16>"
     roof["dominant_kernel"] = {"kernel": kname, "bound": "hbm", "achieved": byt / (tm[2] * 1e-3) / 1e9,
                                "peak": HBM_PEAK_GBS, "unit": "GB/s",

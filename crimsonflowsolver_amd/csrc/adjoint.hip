@@ -301,10 +301,9 @@ __global__ void __launch_bounds__(256) vjp_q_node(int nshg, const int *__restric
 
 // Adjoint of the lumped projection: the element's viscous flux qdi enters
 // q at its nodes with weight m_a / rmass; qdi = mu (grad u + grad u^T)
-// (flux_elem's slots 11,12,13,22,23,33).
+// (elem_flux's slots 11,12,13,22,23,33).
 __global__ void __launch_bounds__(256) vjp_flux_elem(int nel, const int *__restrict__ ien,
                                                      const double *__restrict__ node,
-                                                     const double *__restrict__ frec,
                                                      const double *__restrict__ qbar, double mu0, Ens en,
                                                      double *__restrict__ vrec2) {
   const int e = xcd_block() * blockDim.x + threadIdx.x;
@@ -330,13 +329,12 @@ __global__ void __launch_bounds__(256) vjp_flux_elem(int nel, const int *__restr
 #pragma unroll
     for (int i = 0; i < 3; ++i)
       gu[j][i] = g[0][j] * ul[0][i] + g[1][j] * ul[1][i] + g[2][j] * ul[2][i] + g[3][j] * ul[3][i];
-  const double *fr = frec + (size_t)e * kFRec;
   double qd[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const double *qb = qbar + (size_t)nd[a] * 6;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) qd[k] += flux_weight(fr[0], a) * qb[k];
+    for (int k = 0; k < 6; ++k) qd[k] += flux_weight(W, a) * qb[k];
   }
   double gub[3][3];
   gub[0][0] = 2.0 * mu * qd[0];
@@ -459,7 +457,7 @@ int vjp_run(crimson_form *f, const double *d_y, const double *d_ac, const double
     vjp_e3_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, d_w, f->nshg, c, f->ens(), w.vrec);
   vjp_q_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, w.vrec, w.invm, w.qbar);
   if (f->nel > 0)
-    vjp_flux_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, f->d_frec.p, w.qbar, c.mu, f->ens(), w.vrec2);
+    vjp_flux_elem<<<ge, T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, w.qbar, c.mu, f->ens(), w.vrec2);
   vjp_node<<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, w.vrec, w.vrec2, d_ybar, d_acbar);
   double mb = 0.0;
   if (f->nel > 0) {

@@ -2,26 +2,30 @@
 // of the reference's ElmGMR, elmgmr.f90:82-255).
 //
 // Pipeline, all deterministic (no atomics; every sum runs in element order):
-//   flux_elem   element-parallel: metric + velocity gradients -> per-element
-//               flux record (lumped-mass weights and the viscous flux q_dI of
-//               e3q.f90:100-145)
+//   node_pack   node-parallel: x and u packed as 6 doubles per node
 //   flux_node   node-parallel gather over the node's elements in element
-//               order: qres, rmass, q = qres * (1/rmass)  (asiq.f90 + qpbc.f90);
+//               order, each incidence evaluating its element's metric and
+//               viscous flux q_dI (e3q.f90:100-145) from the packed nodes:
+//               qres, rmass, q = qres * (1/rmass)  (asiq.f90 + qpbc.f90);
 //               writes one 128-byte node record (x, u, p, ac, q)
 //   e3_elem     element-parallel e3 (e3.f90 .. e3lhs.f90) on the four node
 //               records: all four quadrature points in registers; writes a
-//               48-double element record: shape gradients, the scalar part
-//               S_ab of the 3x3 velocity blocks, the element residual rl, and
-//               four scalars from which every tangent entry is rebuilt
+//               12-double record per element incidence (e, a) at the
+//               incidence's place in the node-major incidence list: node
+//               a's row S_a. of the scalar part of the 3x3 velocity blocks,
+//               its residual row rl_a, and the element scalars Tc, Mu, Cm,
+//               tlW from which every tangent entry is rebuilt
 //   asm_row     row-parallel, 16 lanes per CSR row, one lane per block: the
-//               blocks of the row from the row's element incidences in element
-//               order (fillsparseI, fillsparse.f90:29-63), the -G^T column of
-//               Val4 from the same contributions (solfar.f90:225-235 without
-//               its transpose search) and the residual row (asigmr.f90:116)
+//               row's incidence records read as one contiguous run, the
+//               shape gradients rebuilt from the packed nodes, the blocks of
+//               the row from the row's element incidences in element order
+//               (fillsparseI, fillsparse.f90:29-63), the -G^T column of Val4
+//               from the same contributions (solfar.f90:225-235 without its
+//               transpose search) and the residual row (asigmr.f90:116)
 //   res_node    the residual rows alone when no tangent is formed
 //   e3_tangent32, asm_row32   the FP32 tangent of the mixed-precision form
 //
-// The element record factorisation is exact algebra on the reference's
+// The record factorisation is exact algebra on the reference's
 // tangent (e3lhs.f90:85-402, iconvflow = 2 branch :124-129) for P1
 // elements, whose metric and shape gradients do not vary over the element:
 //   K^{ij}_ab = delta_ij S_ab + Tc g_a,i g_b,j + Mu g_a,j g_b,i
@@ -48,26 +52,49 @@
 namespace {
 
 // ---------------------------------------------------------------- flux pass
-// e3q per element (e3q.f90:66-145): qdi = mu (grad u + grad u^T) is the same
-// at every quadrature point of a P1 element; the lumped-mass weights are
-// m_a = sum_q N_a(q) WdetJ.  Record (64 bytes): WdetJ, qdi in slots
-// (11,12,13,22,23,33), pad.
-__global__ void __launch_bounds__(256) flux_elem(int nel, const int *__restrict__ ien,
-                                                 const double *__restrict__ x, int numnp,
-                                                 const double *__restrict__ y, int nshg,
-                                                 double mu0, Ens en, double *__restrict__ frec) {
-  const int e = xcd_block() * blockDim.x + threadIdx.x;
-  if (e >= nel) return;
-  const double mu = ens_mu(en, e, mu0);
+// x and u of every node packed as kXU doubles, so that the gathers of the
+// flux pass and of the assembly take a node in three 16-byte loads.
+__global__ void __launch_bounds__(256) node_pack(int nshg, const double *__restrict__ x, int numnp,
+                                                 const double *__restrict__ y, double *__restrict__ xu) {
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= nshg) return;
+  const size_t N = nshg, NP = numnp;
+  double2 *o = reinterpret_cast<double2 *>(xu + (size_t)i * kXU);
+  o[0] = make_double2(x[i], x[i + NP]);
+  o[1] = make_double2(x[i + 2 * NP], y[i]);
+  o[2] = make_double2(y[i + N], y[i + 2 * N]);
+}
+
+// The coordinates of an element's four nodes (local order) from the packed
+// nodes.
+__device__ inline void gather_x(const double *__restrict__ xu, int4 n4, double (&xl)[4][3]) {
+  const int nd[4] = {n4.x, n4.y, n4.z, n4.w};
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const double *r = xu + (size_t)nd[a] * kXU;
+    const double2 v = *reinterpret_cast<const double2 *>(r);
+    xl[a][0] = v.x;
+    xl[a][1] = v.y;
+    xl[a][2] = r[2];
+  }
+}
+
+// e3q for one element (e3q.f90:66-145): qdi = mu (grad u + grad u^T) is the
+// same at every quadrature point of a P1 element.  Returns WdetJ; qd holds
+// the 6 distinct components in slots (11,12,13,22,23,33).
+__device__ inline double elem_flux(const double *__restrict__ xu, int4 n4, double mu, double (&qd)[6]) {
+  const int nd[4] = {n4.x, n4.y, n4.z, n4.w};
   double xl[4][3], ul[4][3];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int n = ien[a * (size_t)nel + e];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      xl[a][i] = x[n + (size_t)numnp * i];
-      ul[a][i] = y[n + (size_t)nshg * i];
-    }
+    const double2 *r = reinterpret_cast<const double2 *>(xu + (size_t)nd[a] * kXU);
+    const double2 v0 = r[0], v1 = r[1], v2 = r[2];
+    xl[a][0] = v0.x;
+    xl[a][1] = v0.y;
+    xl[a][2] = v1.x;
+    ul[a][0] = v1.y;
+    ul[a][1] = v2.x;
+    ul[a][2] = v2.y;
   }
   double dx[3][3], W, g[4][3];
   p1_metric(xl, dx, W, g);
@@ -77,20 +104,25 @@ __global__ void __launch_bounds__(256) flux_elem(int nel, const int *__restrict_
 #pragma unroll
     for (int i = 0; i < 3; ++i)
       gu[j][i] = g[0][j] * ul[0][i] + g[1][j] * ul[1][i] + g[2][j] * ul[2][i] + g[3][j] * ul[3][i];
-  double2 *r = reinterpret_cast<double2 *>(frec + (size_t)e * kFRec);
-  r[0] = make_double2(W, 2.0 * mu * gu[0][0]);
-  r[1] = make_double2(mu * (gu[0][1] + gu[1][0]), mu * (gu[0][2] + gu[2][0]));
-  r[2] = make_double2(2.0 * mu * gu[1][1], mu * (gu[1][2] + gu[2][1]));
-  r[3] = make_double2(2.0 * mu * gu[2][2], 0.0);
+  qd[0] = 2.0 * mu * gu[0][0];
+  qd[1] = mu * (gu[0][1] + gu[1][0]);
+  qd[2] = mu * (gu[0][2] + gu[2][0]);
+  qd[3] = 2.0 * mu * gu[1][1];
+  qd[4] = mu * (gu[1][2] + gu[2][1]);
+  qd[5] = 2.0 * mu * gu[2][2];
+  return W;
 }
 
-// qres/rmass gather and qpbc inversion.  q(nshg,9) column-major in the
-// reference's component order: (1,4,7) = (q11,q12,q13), ... so component
-// k = i + 3 (j-1) holds mu(du_i/dx_j + du_j/dx_i).  Writes the node record
-// (with the node's x, y, ac) and, when q is not null, q(nshg,9).
+// qres/rmass gather and qpbc inversion (asiq.f90, qpbc.f90): each of the
+// node's incidences (in element order, kNodeLanes lanes) evaluates its
+// element's flux from the packed nodes; nothing per element is stored.
+// q(nshg,9) column-major in the reference's component order: (1,4,7) =
+// (q11,q12,q13), ... so component k = i + 3 (j-1) holds mu(du_i/dx_j +
+// du_j/dx_i).  Writes the node record (with the node's x, y, ac) and, when q
+// is not null, q(nshg,9).
 __global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict__ inc_ptr,
-                                                 const int *__restrict__ inc,
-                                                 const double *__restrict__ frec,
+                                                 const int *__restrict__ inc, const int4 *__restrict__ ienp,
+                                                 const double *__restrict__ xu, double mu0, Ens en,
                                                  const double *__restrict__ x, int numnp,
                                                  const double *__restrict__ y, const double *__restrict__ ac,
                                                  double *__restrict__ node, double *__restrict__ q,
@@ -101,10 +133,10 @@ __global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict
   if (i < nshg) {
     for (int p = inc_ptr[i] + lane; p < inc_ptr[i + 1]; p += kNodeLanes) {
       const int ea = inc[p];
-      const double *r = frec + (size_t)(ea >> 2) * kFRec;
-      const double m = flux_weight(r[0], ea & 3);
+      double qd[6];
+      const double m = flux_weight(elem_flux(xu, ienp[p], ens_mu(en, ea >> 2, mu0), qd), ea & 3);
 #pragma unroll
-      for (int k = 0; k < 6; ++k) s[k] += m * r[1 + k];
+      for (int k = 0; k < 6; ++k) s[k] += m * qd[k];
       rm += m;
     }
   }
@@ -158,24 +190,24 @@ __global__ void __launch_bounds__(256) flux_node(int nshg, const int *__restrict
 }
 
 // ---------------------------------------------------------------- e3
-// With the tangent, the records of the workgroup's (consecutive) elements are
-// assembled in LDS and stored as one contiguous block with coalesced 16-byte
-// stores (written directly, each thread's 384-byte record reached HBM as
-// partially filled lines).
+// With the tangent, the element records of the workgroup's (consecutive)
+// elements are assembled in LDS and stored as one contiguous block with
+// coalesced 16-byte stores (written directly, each thread's 256-byte record
+// reached HBM as partially filled lines); the scalars (32 bytes per element)
+// are stored directly, contiguous across the threads.
 constexpr int kE3Threads = 128;
-constexpr bool kE3Stage = true;
 // LDS stride of a staged record: one double more than the record, so that
 // the threads' stores at the same offset fall in different banks.
 constexpr int kE3Pad = kRec + 1;
 template <bool LHS>
-__global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__restrict__ ien,
+__global__ void __launch_bounds__(kE3Threads) __attribute__((amdgpu_waves_per_eu(2))) e3_elem(int nel, const int *__restrict__ ien,
                                                       const double *__restrict__ node, FormConst c0, Ens en,
-                                                      double *__restrict__ rec) {
-  __shared__ double sh_rec[(LHS && kE3Stage) ? kE3Threads * kE3Pad : 1];
+                                                      double *__restrict__ rec, double *__restrict__ esc) {
+  __shared__ double sh_rec[LHS ? kE3Threads * kE3Pad : 1];
   const int e0 = xcd_block() * kE3Threads;
   const int e = e0 + threadIdx.x;
-  if (!(LHS && kE3Stage) && e >= nel) return;
-  const int ee = e < nel ? e : nel - 1;  // tail threads recompute a valid element; nothing of theirs is stored
+  if (!LHS && e >= nel) return;
+  const int ee = e < nel ? e : nel - 1;  // tail threads evaluate a valid element; nothing of theirs is stored
   FormConst c = c0;
   c.mu = ens_mu(en, ee, c0.mu);
   int nd[4];
@@ -339,19 +371,15 @@ __global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__rest
     }
   }
 
-  double *r = (LHS && kE3Stage) ? sh_rec + threadIdx.x * kE3Pad : rec + (size_t)e * kRec;
+  double *r = LHS ? sh_rec + threadIdx.x * kE3Pad : rec + (size_t)e * kRec;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    r[R_A(a) + 4 + 3] = W * (g[a][0] * SU[0] + g[a][1] * SU[1] + g[a][2] * SU[2]);
+    r[R_A(a) + R_RL + 3] = W * (g[a][0] * SU[0] + g[a][1] * SU[1] + g[a][2] * SU[2]);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-      r[R_A(a) + 4 + i] = -W * (RNa[a][i] + g[a][0] * RGN[i][0] + g[a][1] * RGN[i][1] + g[a][2] * RGN[i][2]);
+      r[R_A(a) + R_RL + i] = -W * (RNa[a][i] + g[a][0] * RGN[i][0] + g[a][1] * RGN[i][1] + g[a][2] * RGN[i][2]);
   }
   if (!LHS) return;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) r[R_G + a * 3 + j] = g[a][j];
   const double tlW = c.lhsFct * W;
   const double tsFct = c.lhmFct * W * rho;
   const double Mu = 4.0 * (tlW * mu);
@@ -367,11 +395,11 @@ __global__ void __launch_bounds__(kE3Threads) e3_elem(int nel, const int *__rest
       const double gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
       r[R_A(a) + b] = tsFct * mm(a, b) + rt * conv + Mu * gab + rt * SUPG[k];
     }
-  r[R_TLW] = tlW;
-  r[R_TC] = tlW * TC;
-  r[R_MU] = Mu;
-  r[R_CM] = tlW * TM / rho;
-  if (!kE3Stage) return;
+  if (e < nel) {
+    double2 *d = reinterpret_cast<double2 *>(esc + (size_t)e * kESc);
+    d[0] = make_double2(tlW * TC, Mu);     // Tc, Mu
+    d[1] = make_double2(tlW * TM / rho, tlW);  // Cm, tlW
+  }
   __syncthreads();
   const int cnt = min(kE3Threads, nel - e0);
   double2 *dst = reinterpret_cast<double2 *>(rec + (size_t)e0 * kRec);
@@ -662,9 +690,12 @@ __global__ void __launch_bounds__(256) res_node(int nshg, const int *__restrict_
   if (i < nshg)
     for (int p = inc_ptr[i] + lane; p < inc_ptr[i + 1]; p += kNodeLanes) {
       const int ea = inc[p];
-      const double *r = rec + (size_t)(ea >> 2) * kRec + R_A(ea & 3) + 4;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) rs[k] += r[k];
+      const double2 *r = reinterpret_cast<const double2 *>(rec + (size_t)(ea >> 2) * kRec + R_A(ea & 3) + R_RL);
+      const double2 r01 = r[0], r23 = r[1];
+      rs[0] += r01.x;
+      rs[1] += r01.y;
+      rs[2] += r23.x;
+      rs[3] += r23.y;
     }
 #pragma unroll
   for (int off = 1; off < kNodeLanes; off <<= 1)
@@ -693,8 +724,8 @@ __device__ inline void block_zero(BlockAcc<T> &A) {
 __device__ inline double sum_n_rt(int a) {
   return a == 0 ? sum_n(0) : a == 1 ? sum_n(1) : a == 2 ? sum_n(2) : sum_n(3);
 }
-// Staged incidence: record line 0 (g, tlW, Tc, Mu, Cm) at 0..15 and node a's
-// S row at 16..19 (the residual row goes straight to the lane that sums it).
+// Staged incidence: g, tlW, Tc, Mu, Cm at 0..15 and node a's S row at 16..19
+// (the residual row goes straight to the lane that sums it).
 constexpr int kStage = 20;
 constexpr int ST_S = 16;
 template <class T>
@@ -831,11 +862,12 @@ __global__ void slot_table_kernel(int n, const int *__restrict__ order, const in
 
 // Row-parallel assembly, kRowLanes lanes per CSR row; rows are taken in the
 // handle's schedule order (neighbouring rows together, so that the element
-// records they share are read from L2).  The row's element incidences are
-// taken in chunks of CH, in element order; each chunk's record data (line 0
-// plus the row node's S and residual half line) is staged in LDS with
-// whole-line loads, 8 lanes per record line, so every record line is read
-// once per row.  Then:
+// records and packed nodes they share are read from L2).  The row's element
+// incidences are taken in chunks of CH, in element order; lane t of a chunk
+// takes incidence t: it loads node a's S row and the element scalars,
+// rebuilds the element's shape gradients from its packed nodes (ienp: the
+// element's four nodes, stored per incidence so that a row reads them as one
+// contiguous run) and stages g, the scalars and the S row in LDS.  Then:
 //  - lane s accumulates off-diagonal block s (and s + 16, ... for long rows)
 //    from its contribution list (lst: local incidence << 4 | a << 2 | b, in
 //    element order, built once from the sparseloc positions): fillsparseI's
@@ -849,10 +881,10 @@ constexpr int kRowsPerBlock = 256 / kRowLanes;
 constexpr int kAsmChunk = 16;  // incidences staged per chunk (even, dividing kRowLanes)
 template <int MODE, int CH = kAsmChunk>
 __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict__ tab, const int *__restrict__ inc,
-                                               const int *__restrict__ lst_ptr,
-                                               const unsigned short *__restrict__ lst,
-                                               const double *__restrict__ rec, double *__restrict__ res,
-                                               AsmOut o, unsigned xc) {
+                                               const int4 *__restrict__ ienp, const double *__restrict__ xu,
+                                               const int *__restrict__ lst_ptr, const unsigned short *__restrict__ lst,
+                                               const double *__restrict__ rec, const double *__restrict__ esc,
+                                               double *__restrict__ res, AsmOut o, unsigned xc) {
   static_assert(CH % 2 == 0 && kRowLanes % CH == 0, "chunk of 2, 4, 8 or 16 incidences");
   __shared__ double st_all[kRowsPerBlock][CH][kStage];
   const int gid = xcd_block(xc) * blockDim.x + threadIdx.x;
@@ -880,12 +912,16 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict_
     BlockAcc<double> A;
     block_zero(A);
     // the block's list range and the first chunk's incidences, loaded
-    // together (lanes without a block read lst_ptr[0..1], an in-bounds pair)
+    // together (lanes without a block read lst_ptr[0..1], an in-bounds pair;
+    // lanes without an incidence read incidence 0, always present, and stage
+    // nothing of it)
     const int kq = own ? k0 + s : 0;
     int q = lst_ptr[kq];
     const int q1r = lst_ptr[kq + 1];
     int nt = min(CH, ninc);
-    int ea = lane < nt ? inc[p0 + lane] : 0;
+    int pi = lane < nt ? p0 + lane : 0;
+    int ea = inc[pi];
+    int4 n4 = ienp[pi];
     const int q1 = own ? q1r : q;
     LstBuf lb;
     lst_fill(lb, lst, q, q1);
@@ -896,53 +932,57 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict_
       // residual row in the same order: chunk c occupies lanes lo..lo+CH-1
       const int lo = CH == kRowLanes ? 0 : tb % kRowLanes;
       const int lon = CH == kRowLanes ? 0 : (tb + CH) % kRowLanes;
+      const bool mine = lane >= lo && lane < lo + CH;  // the lane holds a staging slot of this chunk
+      const int t = lane - lo;
       // the next chunk's incidences, in flight while this chunk is staged
       const int ntn = min(CH, ninc - tb - CH);
-      const int ean = lane >= lon && lane - lon < ntn ? inc[p0 + tb + CH + lane - lon] : 0;
-      wave_lds_sync();  // the previous chunk's readers are done
-      // line 0 (8 pieces of 16 bytes per record) and node a's S row.  No
-      // guard on the loads, so that they are all in flight together: lanes
-      // and pieces past the chunk's end read incidence 0 (ea = 0, always a
-      // valid record) into staging slots nothing reads.
-      double2 pc[CH * 8 / kRowLanes];
-#pragma unroll
-      for (int j = 0; j < CH * 8 / kRowLanes; ++j) {
-        const int t = (j * kRowLanes + lane) >> 3, part = lane & 7;
-        const int et = __shfl(ea, lo + t, kRowLanes);
-        pc[j] = *reinterpret_cast<const double2 *>(rec + (size_t)(et >> 2) * kRec + 2 * part);
-      }
-      const double *ra = rec + (size_t)(ea >> 2) * kRec + R_A(ea & 3);
+      const int pn = lane >= lon && lane - lon < ntn ? p0 + tb + CH + lane - lon : 0;
+      const int ean = inc[pn];
+      const int4 n4n = ienp[pn];
+      // node a's S row, the element scalars and the nodes' coordinates
+      const int a = ea & 3;
+      const double *ra = rec + (size_t)(ea >> 2) * kRec + R_A(a);
+      const double2 *sc = reinterpret_cast<const double2 *>(esc + (size_t)(ea >> 2) * kESc);
       const double2 s01 = *reinterpret_cast<const double2 *>(ra);
       const double2 s23 = *reinterpret_cast<const double2 *>(ra + 2);
+      const double2 sc0 = sc[0], sc1 = sc[1];  // (Tc, Mu), (Cm, tlW)
+      double xl[4][3];
+      gather_x(xu, n4, xl);
+      double dx[3][3], W, gl[4][3];
+      p1_metric(xl, dx, W, gl);
+      wave_lds_sync();  // the previous chunk's readers are done
+      if (mine) {
+        double *d = st[t];
 #pragma unroll
-      for (int j = 0; j < CH * 8 / kRowLanes; ++j) {
-        const int t = (j * kRowLanes + lane) >> 3, part = lane & 7;
-        st[t][2 * part] = pc[j].x;
-        st[t][2 * part + 1] = pc[j].y;
-      }
-      if (lane >= lo && lane < lo + CH) {
-        st[lane - lo][ST_S + 0] = s01.x;
-        st[lane - lo][ST_S + 1] = s01.y;
-        st[lane - lo][ST_S + 2] = s23.x;
-        st[lane - lo][ST_S + 3] = s23.y;
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) d[R_G + 3 * b + j] = gl[b][j];
+        d[R_TC] = sc0.x;
+        d[R_MU] = sc0.y;
+        d[R_CM] = sc1.x;
+        d[R_TLW] = sc1.y;
+        d[ST_S + 0] = s01.x;
+        d[ST_S + 1] = s01.y;
+        d[ST_S + 2] = s23.x;
+        d[ST_S + 3] = s23.y;
       }
       wave_lds_sync();
       // the incidence's own lane: diagonal contribution, and its residual
       // row, loaded here and added after the off-diagonal blocks so that the
       // load is not held in registers while the chunk is staged
-      const bool dl = g == 0 && lane >= lo && lane - lo < nt;
+      const bool dl = g == 0 && mine && t < nt;
       double2 rl01 = make_double2(0.0, 0.0), rl23 = rl01;
       if (dl) {
-        rl01 = *reinterpret_cast<const double2 *>(ra + 4);
-        rl23 = *reinterpret_cast<const double2 *>(ra + 6);
-        diag_add(D, st[lane - lo], ea & 3);
+        rl01 = *reinterpret_cast<const double2 *>(ra + R_RL);
+        rl23 = *reinterpret_cast<const double2 *>(ra + R_RL + 2);
+        diag_add(D, st[t], a);
       }
       while (q < q1) {
         if (lb.n == 0) lst_fill(lb, lst, q, q1);  // blocks with more than 8 contributions
         const unsigned v = lst_front(lb);
-        const int t = (int)(v >> 4) - tb;
-        if (t >= CH) break;
-        block_add(A, st[t], (int)((v >> 2) & 3u), (int)(v & 3u));
+        const int tt = (int)(v >> 4) - tb;
+        if (tt >= CH) break;
+        block_add(A, st[tt], (int)((v >> 2) & 3u), (int)(v & 3u));
         lst_pop(lb);
         ++q;
       }
@@ -954,6 +994,7 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict_
       }
       nt = ntn;
       ea = ean;
+      n4 = n4n;
     }
     if (own) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
   }
@@ -1360,11 +1401,11 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
   const bool t32 = lhs && f->prm.precision == 32;
   if (f->nel > 0) {
     if (lhs && !t32)
-      e3_elem<true><<<crimson::grid_for(f->nel, kE3Threads), kE3Threads, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p,
-                                                                                  c, f->ens(), f->d_rec.p);
+      e3_elem<true><<<crimson::grid_for(f->nel, kE3Threads), kE3Threads, 0, s>>>(
+          f->nel, f->d_ien.p, f->d_node.p, c, f->ens(), f->d_rec.p, f->d_esc.p);
     else
-      e3_elem<false><<<crimson::grid_for(f->nel, kE3Threads), kE3Threads, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p,
-                                                                                   c, f->ens(), f->d_rec.p);
+      e3_elem<false><<<crimson::grid_for(f->nel, kE3Threads), kE3Threads, 0, s>>>(
+          f->nel, f->d_ien.p, f->d_node.p, c, f->ens(), f->d_rec.p, f->d_esc.p);
     if (t32)
       e3_tangent32<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, f->d_node.p, c, f->ens(),
                                                               (float)f->prm.aspect_fp64, f->d_rec32.p);
@@ -1390,18 +1431,18 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
       res_node<1><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
   } else if (mode == 0) {
     if (f->asm_chunk == 8)
-      asm_row<0, 8><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
-                                     d_res, o, f->asm_xcd);
+      asm_row<0, 8><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_ienp.p, f->d_xu.p, f->d_lst_ptr.p,
+                                     f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd);
     else
-      asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
-                                  d_res, o, f->asm_xcd);
+      asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_ienp.p, f->d_xu.p, f->d_lst_ptr.p,
+                                  f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd);
   } else {
     if (f->asm_chunk == 8)
-      asm_row<1, 8><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
-                                     d_res, o, f->asm_xcd);
+      asm_row<1, 8><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_ienp.p, f->d_xu.p, f->d_lst_ptr.p,
+                                     f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd);
     else
-      asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_lst_ptr.p, f->d_lst.p, f->d_rec.p,
-                                  d_res, o, f->asm_xcd);
+      asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_ienp.p, f->d_xu.p, f->d_lst_ptr.p,
+                                  f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd);
   }
   // the wall's rows and tangent (AsBMFG's xKebe goes through bc3lhs too)
   CR_TRY(crimson::form_wall_pass(f, lhs, mode, o.lhsK, o.V, mode == 1 && t32 ? o.V32 : nullptr, o.nnz));
@@ -1432,17 +1473,17 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
 }  // namespace
 
 namespace crimson {
-// The flux pass (flux_elem + flux_node) on the handle's stream: fills the
-// flux and node records; q(nshg,9) and 1/rmass(nshg) when requested.
+// The flux pass (node_pack + flux_node) on the handle's stream: fills the
+// packed nodes and the node records; q(nshg,9) and 1/rmass(nshg) when
+// requested.
 int form_flux_pass(crimson_form *f, const double *d_y, const double *d_ac, const double *d_x, double *d_q,
                    double *d_invm) {
   const unsigned T = 256;
   hipStream_t s = f->stream;
-  if (f->nel > 0)
-    flux_elem<<<crimson::grid_for(f->nel, T), T, 0, s>>>(f->nel, f->d_ien.p, d_x, f->numnp, d_y, f->nshg,
-                                                         f->prm.mu, f->ens(), f->d_frec.p);
+  node_pack<<<crimson::grid_for(f->nshg, T), T, 0, s>>>(f->nshg, d_x, f->numnp, d_y, f->d_xu.p);
   flux_node<<<crimson::grid_for((size_t)f->nshg * kNodeLanes, T), T, 0, s>>>(
-      f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_frec.p, d_x, f->numnp, d_y, d_ac, f->d_node.p, d_q, d_invm);
+      f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_ienp.p, f->d_xu.p, f->prm.mu, f->ens(), d_x, f->numnp, d_y, d_ac,
+      f->d_node.p, d_q, d_invm);
   CR_HIP(hipGetLastError());
   return CRIMSON_OK;
 }
@@ -1630,6 +1671,16 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
     }
   std::vector<int> ien0((size_t)nel * 4);
   for (size_t k = 0; k < ien0.size(); ++k) ien0[k] = ien[k] - 1;
+  // per incidence, its element's nodes (the gathers of the flux pass and of
+  // the assembly's shape gradients, read per row as one contiguous run).  One
+  // spare entry (node 0) when there are no elements, so that every read of
+  // entry 0 is in bounds.
+  std::vector<int4> ienp(std::max<size_t>(inc.size(), 1), make_int4(0, 0, 0, 0));
+  for (size_t p = 0; p < inc.size(); ++p) {
+    const int e = inc[p] >> 2;
+    ienp[p] = make_int4(ien0[e], ien0[(size_t)nel + e], ien0[2 * (size_t)nel + e], ien0[3 * (size_t)nel + e]);
+  }
+  if (inc.empty()) inc.push_back(0);  // the same spare entry (element 0's records are always allocated)
   hipStream_t s = f->stream;
   if ((rc = f->d_ien.upload(ien0.data(), ien0.size(), s)) ||
       (rc = f->d_inc_ptr.upload(inc_ptr.data(), inc_ptr.size(), s)) ||
@@ -1638,9 +1689,10 @@ extern "C" int crimson_form_setup(int nshg, int numnp, int nel, const int *ien, 
       (rc = f->d_rowptr.upload(rp.data(), rp.size(), s)) ||
       (rc = f->d_lst_ptr.upload(lst_ptr.data(), lst_ptr.size(), s)) ||
       (rc = f->d_lst.upload(lst.data(), lst.size(), s)) ||
-      (rc = f->d_frec.alloc((size_t)std::max(nel, 1) * kFRec)) ||
+      (rc = f->d_ienp.upload(ienp.data(), ienp.size(), s)) || (rc = f->d_xu.alloc((size_t)nshg * kXU)) ||
       (rc = f->d_node.alloc((size_t)nshg * kNRec)) || (rc = f->d_q.alloc((size_t)nshg * 9)) ||
-      (rc = f->d_rec.alloc((size_t)std::max(nel, 1) * kRec)))
+      (rc = f->d_rec.alloc((size_t)std::max(nel, 1) * kRec)) ||
+      (rc = f->d_esc.alloc((size_t)std::max(nel, 1) * kESc)))
     return fail(rc);
   if (prm->precision == 32 && (rc = f->d_rec32.alloc((size_t)std::max(nel, 1) * kRec32))) return fail(rc);
   if (iBC) {
@@ -1833,7 +1885,8 @@ std::vector<unsigned long long> graph_signature(const crimson_form *f, const cri
   std::vector<unsigned long long> v = {
       f->d_ien.serial, f->d_inc_ptr.serial, f->d_inc.serial, f->d_diagk.serial, f->d_rowptr.serial,
       f->d_ibc.serial, f->d_lst_ptr.serial, f->d_order.serial, f->d_slot.serial, f->d_lst.serial,
-      f->d_frec.serial, f->d_node.serial, f->d_rec.serial, f->d_rec32.serial, f->d_bc.serial,
+      f->d_ienp.serial, f->d_xu.serial, f->d_node.serial, f->d_rec.serial, f->d_esc.serial,
+      f->d_rec32.serial, f->d_bc.serial,
       f->d_bcl.serial, f->d_bcn.serial, f->d_inst.serial, f->d_mu_inst.serial, f->d_binst.serial,
       f->d_ienb.serial, f->d_bcode.serial, f->d_bcb.serial, f->d_permit.serial, f->d_brec.serial,
       f->d_bflx.serial, f->d_sptr.serial, f->d_selem.serial, f->d_bnode.serial, f->d_binc_ptr.serial,

@@ -26,14 +26,22 @@ __host__ __device__ constexpr double mm(int a, int b) {
   return nq(0, a) * nq(0, b) + nq(1, a) * nq(1, b) + nq(2, a) * nq(2, b) + nq(3, a) * nq(3, b);
 }
 
-// Element record: 48 doubles = three 128-byte lines.  Line 0 holds what
-// every tangent entry needs (shape gradients and the four element scalars);
-// node a's S row and residual sit together in half a line at R_A(a).
-constexpr int kRec = 48;
-constexpr int R_G = 0;                                // g[a][j] at a*3+j (12)
+// Element record: 32 doubles = two 128-byte lines; node a's S row and
+// residual row sit together in half a line at R_A(a).  The shape gradients
+// are not stored: the assembly rebuilds them from the element's nodes.
+constexpr int kRec = 32;
+__host__ __device__ constexpr int R_A(int a) { return 8 * a; }  // S[a][0..3], rl[a][0..3]
+constexpr int R_RL = 4;
+// Element scalars: 4 doubles per element (Tc, Mu, Cm, tlW), from which with
+// the shape gradients every tangent entry is rebuilt.
+constexpr int kESc = 4;
+// Staged incidence in the assembly (LDS): g[a][j] at a*3+j (12), tlW, Tc,
+// Mu, Cm, then node a's S row.
+constexpr int R_G = 0;
 constexpr int R_TLW = 12, R_TC = 13, R_MU = 14, R_CM = 15;
-__host__ __device__ constexpr int R_A(int a) { return 16 + 8 * a; }  // S[a][0..3], rl[a][0..3]
-constexpr int kFRec = 8;  // flux record: half a line (WdetJ, qdi[6], pad)
+// Packed node coordinates and velocity, 6 doubles per node (node_pack), the
+// gathers of the flux pass and of the assembly's shape gradients.
+constexpr int kXU = 6;
 // lumped-mass weight of local node a, m_a = sum_q N_a(q) WdetJ (e3q.f90:120-145)
 __host__ __device__ inline double flux_weight(double W, int a) {
   return a == 0   ? nq(0, 0) * W + nq(1, 0) * W + nq(2, 0) * W + nq(3, 0) * W
@@ -178,7 +186,9 @@ struct crimson_form {
   int asm_chunk = 16;                       // incidences per staged chunk of the FP64 assembly
   unsigned asm_xcd = 0;                     // assembly workgroups dealt per XCD turn (0: one range each)
   crimson::DBuf<unsigned short> d_lst;
-  crimson::DBuf<double> d_frec, d_node, d_q, d_rec;
+  crimson::DBuf<int4> d_ienp;    // per incidence (node-major): the element's four nodes in local order
+  crimson::DBuf<double> d_xu;    // (kXU, nshg): x and u of each node (node_pack)
+  crimson::DBuf<double> d_node, d_q, d_rec, d_esc;  // node records; q; element records and scalars
   crimson::DBuf<float> d_rec32;  // FP32 tangent records (precision 32)
   crimson::DBuf<double> s_y, s_ac, s_x, s_res, s_K, s_P, s_q;  // staging of the host API
   bool has_ibc = false;
