@@ -54,10 +54,79 @@ FLOPS_PER_NODE = 10.0
 FORMATION_KERNELS = ("node_pack", "flux_node", "e3_elem<true>", "asm_row<1,16>", "bnd_elem", "bnd_node<1>")
 
 
+def pmc_kernel_name(name):
+    """A rocprofv3 kernel name without its return type, namespace and
+    arguments, spaces removed (``asm_row<1,16>``)."""
+    n = name[5:] if name.startswith("void ") else name
+    n = n.replace("(anonymous namespace)::", "")
+    depth = 0
+    for i, ch in enumerate(n):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0 and i > 0:
+            n = n[:i]
+            break
+    return n.replace(" ", "")
+
+
+def measure_traffic(grid, steps=5, timeout=600):
+    """HBM bytes per dispatch of the formation bench's kernels, measured now:
+    two child runs of this script's formation alone on the same workload
+    under ``rocprofv3 --pmc`` (FETCH_SIZE, then WRITE_SIZE), each run starting
+    with a plain 1 GiB device copy whose counter reading gives the bytes per
+    counter unit.  Returns {kernel: bytes per dispatch} averaged over the
+    last ``steps`` dispatches (the timed steps), or None when rocprofv3 is
+    missing or a run leaves no counters."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    from collections import defaultdict
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None
+    child = [sys.executable, "-u", os.path.abspath(__file__), "--grid", ",".join(str(g) for g in grid),
+             "--steps", str(steps), "--warmup", "1", "--timesteps", "0", "--cpu-seconds", "0", "--mixed", "0",
+             "--ensemble", "0", "--adjoint", "0", "--config1", "0", "--stretched", "0", "--ensemble-timestep", "0",
+             "--reference-case", "0", "--traffic", "0", "--copy-mb", "1024"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    both = defaultdict(lambda: [0.0, 0.0])
+    for side, counter in enumerate(("FETCH_SIZE", "WRITE_SIZE")):
+        d = tempfile.mkdtemp(prefix="crimson_pmc_", dir="/tmp")
+        try:
+            subprocess.run([exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--"] + child,
+                           cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           timeout=timeout)
+            path = os.path.join(d, "pmc_counter_collection.csv")
+            if not os.path.exists(path):
+                return None
+            per = defaultdict(lambda: defaultdict(float))  # kernel -> dispatch -> counter
+            with open(path) as fh:
+                for row in csv.DictReader(fh):
+                    if row["Counter_Name"] == counter:
+                        per[pmc_kernel_name(row["Kernel_Name"])][int(row["Dispatch_Id"])] += float(row["Counter_Value"])
+            copies = per.get("__amd_rocclr_copyBuffer", {})
+            cal = max(copies.values()) if copies else 0.0
+            if cal <= 0.0:
+                return None
+            scale = float(1 << 30) / cal  # bytes per counter unit
+            for k, dv in per.items():
+                vals = [dv[i] for i in sorted(dv)][-steps:]
+                both[k][side] = sum(vals) / len(vals) * scale
+        except (OSError, subprocess.SubprocessError, ValueError, KeyError):
+            return None
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    return {k: v[0] + v[1] for k, v in both.items()}
+
+
 def pmc_traffic(grid, kernel):
     """HBM bytes per dispatch of ``kernel`` from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated against a
-    plain device copy), or None when no summary exists for it."""
+    plain device copy), or None when no summary exists for it: the fallback
+    when the live measurement (measure_traffic) is off or unavailable."""
     path = os.path.join(ROOT, "profiles", f"formation_{grid[0]}x{grid[1]}x{grid[2]}_pmc_traffic.md")
     if not os.path.exists(path):
         return None
@@ -429,6 +498,9 @@ def main():
                          "and keep the fastest")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MB first (calibrates PMC byte counters)")
+    ap.add_argument("--traffic", type=int, default=1,
+                    help="measure roofline.traffic at the end with two rocprofv3 PMC child runs (0: read the "
+                         "committed profile)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the formation/solve path here has no multi-GPU form "
@@ -1005,6 +1077,24 @@ def main():
                                  "threads": P}
             log(f"first timestep on the full mesh: GPU {ts_info['timestep_ms_fp64_each'][0]:.0f} ms, CPU "
                 f"{cs * 1e3:.0f} ms on {P} threads -> x{cs * 1e3 / ts_info['timestep_ms_fp64_each'][0]:.1f}")
+
+    # ---- roofline.traffic: HBM bytes per launch measured now (two PMC
+    # child runs of the formation on this workload), else from the committed
+    # profile of the same command
+    live = None
+    if args.traffic:
+        t0 = time.time()
+        live = measure_traffic(grid)
+        log(f"traffic: {'measured' if live else 'not measured (rocprofv3 unavailable)'} ({time.time() - t0:.0f}s)")
+    if live:
+        ft = [live.get(k.replace(" ", "")) for k in FORMATION_KERNELS]
+        roof["traffic"] = sum(ft) if all(t is not None for t in ft) else None
+        roof["traffic_by_kernel"] = {k: live.get(k.replace(" ", "")) for k in FORMATION_KERNELS}
+        roof["dominant_kernel"]["traffic"] = live.get(roof["dominant_kernel"]["kernel"].replace(" ", ""))
+        roof["traffic_source"] = "measured: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE child runs of this workload"
+    else:
+        roof["traffic_source"] = (f"profiles/formation_{grid[0]}x{grid[1]}x{grid[2]}_pmc_traffic.md"
+                                  if roof["traffic"] is not None else None)
 
     line = {
         "metric": METRIC, "value": melems, "unit": "Melem/s", "n_gpus": 1, "steps": args.steps,
