@@ -615,16 +615,20 @@ def main():
     y4 = torch.empty_like(x4)
     ms3 = lhs.spmv_dev(3, x3, y3, max(1, args.spmv_reps))
     ms4 = lhs.spmv_dev(4, x4, y4, max(1, args.spmv_reps))
-    b3 = nnz * (72 + 4) + N * (8 + 24 + 24)
-    b4 = nnz * (128 + 4) + N * (8 + 32 + 32)
+    # memLS reads each block's column as a 2-byte offset from its row when
+    # every offset fits 16 bits, else as a 4-byte index
+    rows = np.repeat(np.arange(N), np.diff(colm))
+    cb = 2 if np.abs((np.asarray(rowp) - 1) - rows).max() <= 32767 else 4
+    b3 = nnz * (72 + cb) + N * (8 + 24 + 24)
+    b4 = nnz * (128 + cb) + N * (8 + 32 + 32)
     gbs3, gbs4 = b3 / (ms3 * 1e-3) / 1e9, b4 / (ms4 * 1e-3) / 1e9
     log(f"spmv 3x3: {ms3:.4f} ms {gbs3:.0f} GB/s ; 4x4: {ms4:.4f} ms {gbs4:.0f} GB/s")
     spmv32 = None
     if args.mixed:
         m3 = lhs32.spmv_dev(3, x3, y3, max(1, args.spmv_reps))
         m4 = lhs32.spmv_dev(4, x4, y4, max(1, args.spmv_reps))
-        c3 = nnz * (36 + 4) + N * (4 + 12 + 12)
-        c4 = nnz * (64 + 4) + N * (4 + 16 + 16)
+        c3 = nnz * (36 + cb) + N * (4 + 12 + 12)
+        c4 = nnz * (64 + cb) + N * (4 + 16 + 16)
         spmv32 = {"bsr3x3": c3 / (m3 * 1e-3) / 1e9, "bsr4x4": c4 / (m4 * 1e-3) / 1e9, "ms3": m3, "ms4": m4}
         log(f"fp32 spmv 3x3: {m3:.4f} ms {spmv32['bsr3x3']:.0f} GB/s ; 4x4: {m4:.4f} ms {spmv32['bsr4x4']:.0f} GB/s")
 

@@ -208,6 +208,10 @@ struct crimson_lhs {
   hipStream_t stream = nullptr;
   std::vector<int> h_rowptr, h_col, h_diag, h_tr;  // 0-based
   crimson::DBuf<int> d_rowptr, d_col, d_diag, d_tr;
+  // the column of each block as its offset from the row (col - row), when
+  // every offset fits 16 bits: the products read 2 bytes per block instead
+  // of 4 (empty otherwise)
+  crimson::DBuf<short> d_col16;
   int vdof = 0;                   // block size of the stored matrix
   int prec = 64;                  // 64: d_V holds the matrix; 32: d_V32 does
   crimson::DBuf<double> d_V;      // planes [vdof*vdof][nnz]

@@ -65,11 +65,16 @@ struct MatView {
   int wdof, r0, c0;
 };
 
+// The column of block k of `row`: stored as is (int) or as its offset from
+// the row (short, lhs->d_col16).
+__device__ inline int col_at(const int *__restrict__ col, int k, int row) { return col[k]; }
+__device__ inline int col_at(const short *__restrict__ col, int k, int row) { return row + (int)col[k]; }
+
 // LANES lanes per CSR row: 16 for the 3x3 and 4x4 products, 4 for the
 // narrow ones (G, D, L) whose per-entry work is too small to feed 16 lanes.
-template <int NR, int NC, class T, int LANES>
+template <int NR, int NC, class T, int LANES, class IX>
 __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restrict__ rowptr,
-                                                   const int *__restrict__ col, MatView<T> A,
+                                                   const IX *__restrict__ col, MatView<T> A,
                                                    const T *__restrict__ x, T *__restrict__ y, T ya,
                                                    const T *__restrict__ z, T za,
                                                    const double *__restrict__ skip) {
@@ -83,7 +88,7 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
   if (row < nNo) {
     const int ke = rowptr[row + 1];
     for (int k = rowptr[row] + lane; k < ke; k += LANES) {
-      const int j = col[k];
+      const int j = col_at(col, k, row);
       T xv[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) xv[c] = A.W[A.c0 + c + (size_t)A.wdof * j] * x[c + (size_t)NC * j];
@@ -190,9 +195,9 @@ __global__ void scale_p_kernel(int n, const T *__restrict__ W, int wdof, int off
 // from Ps (fpart[f * kFaceDotBlocks + b]: the b-th workgroup's share, entries
 // s = b * 256 + t + 1024 u), so that no launch between the two row kernels
 // waits for t1; the rest form t1.
-template <class T, int U = kNarrowUnroll>
+template <class T, class IX, int U = kNarrowUnroll>
 __global__ void __launch_bounds__(256) g_product_kernel(int nNo, const int *__restrict__ rowptr,
-                                                        const int *__restrict__ col, MatView<T> G,
+                                                        const IX *__restrict__ col, MatView<T> G,
                                                         const T *__restrict__ Ps, T *__restrict__ t1,
                                                         T *__restrict__ t1s, int nfc,
                                                         const FaceDev *__restrict__ fd,
@@ -241,7 +246,7 @@ __global__ void __launch_bounds__(256) g_product_kernel(int nNo, const int *__re
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         kc[u] = min(k + kNarrowLanes * u, ke - 1);
-        j[u] = col[kc[u]];
+        j[u] = col_at(col, kc[u], row);
       }
       T xv[U], a0[U], a1[U], a2[U];
 #pragma unroll
@@ -275,9 +280,9 @@ __global__ void __launch_bounds__(256) g_product_kernel(int nNo, const int *__re
     }
   }
 }
-template <class T, int U = kNarrowUnroll>
+template <class T, class IX, int U = kNarrowUnroll>
 __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restrict__ rowptr,
-                                                    const int *__restrict__ col, MatView<T> Lv, MatView<T> Gv,
+                                                    const IX *__restrict__ col, MatView<T> Lv, MatView<T> Gv,
                                                     const T *__restrict__ P, const T *__restrict__ Ps,
                                                     const T *__restrict__ t1s, T *__restrict__ SP,
                                                     const double *__restrict__ done) {
@@ -296,7 +301,7 @@ __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restri
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         kc[u] = min(k + kNarrowLanes * u, ke - 1);
-        j[u] = col[kc[u]];
+        j[u] = col_at(col, kc[u], row);
       }
       T l[U], ps[U], b0[U], b1[U], b2[U], x0[U], x1[U], x2[U];
 #pragma unroll
@@ -859,12 +864,20 @@ int spmv(Ctx<T> &c, const MatView<T> &A, int nr, int nc, const T *x, T *y, doubl
   const unsigned G = crimson::grid_for((size_t)c.N * kRowLanes, TB);
   const unsigned Gn = crimson::grid_for((size_t)c.N * kNarrowLanes, TB);
   const int *rp = c.lhs->d_rowptr.p, *cl = c.lhs->d_col.p;
-#define CR_SPMV(R_, C_)                                                                              \
-  do {                                                                                                \
-    if (R_ * C_ >= 9)                                                                                 \
-      spmv_kernel<R_, C_, T, kRowLanes><<<G, TB, 0, c.s>>>(c.N, rp, cl, A, x, y, (T)ya, z, (T)za, skip); \
-    else                                                                                              \
-      spmv_kernel<R_, C_, T, kNarrowLanes><<<Gn, TB, 0, c.s>>>(c.N, rp, cl, A, x, y, (T)ya, z, (T)za, skip); \
+  const short *c16 = c.lhs->d_col16.p;
+#define CR_SPMV_IX(R_, C_, CL_)                                                                           \
+  do {                                                                                                     \
+    if (R_ * C_ >= 9)                                                                                      \
+      spmv_kernel<R_, C_, T, kRowLanes><<<G, TB, 0, c.s>>>(c.N, rp, CL_, A, x, y, (T)ya, z, (T)za, skip);   \
+    else                                                                                                   \
+      spmv_kernel<R_, C_, T, kNarrowLanes><<<Gn, TB, 0, c.s>>>(c.N, rp, CL_, A, x, y, (T)ya, z, (T)za, skip); \
+  } while (0)
+#define CR_SPMV(R_, C_)                  \
+  do {                                   \
+    if (c16)                             \
+      CR_SPMV_IX(R_, C_, c16);           \
+    else                                 \
+      CR_SPMV_IX(R_, C_, cl);            \
   } while (0)
   if (nr == 3 && nc == 3) CR_SPMV(3, 3);
   else if (nr == 4 && nc == 4) CR_SPMV(4, 4);
@@ -879,6 +892,7 @@ int spmv(Ctx<T> &c, const MatView<T> &A, int nr, int nc, const T *x, T *y, doubl
     return CRIMSON_EUNSUPPORTED;
   }
 #undef CR_SPMV
+#undef CR_SPMV_IX
   CR_HIP(hipGetLastError());
   return CRIMSON_OK;
 }
@@ -1290,13 +1304,21 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
   auto schur_op = [&](const T *Pv, T *SPv, const double *scal) -> int {
     const double *done = scal + 5;
     const unsigned gr = crimson::grid_for((size_t)N * kNarrowLanes, 256);
-    g_product_kernel<T><<<gr + nfc * kFaceDotBlocks, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2.p, t1.p,
-                                                                     ws.tmp.p, nfc, c.fdev.p, c.fpart.p, done);
+    if (l->d_col16.p)
+      g_product_kernel<T, short><<<gr + nfc * kFaceDotBlocks, 256, 0, c.s>>>(
+          N, l->d_rowptr.p, l->d_col16.p, mG, t2.p, t1.p, ws.tmp.p, nfc, c.fdev.p, c.fpart.p, done);
+    else
+      g_product_kernel<T, int><<<gr + nfc * kFaceDotBlocks, 256, 0, c.s>>>(
+          N, l->d_rowptr.p, l->d_col.p, mG, t2.p, t1.p, ws.tmp.p, nfc, c.fdev.p, c.fpart.p, done);
     for (int f = 0; f < nfc; ++f)
       if (c.hfdev[f].nNo > 0)
         face_pre_apply_kernel<T><<<crimson::grid_for(c.hfdev[f].nNo, 256), 256, 0, c.s>>>(
           c.hfdev[f], c.fpart.p + f * kFaceDotBlocks, t1.p, ws.tmp.p, c.W.p, 4, done);
-    schur_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2.p, ws.tmp.p, SPv, done);
+    if (l->d_col16.p)
+      schur_kernel<T, short><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col16.p, mL, Gt, Pv, t2.p, ws.tmp.p, SPv,
+                                                  done);
+    else
+      schur_kernel<T, int><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2.p, ws.tmp.p, SPv, done);
     CR_HIP(hipGetLastError());
     return CRIMSON_OK;
   };
@@ -2056,8 +2078,12 @@ int nssolver_graph(NsGraphWork<T> &w, T *Ri) {
   auto schur_op = [&](const T *Pv, T *SPv, const double *scal) -> int {
     const double *done = scal + 5;
     const unsigned gr = crimson::grid_for((size_t)N * kNarrowLanes, 256);
-    g_product_kernel<T><<<gr + nfc * kFaceDotBlocks, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mG, t2, t1, tmp,
-                                                                     nfc, c.fdev.p, c.fpart.p, done);
+    if (l->d_col16.p)
+      g_product_kernel<T, short><<<gr + nfc * kFaceDotBlocks, 256, 0, c.s>>>(
+          N, l->d_rowptr.p, l->d_col16.p, mG, t2, t1, tmp, nfc, c.fdev.p, c.fpart.p, done);
+    else
+      g_product_kernel<T, int><<<gr + nfc * kFaceDotBlocks, 256, 0, c.s>>>(
+          N, l->d_rowptr.p, l->d_col.p, mG, t2, t1, tmp, nfc, c.fdev.p, c.fpart.p, done);
     if (w.disjoint && nfc > 1)
       face_pre_apply_multi_kernel<T><<<dim3(crimson::grid_for(std::max(1, w.maxfn), 256), nfc), 256, 0, c.s>>>(
           c.fdev.p, c.fpart.p, t1, tmp, c.W.p, 4, done);
@@ -2066,7 +2092,10 @@ int nssolver_graph(NsGraphWork<T> &w, T *Ri) {
         if (c.hfdev[f].nNo > 0)
           face_pre_apply_kernel<T><<<crimson::grid_for(c.hfdev[f].nNo, 256), 256, 0, c.s>>>(
               c.hfdev[f], c.fpart.p + f * kFaceDotBlocks, t1, tmp, c.W.p, 4, done);
-    schur_kernel<T><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2, tmp, SPv, done);
+    if (l->d_col16.p)
+      schur_kernel<T, short><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col16.p, mL, Gt, Pv, t2, tmp, SPv, done);
+    else
+      schur_kernel<T, int><<<gr, 256, 0, c.s>>>(N, l->d_rowptr.p, l->d_col.p, mL, Gt, Pv, t2, tmp, SPv, done);
     CR_HIP(hipGetLastError());
     return CRIMSON_OK;
   };
@@ -2504,6 +2533,20 @@ extern "C" int crimson_lhs_create(int gnNo, int nNo, int nnz, const int *gNodes,
       (rc = l->d_diag.upload(l->h_diag.data(), nNo, l->stream)) ||
       (rc = l->d_tr.upload(l->h_tr.data(), nnz, l->stream)))
     return fail(rc);
+  {
+    std::vector<short> c16(nnz);
+    bool fits = true;
+    for (int i = 0; i < nNo && fits; ++i)
+      for (int k = l->h_rowptr[i]; k < l->h_rowptr[i + 1]; ++k) {
+        const int d = l->h_col[k] - i;
+        if (d < -32768 || d > 32767) {
+          fits = false;
+          break;
+        }
+        c16[k] = (short)d;
+      }
+    if (fits && (rc = l->d_col16.upload(c16.data(), nnz, l->stream))) return fail(rc);
+  }
   if (hipStreamSynchronize(l->stream) != hipSuccess) {
     crimson::set_error("upload of the lhs structure failed");
     return fail(CRIMSON_EHIP);

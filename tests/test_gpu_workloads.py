@@ -9,7 +9,9 @@ state with 5 % seeded noise):
     the oracle's memLS on the same system (convergence flag and iteration
     counts equal; the GPU's Gram-Schmidt is classical + DGKS where memLS's
     is modified, so the non-converging GMRESV is compared by the level its
-    residual stalls at).
+    residual stalls at); the NS reading again with the nodes numbered at
+    random (4-byte block columns in memLS's products instead of 2-byte
+    offsets from the row).
 
 64x64x320 cylinder (6,197,760 tets, 1,077,597 nodes) and the workloads built
 on it, where the oracle cannot run over the whole mesh in seconds:
@@ -118,6 +120,52 @@ def test_cylinder_20x20x100_solve_matches_oracle(env, oracle_lib, cyl20, reading
         ratio = (g["fNorm"] / g["iNorm"]) / (o["fNorm"] / o["iNorm"])
         print("stall level gpu/oracle", ratio)
         assert 0.5 <= ratio <= 2.0
+
+
+def test_cylinder_20x20x100_ns_solve_scattered_numbering(env, oracle_lib, cyl20):
+    """The NS reading on the same cylinder with its nodes numbered at random:
+    some block columns lie more than 32,767 rows away, so memLS keeps 4-byte
+    column indices for its products, where the numbered cylinder's products
+    read 2-byte offsets from the row; against the oracle on the renumbered
+    system (the same checks as the NS reading above)."""
+    cfa, _ = env
+    from crimsonflowsolver_amd import mesh as M
+    m = cyl20["m"]
+    n = m.numnp
+    perm = np.random.default_rng(2020).permutation(n)  # new number of old node i
+    inv = np.argsort(perm)
+    ien = np.asfortranarray((perm[np.asarray(m.ien) - 1] + 1).astype(np.int32))
+    x = np.asfortranarray(m.x[inv])
+    y0, ac0 = M.poiseuille_state(m)
+    y, ac = np.asfortranarray(y0[inv]), np.asfortranarray(ac0[inv])
+    ibc = np.ascontiguousarray(cyl20["ibc"][inv])
+    bc = np.asfortranarray(cyl20["bc"][inv])
+    colm, rowp = cfa.genadj(ien, n)
+    rows = np.repeat(np.arange(n), np.diff(colm))
+    assert np.abs((rowp - 1) - rows).max() > 32767
+    rres, rK, rP = oracle_lib.elmgmr_interior(ien, y, ac, x, colm, rowp, oracle_lib.make_params())
+    R = np.asfortranarray(oracle_lib.bc3res(ibc, rres, bc).T)
+    val = oracle_lib.pack_val4(colm, rowp, rK, rP, mt=True)
+    kw = dict(relTol=1e-6, dimKry=50, maxItr=10)
+    fixed = np.nonzero(ibc)[0] + 1
+    zero = np.zeros((3, fixed.size), order="F")
+    ls = cfa.memLS_LS_CREATE(cfa.LS_TYPE_NS, **kw)
+    lhs = cfa.memLS_LHS_CREATE(n, n, rowp.size, None, colm, rowp, 1)
+    cfa.memLS_BC_CREATE(lhs, 1, fixed.size, 3, cfa.BC_TYPE_Dir, fixed, zero)
+    Ri = R.copy(order="F")
+    cfa.memLS_SOLVE(lhs, ls, 4, Ri, val)
+    ref = oracle_lib.MemLS(colm, rowp, 1, oracle_lib.LS_TYPE_NS, mt=True, **kw)
+    ref.bc_create(1, fixed, 3, oracle_lib.BC_TYPE_Dir, zero)
+    xo = ref.solve(R, val)
+    g, o = ls.RI, ref.stats(0)
+    print("scattered numbering: gpu", g, "oracle", o)
+    assert g["suc"] == o["suc"] == 1
+    assert abs(g["iNorm"] - o["iNorm"]) <= 1e-12 * o["iNorm"]
+    for sub, name in ((0, "RI"), (1, "GM"), (2, "CG")):
+        gi, oi = getattr(ls, name)["itr"], ref.stats(sub)["itr"]
+        print(name, "itr gpu", gi, "oracle", oi)
+        assert abs(gi - oi) <= max(2, 0.02 * oi), (name, gi, oi)
+    assert np.abs(Ri - xo).max() <= 1e-4 * np.abs(xo).max()
 
 
 # ------------------------------------------------------------ 64x64x320
