@@ -69,6 +69,29 @@ struct MatView {
 // the row (short, lhs->d_col16).
 __device__ inline int col_at(const int *__restrict__ col, int k, int row) { return col[k]; }
 __device__ inline int col_at(const short *__restrict__ col, int k, int row) { return row + (int)col[k]; }
+// One 4-wide node entry (32 bytes in FP64, 16 in FP32; aligned) in 16-byte
+// pieces.
+__device__ inline void load4(const double *__restrict__ p, double (&z)[4]) {
+  const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+  z[0] = a.x;
+  z[1] = a.y;
+  z[2] = b.x;
+  z[3] = b.y;
+}
+__device__ inline void load4(const float *__restrict__ p, float (&z)[4]) {
+  const float4 a = *reinterpret_cast<const float4 *>(p);
+  z[0] = a.x;
+  z[1] = a.y;
+  z[2] = a.z;
+  z[3] = a.w;
+}
+__device__ inline void store4(double *__restrict__ p, const double (&z)[4]) {
+  reinterpret_cast<double2 *>(p)[0] = make_double2(z[0], z[1]);
+  reinterpret_cast<double2 *>(p)[1] = make_double2(z[2], z[3]);
+}
+__device__ inline void store4(float *__restrict__ p, const float (&z)[4]) {
+  *reinterpret_cast<float4 *>(p) = make_float4(z[0], z[1], z[2], z[3]);
+}
 
 // LANES lanes per CSR row: 16 for the 3x3 and 4x4 products, 4 for the
 // narrow ones (G, D, L) whose per-entry work is too small to feed 16 lanes.
@@ -117,8 +140,9 @@ __global__ void __launch_bounds__(256) spmv_kernel(int nNo, const int *__restric
 // The Schur product of NSSOLVER's CG (memLS.f90:1719-1735) after G P:
 // SP = L P - Gt t1 with t1 = G P, both scaled views of the same CSR rows, in
 // one pass: SP_i = W_i sum_j L_ij (W_j P_j) + fix_i P_i - W_i sum_j Gt_ij (W_j t1_j)
-// with Ps = W o P and t1s = W o t1 precomputed (the same operations, in the
-// same order, as the two separate products).
+// with Ps = W o P and W o t1 precomputed, gathered per column as one entry
+// Z(0:3, j) = (Ps_j, W o t1_j) (the same operations, in the same order, as
+// the two separate products).
 constexpr int kNarrowLanes = 4;
 
 // A coupled face of the Schur CG (ADDBCMUL's PRE, memLS.f90:1719-1735):
@@ -144,8 +168,10 @@ constexpr int kFaceDotBlocks = 4;
 // The Schur operator's vectors pre-multiplied by their column scaling, so
 // that its two row kernels gather no W: Ps = W_p o P (written by the CG's
 // direction update, scale_p_kernel for the first direction); the G product
-// writes t1 = W_u o (G Ps) and t1s = W_u o t1 (the operand of Gt).  Same
-// products in the same order as the unscaled kernels.
+// writes t1 = W_u o (G Ps) and, per node i, Z(0:3, i) = (Ps_i, W_u o t1_i):
+// the Schur pass gathers both of its operands of a column as one 4-wide
+// entry (two 16-byte loads in FP64, one in FP32).  Same products in the
+// same order as the unscaled kernels.
 
 // Wave-level sum (64 lanes) then across the 4 waves of a 256-thread group.
 template <class T>
@@ -270,14 +296,17 @@ __global__ void __launch_bounds__(256) g_product_kernel(int nNo, const int *__re
 #pragma unroll
     for (int r = 0; r < 3; ++r) acc[r] += __shfl_xor(acc[r], off, kNarrowLanes);
   if (row < nNo && lane == 0) {
+    T z[4];
+    z[0] = Ps[row];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const T w = G.W[G.r0 + r + (size_t)G.wdof * row];
       T v = w * acc[r];
       v *= T(1);
       t1[r + (size_t)3 * row] = v;
-      t1s[r + (size_t)3 * row] = w * v;
+      z[1 + r] = w * v;
     }
+    store4(t1s + (size_t)4 * row, z);
   }
 }
 template <class T, class IX, int U = kNarrowUnroll>
@@ -307,13 +336,15 @@ __global__ void __launch_bounds__(256) schur_kernel(int nNo, const int *__restri
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         l[u] = lp[kc[u]];
-        ps[u] = Ps[j[u]];
         b0[u] = d0[kc[u]];
         b1[u] = d1[kc[u]];
         b2[u] = d2[kc[u]];
-        x0[u] = t1s[(size_t)3 * j[u]];
-        x1[u] = t1s[(size_t)3 * j[u] + 1];
-        x2[u] = t1s[(size_t)3 * j[u] + 2];
+        T z[4];
+        load4(t1s + (size_t)4 * j[u], z);  // (Ps_j, W_u o t1_j)
+        ps[u] = z[0];
+        x0[u] = z[1];
+        x1[u] = z[2];
+        x2[u] = z[3];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -553,7 +584,7 @@ __global__ void face_axpy_kernel(int n, int xdof, int fdof, const int *__restric
 // A coupled face on the device, with its PRE coefficient -res/(1 + res nS)
 // (memLS.f90:1096-1118 with BCPRE's nS).
 // ADDBCMUL's PRE on the Schur CG's G product t1 (3, N): face by face
-// t1 += coef_f S_f valM_f, with t1s = W_u o t1 rewritten at the same nodes
+// t1 += coef_f S_f valM_f, with Z(1:3) = W_u o t1 rewritten at the same nodes
 // (one launch per face, since faces may share nodes).  S_f, every face's dot
 // of the unmodified t1, is the sum of its kFaceDotBlocks partials from the G
 // product (summed here in a fixed order).
@@ -573,7 +604,7 @@ __global__ void __launch_bounds__(256) face_pre_apply_kernel(FaceDev F, const do
   const size_t g = F.glob[a];
   for (int i = 0; i < nc; ++i) t1[i + 3 * g] += (T)(F.valM[i + (size_t)F.dof * a] * sc);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) t1s[c + 3 * g] = W[c + (size_t)wdof * g] * t1[c + 3 * g];
+  for (int c = 0; c < 3; ++c) t1s[1 + c + 4 * g] = W[c + (size_t)wdof * g] * t1[c + 3 * g];
 }
 // The same three face operations for every coupled face in one launch, when
 // the faces share no node (then the faces' updates commute): one
@@ -631,7 +662,7 @@ __global__ void __launch_bounds__(256) face_pre_apply_multi_kernel(const FaceDev
   const size_t g = F.glob[a];
   for (int i = 0; i < nc; ++i) t1[i + 3 * g] += (T)(F.valM[i + (size_t)F.dof * a] * sc);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) t1s[c + 3 * g] = W[c + (size_t)wdof * g] * t1[c + 3 * g];
+  for (int c = 0; c < 3; ++c) t1s[1 + c + 4 * g] = W[c + (size_t)wdof * g] * t1[c + 3 * g];
 }
 // The face functional's coefficients q (FaceDev), one thread per support
 // column, its pairs in their stored order, in FP64.
@@ -1229,7 +1260,7 @@ int nssolver_dev(Ctx<T> &c, crimson_ls *ls, T *Ri) {
   CR_TRY(t4.alloc(N));
   CR_TRY(dxB.alloc(nB));
   Workspace<T> ws;
-  CR_TRY(ws.tmp.alloc(mn));
+  CR_TRY(ws.tmp.alloc(std::max(mn, (size_t)4 * N)));  // GMRES scratch; the Schur CG's Z(4, N)
   const unsigned gN = crimson::grid_for(N, 256);
   extract_kernel<<<gN, 256, 0, c.s>>>(N, 4, 0, 3, Ri, Rmi.p);
   extract_kernel<<<gN, 256, 0, c.s>>>(N, 4, 3, 1, Ri, Rci.p);
@@ -2211,7 +2242,7 @@ int ns_graph_setup(NsGraphWork<T> &w, crimson_lhs *l, crimson_ls *ls, const int 
   CR_TRY(w.t2.alloc(N));
   CR_TRY(w.t3.alloc(N));
   CR_TRY(w.t4.alloc(N));
-  CR_TRY(w.gmTmp.alloc(mn));
+  CR_TRY(w.gmTmp.alloc(std::max(mn, (size_t)4 * N)));  // GMRES scratch; the Schur CG's Z(4, N)
   CR_TRY(w.gmU.alloc(mn * (w.sD + 1)));
   if (sizeof(T) != sizeof(double)) CR_TRY(w.Rw.alloc((size_t)4 * N));
   size_t o = G_ARR;
