@@ -52,8 +52,8 @@
 namespace {
 
 // ---------------------------------------------------------------- flux pass
-// x and u of every node packed as kXU doubles, so that the gathers of the
-// flux pass and of the assembly take a node in three 16-byte loads.
+// x and u of every node packed as kXU doubles: the flux pass gathers a
+// node's x and u in three 16-byte loads, the assembly its x in two loads.
 __global__ void __launch_bounds__(256) node_pack(int nshg, const double *__restrict__ x, int numnp,
                                                  const double *__restrict__ y, double *__restrict__ xu) {
   const int i = xcd_block() * blockDim.x + threadIdx.x;

@@ -35,8 +35,9 @@ constexpr int R_RL = 4;
 // Element scalars: 4 doubles per element (Tc, Mu, Cm, tlW), from which with
 // the shape gradients every tangent entry is rebuilt.
 constexpr int kESc = 4;
-// Staged incidence in the assembly (LDS): g[a][j] at a*3+j (12), tlW, Tc,
-// Mu, Cm, then node a's S row.
+// Staged incidence in the FP64 assembly (LDS), and the first 16 floats of
+// the FP32 tangent record: g[a][j] at a*3+j (12), tlW, Tc, Mu, Cm; the FP64
+// stage then holds node a's S row.
 constexpr int R_G = 0;
 constexpr int R_TLW = 12, R_TC = 13, R_MU = 14, R_CM = 15;
 // Packed node coordinates and velocity, 6 doubles per node (node_pack), the
