@@ -71,14 +71,15 @@ def pmc_kernel_name(name):
     return n.replace(" ", "")
 
 
-def measure_traffic(grid, steps=5, timeout=600):
+def measure_traffic(grid, steps=5, timeout=180):
     """HBM bytes per dispatch of the formation bench's kernels, measured now:
     two child runs of this script's formation alone on the same workload
     under ``rocprofv3 --pmc`` (FETCH_SIZE, then WRITE_SIZE), each run starting
     with a plain 1 GiB device copy whose counter reading gives the bytes per
     counter unit.  Returns {kernel: bytes per dispatch} averaged over the
     last ``steps`` dispatches (the timed steps), or None when rocprofv3 is
-    missing or a run leaves no counters."""
+    missing, this process already runs under a profiler, or a run leaves no
+    counters."""
     import csv
     import shutil
     import subprocess
@@ -86,6 +87,11 @@ def measure_traffic(grid, steps=5, timeout=600):
     from collections import defaultdict
     exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(exe):
+        return None
+    # not from a process that rocprofv3 already instruments (it exports its
+    # ROCPROF* settings to the application): the child profiler would
+    # inherit the tool
+    if any(k.startswith("ROCPROF") for k in os.environ):
         return None
     child = [sys.executable, "-u", os.path.abspath(__file__), "--grid", ",".join(str(g) for g in grid),
              "--steps", str(steps), "--warmup", "1", "--timesteps", "0", "--cpu-seconds", "0", "--mixed", "0",
