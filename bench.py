@@ -413,10 +413,14 @@ def implicit_adjoint(cfa, M, torch, m, nsteps, every, R=1333.0):
     y, ac = y0, ac0
     for n in range(nsteps):
         y, ac, _ = case.step(y, ac, (n + 1) * case.params.Delt, iters=3)
+        if (n + 1) % 10 == 0:
+            log(f"implicit adjoint: forward step {n + 1} of {nsteps}")
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
-    J, yb, ab, mub, rb = adj.gradient(y0, ac0, 0.0, nsteps, every, cy, ca, step_kw=dict(iters=3))
+    J, yb, ab, mub, rb = adj.gradient(y0, ac0, 0.0, nsteps, every, cy, ca, step_kw=dict(iters=3),
+                                      progress=lambda ph, n: log(f"implicit adjoint gradient: {ph} pass "
+                                                                 f"{'done' if ph == 'forward' else f'back to step {n}'}"))
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
     its = [s[0] for s in adj.stats]

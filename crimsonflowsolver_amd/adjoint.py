@@ -234,10 +234,12 @@ class StepAdjoint:
         acbar0[:3] = cf * acbar1[:3] - (1.0 - p.almi + p.almi * cf) * ab[:3]
         return ybar0, acbar0, -mub, -rbar
 
-    def gradient(self, y0, ac0, t0, nsteps, every, cy, ca, step_kw=None):
+    def gradient(self, y0, ac0, t0, nsteps, every, cy, ca, step_kw=None, progress=None):
         """J = <cy, y_N> + <ca, ac_N> after nsteps steps from (y0, ac0) at
         time t0 (step_kw: ResistanceCylinder.step's keywords).  Returns (J,
-        dJ/dy0, dJ/dac0, dJ/dmu, dJ/dR)."""
+        dJ/dy0, dJ/dac0, dJ/dmu, dJ/dR).  progress(phase, step), when given,
+        is called after the forward pass ("forward", nsteps) and after each
+        checkpoint segment's reverse sweep ("reverse", its first step)."""
         c = self.case
         kw = dict(step_kw or {})
         dt = c.params.Delt
@@ -248,6 +250,8 @@ class StepAdjoint:
                 ck[n] = (y.clone(), ac.clone())
             y, ac, _ = c.step(y, ac, t0 + (n + 1) * dt, **kw)
         J = float((cy * y).sum() + (ca * ac).sum())
+        if progress:
+            progress("forward", nsteps)
         yb, ab = cy.clone(), ca.clone()
         mub = rb = 0.0
         self.stats = []
@@ -262,4 +266,6 @@ class StepAdjoint:
                 mub += dm
                 rb += dr
             del states
+            if progress:
+                progress("reverse", s0)
         return J, yb, ab, mub, rb
