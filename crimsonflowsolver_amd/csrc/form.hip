@@ -620,6 +620,30 @@ struct AsmOut {
   size_t nnz;
 };
 
+// The 16 plane values of one block (row-major 4x4 of Val4): solfar.f90:216-221:
+// Val4(1:3)=lhsK(1:3), Val4(5:7)=lhsK(4:6), Val4(9:11)=lhsK(7:9),
+// Val4(13:15)=lhsP(1:3), Val4(16)=lhsP(4); Val4(4:12:4) = -lhsP(1:3) of the
+// transposed block (:225-235).
+template <class T>
+__device__ inline void block_planes(const T (&K)[9], const T (&Gk)[3], T C, const T (&Gtr)[3], T (&v)[16]) {
+  v[0] = K[0];
+  v[1] = K[1];
+  v[2] = K[2];
+  v[3] = -Gtr[0];
+  v[4] = K[3];
+  v[5] = K[4];
+  v[6] = K[5];
+  v[7] = -Gtr[1];
+  v[8] = K[6];
+  v[9] = K[7];
+  v[10] = K[8];
+  v[11] = -Gtr[2];
+  v[12] = Gk[0];
+  v[13] = Gk[1];
+  v[14] = Gk[2];
+  v[15] = C;
+}
+
 // Write one block given K (PHASTA column-major order m = i + 3j), the
 // pressure-gradient column G(k) = lhsP(1:3) of this block, C = lhsP(4), and
 // the transposed block's lhsP(1:3) (for Val4's -G^T column).
@@ -636,27 +660,12 @@ __device__ inline void put_block(const AsmOut &o, int k, const T (&K)[9], const 
     pp[2] = Gk[2];
     pp[3] = C;
   } else {
-    // solfar.f90:216-221: Val4(1:3)=lhsK(1:3), Val4(5:7)=lhsK(4:6),
-    // Val4(9:11)=lhsK(7:9), Val4(13:15)=lhsP(1:3), Val4(16)=lhsP(4);
-    // Val4(4:12:4) = -lhsP(1:3) of the transposed block (:225-235).
     const size_t S = o.nnz;
     T *V = (sizeof(T) == 4 ? (T *)o.V32 : (T *)o.V) + k;
-    V[0 * S] = K[0];
-    V[1 * S] = K[1];
-    V[2 * S] = K[2];
-    V[3 * S] = -Gtr[0];
-    V[4 * S] = K[3];
-    V[5 * S] = K[4];
-    V[6 * S] = K[5];
-    V[7 * S] = -Gtr[1];
-    V[8 * S] = K[6];
-    V[9 * S] = K[7];
-    V[10 * S] = K[8];
-    V[11 * S] = -Gtr[2];
-    V[12 * S] = Gk[0];
-    V[13 * S] = Gk[1];
-    V[14 * S] = Gk[2];
-    V[15 * S] = C;
+    T v[16];
+    block_planes(K, Gk, C, Gtr, v);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) V[m * S] = v[m];
   }
 }
 
@@ -884,7 +893,7 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict_
                                                const int4 *__restrict__ ienp, const double *__restrict__ xu,
                                                const int *__restrict__ lst_ptr, const unsigned short *__restrict__ lst,
                                                const double *__restrict__ rec, const double *__restrict__ esc,
-                                               double *__restrict__ res, AsmOut o, unsigned xc) {
+                                               double *__restrict__ res, AsmOut o, unsigned xc, int runs) {
   static_assert(CH % 2 == 0 && kRowLanes % CH == 0, "chunk of 2, 4, 8 or 16 incidences");
   __shared__ double st_all[kRowsPerBlock][CH][kStage];
   const int gid = xcd_block(xc) * blockDim.x + threadIdx.x;
@@ -903,13 +912,21 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict_
   }
   const int nchunk = (ninc + CH - 1) / CH;
   const int ngroup = (nslot + kRowLanes - 1) / kRowLanes;
+  // Staged plane stores (planes, chunks of 16): when the wave's four rows
+  // are consecutive rows (runs: the schedule is the numbering) of at most
+  // 16 blocks each, their blocks form one range of every plane; the wave
+  // gathers them, the diagonal blocks included, in its staging LDS after
+  // the rows are summed and stores each plane's range at once, instead of
+  // the off-diagonal blocks and, later, the diagonal ones into the holes
+  // they left.
+  const bool wstg = MODE == 1 && CH == kRowLanes && runs && __all(nslot <= kRowLanes);
   DiagAcc<double> D;
   diag_zero(D);
   double rs[4] = {0, 0, 0, 0};
+  BlockAcc<double> A;
   for (int g = 0; g < ngroup; ++g) {
     const int s = lane + g * kRowLanes;
     const bool own = s < nslot && s != sdiag;
-    BlockAcc<double> A;
     block_zero(A);
     // the block's list range and the first chunk's incidences, loaded
     // together (lanes without a block read lst_ptr[0..1], an in-bounds pair;
@@ -996,13 +1013,48 @@ __global__ void __launch_bounds__(256) asm_row(int nshg, const int4 *__restrict_
       ea = ean;
       n4 = n4n;
     }
-    if (own) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
+    if (own && !wstg) put_block<MODE>(o, k0 + s, A.K, A.Gij, A.C, A.Gji);
   }
   diag_reduce(D);
 #pragma unroll
   for (int off = kRowLanes / 2; off > 0; off >>= 1)
 #pragma unroll
     for (int k = 0; k < 4; ++k) rs[k] += __shfl_xor(rs[k], off, kRowLanes);
+  if constexpr (MODE == 1 && CH == kRowLanes) {
+    if (wstg) {
+      // the wave's range [kA, kB) of every plane (its live rows are
+      // consecutive, so their blocks are)
+      int kA = live ? k0 : 0x7fffffff, kB = live ? k0 + nslot : 0;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        kA = min(kA, __shfl_xor(kA, off, 64));
+        kB = max(kB, __shfl_xor(kB, off, 64));
+      }
+      const int nb = kB - kA;  // at most 4 rows x 16 blocks
+      double *ob = &st_all[threadIdx.x / 64 * (64 / kRowLanes)][0][0];  // the wave's rows' staging: [plane][block - kA]
+      wave_lds_sync();  // the last chunk's readers are done
+      double v[16];
+      if (lane < nslot && lane != sdiag) {
+        block_planes(A.K, A.Gij, A.C, A.Gji, v);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) ob[m * nb + k0 + lane - kA] = v[m];
+      }
+      if (live && lane == 0) {
+        double K[9];
+        diag_full(D, K);
+        block_planes(K, D.G, D.C, D.G, v);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) ob[m * nb + k0 + sdiag - kA] = v[m];
+        put_res<MODE>(res, row, nshg, rs);
+      }
+      wave_lds_sync();
+      const int wl = threadIdx.x % 64;
+      if (wl < nb)
+#pragma unroll
+        for (int m = 0; m < 16; ++m) o.V[m * o.nnz + kA + wl] = ob[m * nb + wl];
+      return;
+    }
+  }
   if (live && lane == 0) {
     double K[9];
     diag_full(D, K);
@@ -1414,6 +1466,7 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
   if (timed) CR_HIP(hipEventRecord(f->ev[2], s));
   const unsigned gn = crimson::grid_for((size_t)f->nshg * kNodeLanes, T);
   const unsigned gr = crimson::grid_for((size_t)f->nshg * kRowLanes, T);
+  const int runs = f->sched == kSchedNumbering;  // slot i is row i: a wave's rows are consecutive
   if (t32) {
     if (mode == 0) {
       res_node<0><<<gn, T, 0, s>>>(f->nshg, f->d_inc_ptr.p, f->d_inc.p, f->d_rec.p, d_res);
@@ -1432,17 +1485,17 @@ int run_formation(crimson_form *f, const double *d_y, const double *d_ac, const 
   } else if (mode == 0) {
     if (f->asm_chunk == 8)
       asm_row<0, 8><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_ienp.p, f->d_xu.p, f->d_lst_ptr.p,
-                                     f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd);
+                                     f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd, runs);
     else
       asm_row<0><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_ienp.p, f->d_xu.p, f->d_lst_ptr.p,
-                                  f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd);
+                                  f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd, runs);
   } else {
     if (f->asm_chunk == 8)
       asm_row<1, 8><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_ienp.p, f->d_xu.p, f->d_lst_ptr.p,
-                                     f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd);
+                                     f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd, runs);
     else
       asm_row<1><<<gr, T, 0, s>>>(f->nshg, f->d_slot.p, f->d_inc.p, f->d_ienp.p, f->d_xu.p, f->d_lst_ptr.p,
-                                  f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd);
+                                  f->d_lst.p, f->d_rec.p, f->d_esc.p, d_res, o, f->asm_xcd, runs);
   }
   // the wall's rows and tangent (AsBMFG's xKebe goes through bc3lhs too)
   CR_TRY(crimson::form_wall_pass(f, lhs, mode, o.lhsK, o.V, mode == 1 && t32 ? o.V32 : nullptr, o.nnz));
