@@ -1103,7 +1103,8 @@ def main():
     if args.traffic:
         t0 = time.time()
         live = measure_traffic(grid)
-        log(f"traffic: {'measured' if live else 'not measured (rocprofv3 unavailable)'} ({time.time() - t0:.0f}s)")
+        log(f"traffic: {'measured' if live else 'not measured (no rocprofv3, or running under it)'} "
+            f"({time.time() - t0:.0f}s)")
     if live:
         ft = [live.get(k.replace(" ", "")) for k in FORMATION_KERNELS]
         roof["traffic"] = sum(ft) if all(t is not None for t in ft) else None
